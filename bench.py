@@ -1,0 +1,265 @@
+#!/usr/bin/env python3
+"""Element-block hot-path benchmark on one MI355X.
+
+Workload (default): 60x60x60 8-node bricks (216,000 elements, 2x2x2 Gauss
+points with B-bar: 1,728,000 points), interior nodes jittered +-5 % of the
+edge (seeded), mm01 bilinear Mises (E = 200 GPa, nu = 0.3, sigma_y =
+350 MPa, H' = E/100), a seeded displacement increment at 0.75 % mean axial
+strain (the mid-point of a 10-step ramp to 1.5 %) that puts most points
+in active plastic flow.
+
+One step = one Newton iteration of the element-block hot path, all
+device-resident: strain/stress recovery + mm01 + element internal forces,
+the assembly of the element forces, the block tangents and their assembly
+into the VSS sparse matrix.  The metric is Gauss-point stress+tangent
+updates per second; `value` is the path with the FP32 tangent (the library's
+mixed-precision path), reported beside the plain FP64 path measured in the
+same run.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0         # MI355X HBM3E
+FP64_PEAK_TFS = 78.6          # MI355X FP64 (vector and matrix)
+FP32_MFMA_PEAK_TFS = 157.3    # MI355X dense FP32 matrix
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_workload(n, seed=2024):
+    from warp3d_amd import _abi
+    from warp3d_amd import mesh as M
+    m = M.box_hex8(n, jitter=0.05, seed=seed)
+    mat = _abi.mm01_material(200e3, 0.3, 350.0, hprime=2000.0)
+    arr = m.model(mat, auto_size=128)
+    x = m.coords
+    nd = 3 * m.nonode
+    base = np.abs(x[:, 2]) < 1e-9
+    cst = np.zeros(nd, np.int32)
+    for k in range(3):
+        cst[k::3][base] = 1
+    rng = np.random.default_rng(seed)
+    # seeded top-face field, mean axial strain 0.75 %, spread through the
+    # height as the linear-elastic solution would
+    top = rng.uniform(0.5, 1.5, size=m.nonode)
+    du = np.zeros_like(x)
+    du[:, 2] = 0.0075 * x[:, 2] * top
+    du[:, 0] = -0.3 * 0.0075 * (x[:, 0] - 0.5) * x[:, 2]
+    du[:, 1] = -0.3 * 0.0075 * (x[:, 1] - 0.5) * x[:, 2]
+    du[base] = 0.0
+    return m, arr, cst, du.ravel()
+
+
+def algorithmic(m, ncontrib, nout, ninv, ke_bytes):
+    """Algorithmic work per launch of each phase (bytes or flops)."""
+    ngp = m.noelem * 8
+    ne = m.noelem
+    return {
+        # per point: sigma_n 9 + eps_n 6 + hist_n 11 read, sigma 9 + eps 6 +
+        # hist 11 + [D] 21 written = 73 words; per element: coordinates and
+        # increments gathered (2 x 24 words), connectivity 8 ints, element
+        # forces 24 words and volume 1 word written
+        "recover": ("hbm", ngp * 73 * 8 + ne * ((24 + 24 + 24 + 1) * 8 + 32)),
+        # per structure node: its element list (8-byte offset + 4-byte
+        # stride, 3 words read each) and 3 words written
+        "ifv_gather": ("hbm", ninv * (12 + 24) + m.nonode * 24),
+        # upper-triangle B^T D B as bdbtgen forms it: 2(36 T + 6 utsz)
+        "tangent": ("flop", ngp * 2 * (36 * 24 + 6 * 300)),
+        # one index + one Ke word per contribution, the output pointer and
+        # the assembled word per output
+        "assemble": ("hbm", ncontrib * (4 + ke_bytes) + nout * (8 + 8)),
+    }
+
+
+def run_device(arr, cst, du, steps, warmup, precision):
+    from warp3d_amd import hip
+    from warp3d_amd.model import BlockModel
+    dev = BlockModel(arr)
+    neq, nco = dev.pattern(cst)
+    d_du = hip.DeviceArray.from_host(du)
+    d_u = hip.DeviceArray(du.size)
+    d_ifv = hip.DeviceArray(du.size)
+    d_kd = hip.DeviceArray(neq)
+    d_kc = hip.DeviceArray(max(nco, 1))
+    # state at the start of the timed region: one pseudo-update and one
+    # nonlinear update already done (iterations of the current step)
+    dev.recover_dev(1, 0, d_u.ptr, d_du.ptr, d_ifv.ptr)
+    dev.tangent(1, 0, precision)
+
+    def one(it):
+        dev.recover_dev(1, it, d_u.ptr, d_du.ptr, d_ifv.ptr)
+        dev.tangent(1, it, precision)
+        dev.assemble_vss_dev(d_kd.ptr, d_kc.ptr)
+
+    for w in range(warmup):
+        one(1 + w)
+    hip.synchronize()
+    dev.set_profiling(True)
+    dev.reset_phase_times()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        one(1 + warmup + s)
+    hip.synchronize()
+    t1 = time.perf_counter()
+    phases = dev.phase_times()
+    dev.set_profiling(False)
+    ifv = d_ifv.download()
+    info = {"neqns": neq, "ncoeff": nco, "ifv_norm": float(np.linalg.norm(ifv))}
+    return (t1 - t0) * 1e3 / steps, phases, info
+
+
+def contrib_counts(m, cst):
+    """Sizes of the assembly map (same rules as w3d_pattern)."""
+    # upper-triangle contributions per element: pairs (erow, ecol) with both
+    # dofs free and scol >= srow; for a brick every free pair counts once
+    conn = m.conn - 1
+    dofs = np.concatenate([conn * 3, conn * 3 + 1, conn * 3 + 2], axis=1)
+    free = cst[dofs] == 0
+    nf = free.sum(axis=1)
+    return int(np.sum(nf * (nf + 1) // 2))
+
+
+def cpu_baseline(arr, du, budget_s=10.0):
+    from oracle import oracle as O
+    threads = len(os.sched_getaffinity(0))
+    om = O.OracleModel(arr, threads=threads)
+    u = np.zeros_like(du)
+    om.recover(1, 0, u, du)
+    om.tangent(1, 0)
+    it = 0
+    t0 = time.perf_counter()
+    while True:
+        it += 1
+        om.recover(1, it, u, du)
+        om.tangent(1, it)
+        if time.perf_counter() - t0 > budget_s and it >= 2:
+            break
+    t1 = time.perf_counter()
+    ngp = arr.noelem * 8
+    return {"value": ngp * it / (t1 - t0), "unit": "GP updates/s",
+            "cores": threads, "kind": "port",
+            "sample": f"oracle (C, OpenMP over blocks of 128) recovery + mm01 "
+                      f"+ element forces + FP64 tangent on the full workload, "
+                      f"{it} iterations in {t1 - t0:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=60)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not "
+              "supported (the element-block path runs on a single device "
+              "here; see DESIGN.md for how it would shard)", file=sys.stderr)
+        return 2
+    from warp3d_amd import _abi, hip
+    if hip.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        return 1
+    t = time.perf_counter()
+    m, arr, cst, du = build_workload(args.n)
+    log(f"workload: {m.noelem} elements, {m.noelem * 8} points, "
+        f"{m.nonode} nodes ({time.perf_counter() - t:.1f} s)")
+    res = {}
+    for prec, name in ((_abi.W3D_FP64, "fp64"), (_abi.W3D_FP32, "fp32")):
+        t = time.perf_counter()
+        ms, phases, info = run_device(arr, cst, du, args.steps, args.warmup,
+                                      prec)
+        res[name] = (ms, phases, info)
+        log(f"{name}: {ms:.3f} ms/step  phases "
+            + ", ".join(f"{k} {v[1] / max(v[0], 1):.3f} ms" for k, v in
+                        phases.items() if v[0]) +
+            f"  ({time.perf_counter() - t:.1f} s)")
+    ngp = m.noelem * 8
+    ms32, ph32, info = res["fp32"]
+    ms64, ph64, _ = res["fp64"]
+    ninv = int(m.conn.size)
+    ncontrib = contrib_counts(m, cst)
+    nout = info["neqns"] + info["ncoeff"]
+    alg = algorithmic(m, ncontrib, nout, ninv, 4)
+    # dominant kernel of the reported path
+    avg = {k: v[1] / v[0] for k, v in ph32.items() if v[0]}
+    dom = max(avg, key=avg.get)
+    kind, work = alg.get(dom, ("hbm", 0))
+    dur_s = avg[dom] * 1e-3
+    if kind == "hbm":
+        achieved = work / dur_s / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None}
+    else:
+        achieved = work / dur_s / 1e12
+        roof = {"bound": "mfma", "achieved": round(achieved, 2),
+                "peak": FP32_MFMA_PEAK_TFS, "unit": "TFLOP/s",
+                "frac": round(achieved / FP32_MFMA_PEAK_TFS, 4),
+                "traffic": None}
+    roof["kernel"] = dom
+    phase_report = {}
+    for tag, ph, kb in (("fp32", ph32, 4), ("fp64", ph64, 8)):
+        a = algorithmic(m, ncontrib, nout, ninv, kb)
+        rep = {}
+        for k, (n, tot) in ph.items():
+            if not n:
+                continue
+            d = tot / n
+            entry = {"ms": round(d, 4)}
+            if k in a:
+                kd, w = a[k]
+                if kd == "hbm":
+                    entry["GB/s"] = round(w / (d * 1e-3) / 1e9, 1)
+                else:
+                    entry["TFLOP/s"] = round(w / (d * 1e-3) / 1e12, 2)
+            rep[k] = entry
+        phase_report[tag] = rep
+    out = {
+        "metric": "Gauss-pt stress+tangent updates/s",
+        "value": round(ngp / (ms32 * 1e-3), 1),
+        "unit": "GP updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms32, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (recovery, forces, assembly) + f32 (tangent)",
+        "data": "synthetic (seeded jittered mesh and displacement field)",
+        "config": {"workload": "hex8 60x60x60 box, 216000 elements, 1728000 "
+                               "Gauss points, mm01, blocks of 128, one Newton "
+                               "iteration of the element-block path per step",
+                   "elements": m.noelem, "gauss_points": ngp,
+                   "equations": info["neqns"], "matrix_terms": info["ncoeff"]},
+        "plain_fp64": {"value": round(ngp / (ms64 * 1e-3), 1),
+                       "ms_per_step": round(ms64, 4)},
+        "speedup_vs_fp64": round(ms64 / ms32, 3),
+        "roofline": roof,
+        "phases": phase_report,
+    }
+    if not args.no_cpu:
+        t = time.perf_counter()
+        out["cpu_baseline"] = cpu_baseline(arr, du)
+        log(f"cpu baseline done ({time.perf_counter() - t:.1f} s)")
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,264 @@
+/*
+ * w3d.h -- C ABI of libw3dblk, the MI355X (gfx950) element-block library.
+ *
+ * The library replaces the element-block hot path of WARP3D (rhdodds/warp3d):
+ * Gauss-point strain/stress recovery with the mm01 (bilinear Mises) and mm10
+ * (crystal plasticity) material updates, the element internal-force vector,
+ * the block tangent Ke = sum B^T D B w |J| and the assembly of the blocks into
+ * the upper-triangular "NASA-VSS" sparse matrix.
+ *
+ * Conventions (all follow the reference's Fortran data):
+ *   - every integer that names a node, element, block, dof or equation is
+ *     1-based, exactly as in the reference arrays it mirrors;
+ *   - every array is plain memory: host pointers in the w3d_* entry points,
+ *     device pointers in the w3d_*_dev entry points (device pointers must come
+ *     from hipMalloc on the context's device);
+ *   - every entry point returns W3D_OK (0), W3D_CUT_STEP (1: a material or
+ *     kinematic failure asks the caller to cut the load step) or a negative
+ *     error code; w3d_last_error() gives the message of the last failure on
+ *     the calling thread.
+ *
+ * Reference interfaces replaced (file:line under the reference's src/):
+ *   w3d_create            model set-up in incomp.f:14 / setup.f / inelbk.f:282
+ *   w3d_recover           drive_eps_sig_internal_forces.f:20
+ *   w3d_recover_blocks    do_nleps_block, drive_eps_sig_internal_forces.f:399
+ *   w3d_tangent           tanstf.f:17
+ *   w3d_tangent_blocks    do_nlek_block, tanstf.f:119
+ *   w3d_pattern           drive_assemble_solve.f:358 (ds_setup_sparsity_local)
+ *   w3d_assemble_vss      drive_assemble_solve.f:654 (ds_root_assembly ->
+ *                         assemble_code.f:659 assem_by_row)
+ *   w3d_commit_step       update.f:15 (n+1 -> n roll-over of block state)
+ *   w3d_get_block_state   the block arrays of mod_eleblocks.f:17-161
+ */
+#ifndef W3D_H
+#define W3D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define W3D_ABI_VERSION 1
+
+/* ---- status codes ---------------------------------------------------- */
+enum {
+  W3D_OK = 0,
+  W3D_CUT_STEP = 1,          /* material_cut_step = .true. in the reference */
+  W3D_ERR_ARG = -1,          /* invalid argument / inconsistent model       */
+  W3D_ERR_UNSUPPORTED = -2,  /* feature outside the implemented scope       */
+  W3D_ERR_DEVICE = -3,       /* HIP runtime failure                         */
+  W3D_ERR_NOMEM = -4,        /* device or host allocation failure           */
+  W3D_ERR_NUMERIC = -5       /* fatal numerical failure (reference die_abort) */
+};
+
+/* ---- element types: reference iprops(1,elem) (elprp.f) ---------------- */
+enum {
+  W3D_Q3DISOP = 1,  /* 20-node hexahedron                                 */
+  W3D_L3DISOP = 2,  /* 8-node hexahedron                                  */
+  W3D_TET10 = 6     /* 10-node tetrahedron                                */
+};
+
+/* ---- material models: reference iprops(25,elem) ---------------------- */
+enum {
+  W3D_MM01 = 1,     /* bilinear Mises, mixed iso/kinematic hardening      */
+  W3D_MM10 = 10     /* crystal plasticity (FCC, Voce hardening)           */
+};
+
+/* ---- tangent precision ----------------------------------------------- */
+enum {
+  W3D_FP64 = 0,     /* B^T D B formed and stored in FP64                  */
+  W3D_FP32 = 1      /* B^T D B formed on MFMA in FP32 with FP32 accumulate;
+                       Ke stored in FP32 and widened during assembly      */
+};
+
+/* ---- block state selectors for w3d_get_block_state ------------------- */
+enum {
+  W3D_STRESS_N = 0,   /* urcs_n_blocks   (9, ngp, span)                   */
+  W3D_STRESS_N1 = 1,  /* urcs_n1_blocks  (9, ngp, span)                   */
+  W3D_STRAIN_N = 2,   /* eps_n_blocks    (6, ngp, span)                   */
+  W3D_STRAIN_N1 = 3,  /* eps_n1_blocks   (6, ngp, span)                   */
+  W3D_HIST_N = 4,     /* history_blocks  (hist, ngp, span)                */
+  W3D_HIST_N1 = 5,    /* history1_blocks (hist, ngp, span)                */
+  W3D_CEP = 6,        /* cep_blocks      (21, span, ngp)                  */
+  W3D_ESTIFF = 7,     /* estiff_blocks   (utsz, span)                     */
+  W3D_EINFVEC = 8,    /* einfvec_blocks  (span, totdof)                   */
+  W3D_ELEM_VOL = 9    /* element_vol_blocks (span)                        */
+};
+
+/*
+ * One material.  The mm01 fields mirror the reference's REAL matprp rows
+ * (mod_main.f:373): e (1), nu (2), beta (3), tan_e (4), yld_pt (5); they are
+ * single precision on purpose, because the reference rounds them through
+ * REAL and forms H' = e*tan_e/(e - tan_e) in REAL (elprp.f:722,872).
+ * The mm10 parameters are double precision (mod_crystals.f:149-221).
+ */
+typedef struct w3d_material {
+  int32_t model;            /* W3D_MM01 or W3D_MM10                        */
+  /* mm01 */
+  float e, nu, beta, tan_e, yld_pt;
+  int32_t linear_elastic;   /* lprops(17): response always linear elastic  */
+  /* mm10 (FCC slip, Voce hardening); ignored for mm01 */
+  double c11, c12, c44;     /* cubic crystal elastic constants            */
+  double tau_y, tau_v;      /* Voce initial slip resistance, saturation gap */
+  double voce_m, theta_0;   /* Voce exponent m and initial hardening rate  */
+  double rate_n;            /* slip-rate sensitivity exponent (harden_n)   */
+  double iD_v;              /* Voce diffusion term (0 in the configs)      */
+  double atol, atol1, rtol, rtol1;  /* local Newton tolerances             */
+  int32_t miter;            /* local Newton iteration limit                */
+  int32_t angle_convention; /* 1 = Kocks (mm10_rotation_matrix)            */
+} w3d_material;
+
+/*
+ * The mesh and its element blocking.  Arrays are read during w3d_create and
+ * not retained.
+ */
+typedef struct w3d_model {
+  int32_t abi_version;      /* W3D_ABI_VERSION                              */
+  int32_t nonode;           /* number of structure nodes                    */
+  int32_t noelem;           /* number of elements                           */
+  const double* c;          /* (3*nonode) x,y,z of node n at c[3(n-1)+k]    */
+  const int32_t* incid;     /* packed element incidences, 1-based nodes     */
+  const int32_t* incmap;    /* (noelem) 1-based start of element e in incid */
+  const int32_t* etype;     /* (noelem) element type                        */
+  const int32_t* int_order; /* (noelem) integration order, iprops(5)        */
+  const int32_t* matnum;    /* (noelem) 1-based index into mats             */
+  const int32_t* bbar;      /* (noelem) mean-dilatation B-bar (hex8 only)   */
+  const double* angles;     /* (3*noelem) crystal Euler angles, degrees;
+                               may be NULL when no element uses mm10        */
+  int32_t nmat;
+  const w3d_material* mats;
+  int32_t nelblk;           /* >0: user blocking table below; 0: automatic  */
+  const int32_t* elblks;    /* (4*nelblk) span, felem, owner, unused        */
+  int32_t auto_size;        /* automatic block size (inelbk.f:219)          */
+  double eps_bbar;          /* B-bar stabilisation, indypm.f:713 (0.0)      */
+} w3d_model;
+
+typedef struct w3d_ctx w3d_ctx;
+
+/* Thread-local message of the last failing call (never NULL). */
+const char* w3d_last_error(void);
+
+/* Library version string. */
+const char* w3d_version(void);
+
+/*
+ * Build the device-resident model on HIP device `device`: blocking
+ * (inelbk.f:282 rules for automatic blocking, or the given table checked
+ * against the same compatibility rules), connectivity, dof destinations
+ * (edest), inverse incidences (setup.f:160-190), element constants and the
+ * zero initial state.
+ */
+int w3d_create(w3d_ctx** ctx, const w3d_model* model, int device);
+int w3d_destroy(w3d_ctx* ctx);
+
+/* Blocking result: nelblk and the (4, nelblk) elblks table. */
+int w3d_get_blocking(const w3d_ctx* ctx, int32_t* nelblk, int32_t* elblks);
+/* Per-block sizes: span, ngp, hist size, totdof, utsz. */
+int w3d_block_info(const w3d_ctx* ctx, int32_t blk, int32_t* info5);
+
+/*
+ * Strain/stress recovery, material update and internal forces for every
+ * block, then the deterministic assembly of the element forces into the
+ * structure vector ifv (length 3*nonode).  u is the displacement at n, du the
+ * increment n -> n+1 (both 3*nonode, structure dof order).
+ *   iter = 0 is the reference's pseudo-update (linear-elastic [D] unless
+ *   extrapolated != 0, rstgp1.f:835); history is scattered only for iter > 0
+ *   (rplstr.f).
+ * energy / plastic_work receive the block-summed internal energy and plastic
+ * work (rstgp1_b); either may be NULL.
+ * Returns W3D_CUT_STEP when any Gauss point asks for a step cut.
+ */
+int w3d_recover(w3d_ctx* ctx, int32_t step, int32_t iter, int32_t extrapolated,
+                const double* u, const double* du, double* ifv,
+                double* energy, double* plastic_work);
+int w3d_recover_dev(w3d_ctx* ctx, int32_t step, int32_t iter,
+                    int32_t extrapolated, const double* d_u,
+                    const double* d_du, double* d_ifv, double* energy,
+                    double* plastic_work);
+
+/*
+ * The per-block form of the recovery (do_nleps_block): processes blocks
+ * first_blk .. first_blk+nblk-1 and leaves the element forces in the block's
+ * einfvec array (read with w3d_get_block_state) without assembling them.
+ */
+int w3d_recover_blocks(w3d_ctx* ctx, int32_t first_blk, int32_t nblk,
+                       int32_t step, int32_t iter, int32_t extrapolated,
+                       const double* u, const double* du);
+
+/*
+ * Block tangent stiffness Ke = sum_gp B^T D B w |J| for every block, from the
+ * [D] left by the last recovery (mm01: cep_blocks; mm10: history1(1:36)).
+ * precision selects W3D_FP64 or W3D_FP32 (MFMA).
+ */
+int w3d_tangent(w3d_ctx* ctx, int32_t step, int32_t iter, int32_t precision);
+int w3d_tangent_blocks(w3d_ctx* ctx, int32_t first_blk, int32_t nblk,
+                       int32_t step, int32_t iter, int32_t precision);
+
+/*
+ * Sparsity of the assembled upper triangle for the absolute constraints in
+ * cstmap (3*nonode, nonzero = constrained): dof_map (assemble_code.f:561),
+ * count_profile_symmetric (:20) and build_col_sparse_symm (:260).
+ * Outputs neqns and ncoeff (terms strictly right of the diagonal).
+ */
+int w3d_pattern(w3d_ctx* ctx, const int32_t* cstmap, int32_t* neqns,
+                int64_t* ncoeff);
+/* Copy the pattern out: dof_eqn_map (3*nonode), eqn_node_map (neqns),
+   k_ptrs (neqns, row counts), k_indexes (ncoeff, 1-based columns).  Any
+   pointer may be NULL. */
+int w3d_get_pattern(const w3d_ctx* ctx, int32_t* dof_eqn_map,
+                    int32_t* eqn_node_map, int32_t* k_ptrs,
+                    int32_t* k_indexes);
+
+/*
+ * Row-by-row assembly of the current block tangents into k_diag (neqns) and
+ * k_coeffs (ncoeff) following assem_by_row (assemble_code.f:659).
+ */
+int w3d_assemble_vss(w3d_ctx* ctx, double* k_diag, double* k_coeffs);
+int w3d_assemble_vss_dev(w3d_ctx* ctx, double* d_k_diag, double* d_k_coeffs);
+
+/* n+1 -> n roll-over of stresses, strains and history (update.f:15). */
+int w3d_commit_step(w3d_ctx* ctx);
+/* Discard the n+1 state (adaptive step restart): n+1 <- n. */
+int w3d_rollback(w3d_ctx* ctx);
+
+/*
+ * Host copy of one block's state in the reference's layout (see the
+ * W3D_* selectors).  n_out is the capacity of out in doubles; W3D_ESTIFF of
+ * an FP32 tangent is widened to double.
+ */
+int w3d_get_block_state(const w3d_ctx* ctx, int32_t blk, int32_t which,
+                        double* out, int64_t n_out);
+/* Overwrite one block's n-state (restart / initial state).  which is
+   W3D_STRESS_N, W3D_STRAIN_N or W3D_HIST_N. */
+int w3d_set_block_state(w3d_ctx* ctx, int32_t blk, int32_t which,
+                        const double* in, int64_t n_in);
+
+/* Wait for all device work of the context. */
+int w3d_synchronize(w3d_ctx* ctx);
+
+/*
+ * Per-phase device timing, the counterpart of the reference's thyme()
+ * accumulators (thyme.f:15): when enabled, every kernel launch of a phase is
+ * bracketed by HIP events on the context's stream.  Phases: 0 recovery,
+ * 1 element-force gather, 2 energy reduction, 3 tangent, 4 assembly.
+ */
+enum {
+  W3D_PHASE_RECOVER = 0,
+  W3D_PHASE_IFV = 1,
+  W3D_PHASE_ENERGY = 2,
+  W3D_PHASE_TANGENT = 3,
+  W3D_PHASE_ASSEMBLE = 4,
+  W3D_NUM_PHASES = 5
+};
+int w3d_set_profiling(w3d_ctx* ctx, int32_t on);
+/* launches and total device milliseconds of one phase since the last reset */
+int w3d_get_phase_time(const w3d_ctx* ctx, int32_t phase, int64_t* launches,
+                       double* total_ms);
+int w3d_reset_phase_times(w3d_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* W3D_H */

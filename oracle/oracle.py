@@ -1,0 +1,241 @@
+"""ctypes wrapper of the CPU oracle (oracle/libw3d_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker; the product path never uses it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from warp3d_amd._abi import (PROTOS, W3DModel, dptr, iptr, lptr)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "libw3d_oracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run make -C oracle")
+        L = C.CDLL(path)
+        for name, (res, args) in PROTOS.items():
+            f = getattr(L, "orc_" + name)
+            f.restype = res
+            f.argtypes = args
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [C.POINTER(W3DModel)]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_tangent.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.orc_ngp.restype = C.c_int
+        L.orc_ngp.argtypes = [C.c_int, C.c_int]
+        L.orc_getgpts.argtypes = [C.c_int, C.c_int, C.c_int] + \
+            [C.POINTER(C.c_double)] * 4
+        L.orc_derivs.argtypes = [C.c_int, C.c_double, C.c_double,
+                                 C.c_double] + [C.POINTER(C.c_double)] * 3
+        L.orc_jacob1.argtypes = [C.c_int] + [C.POINTER(C.c_double)] * 7
+        L.orc_bmatrix.argtypes = [C.c_int] + [C.POINTER(C.c_double)] * 5 + \
+            [C.c_double, C.POINTER(C.c_double)]
+        L.orc_mm01_gp.restype = C.c_int
+        L.orc_mm01_gp.argtypes = [C.c_int, C.c_double, C.c_double,
+                                  C.c_double, C.c_double, C.c_int,
+                                  C.c_double] + \
+            [C.POINTER(C.c_double)] * 5 + [C.c_double, C.POINTER(C.c_double)]
+        L.orc_hprime.restype = C.c_float
+        L.orc_hprime.argtypes = [C.c_float, C.c_float]
+        _LIB = L
+    return _LIB
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def _chk(rc, what):
+    if rc < 0:
+        raise OracleError(f"{what}: {lib().orc_last_error().decode()}")
+    return rc
+
+
+# ---------------------------------------------------------------- leaves
+def getgpts(etype, order, gpn):
+    d = [C.c_double() for _ in range(4)]
+    lib().orc_getgpts(etype, order, gpn, *[C.byref(x) for x in d])
+    return tuple(x.value for x in d)
+
+
+def ngp(etype, order):
+    return lib().orc_ngp(etype, order)
+
+
+def derivs(etype, xi, eta, zeta, nnode):
+    a = [np.zeros(nnode) for _ in range(3)]
+    lib().orc_derivs(etype, xi, eta, zeta, *[dptr(x) for x in a])
+    return a
+
+
+def jacob1(nxi, neta, nzeta, ce):
+    nn = len(nxi)
+    jac = np.zeros(9)
+    gama = np.zeros(9)
+    dj = C.c_double()
+    lib().orc_jacob1(nn, dptr(np.ascontiguousarray(nxi)),
+                     dptr(np.ascontiguousarray(neta)),
+                     dptr(np.ascontiguousarray(nzeta)),
+                     dptr(np.ascontiguousarray(ce, dtype=np.float64)),
+                     dptr(jac), C.byref(dj), dptr(gama))
+    return jac.reshape(3, 3), dj.value, gama.reshape(3, 3)
+
+
+def bmatrix(gama, nxi, neta, nzeta, vol=None, eps_bbar=0.0):
+    nn = len(nxi)
+    b = np.zeros((6, 3 * nn))
+    g = np.ascontiguousarray(gama, dtype=np.float64).ravel()
+    v = (C.POINTER(C.c_double)() if vol is None
+         else dptr(np.ascontiguousarray(vol, dtype=np.float64).ravel()))
+    lib().orc_bmatrix(nn, dptr(g), dptr(np.ascontiguousarray(nxi)),
+                      dptr(np.ascontiguousarray(neta)),
+                      dptr(np.ascontiguousarray(nzeta)), v, eps_bbar,
+                      dptr(b))
+    return b
+
+
+def hprime(e, et):
+    return lib().orc_hprime(e, et)
+
+
+def mm01_gp(step, e, nu, beta, hp, yld, cgn, deps, hist, hist1=None,
+            first_hist11=None, lnelas=False):
+    cgn = np.array(cgn, dtype=np.float64)
+    cgn1 = np.zeros(9)
+    hist = np.array(hist, dtype=np.float64)
+    hist1 = np.zeros(11) if hist1 is None else np.array(hist1, float)
+    cep = np.zeros(36)
+    f11 = hist[10] if first_hist11 is None else first_hist11
+    rc = lib().orc_mm01_gp(step, e, nu, beta, hp, int(lnelas), yld,
+                           dptr(cgn), dptr(cgn1),
+                           dptr(np.ascontiguousarray(deps, dtype=float)),
+                           dptr(hist), dptr(hist1), f11, dptr(cep))
+    if rc != 0:
+        raise OracleError("mm01 negative plastic multiplier")
+    return cgn1, hist1, cep.reshape(6, 6)
+
+
+# ---------------------------------------------------------------- model
+class OracleModel:
+    """The oracle behind the same calls as warp3d_amd.BlockModel."""
+
+    def __init__(self, arrays, threads=0):
+        self._arrays = arrays
+        self._m = arrays.struct()
+        lib().orc_set_threads(threads)
+        self.h = lib().orc_create(C.byref(self._m))
+        if not self.h:
+            raise OracleError(lib().orc_last_error().decode())
+        self.nonode = arrays.nonode
+        nb = C.c_int32()
+        lib().orc_get_blocking(self.h, C.byref(nb), None)
+        self.nelblk = nb.value
+        self.elblks = np.zeros(4 * self.nelblk, np.int32)
+        lib().orc_get_blocking(self.h, C.byref(nb), iptr(self.elblks))
+        self.elblks = self.elblks.reshape(self.nelblk, 4)
+        self.neqns = None
+        self.ncoeff = None
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def block_info(self, blk):
+        info = np.zeros(5, np.int32)
+        _chk(lib().orc_block_info(self.h, blk, iptr(info)), "block_info")
+        return dict(zip(("span", "ngp", "hist", "totdof", "utsz"),
+                        info.tolist()))
+
+    def recover(self, step, iter_, u, du, extrapolated=False):
+        ifv = np.zeros(3 * self.nonode)
+        en = C.c_double()
+        pw = C.c_double()
+        rc = _chk(lib().orc_recover(
+            self.h, step, iter_, int(extrapolated),
+            dptr(np.ascontiguousarray(u, float)),
+            dptr(np.ascontiguousarray(du, float)), dptr(ifv),
+            C.cast(C.byref(en), C.POINTER(C.c_double)),
+            C.cast(C.byref(pw), C.POINTER(C.c_double))), "recover")
+        return ifv, en.value, pw.value, rc
+
+    def recover_blocks(self, first, n, step, iter_, u, du,
+                       extrapolated=False):
+        return _chk(lib().orc_recover_blocks(
+            self.h, first, n, step, iter_, int(extrapolated),
+            dptr(np.ascontiguousarray(u, float)),
+            dptr(np.ascontiguousarray(du, float))), "recover_blocks")
+
+    def tangent(self, step, iter_, precision=0):
+        return _chk(lib().orc_tangent(self.h, step, iter_), "tangent")
+
+    def tangent_blocks(self, first, n, step, iter_):
+        return _chk(lib().orc_tangent_blocks(self.h, first, n, step, iter_),
+                    "tangent_blocks")
+
+    def pattern(self, cstmap):
+        neq = C.c_int32()
+        nco = C.c_int64()
+        _chk(lib().orc_pattern(self.h, iptr(np.ascontiguousarray(
+            cstmap, np.int32)), C.byref(neq), C.byref(nco)), "pattern")
+        self.neqns, self.ncoeff = neq.value, nco.value
+        return self.neqns, self.ncoeff
+
+    def get_pattern(self):
+        d = np.zeros(3 * self.nonode, np.int32)
+        e = np.zeros(self.neqns, np.int32)
+        p = np.zeros(self.neqns, np.int32)
+        k = np.zeros(max(self.ncoeff, 1), np.int32)
+        _chk(lib().orc_get_pattern(self.h, iptr(d), iptr(e), iptr(p),
+                                   iptr(k)), "get_pattern")
+        return d, e, p, k[:self.ncoeff]
+
+    def assemble_vss(self):
+        kd = np.zeros(self.neqns)
+        kc = np.zeros(max(self.ncoeff, 1))
+        _chk(lib().orc_assemble_vss(self.h, dptr(kd), dptr(kc)), "assemble")
+        return kd, kc[:self.ncoeff]
+
+    def commit_step(self):
+        _chk(lib().orc_commit_step(self.h), "commit")
+
+    def rollback(self):
+        _chk(lib().orc_rollback(self.h), "rollback")
+
+    def state_size(self, blk, which):
+        i = self.block_info(blk)
+        nq = i["span"] * i["ngp"]
+        return {0: 9 * nq, 1: 9 * nq, 2: 6 * nq, 3: 6 * nq, 4: i["hist"] * nq,
+                5: i["hist"] * nq, 6: 21 * nq, 7: i["utsz"] * i["span"],
+                8: i["totdof"] * i["span"], 9: i["span"]}[which]
+
+    def get_block_state(self, blk, which):
+        n = self.state_size(blk, which)
+        out = np.zeros(n)
+        _chk(lib().orc_get_block_state(self.h, blk, which, dptr(out), n),
+             "get_block_state")
+        return out
+
+    def set_block_state(self, blk, which, data):
+        data = np.ascontiguousarray(data, float)
+        _chk(lib().orc_set_block_state(self.h, blk, which, dptr(data),
+                                       data.size), "set_block_state")
+
+
+def load_ref_leaf():
+    """The reference's own leaf routines built by `make -C oracle ref`, or
+    None when they are not built (no reference tree on this machine)."""
+    path = os.path.join(_HERE, "_ref", "libw3d_refleaf.so")
+    if not os.path.exists(path):
+        return None
+    return C.CDLL(path, mode=os.RTLD_LAZY)

@@ -1,0 +1,866 @@
+/*
+ * orc_block.c -- oracle: the block model, the recovery / internal-force
+ * driver, the block tangent, automatic blocking and the VSS sparse pattern
+ * and row-by-row assembly.
+ *
+ * TEST INFRASTRUCTURE ONLY (see w3d_oracle.h).  Restates, in the reference's
+ * own block loop order and array layouts:
+ *   inelbk.f:282-548 (inelbk_simple_blocking, inelbk_chk_match)
+ *   setup.f:160-190, 268-276 (inverse incidences, cp/dcp/icp)
+ *   reopen.f:1315 (edest_init)
+ *   drive_eps_sig_internal_forces.f:20, :399, :1653 (driver, do_nleps_block,
+ *     dupstr_blocked), rknstr.f:563 (rknstr_sm_displ), rstgp1.f:351 (rstgp2),
+ *     gtlsn1.f:25, rstgp1.f:614 (drive_01_update), :998 (drive_01_update_b),
+ *     :5189 (rstgp1_b), :5360 (rstgp1_store_cep), rplstr.f:15,
+ *     rknifv.f:141 (rknifv_std), gpifv1.f:16, addifv.f:16
+ *   tanstf.f:119, rktstf.f:15, gptns1.f:16, :1174 (extract_D_symmetric),
+ *     :1628 (bdbtgen), gpmas1.f:399 (adjust_cnst), rktstf.f:221 (transpose)
+ *   assemble_code.f:20, :151, :260, :355, :561, :659, :785
+ *
+ * The scatter of element forces into the structure vector (addifv) uses
+ * OpenMP atomics in the reference, whose summation order is not fixed; the
+ * oracle sums each structure dof over its inverse-incidence list in element
+ * order, which is one of the orders the reference can take and the order the
+ * device library uses.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+#include "w3d_oracle.h"
+
+#define MXND 20
+#define MXDOF 60
+#define MXGP 27
+
+static __thread char g_err[512] = "no error";
+const char* orc_last_error(void) { return g_err; }
+static int fail(int code, const char* msg) {
+  snprintf(g_err, sizeof g_err, "%s", msg);
+  return code;
+}
+
+typedef struct {
+  int span, felem, etype, order, ngp, nnode, totdof, utsz, mat, model,
+      hist, bbar;
+  /* device-free host state in the reference layouts */
+  double *urcs_n, *urcs_n1;   /* (9, ngp, span) */
+  double *eps_n, *eps_n1;     /* (6, ngp, span) */
+  double *hist_n, *hist_n1;   /* (hist, ngp, span) */
+  double *cep;                /* (21, span, ngp) */
+  double *ek;                 /* (utsz, span) */
+  double *einfvec;            /* (span, totdof) */
+  double *evol;               /* (span) */
+  int32_t* edest;             /* (totdof, span) 1-based structure dofs */
+} orc_block;
+
+struct orc_model {
+  int nonode, noelem, nelblk, nmat;
+  double* c;
+  int32_t *incid, *incmap, *etype, *order, *matnum, *bbar;
+  w3d_material* mats;
+  double eps_bbar;
+  int32_t* elblks; /* (4, nelblk) */
+  orc_block* blk;
+  int32_t* elem_blk; /* (noelem) 0-based block of each element */
+  /* inverse incidences: per node, elements in ascending order */
+  int32_t *inv_ptr, *inv_list;
+  /* sparsity */
+  int neqns;
+  int64_t ncoeff;
+  int32_t *dof_eqn_map, *eqn_node_map, *k_ptrs, *k_indexes;
+  int64_t* row_start;
+};
+
+static int g_threads = 0;
+void orc_set_threads(int n) { g_threads = n; }
+
+static int hist_size(int model) { return model == W3D_MM01 ? 11 : 158; }
+
+/* ------------------------------------------------------------------ */
+/* blocking: inelbk_simple_blocking / inelbk_chk_match                 */
+/* ------------------------------------------------------------------ */
+static int compatible(const orc_model* om, int a, int b) {
+  if (om->etype[a] != om->etype[b]) return 0;
+  if (om->matnum[a] != om->matnum[b]) return 0;
+  if (om->order[a] != om->order[b]) return 0;
+  if (orc_ngp(om->etype[a], om->order[a]) !=
+      orc_ngp(om->etype[b], om->order[b]))
+    return 0;
+  if (om->bbar[a] != om->bbar[b]) return 0;
+  return 1;
+}
+
+static int auto_blocking(orc_model* om, int auto_size) {
+  int cap = om->noelem + 1;
+  int32_t* t = (int32_t*)malloc(sizeof(int32_t) * 4 * cap);
+  int nb = 1, cur = 1, felem = 1;
+  t[0] = 1; /* span */
+  t[1] = 1; /* felem */
+  t[2] = 0;
+  t[3] = 0;
+  for (int e = 2; e <= om->noelem; ++e) {
+    int newblk = 0;
+    cur = cur + 1;
+    if (cur > auto_size) newblk = 1;
+    if (!newblk && !compatible(om, felem - 1, e - 1)) newblk = 1;
+    if (!newblk) {
+      t[4 * (nb - 1) + 0] = cur;
+      continue;
+    }
+    nb += 1;
+    felem = e;
+    t[4 * (nb - 1) + 0] = 1;
+    t[4 * (nb - 1) + 1] = felem;
+    t[4 * (nb - 1) + 2] = 0;
+    t[4 * (nb - 1) + 3] = 0;
+    cur = 1;
+  }
+  om->nelblk = nb;
+  om->elblks = t;
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* model creation                                                     */
+/* ------------------------------------------------------------------ */
+orc_model* orc_create(const w3d_model* m) {
+  if (!m || m->abi_version != W3D_ABI_VERSION) {
+    fail(W3D_ERR_ARG, "orc_create: bad model or ABI version");
+    return NULL;
+  }
+  orc_model* om = (orc_model*)calloc(1, sizeof(orc_model));
+  om->nonode = m->nonode;
+  om->noelem = m->noelem;
+  om->nmat = m->nmat;
+  om->eps_bbar = m->eps_bbar;
+  om->c = (double*)malloc(sizeof(double) * 3 * m->nonode);
+  memcpy(om->c, m->c, sizeof(double) * 3 * m->nonode);
+  om->etype = (int32_t*)malloc(sizeof(int32_t) * m->noelem);
+  om->order = (int32_t*)malloc(sizeof(int32_t) * m->noelem);
+  om->matnum = (int32_t*)malloc(sizeof(int32_t) * m->noelem);
+  om->bbar = (int32_t*)malloc(sizeof(int32_t) * m->noelem);
+  om->incmap = (int32_t*)malloc(sizeof(int32_t) * m->noelem);
+  int64_t ninc = 0;
+  for (int e = 0; e < m->noelem; ++e) {
+    int nn = orc_nnode(m->etype[e]);
+    if (nn == 0 || orc_ngp(m->etype[e], m->int_order[e]) == 0) {
+      fail(W3D_ERR_UNSUPPORTED, "orc_create: element type/order");
+      orc_destroy(om);
+      return NULL;
+    }
+    ninc += nn;
+  }
+  om->incid = (int32_t*)malloc(sizeof(int32_t) * ninc);
+  int64_t pos = 0;
+  for (int e = 0; e < m->noelem; ++e) {
+    int nn = orc_nnode(m->etype[e]);
+    om->etype[e] = m->etype[e];
+    om->order[e] = m->int_order[e];
+    om->matnum[e] = m->matnum[e];
+    om->bbar[e] = (m->etype[e] == W3D_L3DISOP) && m->bbar && m->bbar[e];
+    om->incmap[e] = (int32_t)pos + 1;
+    for (int j = 0; j < nn; ++j)
+      om->incid[pos + j] = m->incid[m->incmap[e] - 1 + j];
+    pos += nn;
+  }
+  om->mats = (w3d_material*)malloc(sizeof(w3d_material) * m->nmat);
+  memcpy(om->mats, m->mats, sizeof(w3d_material) * m->nmat);
+  for (int e = 0; e < m->noelem; ++e) {
+    if (om->matnum[e] < 1 || om->matnum[e] > m->nmat) {
+      fail(W3D_ERR_ARG, "orc_create: bad material number");
+      orc_destroy(om);
+      return NULL;
+    }
+    int mod = om->mats[om->matnum[e] - 1].model;
+    if (mod != W3D_MM01) {
+      fail(W3D_ERR_UNSUPPORTED, "orc_create: material model");
+      orc_destroy(om);
+      return NULL;
+    }
+  }
+  /* blocking */
+  if (m->nelblk > 0) {
+    om->nelblk = m->nelblk;
+    om->elblks = (int32_t*)malloc(sizeof(int32_t) * 4 * m->nelblk);
+    memcpy(om->elblks, m->elblks, sizeof(int32_t) * 4 * m->nelblk);
+    int next = 1;
+    for (int b = 0; b < m->nelblk; ++b) {
+      int span = om->elblks[4 * b], felem = om->elblks[4 * b + 1];
+      if (span < 1 || felem != next) {
+        fail(W3D_ERR_ARG, "orc_create: blocks must tile the elements");
+        orc_destroy(om);
+        return NULL;
+      }
+      for (int i = 1; i < span; ++i)
+        if (!compatible(om, felem - 1, felem - 1 + i)) {
+          fail(W3D_ERR_ARG, "orc_create: incompatible elements in block");
+          orc_destroy(om);
+          return NULL;
+        }
+      next = felem + span;
+    }
+    if (next != m->noelem + 1) {
+      fail(W3D_ERR_ARG, "orc_create: blocks must tile the elements");
+      orc_destroy(om);
+      return NULL;
+    }
+  } else {
+    int as = m->auto_size > 0 ? m->auto_size : 128;
+    auto_blocking(om, as);
+  }
+  /* blocks */
+  om->blk = (orc_block*)calloc(om->nelblk, sizeof(orc_block));
+  om->elem_blk = (int32_t*)malloc(sizeof(int32_t) * m->noelem);
+  for (int b = 0; b < om->nelblk; ++b) {
+    orc_block* B = &om->blk[b];
+    B->span = om->elblks[4 * b];
+    B->felem = om->elblks[4 * b + 1];
+    int fe = B->felem - 1;
+    B->etype = om->etype[fe];
+    B->order = om->order[fe];
+    B->ngp = orc_ngp(B->etype, B->order);
+    B->nnode = orc_nnode(B->etype);
+    B->totdof = 3 * B->nnode;
+    B->utsz = B->totdof * (B->totdof + 1) / 2;
+    B->mat = om->matnum[fe] - 1;
+    B->model = om->mats[B->mat].model;
+    B->hist = hist_size(B->model);
+    B->bbar = om->bbar[fe];
+    int64_t nq = (int64_t)B->ngp * B->span;
+    B->urcs_n = (double*)calloc(9 * nq, sizeof(double));
+    B->urcs_n1 = (double*)calloc(9 * nq, sizeof(double));
+    B->eps_n = (double*)calloc(6 * nq, sizeof(double));
+    B->eps_n1 = (double*)calloc(6 * nq, sizeof(double));
+    B->hist_n = (double*)calloc(B->hist * nq, sizeof(double));
+    B->hist_n1 = (double*)calloc(B->hist * nq, sizeof(double));
+    B->cep = (double*)calloc(21 * nq, sizeof(double));
+    B->ek = (double*)calloc((int64_t)B->utsz * B->span, sizeof(double));
+    B->einfvec = (double*)calloc((int64_t)B->totdof * B->span, sizeof(double));
+    B->evol = (double*)calloc(B->span, sizeof(double));
+    B->edest = (int32_t*)malloc(sizeof(int32_t) * B->totdof * B->span);
+    for (int i = 0; i < B->span; ++i) {
+      int e = fe + i;
+      om->elem_blk[e] = b;
+      const int32_t* nodes = om->incid + om->incmap[e] - 1;
+      for (int j = 0; j < B->nnode; ++j) {
+        int d = 3 * (nodes[j] - 1) + 1;
+        B->edest[i * B->totdof + j] = d;
+        B->edest[i * B->totdof + B->nnode + j] = d + 1;
+        B->edest[i * B->totdof + 2 * B->nnode + j] = d + 2;
+      }
+    }
+  }
+  /* inverse incidences (setup.f:160-190): repeated nodes of an element
+     are listed once */
+  om->inv_ptr = (int32_t*)calloc(m->nonode + 1, sizeof(int32_t));
+  for (int e = 0; e < m->noelem; ++e) {
+    int nn = orc_nnode(om->etype[e]);
+    const int32_t* nodes = om->incid + om->incmap[e] - 1;
+    for (int j = 0; j < nn; ++j) {
+      int rep = 0;
+      for (int k = 0; k < j; ++k) rep |= nodes[k] == nodes[j];
+      if (!rep) om->inv_ptr[nodes[j]] += 1;
+    }
+  }
+  for (int n = 0; n < m->nonode; ++n) om->inv_ptr[n + 1] += om->inv_ptr[n];
+  om->inv_list = (int32_t*)malloc(sizeof(int32_t) * om->inv_ptr[m->nonode]);
+  int32_t* fillp = (int32_t*)malloc(sizeof(int32_t) * m->nonode);
+  for (int n = 0; n < m->nonode; ++n) fillp[n] = om->inv_ptr[n];
+  for (int e = 0; e < m->noelem; ++e) {
+    int nn = orc_nnode(om->etype[e]);
+    const int32_t* nodes = om->incid + om->incmap[e] - 1;
+    for (int j = 0; j < nn; ++j) {
+      int rep = 0;
+      for (int k = 0; k < j; ++k) rep |= nodes[k] == nodes[j];
+      if (!rep) om->inv_list[fillp[nodes[j] - 1]++] = e + 1;
+    }
+  }
+  free(fillp);
+  return om;
+}
+
+void orc_destroy(orc_model* om) {
+  if (!om) return;
+  if (om->blk)
+    for (int b = 0; b < om->nelblk; ++b) {
+      orc_block* B = &om->blk[b];
+      free(B->urcs_n); free(B->urcs_n1); free(B->eps_n); free(B->eps_n1);
+      free(B->hist_n); free(B->hist_n1); free(B->cep); free(B->ek);
+      free(B->einfvec); free(B->evol); free(B->edest);
+    }
+  free(om->blk); free(om->elem_blk); free(om->c); free(om->incid);
+  free(om->incmap); free(om->etype); free(om->order); free(om->matnum);
+  free(om->bbar); free(om->mats); free(om->elblks); free(om->inv_ptr);
+  free(om->inv_list); free(om->dof_eqn_map); free(om->eqn_node_map);
+  free(om->k_ptrs); free(om->k_indexes); free(om->row_start);
+  free(om);
+}
+
+int orc_get_blocking(const orc_model* om, int32_t* nelblk, int32_t* elblks) {
+  if (nelblk) *nelblk = om->nelblk;
+  if (elblks) memcpy(elblks, om->elblks, sizeof(int32_t) * 4 * om->nelblk);
+  return 0;
+}
+
+int orc_block_info(const orc_model* om, int blk, int32_t* info5) {
+  if (blk < 1 || blk > om->nelblk) return fail(W3D_ERR_ARG, "bad block");
+  const orc_block* B = &om->blk[blk - 1];
+  info5[0] = B->span;
+  info5[1] = B->ngp;
+  info5[2] = B->hist;
+  info5[3] = B->totdof;
+  info5[4] = B->utsz;
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* shared per-block kinematics                                         */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  double w[MXGP];
+  double nxi[MXGP][MXND], neta[MXGP][MXND], nzeta[MXGP][MXND];
+} gp_table;
+
+static void make_table(const orc_block* B, gp_table* t) {
+  for (int g = 0; g < B->ngp; ++g) {
+    double xi, eta, zeta;
+    orc_getgpts(B->etype, B->order, g + 1, &xi, &eta, &zeta, &t->w[g]);
+    orc_derivs(B->etype, xi, eta, zeta, t->nxi[g], t->neta[g], t->nzeta[g]);
+  }
+}
+
+/* ce (3N: x-block, y-block, z-block) of element i of block B */
+static void gather_coords(const orc_model* om, const orc_block* B, int i,
+                          double* ce) {
+  const int32_t* ed = B->edest + (int64_t)i * B->totdof;
+  for (int k = 0; k < B->totdof; ++k) ce[k] = om->c[ed[k] - 1];
+}
+
+/* Jacobians at every point of one element, then (hex8 B-bar) the
+   volume-averaged derivatives: rknstr_sm_displ / rktstf / rknifv_std. */
+static void elem_kinematics(const orc_block* B, const gp_table* t,
+                            const double* ce, double dj[MXGP],
+                            double gama[MXGP][9], double vol[24]) {
+  double volume = 0.0;
+  if (B->bbar) memset(vol, 0, sizeof(double) * 24);
+  for (int g = 0; g < B->ngp; ++g) {
+    double jac[9];
+    orc_jacob1(B->nnode, t->nxi[g], t->neta[g], t->nzeta[g], ce, jac, &dj[g],
+               gama[g]);
+    if (B->bbar) { /* vol_terms, vol_terms.f:14 */
+      const double* a = gama[g];
+      double d = dj[g];
+      for (int k = 0; k < 3; ++k)
+        for (int j = 0; j < 8; ++j)
+          vol[3 * j + k] = vol[3 * j + k] + (a[3 * k] * t->nxi[g][j] +
+                                             a[3 * k + 1] * t->neta[g][j] +
+                                             a[3 * k + 2] * t->nzeta[g][j]) *
+                                                d;
+      volume = volume + d;
+    }
+  }
+  if (B->bbar) { /* vol_avg, vol_avg.f:13 */
+    double voli = 1.0 / volume;
+    for (int k = 0; k < 3; ++k)
+      for (int j = 0; j < 8; ++j) vol[3 * j + k] = vol[3 * j + k] * voli;
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* recovery: do_nleps_block for one block                              */
+/* ------------------------------------------------------------------ */
+static int recover_block(orc_model* om, int b, int step, int iter,
+                         int extrapolated, const double* u, const double* du,
+                         double* energy, double* pwork) {
+  (void)u; /* small-displacement form: u enters only through du */
+  orc_block* B = &om->blk[b];
+  const w3d_material* mat = &om->mats[B->mat];
+  gp_table t;
+  make_table(B, &t);
+  const int S = B->span, G = B->ngp, N = B->nnode, T = B->totdof,
+            H = B->hist;
+  const double scalar = B->etype == W3D_TET10 ? 1.0 / 6.0 : 1.0;
+  /* material property vectors: setup_mm01_rknstr (rknstr.f:1018) */
+  const double e = (double)mat->e, nu = (double)mat->nu,
+               beta = (double)mat->beta,
+               hp = (double)orc_hprime(mat->e, mat->tan_e),
+               yld = (double)mat->yld_pt;
+  const int lnelas = mat->linear_elastic;
+  const int nonlinear = iter >= 1 || extrapolated;
+  double* ce = (double*)malloc(sizeof(double) * S * T);
+  double* due = (double*)malloc(sizeof(double) * S * T);
+  double(*dj)[MXGP] = malloc(sizeof(double) * S * MXGP);
+  double(*gama)[MXGP][9] = malloc(sizeof(double) * S * MXGP * 9);
+  double(*vol)[24] = malloc(sizeof(double) * S * 24);
+  /* local n-state copies (gathered, then possibly reset at step 1) */
+  double* hloc = (double*)malloc(sizeof(double) * S * G * H);
+  double* sloc = (double*)malloc(sizeof(double) * S * G * 9);
+  memcpy(hloc, B->hist_n, sizeof(double) * S * G * H);
+  memcpy(sloc, B->urcs_n, sizeof(double) * S * G * 9);
+  for (int i = 0; i < S; ++i) {
+    gather_coords(om, B, i, ce + i * T);
+    const int32_t* ed = B->edest + (int64_t)i * T;
+    for (int k = 0; k < T; ++k) due[i * T + k] = du[ed[k] - 1];
+    elem_kinematics(B, &t, ce + i * T, dj[i], gama[i], vol[i]);
+  }
+  double blk_energy = 0.0, blk_pwork = 0.0;
+  double bmat[6 * MXDOF];
+  int rc = 0;
+  /* mm01_set_history at step 1 resets the gathered n-history of every
+     element before any element is updated: the first element's word 11
+     read by mm01_general (mm01.f:845) is then zero. */
+  for (int g = 0; g < G; ++g) {
+    double first_hist11 = hloc[((int64_t)0 * G + g) * H + 10];
+    for (int i = 0; i < S; ++i) {
+      int64_t q = (int64_t)i * G + g;
+      /* gtlsn1: B, B-bar, deps = B due */
+      orc_bmatrix(N, gama[i][g], t.nxi[g], t.neta[g], t.nzeta[g],
+                  B->bbar ? vol[i] : NULL, om->eps_bbar, bmat);
+      const double* d = due + i * T;
+      double deps[6] = {0, 0, 0, 0, 0, 0};
+      for (int j = 0; j < N; ++j) {
+        int j1 = j, j2 = N + j, j3 = 2 * N + j;
+        deps[0] = deps[0] + bmat[0 * T + j1] * d[j1] + bmat[0 * T + j2] * d[j2] +
+                  bmat[0 * T + j3] * d[j3];
+        deps[1] = deps[1] + bmat[1 * T + j1] * d[j1] + bmat[1 * T + j2] * d[j2] +
+                  bmat[1 * T + j3] * d[j3];
+        deps[2] = deps[2] + bmat[2 * T + j1] * d[j1] + bmat[2 * T + j2] * d[j2] +
+                  bmat[2 * T + j3] * d[j3];
+        deps[3] = deps[3] + bmat[3 * T + j1] * d[j1] + bmat[3 * T + j2] * d[j2];
+        deps[4] = deps[4] + bmat[4 * T + j2] * d[j2] + bmat[4 * T + j3] * d[j3];
+        deps[5] = deps[5] + bmat[5 * T + j1] * d[j1] + bmat[5 * T + j3] * d[j3];
+      }
+      /* rstgp1_update_strains */
+      for (int k = 0; k < 6; ++k)
+        B->eps_n1[q * 6 + k] = B->eps_n[q * 6 + k] + deps[k];
+      double* cgn = sloc + q * 9;
+      double* cgn1 = B->urcs_n1 + q * 9;
+      double cep[36];
+      if (nonlinear) {
+        if (step == 1) first_hist11 = 0.0;
+        double h1[11];
+        if (orc_mm01_gp(step, e, nu, beta, hp, lnelas, yld, cgn, cgn1, deps,
+                        hloc + q * H, h1, first_hist11, cep) != 0)
+          rc = W3D_ERR_NUMERIC;
+        /* rplstr: history is scattered for iter > 0 only */
+        if (iter > 0) memcpy(B->hist_n1 + q * H, h1, sizeof h1);
+      } else {
+        /* drive_01_update_a / drive_01_update_b: linear-elastic [D] and
+           sigma_n+1(1:6) = sigma_n + [D] deps; words 7-9 stay zero */
+        const double one = 1.0, two = 2.0;
+        double c1 = (e / ((one + nu) * (one - two * nu)));
+        double c2 = (one - nu) * c1;
+        double c3 = ((one - two * nu) / two) * c1;
+        double c4 = nu * c1;
+        memset(cep, 0, sizeof cep);
+        cep[0] = c2; cep[7] = c2; cep[14] = c2;
+        cep[21] = c3; cep[28] = c3; cep[35] = c3;
+        cep[1] = c4; cep[2] = c4; cep[6] = c4; cep[12] = c4; cep[8] = c4;
+        cep[13] = c4;
+        for (int k = 0; k < 6; ++k) cgn1[k] = cgn[k];
+        for (int k = 0; k < 6; ++k)
+          for (int m = 0; m < 6; ++m)
+            cgn1[k] = cgn1[k] + cep[6 * m + k] * deps[m];
+        cgn1[6] = cgn1[7] = cgn1[8] = 0.0;
+      }
+      /* rstgp1_store_cep: 21-term lower triangle, layout (21, span, ngp) */
+      double* cs = B->cep + ((int64_t)g * S + i) * 21;
+      int k = 0;
+      for (int r = 0; r < 6; ++r)
+        for (int cc = 0; cc <= r; ++cc) cs[k++] = cep[6 * r + cc];
+    }
+    /* rstgp1_b (small strain), only for iter > 0 */
+    if (iter > 0) {
+      for (int i = 0; i < S; ++i) {
+        int64_t q = (int64_t)i * G + g;
+        double factor = dj[i][g] * 1.0 * t.w[g];
+        blk_energy = blk_energy + B->urcs_n1[q * 9 + 6] * factor;
+        blk_pwork = blk_pwork + B->urcs_n1[q * 9 + 7] * factor;
+      }
+    }
+  }
+  /* rknifv_std: element internal forces with the updated stresses (the
+     coordinates at n+1 equal the initial ones in the small-displacement
+     form, so the Jacobians above are reused) */
+  memset(B->einfvec, 0, sizeof(double) * S * T);
+  memset(B->evol, 0, sizeof(double) * S);
+  for (int g = 0; g < G; ++g) {
+    double w = t.w[g] * scalar;
+    for (int i = 0; i < S; ++i) {
+      int64_t q = (int64_t)i * G + g;
+      orc_bmatrix(N, gama[i][g], t.nxi[g], t.neta[g], t.nzeta[g],
+                  B->bbar ? vol[i] : NULL, om->eps_bbar, bmat);
+      const double* s = B->urcs_n1 + q * 9;
+      double d = dj[i][g];
+      for (int j = 0; j < T; ++j) {
+        double* f = &B->einfvec[(int64_t)j * S + i];
+        *f = *f + bmat[0 * T + j] * s[0] * w * d + bmat[1 * T + j] * s[1] * w * d +
+             bmat[2 * T + j] * s[2] * w * d + bmat[3 * T + j] * s[3] * w * d +
+             bmat[4 * T + j] * s[4] * w * d + bmat[5 * T + j] * s[5] * w * d;
+      }
+      B->evol[i] = B->evol[i] + w * d;
+    }
+  }
+  /* rplstr: the gathered history of iter <= 0 is not scattered back; the
+     n+1 history array keeps its previous contents */
+  free(ce); free(due); free(dj); free(gama); free(vol); free(hloc);
+  free(sloc);
+  *energy = blk_energy;
+  *pwork = blk_pwork;
+  return rc;
+}
+
+int orc_recover_blocks(orc_model* om, int first, int n, int step, int iter,
+                       int extrapolated, const double* u, const double* du) {
+  if (first < 1 || n < 0 || first - 1 + n > om->nelblk)
+    return fail(W3D_ERR_ARG, "orc_recover_blocks: block range");
+  int rc = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(min : rc) \
+    num_threads(g_threads > 0 ? g_threads : omp_get_max_threads())
+  for (int b = first - 1; b < first - 1 + n; ++b) {
+    double en, pw;
+    int r = recover_block(om, b, step, iter, extrapolated, u, du, &en, &pw);
+    if (r < rc) rc = r;
+  }
+  if (rc) return fail(rc, "mm01: negative plastic multiplier");
+  return 0;
+}
+
+int orc_recover(orc_model* om, int step, int iter, int extrapolated,
+                const double* u, const double* du, double* ifv,
+                double* energy, double* plastic_work) {
+  int nb = om->nelblk;
+  double* be = (double*)calloc(nb, sizeof(double));
+  double* bp = (double*)calloc(nb, sizeof(double));
+  int rc = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(min : rc) \
+    num_threads(g_threads > 0 ? g_threads : omp_get_max_threads())
+  for (int b = 0; b < nb; ++b) {
+    int r = recover_block(om, b, step, iter, extrapolated, u, du, &be[b],
+                          &bp[b]);
+    if (r < rc) rc = r;
+  }
+  double en = 0.0, pw = 0.0;
+  for (int b = 0; b < nb; ++b) {
+    en = en + be[b];
+    pw = pw + bp[b];
+  }
+  free(be);
+  free(bp);
+  if (energy) *energy = en;
+  if (plastic_work) *plastic_work = pw;
+  if (rc) return fail(rc, "mm01: negative plastic multiplier");
+  /* assembly of element forces: per structure dof, over the node's
+     elements in ascending order */
+  if (ifv) {
+#pragma omp parallel for schedule(static) \
+    num_threads(g_threads > 0 ? g_threads : omp_get_max_threads())
+    for (int n = 0; n < om->nonode; ++n) {
+      double f[3] = {0, 0, 0};
+      for (int p = om->inv_ptr[n]; p < om->inv_ptr[n + 1]; ++p) {
+        int e = om->inv_list[p] - 1;
+        const orc_block* B = &om->blk[om->elem_blk[e]];
+        int i = e - (B->felem - 1);
+        const int32_t* ed = B->edest + (int64_t)i * B->totdof;
+        for (int j = 0; j < B->totdof; ++j) {
+          int d = ed[j] - 1;
+          if (d / 3 == n) f[d % 3] += B->einfvec[(int64_t)j * B->span + i];
+        }
+      }
+      ifv[3 * n] = f[0];
+      ifv[3 * n + 1] = f[1];
+      ifv[3 * n + 2] = f[2];
+    }
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* tangent: do_nlek_block / rktstf / gptns1 / bdbtgen                  */
+/* ------------------------------------------------------------------ */
+static void tangent_block(orc_model* om, int b) {
+  orc_block* B = &om->blk[b];
+  gp_table t;
+  make_table(B, &t);
+  const int S = B->span, G = B->ngp, N = B->nnode, T = B->totdof,
+            U = B->utsz;
+  const double scalar = B->etype == W3D_TET10 ? 1.0 / 6.0 : 1.0;
+  double ce[MXDOF], dj[MXGP], gama[MXGP][9], vol[24];
+  double bmat[6 * MXDOF], bd[6 * MXDOF];
+  /* icp table, setup.f:268-276 */
+  int* icp_r = (int*)malloc(sizeof(int) * U);
+  int* icp_c = (int*)malloc(sizeof(int) * U);
+  for (int j = 1; j <= T; ++j)
+    for (int i = 1; i <= j; ++i) {
+      int pos = (j - 1) * j / 2 + i;
+      icp_r[pos - 1] = i;
+      icp_c[pos - 1] = j;
+    }
+  for (int i = 0; i < S; ++i) {
+    gather_coords(om, B, i, ce);
+    elem_kinematics(B, &t, ce, dj, gama, vol);
+    double* ek = B->ek + (int64_t)i * U; /* transposed layout (utsz,span) */
+    memset(ek, 0, sizeof(double) * U);
+    for (int g = 0; g < G; ++g) {
+      orc_bmatrix(N, gama[g], t.nxi[g], t.neta[g], t.nzeta[g],
+                  B->bbar ? vol : NULL, om->eps_bbar, bmat);
+      /* extract_D_symmetric: f = w |J| times the stored lower triangle */
+      double f = t.w[g] * dj[g];
+      const double* cs = B->cep + ((int64_t)g * S + i) * 21;
+      double dm[36];
+      int k = 0;
+      for (int r = 0; r < 6; ++r)
+        for (int c = 0; c <= r; ++c) {
+          double v = f * cs[k++];
+          dm[6 * r + c] = v;
+          dm[6 * c + r] = v;
+        }
+      if (scalar != 1.0) /* adjust_cnst */
+        for (int q = 0; q < 36; ++q) dm[q] = scalar * dm[q];
+      /* bdbtgen: bd(j,k) = sum_m d(m,k) b(j,m) */
+      for (int j = 0; j < T; ++j)
+        for (int kk = 0; kk < 6; ++kk)
+          bd[kk * T + j] = dm[0 * 6 + kk] * bmat[0 * T + j] +
+                           dm[1 * 6 + kk] * bmat[1 * T + j] +
+                           dm[2 * 6 + kk] * bmat[2 * T + j] +
+                           dm[3 * 6 + kk] * bmat[3 * T + j] +
+                           dm[4 * 6 + kk] * bmat[4 * T + j] +
+                           dm[5 * 6 + kk] * bmat[5 * T + j];
+      for (int j = 0; j < U; ++j) {
+        int row = icp_r[j] - 1, col = icp_c[j] - 1;
+        ek[j] = ek[j] + bmat[0 * T + col] * bd[0 * T + row] +
+                bmat[1 * T + col] * bd[1 * T + row] +
+                bmat[2 * T + col] * bd[2 * T + row] +
+                bmat[3 * T + col] * bd[3 * T + row] +
+                bmat[4 * T + col] * bd[4 * T + row] +
+                bmat[5 * T + col] * bd[5 * T + row];
+      }
+    }
+  }
+  free(icp_r);
+  free(icp_c);
+}
+
+int orc_tangent_blocks(orc_model* om, int first, int n, int step, int iter) {
+  (void)step;
+  (void)iter;
+  if (first < 1 || n < 0 || first - 1 + n > om->nelblk)
+    return fail(W3D_ERR_ARG, "orc_tangent_blocks: block range");
+#pragma omp parallel for schedule(dynamic, 1) \
+    num_threads(g_threads > 0 ? g_threads : omp_get_max_threads())
+  for (int b = first - 1; b < first - 1 + n; ++b) tangent_block(om, b);
+  return 0;
+}
+
+int orc_tangent(orc_model* om, int step, int iter) {
+  return orc_tangent_blocks(om, 1, om->nelblk, step, iter);
+}
+
+/* ------------------------------------------------------------------ */
+/* sparsity and assembly                                               */
+/* ------------------------------------------------------------------ */
+static int cmp_i32(const void* a, const void* b) {
+  int32_t x = *(const int32_t*)a, y = *(const int32_t*)b;
+  return (x > y) - (x < y);
+}
+
+int orc_pattern(orc_model* om, const int32_t* cstmap, int32_t* neqns_out,
+                int64_t* ncoeff_out) {
+  int nd = 3 * om->nonode;
+  free(om->dof_eqn_map); free(om->eqn_node_map); free(om->k_ptrs);
+  free(om->k_indexes); free(om->row_start);
+  om->dof_eqn_map = (int32_t*)calloc(nd, sizeof(int32_t));
+  om->eqn_node_map = (int32_t*)calloc(nd, sizeof(int32_t));
+  /* dof_map, assemble_code.f:561 */
+  int neq = 0;
+  for (int d = 1; d <= nd; ++d) {
+    if (cstmap[d - 1] != 0) continue;
+    neq = neq + 1;
+    om->dof_eqn_map[d - 1] = neq;
+    om->eqn_node_map[neq - 1] = (d - 1) / 3 + 1;
+  }
+  om->neqns = neq;
+  om->k_ptrs = (int32_t*)calloc(neq > 0 ? neq : 1, sizeof(int32_t));
+  /* build_col_sparse_symm_srow, per row: collect, sort, unique */
+  int32_t** rows = (int32_t**)calloc(neq > 0 ? neq : 1, sizeof(int32_t*));
+#pragma omp parallel for schedule(dynamic, 256) \
+    num_threads(g_threads > 0 ? g_threads : omp_get_max_threads())
+  for (int srow = 1; srow <= neq - 1; ++srow) {
+    int snode = om->eqn_node_map[srow - 1];
+    int cap = 0;
+    for (int p = om->inv_ptr[snode - 1]; p < om->inv_ptr[snode]; ++p) {
+      int e = om->inv_list[p] - 1;
+      cap += 3 * orc_nnode(om->etype[e]);
+    }
+    int32_t* list = (int32_t*)malloc(sizeof(int32_t) * (cap + 1));
+    int nused = 0;
+    for (int p = om->inv_ptr[snode - 1]; p < om->inv_ptr[snode]; ++p) {
+      int e = om->inv_list[p] - 1;
+      const orc_block* B = &om->blk[om->elem_blk[e]];
+      int i = e - (B->felem - 1);
+      const int32_t* ed = B->edest + (int64_t)i * B->totdof;
+      for (int erow = 0; erow < B->totdof; ++erow) {
+        if (om->dof_eqn_map[ed[erow] - 1] != srow) continue;
+        for (int ecol = 0; ecol < B->totdof; ++ecol) {
+          int scol = om->dof_eqn_map[ed[ecol] - 1];
+          if (scol <= srow) continue;
+          list[nused++] = scol;
+        }
+        break; /* no repeated incidences in these meshes */
+      }
+    }
+    if (nused > 1) qsort(list, nused, sizeof(int32_t), cmp_i32);
+    int nu = 0;
+    for (int k = 0; k < nused; ++k)
+      if (k == 0 || list[k] != list[k - 1]) list[nu++] = list[k];
+    rows[srow - 1] = list;
+    om->k_ptrs[srow - 1] = nu;
+  }
+  if (neq > 0) om->k_ptrs[neq - 1] = 0;
+  om->row_start = (int64_t*)malloc(sizeof(int64_t) * (neq + 1));
+  om->row_start[0] = 0;
+  for (int r = 0; r < neq; ++r)
+    om->row_start[r + 1] = om->row_start[r] + om->k_ptrs[r];
+  om->ncoeff = om->row_start[neq];
+  om->k_indexes = (int32_t*)malloc(sizeof(int32_t) * (om->ncoeff + 1));
+  for (int r = 0; r < neq - 1; ++r) {
+    if (om->k_ptrs[r])
+      memcpy(om->k_indexes + om->row_start[r], rows[r],
+             sizeof(int32_t) * om->k_ptrs[r]);
+    free(rows[r]);
+  }
+  free(rows);
+  if (neqns_out) *neqns_out = neq;
+  if (ncoeff_out) *ncoeff_out = om->ncoeff;
+  return 0;
+}
+
+int orc_get_pattern(const orc_model* om, int32_t* dof_eqn_map,
+                    int32_t* eqn_node_map, int32_t* k_ptrs,
+                    int32_t* k_indexes) {
+  if (!om->dof_eqn_map) return fail(W3D_ERR_ARG, "no pattern");
+  if (dof_eqn_map)
+    memcpy(dof_eqn_map, om->dof_eqn_map, sizeof(int32_t) * 3 * om->nonode);
+  if (eqn_node_map)
+    memcpy(eqn_node_map, om->eqn_node_map, sizeof(int32_t) * om->neqns);
+  if (k_ptrs) memcpy(k_ptrs, om->k_ptrs, sizeof(int32_t) * om->neqns);
+  if (k_indexes)
+    memcpy(k_indexes, om->k_indexes, sizeof(int32_t) * om->ncoeff);
+  return 0;
+}
+
+/* assem_by_row / assem_a_row, assemble_code.f:659,785 */
+int orc_assemble_vss(orc_model* om, double* k_diag, double* k_coeffs) {
+  if (!om->dof_eqn_map) return fail(W3D_ERR_ARG, "no pattern");
+  int neq = om->neqns;
+#pragma omp parallel num_threads(g_threads > 0 ? g_threads : omp_get_max_threads())
+  {
+    double* coeff_row = (double*)calloc(neq + 1, sizeof(double));
+#pragma omp for schedule(dynamic, 256)
+    for (int srow = 1; srow <= neq; ++srow) {
+      int snode = om->eqn_node_map[srow - 1];
+      for (int p = om->inv_ptr[snode - 1]; p < om->inv_ptr[snode]; ++p) {
+        int e = om->inv_list[p] - 1;
+        const orc_block* B = &om->blk[om->elem_blk[e]];
+        int i = e - (B->felem - 1);
+        const int32_t* ed = B->edest + (int64_t)i * B->totdof;
+        const double* emat = B->ek + (int64_t)i * B->utsz;
+        int T = B->totdof;
+        int lsc[MXDOF];
+        for (int k = 0; k < T; ++k) lsc[k] = om->dof_eqn_map[ed[k] - 1];
+        for (int erow = 1; erow <= T; ++erow) {
+          if (lsc[erow - 1] != srow) continue;
+          for (int ecol = 1; ecol <= T; ++ecol) {
+            int scol = lsc[ecol - 1];
+            if (scol < srow) continue;
+            int mx = ecol > erow ? ecol : erow;
+            int dcp = (mx - 1) * mx / 2 + mx;
+            int kk = dcp - abs(ecol - erow);
+            coeff_row[scol] = coeff_row[scol] + emat[kk - 1];
+          }
+          break;
+        }
+      }
+      k_diag[srow - 1] = coeff_row[srow];
+      coeff_row[srow] = 0.0;
+      if (srow == neq) continue;
+      int64_t st = om->row_start[srow - 1];
+      for (int j = 0; j < om->k_ptrs[srow - 1]; ++j) {
+        int col = om->k_indexes[st + j];
+        k_coeffs[st + j] = coeff_row[col];
+        coeff_row[col] = 0.0;
+      }
+    }
+    free(coeff_row);
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* state roll-over and host views                                      */
+/* ------------------------------------------------------------------ */
+int orc_commit_step(orc_model* om) {
+  for (int b = 0; b < om->nelblk; ++b) {
+    orc_block* B = &om->blk[b];
+    int64_t nq = (int64_t)B->span * B->ngp;
+    memcpy(B->urcs_n, B->urcs_n1, sizeof(double) * 9 * nq);
+    memcpy(B->eps_n, B->eps_n1, sizeof(double) * 6 * nq);
+    memcpy(B->hist_n, B->hist_n1, sizeof(double) * B->hist * nq);
+  }
+  return 0;
+}
+
+int orc_rollback(orc_model* om) {
+  for (int b = 0; b < om->nelblk; ++b) {
+    orc_block* B = &om->blk[b];
+    int64_t nq = (int64_t)B->span * B->ngp;
+    memcpy(B->urcs_n1, B->urcs_n, sizeof(double) * 9 * nq);
+    memcpy(B->eps_n1, B->eps_n, sizeof(double) * 6 * nq);
+    memcpy(B->hist_n1, B->hist_n, sizeof(double) * B->hist * nq);
+  }
+  return 0;
+}
+
+static double* state_ptr(const orc_block* B, int which, int64_t* n) {
+  int64_t nq = (int64_t)B->span * B->ngp;
+  switch (which) {
+    case W3D_STRESS_N: *n = 9 * nq; return B->urcs_n;
+    case W3D_STRESS_N1: *n = 9 * nq; return B->urcs_n1;
+    case W3D_STRAIN_N: *n = 6 * nq; return B->eps_n;
+    case W3D_STRAIN_N1: *n = 6 * nq; return B->eps_n1;
+    case W3D_HIST_N: *n = B->hist * nq; return B->hist_n;
+    case W3D_HIST_N1: *n = B->hist * nq; return B->hist_n1;
+    case W3D_CEP: *n = 21 * nq; return B->cep;
+    case W3D_ESTIFF: *n = (int64_t)B->utsz * B->span; return B->ek;
+    case W3D_EINFVEC: *n = (int64_t)B->totdof * B->span; return B->einfvec;
+    case W3D_ELEM_VOL: *n = B->span; return B->evol;
+    default: *n = 0; return NULL;
+  }
+}
+
+int orc_get_block_state(const orc_model* om, int blk, int which, double* out,
+                        int64_t n_out) {
+  if (blk < 1 || blk > om->nelblk) return fail(W3D_ERR_ARG, "bad block");
+  int64_t n;
+  double* p = state_ptr(&om->blk[blk - 1], which, &n);
+  if (!p || n_out < n) return fail(W3D_ERR_ARG, "bad selector or size");
+  memcpy(out, p, sizeof(double) * n);
+  return 0;
+}
+
+int orc_set_block_state(orc_model* om, int blk, int which, const double* in,
+                        int64_t n_in) {
+  if (blk < 1 || blk > om->nelblk) return fail(W3D_ERR_ARG, "bad block");
+  if (which != W3D_STRESS_N && which != W3D_STRAIN_N && which != W3D_HIST_N)
+    return fail(W3D_ERR_ARG, "bad selector");
+  int64_t n;
+  double* p = state_ptr(&om->blk[blk - 1], which, &n);
+  if (n_in < n) return fail(W3D_ERR_ARG, "bad size");
+  memcpy(p, in, sizeof(double) * n);
+  return 0;
+}

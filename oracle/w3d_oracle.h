@@ -1,0 +1,86 @@
+/*
+ * w3d_oracle.h -- CPU FP64 restatement of WARP3D's element-block hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  The oracle is the checker for libw3dblk: only
+ * the tests, the smoke check and the CPU-baseline leg of bench.py load it.
+ * The product path never calls it.
+ *
+ * Every function here restates a reference routine (file:line under the
+ * reference's src/), in the reference's data layouts, so the tests can compare
+ * libw3dblk's device results with it array by array.
+ */
+#ifndef W3D_ORACLE_H
+#define W3D_ORACLE_H
+
+#include <stdint.h>
+#include "../include/w3d.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_model orc_model;
+
+/* ---- leaf routines (exported for the pinning tests) ------------------ */
+int orc_ngp(int etype, int order);
+int orc_nnode(int etype);
+void orc_getgpts(int etype, int order, int gpn, double* xi, double* eta,
+                 double* zeta, double* w);
+void orc_derivs(int etype, double xi, double eta, double zeta, double* nxi,
+                double* neta, double* nzeta);
+/* jacob1 for one element: ce is (3N) x-block,y-block,z-block. */
+void orc_jacob1(int nnode, const double* nxi, const double* neta,
+                const double* nzeta, const double* ce, double jac[9],
+                double* dj, double gama[9]);
+/* B (6 x 3N, row-major [r][dof]) of blcmp1 (+ bmod when vol != NULL). */
+void orc_bmatrix(int nnode, const double gama[9], const double* nxi,
+                 const double* neta, const double* nzeta, const double* vol,
+                 double eps_bbar, double* b);
+/*
+ * mm01 + cnst1 for one Gauss point of one element (mm01.f:24, :1196).
+ * hist/hist1 are 11 words, cgn/cgn1 9 words, deps 6, cep 36 (6x6 row-major).
+ * first_hist11 is history(1,11) of the block's first element at this point
+ * (the quirk of mm01.f:845).  Returns 0, or -1 on a negative plastic
+ * multiplier (die_abort in the reference).
+ */
+int orc_mm01_gp(int step, double e, double nu, double beta, double hprime,
+                int lnelas, double yld, double* cgn, double* cgn1,
+                const double* deps, double* hist, double* hist1,
+                double first_hist11, double* cep);
+/* H' = e*et/(e-et) in REAL arithmetic (elprp.f:872). */
+float orc_hprime(float e, float et);
+
+/* ---- block model ------------------------------------------------------ */
+orc_model* orc_create(const w3d_model* m);
+void orc_destroy(orc_model* om);
+const char* orc_last_error(void);
+int orc_get_blocking(const orc_model* om, int32_t* nelblk, int32_t* elblks);
+int orc_block_info(const orc_model* om, int blk, int32_t* info5);
+void orc_set_threads(int n);
+
+int orc_recover(orc_model* om, int step, int iter, int extrapolated,
+                const double* u, const double* du, double* ifv,
+                double* energy, double* plastic_work);
+/* Recovery of blocks [first, first+n) only, element forces left in einfvec. */
+int orc_recover_blocks(orc_model* om, int first, int n, int step, int iter,
+                       int extrapolated, const double* u, const double* du);
+int orc_tangent(orc_model* om, int step, int iter);
+int orc_tangent_blocks(orc_model* om, int first, int n, int step, int iter);
+int orc_pattern(orc_model* om, const int32_t* cstmap, int32_t* neqns,
+                int64_t* ncoeff);
+int orc_get_pattern(const orc_model* om, int32_t* dof_eqn_map,
+                    int32_t* eqn_node_map, int32_t* k_ptrs,
+                    int32_t* k_indexes);
+int orc_assemble_vss(orc_model* om, double* k_diag, double* k_coeffs);
+int orc_commit_step(orc_model* om);
+int orc_rollback(orc_model* om);
+int orc_get_block_state(const orc_model* om, int blk, int which, double* out,
+                        int64_t n_out);
+int orc_set_block_state(orc_model* om, int blk, int which, const double* in,
+                        int64_t n_in);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

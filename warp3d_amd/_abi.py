@@ -1,0 +1,171 @@
+"""ctypes mirror of include/w3d.h (structures, enums and prototypes).
+
+The C header is the boundary; this module only restates its layouts so the
+Python host code and the tests can fill a ``w3d_model`` from numpy arrays.
+"""
+import ctypes as C
+
+import numpy as np
+
+W3D_ABI_VERSION = 1
+
+W3D_OK = 0
+W3D_CUT_STEP = 1
+
+W3D_Q3DISOP = 1
+W3D_L3DISOP = 2
+W3D_TET10 = 6
+
+W3D_MM01 = 1
+W3D_MM10 = 10
+
+W3D_FP64 = 0
+W3D_FP32 = 1
+
+STRESS_N, STRESS_N1, STRAIN_N, STRAIN_N1, HIST_N, HIST_N1, CEP, ESTIFF, \
+    EINFVEC, ELEM_VOL = range(10)
+
+NNODE = {W3D_Q3DISOP: 20, W3D_L3DISOP: 8, W3D_TET10: 10}
+
+
+class W3DMaterial(C.Structure):
+    _fields_ = [
+        ("model", C.c_int32),
+        ("e", C.c_float), ("nu", C.c_float), ("beta", C.c_float),
+        ("tan_e", C.c_float), ("yld_pt", C.c_float),
+        ("linear_elastic", C.c_int32),
+        ("c11", C.c_double), ("c12", C.c_double), ("c44", C.c_double),
+        ("tau_y", C.c_double), ("tau_v", C.c_double),
+        ("voce_m", C.c_double), ("theta_0", C.c_double),
+        ("rate_n", C.c_double), ("iD_v", C.c_double),
+        ("atol", C.c_double), ("atol1", C.c_double),
+        ("rtol", C.c_double), ("rtol1", C.c_double),
+        ("miter", C.c_int32), ("angle_convention", C.c_int32),
+    ]
+
+
+class W3DModel(C.Structure):
+    _fields_ = [
+        ("abi_version", C.c_int32),
+        ("nonode", C.c_int32),
+        ("noelem", C.c_int32),
+        ("c", C.POINTER(C.c_double)),
+        ("incid", C.POINTER(C.c_int32)),
+        ("incmap", C.POINTER(C.c_int32)),
+        ("etype", C.POINTER(C.c_int32)),
+        ("int_order", C.POINTER(C.c_int32)),
+        ("matnum", C.POINTER(C.c_int32)),
+        ("bbar", C.POINTER(C.c_int32)),
+        ("angles", C.POINTER(C.c_double)),
+        ("nmat", C.c_int32),
+        ("mats", C.POINTER(W3DMaterial)),
+        ("nelblk", C.c_int32),
+        ("elblks", C.POINTER(C.c_int32)),
+        ("auto_size", C.c_int32),
+        ("eps_bbar", C.c_double),
+    ]
+
+
+def dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def iptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def lptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+def mm01_material(e, nu, yld_pt, tan_e=None, hprime=None, beta=1.0,
+                  linear_elastic=False):
+    """An mm01 material.  Give either the tangent modulus tan_e (the
+    reference's input) or a target H' (then tan_e = E H / (E + H), rounded
+    to REAL like the reference's input would be)."""
+    m = W3DMaterial()
+    m.model = W3D_MM01
+    m.e = e
+    m.nu = nu
+    m.yld_pt = yld_pt
+    if tan_e is None:
+        tan_e = float(np.float32(e * hprime / (e + hprime)))
+    m.tan_e = tan_e
+    m.beta = beta
+    m.linear_elastic = int(bool(linear_elastic))
+    return m
+
+
+class ModelArrays:
+    """Holds the numpy arrays a w3d_model points into (keeps them alive)."""
+
+    def __init__(self, coords, incid, incmap, etype, int_order, matnum,
+                 bbar, materials, elblks=None, auto_size=128, eps_bbar=0.0,
+                 angles=None):
+        self.coords = np.ascontiguousarray(coords, dtype=np.float64).ravel()
+        self.incid = np.ascontiguousarray(incid, dtype=np.int32).ravel()
+        self.incmap = np.ascontiguousarray(incmap, dtype=np.int32)
+        self.etype = np.ascontiguousarray(etype, dtype=np.int32)
+        self.int_order = np.ascontiguousarray(int_order, dtype=np.int32)
+        self.matnum = np.ascontiguousarray(matnum, dtype=np.int32)
+        self.bbar = np.ascontiguousarray(bbar, dtype=np.int32)
+        self.angles = (None if angles is None else
+                       np.ascontiguousarray(angles, dtype=np.float64).ravel())
+        self.materials = list(materials)
+        self.mats = (W3DMaterial * len(self.materials))(*self.materials)
+        self.elblks = (None if elblks is None else
+                       np.ascontiguousarray(elblks, dtype=np.int32).ravel())
+        self.auto_size = auto_size
+        self.eps_bbar = eps_bbar
+        self.nonode = self.coords.size // 3
+        self.noelem = self.etype.size
+
+    def struct(self):
+        m = W3DModel()
+        m.abi_version = W3D_ABI_VERSION
+        m.nonode = self.nonode
+        m.noelem = self.noelem
+        m.c = dptr(self.coords)
+        m.incid = iptr(self.incid)
+        m.incmap = iptr(self.incmap)
+        m.etype = iptr(self.etype)
+        m.int_order = iptr(self.int_order)
+        m.matnum = iptr(self.matnum)
+        m.bbar = iptr(self.bbar)
+        m.angles = (dptr(self.angles) if self.angles is not None
+                    else C.POINTER(C.c_double)())
+        m.nmat = len(self.materials)
+        m.mats = C.cast(self.mats, C.POINTER(W3DMaterial))
+        if self.elblks is not None:
+            m.nelblk = self.elblks.size // 4
+            m.elblks = iptr(self.elblks)
+        else:
+            m.nelblk = 0
+            m.elblks = C.POINTER(C.c_int32)()
+        m.auto_size = self.auto_size
+        m.eps_bbar = self.eps_bbar
+        return m
+
+
+# prototypes shared by the device library and the oracle (same signatures
+# with a different prefix)
+_P = C.c_void_p
+_I = C.c_int32
+_D = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int32)
+_LP = C.POINTER(C.c_int64)
+
+PROTOS = {
+    "get_blocking": (_I, [_P, _IP, _IP]),
+    "block_info": (_I, [_P, _I, _IP]),
+    "recover": (_I, [_P, _I, _I, _I, _D, _D, _D, _D, _D]),
+    "recover_blocks": (_I, [_P, _I, _I, _I, _I, _I, _D, _D]),
+    "tangent_blocks": (_I, [_P, _I, _I, _I, _I]),
+    "pattern": (_I, [_P, _IP, _IP, _LP]),
+    "get_pattern": (_I, [_P, _IP, _IP, _IP, _IP]),
+    "assemble_vss": (_I, [_P, _D, _D]),
+    "commit_step": (_I, [_P]),
+    "rollback": (_I, [_P]),
+    "get_block_state": (_I, [_P, _I, _I, _D, C.c_int64]),
+    "set_block_state": (_I, [_P, _I, _I, _D, C.c_int64]),
+}

@@ -1,0 +1,1094 @@
+// w3d_api.cpp -- the C ABI of libw3dblk (include/w3d.h): model set-up,
+// blocking, device state, orchestration of the kernels, sparsity pattern and
+// the host views of block state.
+//
+// Reference set-up steps restated here (file:line under the reference's
+// src/): inelbk.f:282-548 automatic blocking and its compatibility rules,
+// setup.f:160-190 inverse incidences, reopen.f:1315 edest_init,
+// elprp.f:872 H' = E Et / (E - Et) in REAL, rknstr.f:1018
+// setup_mm01_rknstr, assemble_code.f:561 dof_map, :20 / :260
+// count_profile_symmetric / build_col_sparse_symm, update.f:15.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "w3d_internal.h"
+
+namespace w3d {
+bool fill_gauss_table(int etype, int order, GaussTable* t);
+int ngp_of(int etype, int order);
+int nnode_of(int etype);
+}  // namespace w3d
+
+using namespace w3d;
+
+namespace {
+
+thread_local std::string g_err = "no error";
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                        \
+  do {                                                                   \
+    hipError_t e_ = (x);                                                 \
+    if (e_ != hipSuccess)                                                \
+      return fail(W3D_ERR_DEVICE, std::string(#x) + ": " +               \
+                                      hipGetErrorString(e_));            \
+  } while (0)
+
+template <typename T>
+int dalloc(T** p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess)
+    return fail(W3D_ERR_NOMEM, std::string("hipMalloc: ") +
+                                   hipGetErrorString(e));
+  return W3D_OK;
+}
+
+template <typename T>
+void dfree(T*& p) {
+  if (p) hipFree((void*)p);
+  p = nullptr;
+}
+
+float hprime_real(float e, float et) {
+  volatile float num = e * et;  // REAL arithmetic as elprp.f:872
+  volatile float den = e - et;
+  return num / den;
+}
+
+}  // namespace
+
+struct w3d_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int nonode = 0, noelem = 0, nelblk = 0;
+  std::vector<int32_t> etype, order, matnum, bbar;
+  std::vector<int32_t> elblks;
+  std::vector<BlockDesc> blocks;
+  std::vector<MatDesc> mats;
+  std::vector<GaussTable> tables;
+  std::vector<int32_t> conn;      // 0-based nodes, block order
+  std::vector<int32_t> elem_blk;  // block of each element (0-based)
+  // inverse incidences: per node, (element, local node) ascending element
+  std::vector<int64_t> inv_ptr;
+  std::vector<int32_t> inv_elem, inv_ln;
+  int64_t nq = 0, nhist = 0, nek = 0, nifv = 0;
+  // device memory
+  BlockDesc* d_blocks = nullptr;
+  MatDesc* d_mats = nullptr;
+  GaussTable* d_tables = nullptr;
+  double* d_coords = nullptr;
+  int32_t* d_conn = nullptr;
+  double *d_stress_n = nullptr, *d_stress_n1 = nullptr;
+  double *d_strain_n = nullptr, *d_strain_n1 = nullptr;
+  double *d_hist_n = nullptr, *d_hist_n1 = nullptr;
+  double* d_cep = nullptr;
+  double* d_ek64 = nullptr;
+  float* d_ek32 = nullptr;
+  double *d_einfvec = nullptr, *d_evol = nullptr, *d_energy_q = nullptr;
+  int32_t* d_flags = nullptr;
+  WorkItem *d_work_rec = nullptr, *d_work_tan = nullptr;
+  std::vector<int> rec_groups, rec_start, rec_count;
+  std::vector<int> tan_groups, tan_start, tan_count;
+  int64_t *d_inv_ptr = nullptr, *d_inv_off = nullptr;
+  int32_t* d_inv_nn = nullptr;
+  int64_t* d_blk_q_off = nullptr;
+  int32_t *d_blk_ngp = nullptr, *d_blk_span = nullptr;
+  double* d_blk_sums = nullptr;
+  double *d_u = nullptr, *d_du = nullptr, *d_ifv = nullptr;
+  // sparsity + assembly map
+  int neqns = 0;
+  int64_t ncoeff = 0;
+  std::vector<int32_t> dof_eqn_map, eqn_node_map, k_ptrs, k_indexes;
+  std::vector<int64_t> row_start;
+  int64_t* d_asm_ptr = nullptr;
+  uint32_t* d_asm_src = nullptr;
+  double* d_kout = nullptr;  // neqns diagonal then ncoeff coefficients
+  int tangent_precision = -1;
+  double eps_bbar = 0.0;
+  // phase timing (w3d_set_profiling)
+  bool profiling = false;
+  hipEvent_t ev_a[W3D_NUM_PHASES] = {}, ev_b[W3D_NUM_PHASES] = {};
+  bool pending[W3D_NUM_PHASES] = {};
+  int64_t prof_n[W3D_NUM_PHASES] = {};
+  double prof_ms[W3D_NUM_PHASES] = {};
+
+  DevModel dev() const {
+    DevModel M;
+    M.blocks = d_blocks;
+    M.mats = d_mats;
+    M.tables = d_tables;
+    M.coords = d_coords;
+    M.conn = d_conn;
+    M.stress_n = d_stress_n;
+    M.stress_n1 = d_stress_n1;
+    M.strain_n = d_strain_n;
+    M.strain_n1 = d_strain_n1;
+    M.hist_n = d_hist_n;
+    M.hist_n1 = d_hist_n1;
+    M.cep = d_cep;
+    M.ek64 = d_ek64;
+    M.ek32 = d_ek32;
+    M.einfvec = d_einfvec;
+    M.evol = d_evol;
+    M.energy_q = d_energy_q;
+    M.flags = d_flags;
+    M.eps_bbar = eps_bbar;
+    return M;
+  }
+};
+
+namespace {
+
+bool compatible(const w3d_ctx* c, int a, int b) {
+  return c->etype[a] == c->etype[b] && c->matnum[a] == c->matnum[b] &&
+         c->order[a] == c->order[b] &&
+         ngp_of(c->etype[a], c->order[a]) == ngp_of(c->etype[b], c->order[b]) &&
+         c->bbar[a] == c->bbar[b];
+}
+
+// inelbk_simple_blocking (inelbk.f:282): one sequential pass; a new block
+// opens when the current one is full or the element is incompatible with
+// the block's first element.
+void auto_blocking(w3d_ctx* c, int auto_size) {
+  c->elblks.assign({1, 1, 0, 0});
+  int nb = 1, cur = 1, felem = 1;
+  for (int e = 2; e <= c->noelem; ++e) {
+    bool newblk = false;
+    cur = cur + 1;
+    if (cur > auto_size) newblk = true;
+    if (!newblk && !compatible(c, felem - 1, e - 1)) newblk = true;
+    if (!newblk) {
+      c->elblks[4 * (nb - 1)] = cur;
+      continue;
+    }
+    nb += 1;
+    felem = e;
+    c->elblks.insert(c->elblks.end(), {1, felem, 0, 0});
+    cur = 1;
+  }
+  c->nelblk = nb;
+}
+
+// Work lists: chunks of consecutive elements of one block, grouped by
+// kernel instantiation.
+void build_work(const w3d_ctx* c, bool tangent, std::vector<WorkItem>& items,
+                std::vector<int>& groups, std::vector<int>& start,
+                std::vector<int>& count) {
+  std::vector<std::pair<int, WorkItem>> all;
+  for (int b = 0; b < c->nelblk; ++b) {
+    const BlockDesc& B = c->blocks[b];
+    int E = tangent ? tan_elems(B.nnode) : rec_elems(B.ngp);
+    int key = group_key(B.nnode, B.ngp, B.bbar);
+    for (int f = 0; f < B.span; f += E) {
+      WorkItem w;
+      w.blk = b;
+      w.first = f;
+      w.count = std::min(E, B.span - f);
+      w.pad_ = 0;
+      all.push_back({key, w});
+    }
+  }
+  std::stable_sort(all.begin(), all.end(),
+                   [](const auto& x, const auto& y) { return x.first < y.first; });
+  items.clear();
+  groups.clear();
+  start.clear();
+  count.clear();
+  for (size_t k = 0; k < all.size(); ++k) {
+    if (groups.empty() || groups.back() != all[k].first) {
+      groups.push_back(all[k].first);
+      start.push_back((int)k);
+      count.push_back(0);
+    }
+    count.back() += 1;
+    items.push_back(all[k].second);
+  }
+}
+
+void free_ctx(w3d_ctx* c) {
+  dfree(c->d_blocks); dfree(c->d_mats); dfree(c->d_tables);
+  dfree(c->d_coords); dfree(c->d_conn);
+  dfree(c->d_stress_n); dfree(c->d_stress_n1);
+  dfree(c->d_strain_n); dfree(c->d_strain_n1);
+  dfree(c->d_hist_n); dfree(c->d_hist_n1); dfree(c->d_cep);
+  dfree(c->d_ek64); dfree(c->d_ek32); dfree(c->d_einfvec);
+  dfree(c->d_evol); dfree(c->d_energy_q); dfree(c->d_flags);
+  dfree(c->d_work_rec); dfree(c->d_work_tan);
+  dfree(c->d_inv_ptr); dfree(c->d_inv_off); dfree(c->d_inv_nn);
+  dfree(c->d_blk_q_off); dfree(c->d_blk_ngp); dfree(c->d_blk_span);
+  dfree(c->d_blk_sums); dfree(c->d_u); dfree(c->d_du); dfree(c->d_ifv);
+  dfree(c->d_asm_ptr); dfree(c->d_asm_src); dfree(c->d_kout);
+  for (int ph = 0; ph < W3D_NUM_PHASES; ++ph) {
+    if (c->ev_a[ph]) hipEventDestroy(c->ev_a[ph]);
+    if (c->ev_b[ph]) hipEventDestroy(c->ev_b[ph]);
+    c->ev_a[ph] = c->ev_b[ph] = nullptr;
+  }
+  if (c->stream) hipStreamDestroy(c->stream);
+  c->stream = nullptr;
+}
+
+template <typename T>
+int upload(T** d, const std::vector<T>& h) {
+  int rc = dalloc(d, h.size());
+  if (rc) return rc;
+  if (!h.empty())
+    HIPCHK(hipMemcpy(*d, h.data(), h.size() * sizeof(T),
+                     hipMemcpyHostToDevice));
+  return W3D_OK;
+}
+
+int create_impl(w3d_ctx* c, const w3d_model* m) {
+  if (m->nonode <= 0 || m->noelem <= 0 || !m->c || !m->incid ||
+      !m->incmap || !m->etype || !m->int_order || !m->matnum || !m->mats ||
+      m->nmat <= 0)
+    return fail(W3D_ERR_ARG, "w3d_create: incomplete model");
+  c->nonode = m->nonode;
+  c->noelem = m->noelem;
+  c->eps_bbar = m->eps_bbar;
+  c->etype.resize(m->noelem);
+  c->order.resize(m->noelem);
+  c->matnum.resize(m->noelem);
+  c->bbar.resize(m->noelem);
+  for (int e = 0; e < m->noelem; ++e) {
+    c->etype[e] = m->etype[e];
+    c->order[e] = m->int_order[e];
+    c->matnum[e] = m->matnum[e];
+    c->bbar[e] = m->etype[e] == W3D_L3DISOP && m->bbar && m->bbar[e];
+    if (nnode_of(m->etype[e]) == 0 || ngp_of(m->etype[e], m->int_order[e]) == 0)
+      return fail(W3D_ERR_UNSUPPORTED,
+                  "w3d_create: element type / integration order of element " +
+                      std::to_string(e + 1));
+    if (m->matnum[e] < 1 || m->matnum[e] > m->nmat)
+      return fail(W3D_ERR_ARG, "w3d_create: material number of element " +
+                                   std::to_string(e + 1));
+  }
+  // materials (setup_mm01_rknstr: REAL props widened to FP64)
+  c->mats.resize(m->nmat);
+  for (int k = 0; k < m->nmat; ++k) {
+    const w3d_material& s = m->mats[k];
+    MatDesc& d = c->mats[k];
+    std::memset(&d, 0, sizeof d);
+    d.model = s.model;
+    if (s.model == W3D_MM01) {
+      if (!(s.e > s.tan_e) || s.e <= 0.0f || s.nu <= -1.0f || s.nu >= 0.5f)
+        return fail(W3D_ERR_ARG, "w3d_create: mm01 properties of material " +
+                                     std::to_string(k + 1));
+      d.lnelas = s.linear_elastic;
+      d.e = (double)s.e;
+      d.nu = (double)s.nu;
+      d.beta = (double)s.beta;
+      d.hprime = (double)hprime_real(s.e, s.tan_e);
+      d.yld = (double)s.yld_pt;
+    } else {
+      return fail(W3D_ERR_UNSUPPORTED,
+                  "w3d_create: material model " + std::to_string(s.model) +
+                      " of material " + std::to_string(k + 1));
+    }
+  }
+  // blocking
+  if (m->nelblk > 0) {
+    if (!m->elblks) return fail(W3D_ERR_ARG, "w3d_create: elblks missing");
+    c->nelblk = m->nelblk;
+    c->elblks.assign(m->elblks, m->elblks + 4 * m->nelblk);
+    int next = 1;
+    for (int b = 0; b < c->nelblk; ++b) {
+      int span = c->elblks[4 * b], felem = c->elblks[4 * b + 1];
+      if (span < 1 || felem != next)
+        return fail(W3D_ERR_ARG, "w3d_create: blocks must tile the elements");
+      for (int i = 1; i < span; ++i)
+        if (!compatible(c, felem - 1, felem - 1 + i))
+          return fail(W3D_ERR_ARG,
+                      "w3d_create: incompatible elements in block " +
+                          std::to_string(b + 1));
+      next = felem + span;
+    }
+    if (next != c->noelem + 1)
+      return fail(W3D_ERR_ARG, "w3d_create: blocks must tile the elements");
+  } else {
+    auto_blocking(c, m->auto_size > 0 ? m->auto_size : 128);
+  }
+  // Gauss tables
+  std::vector<std::pair<int, int>> tkeys;
+  auto table_of = [&](int et, int ord) {
+    for (size_t k = 0; k < tkeys.size(); ++k)
+      if (tkeys[k].first == et && tkeys[k].second == ord) return (int)k;
+    tkeys.push_back({et, ord});
+    GaussTable t;
+    fill_gauss_table(et, ord, &t);
+    c->tables.push_back(t);
+    return (int)tkeys.size() - 1;
+  };
+  // blocks
+  c->blocks.resize(c->nelblk);
+  c->elem_blk.resize(c->noelem);
+  int64_t q = 0, h = 0, ek = 0, fv = 0, cn = 0;
+  for (int b = 0; b < c->nelblk; ++b) {
+    BlockDesc& B = c->blocks[b];
+    std::memset(&B, 0, sizeof B);
+    B.span = c->elblks[4 * b];
+    B.felem = c->elblks[4 * b + 1];
+    int fe = B.felem - 1;
+    B.etype = c->etype[fe];
+    B.order = c->order[fe];
+    B.ngp = ngp_of(B.etype, B.order);
+    B.nnode = nnode_of(B.etype);
+    B.totdof = 3 * B.nnode;
+    B.utsz = B.totdof * (B.totdof + 1) / 2;
+    B.mat = c->matnum[fe] - 1;
+    B.model = c->mats[B.mat].model;
+    B.hist = B.model == W3D_MM01 ? kMM01Hist : kMM10Hist;
+    B.bbar = c->bbar[fe];
+    B.table = table_of(B.etype, B.order);
+    B.q_off = q;
+    B.hist_off = h;
+    B.ek_off = ek;
+    B.ifv_off = fv;
+    B.conn_off = cn;
+    q += (int64_t)B.span * B.ngp;
+    h += (int64_t)B.span * B.ngp * B.hist;
+    ek += (int64_t)B.span * B.utsz;
+    fv += (int64_t)B.span * B.totdof;
+    cn += (int64_t)B.span * B.nnode;
+    for (int i = 0; i < B.span; ++i) c->elem_blk[fe + i] = b;
+  }
+  c->nq = q;
+  c->nhist = h;
+  c->nek = ek;
+  c->nifv = fv;
+  if (ek >= (int64_t)UINT32_MAX)
+    return fail(W3D_ERR_UNSUPPORTED, "w3d_create: too many Ke words");
+  c->conn.resize(cn);
+  for (int b = 0; b < c->nelblk; ++b) {
+    const BlockDesc& B = c->blocks[b];
+    for (int i = 0; i < B.span; ++i) {
+      int e = B.felem - 1 + i;
+      for (int j = 0; j < B.nnode; ++j) {
+        int node = m->incid[m->incmap[e] - 1 + j];
+        if (node < 1 || node > c->nonode)
+          return fail(W3D_ERR_ARG, "w3d_create: bad node in element " +
+                                       std::to_string(e + 1));
+        c->conn[B.conn_off + (int64_t)i * B.nnode + j] = node - 1;
+      }
+    }
+  }
+  // inverse incidences (element order; repeated nodes listed once)
+  c->inv_ptr.assign(c->nonode + 1, 0);
+  for (int b = 0; b < c->nelblk; ++b) {
+    const BlockDesc& B = c->blocks[b];
+    for (int i = 0; i < B.span; ++i) {
+      const int32_t* nd = &c->conn[B.conn_off + (int64_t)i * B.nnode];
+      for (int j = 0; j < B.nnode; ++j) {
+        bool rep = false;
+        for (int k = 0; k < j; ++k) rep |= nd[k] == nd[j];
+        if (rep)
+          return fail(W3D_ERR_UNSUPPORTED,
+                      "w3d_create: repeated nodes in an element");
+        c->inv_ptr[nd[j] + 1] += 1;
+      }
+    }
+  }
+  for (int n = 0; n < c->nonode; ++n) c->inv_ptr[n + 1] += c->inv_ptr[n];
+  c->inv_elem.resize(c->inv_ptr[c->nonode]);
+  c->inv_ln.resize(c->inv_ptr[c->nonode]);
+  {
+    std::vector<int64_t> fill(c->inv_ptr.begin(), c->inv_ptr.end() - 1);
+    for (int b = 0; b < c->nelblk; ++b) {
+      const BlockDesc& B = c->blocks[b];
+      for (int i = 0; i < B.span; ++i) {
+        const int32_t* nd = &c->conn[B.conn_off + (int64_t)i * B.nnode];
+        for (int j = 0; j < B.nnode; ++j) {
+          int64_t p = fill[nd[j]]++;
+          c->inv_elem[p] = B.felem - 1 + i;
+          c->inv_ln[p] = j;
+        }
+      }
+    }
+  }
+  for (int n = 0; n < c->nonode; ++n)
+    if (c->inv_ptr[n + 1] == c->inv_ptr[n])
+      return fail(W3D_ERR_ARG, "w3d_create: node " + std::to_string(n + 1) +
+                                   " has no elements");
+
+  // ---- device ----
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  int rc;
+  if ((rc = upload(&c->d_blocks, c->blocks))) return rc;
+  if ((rc = upload(&c->d_mats, c->mats))) return rc;
+  if ((rc = upload(&c->d_tables, c->tables))) return rc;
+  if ((rc = upload(&c->d_conn, c->conn))) return rc;
+  {
+    std::vector<double> xyz(m->c, m->c + 3 * (int64_t)m->nonode);
+    if ((rc = upload(&c->d_coords, xyz))) return rc;
+  }
+  struct A {
+    double** p;
+    int64_t n;
+  } arrs[] = {{&c->d_stress_n, 9 * q},  {&c->d_stress_n1, 9 * q},
+              {&c->d_strain_n, 6 * q},  {&c->d_strain_n1, 6 * q},
+              {&c->d_hist_n, h},        {&c->d_hist_n1, h},
+              {&c->d_cep, 21 * q},      {&c->d_ek64, ek},
+              {&c->d_einfvec, fv},      {&c->d_evol, c->noelem},
+              {&c->d_energy_q, 2 * q},  {&c->d_u, 3 * (int64_t)c->nonode},
+              {&c->d_du, 3 * (int64_t)c->nonode},
+              {&c->d_ifv, 3 * (int64_t)c->nonode},
+              {&c->d_blk_sums, 2 * (int64_t)c->nelblk}};
+  for (auto& a : arrs) {
+    if ((rc = dalloc(a.p, a.n))) return rc;
+    HIPCHK(hipMemset(*a.p, 0, std::max<int64_t>(a.n, 1) * sizeof(double)));
+  }
+  if ((rc = dalloc(&c->d_ek32, ek))) return rc;
+  HIPCHK(hipMemset(c->d_ek32, 0, std::max<int64_t>(ek, 1) * sizeof(float)));
+  if ((rc = dalloc(&c->d_flags, 4))) return rc;
+  HIPCHK(hipMemset(c->d_flags, 0, 4 * sizeof(int32_t)));
+  {
+    std::vector<WorkItem> items;
+    build_work(c, false, items, c->rec_groups, c->rec_start, c->rec_count);
+    if ((rc = upload(&c->d_work_rec, items))) return rc;
+    build_work(c, true, items, c->tan_groups, c->tan_start, c->tan_count);
+    if ((rc = upload(&c->d_work_tan, items))) return rc;
+  }
+  {
+    std::vector<int64_t> off(c->inv_elem.size());
+    std::vector<int32_t> nn(c->inv_elem.size());
+    for (size_t p = 0; p < off.size(); ++p) {
+      int e = c->inv_elem[p];
+      const BlockDesc& B = c->blocks[c->elem_blk[e]];
+      int i = e - (B.felem - 1);
+      off[p] = B.ifv_off + (int64_t)i * B.totdof + c->inv_ln[p];
+      nn[p] = B.nnode;
+    }
+    if ((rc = upload(&c->d_inv_ptr, c->inv_ptr))) return rc;
+    if ((rc = upload(&c->d_inv_off, off))) return rc;
+    if ((rc = upload(&c->d_inv_nn, nn))) return rc;
+  }
+  {
+    std::vector<int64_t> qo(c->nelblk);
+    std::vector<int32_t> ng(c->nelblk), sp(c->nelblk);
+    for (int b = 0; b < c->nelblk; ++b) {
+      qo[b] = c->blocks[b].q_off;
+      ng[b] = c->blocks[b].ngp;
+      sp[b] = c->blocks[b].span;
+    }
+    if ((rc = upload(&c->d_blk_q_off, qo))) return rc;
+    if ((rc = upload(&c->d_blk_ngp, ng))) return rc;
+    if ((rc = upload(&c->d_blk_span, sp))) return rc;
+  }
+  return W3D_OK;
+}
+
+void prof_begin(w3d_ctx* c, int ph) {
+  if (!c->profiling) return;
+  hipEventRecord(c->ev_a[ph], c->stream);
+}
+void prof_end(w3d_ctx* c, int ph) {
+  if (!c->profiling) return;
+  hipEventRecord(c->ev_b[ph], c->stream);
+  c->pending[ph] = true;
+}
+// called after the stream has been synchronised
+void prof_collect(w3d_ctx* c) {
+  if (!c->profiling) return;
+  for (int ph = 0; ph < W3D_NUM_PHASES; ++ph) {
+    if (!c->pending[ph]) continue;
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, c->ev_a[ph], c->ev_b[ph]) == hipSuccess) {
+      c->prof_ms[ph] += ms;
+      c->prof_n[ph] += 1;
+    }
+    c->pending[ph] = false;
+  }
+}
+
+int check_flags(w3d_ctx* c) {
+  int32_t f[4];
+  HIPCHK(hipMemcpyAsync(f, c->d_flags, sizeof f, hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  prof_collect(c);
+  if (f[0]) return fail(W3D_ERR_NUMERIC, "mm01: negative plastic multiplier");
+  if (f[1]) return W3D_CUT_STEP;
+  return W3D_OK;
+}
+
+int recover_impl(w3d_ctx* c, int step, int iter, int extrapolated,
+                 const double* d_du, double* d_ifv, double* energy,
+                 double* plastic_work, int first_blk, int nblk) {
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemsetAsync(c->d_flags, 0, 4 * sizeof(int32_t), c->stream));
+  const int nonlinear = (iter >= 1 || extrapolated) ? 1 : 0;
+  const int want_energy = (iter > 0) ? 1 : 0;
+  DevModel M = c->dev();
+  int rc;
+  if (first_blk == 1 && nblk == c->nelblk) {
+    prof_begin(c, W3D_PHASE_RECOVER);
+    rc = launch_recover(M, c->blocks, c->d_work_rec, c->rec_groups,
+                        c->rec_start, c->rec_count, step, iter, nonlinear,
+                        d_du, want_energy, c->stream);
+    prof_end(c, W3D_PHASE_RECOVER);
+  } else {
+    // restrict the work list to the requested blocks
+    std::vector<WorkItem> items;
+    std::vector<int> g, s, n;
+    build_work(c, false, items, g, s, n);
+    std::vector<WorkItem> sel;
+    std::vector<int> sg, ss, sn;
+    for (size_t k = 0; k < g.size(); ++k) {
+      int st = (int)sel.size();
+      for (int i = s[k]; i < s[k] + n[k]; ++i)
+        if (items[i].blk >= first_blk - 1 && items[i].blk < first_blk - 1 + nblk)
+          sel.push_back(items[i]);
+      if ((int)sel.size() > st) {
+        sg.push_back(g[k]);
+        ss.push_back(st);
+        sn.push_back((int)sel.size() - st);
+      }
+    }
+    WorkItem* d_sel = nullptr;
+    if ((rc = upload(&d_sel, sel))) return rc;
+    rc = launch_recover(M, c->blocks, d_sel, sg, ss, sn, step, iter,
+                        nonlinear, d_du, want_energy, c->stream);
+    hipStreamSynchronize(c->stream);
+    dfree(d_sel);
+  }
+  if (rc) return fail(rc, "recover: kernel launch failed");
+  if (d_ifv) {
+    prof_begin(c, W3D_PHASE_IFV);
+    rc = launch_ifv_gather(c->d_einfvec, c->d_inv_ptr, c->d_inv_off,
+                           c->d_inv_nn, c->nonode, d_ifv, c->stream);
+    prof_end(c, W3D_PHASE_IFV);
+    if (rc) return fail(rc, "recover: ifv gather launch failed");
+  }
+  double en = 0.0, pw = 0.0;
+  if ((energy || plastic_work) && want_energy) {
+    prof_begin(c, W3D_PHASE_ENERGY);
+    rc = launch_energy_reduce(c->d_energy_q, c->d_blk_q_off, c->d_blk_ngp,
+                              c->d_blk_span, c->nelblk, c->d_blk_sums,
+                              c->stream);
+    prof_end(c, W3D_PHASE_ENERGY);
+    if (rc) return fail(rc, "recover: energy reduction launch failed");
+    std::vector<double> sums(2 * c->nelblk);
+    HIPCHK(hipMemcpyAsync(sums.data(), c->d_blk_sums,
+                          sums.size() * sizeof(double),
+                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int b = 0; b < c->nelblk; ++b) {
+      en = en + sums[2 * b];
+      pw = pw + sums[2 * b + 1];
+    }
+  }
+  if (energy) *energy = en;
+  if (plastic_work) *plastic_work = pw;
+  return check_flags(c);
+}
+
+// state views ----------------------------------------------------------
+struct View {
+  double* d = nullptr;
+  float* f = nullptr;
+  int64_t n = 0;
+};
+
+}  // namespace
+
+extern "C" {
+
+const char* w3d_last_error(void) { return g_err.c_str(); }
+
+const char* w3d_version(void) { return "libw3dblk 0.1 (gfx950)"; }
+
+int w3d_create(w3d_ctx** ctx, const w3d_model* model, int device) {
+  if (!ctx || !model) return fail(W3D_ERR_ARG, "w3d_create: null argument");
+  *ctx = nullptr;
+  if (model->abi_version != W3D_ABI_VERSION)
+    return fail(W3D_ERR_ARG, "w3d_create: ABI version mismatch");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(W3D_ERR_DEVICE, "w3d_create: no HIP device");
+  if (device < 0 || device >= ndev)
+    return fail(W3D_ERR_ARG, "w3d_create: bad device ordinal");
+  w3d_ctx* c = new w3d_ctx();
+  c->device = device;
+  int rc = create_impl(c, model);
+  if (rc) {
+    std::string keep = g_err;
+    free_ctx(c);
+    delete c;
+    g_err = keep;
+    return rc;
+  }
+  *ctx = c;
+  return W3D_OK;
+}
+
+int w3d_destroy(w3d_ctx* ctx) {
+  if (!ctx) return W3D_OK;
+  hipSetDevice(ctx->device);
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  free_ctx(ctx);
+  delete ctx;
+  return W3D_OK;
+}
+
+int w3d_get_blocking(const w3d_ctx* c, int32_t* nelblk, int32_t* elblks) {
+  if (!c) return fail(W3D_ERR_ARG, "null context");
+  if (nelblk) *nelblk = c->nelblk;
+  if (elblks) std::memcpy(elblks, c->elblks.data(), 16 * (size_t)c->nelblk);
+  return W3D_OK;
+}
+
+int w3d_block_info(const w3d_ctx* c, int32_t blk, int32_t* info5) {
+  if (!c || !info5 || blk < 1 || blk > c->nelblk)
+    return fail(W3D_ERR_ARG, "w3d_block_info: bad block");
+  const BlockDesc& B = c->blocks[blk - 1];
+  info5[0] = B.span;
+  info5[1] = B.ngp;
+  info5[2] = B.hist;
+  info5[3] = B.totdof;
+  info5[4] = B.utsz;
+  return W3D_OK;
+}
+
+int w3d_recover_dev(w3d_ctx* c, int32_t step, int32_t iter,
+                    int32_t extrapolated, const double* d_u,
+                    const double* d_du, double* d_ifv, double* energy,
+                    double* plastic_work) {
+  (void)d_u;  // the small-displacement update uses only the increment
+  if (!c || !d_du) return fail(W3D_ERR_ARG, "w3d_recover_dev: null argument");
+  if (step < 1 || iter < 0) return fail(W3D_ERR_ARG, "w3d_recover: step/iter");
+  return recover_impl(c, step, iter, extrapolated, d_du, d_ifv, energy,
+                      plastic_work, 1, c->nelblk);
+}
+
+int w3d_recover(w3d_ctx* c, int32_t step, int32_t iter, int32_t extrapolated,
+                const double* u, const double* du, double* ifv,
+                double* energy, double* plastic_work) {
+  if (!c || !du) return fail(W3D_ERR_ARG, "w3d_recover: null argument");
+  if (step < 1 || iter < 0) return fail(W3D_ERR_ARG, "w3d_recover: step/iter");
+  size_t nb = 3 * (size_t)c->nonode * sizeof(double);
+  HIPCHK(hipSetDevice(c->device));
+  if (u)
+    HIPCHK(hipMemcpyAsync(c->d_u, u, nb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_du, du, nb, hipMemcpyHostToDevice, c->stream));
+  int rc = recover_impl(c, step, iter, extrapolated, c->d_du,
+                        ifv ? c->d_ifv : nullptr, energy, plastic_work, 1,
+                        c->nelblk);
+  if (rc < 0) return rc;
+  if (ifv) {
+    HIPCHK(hipMemcpyAsync(ifv, c->d_ifv, nb, hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  return rc;
+}
+
+int w3d_recover_blocks(w3d_ctx* c, int32_t first_blk, int32_t nblk,
+                       int32_t step, int32_t iter, int32_t extrapolated,
+                       const double* u, const double* du) {
+  (void)u;
+  if (!c || !du) return fail(W3D_ERR_ARG, "w3d_recover_blocks: null argument");
+  if (first_blk < 1 || nblk < 1 || first_blk - 1 + nblk > c->nelblk)
+    return fail(W3D_ERR_ARG, "w3d_recover_blocks: block range");
+  size_t nb = 3 * (size_t)c->nonode * sizeof(double);
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemcpyAsync(c->d_du, du, nb, hipMemcpyHostToDevice, c->stream));
+  return recover_impl(c, step, iter, extrapolated, c->d_du, nullptr, nullptr,
+                      nullptr, first_blk, nblk);
+}
+
+int w3d_tangent_blocks(w3d_ctx* c, int32_t first_blk, int32_t nblk,
+                       int32_t step, int32_t iter, int32_t precision) {
+  (void)step;
+  (void)iter;
+  if (!c) return fail(W3D_ERR_ARG, "w3d_tangent: null context");
+  if (precision != W3D_FP64 && precision != W3D_FP32)
+    return fail(W3D_ERR_ARG, "w3d_tangent: precision");
+  if (first_blk < 1 || nblk < 1 || first_blk - 1 + nblk > c->nelblk)
+    return fail(W3D_ERR_ARG, "w3d_tangent_blocks: block range");
+  HIPCHK(hipSetDevice(c->device));
+  DevModel M = c->dev();
+  int rc;
+  if (first_blk == 1 && nblk == c->nelblk) {
+    prof_begin(c, W3D_PHASE_TANGENT);
+    rc = launch_tangent(M, c->blocks, c->d_work_tan, c->tan_groups,
+                        c->tan_start, c->tan_count, precision, c->stream);
+    prof_end(c, W3D_PHASE_TANGENT);
+  } else {
+    std::vector<WorkItem> items;
+    std::vector<int> g, s, n;
+    build_work(c, true, items, g, s, n);
+    std::vector<WorkItem> sel;
+    std::vector<int> sg, ss, sn;
+    for (size_t k = 0; k < g.size(); ++k) {
+      int st = (int)sel.size();
+      for (int i = s[k]; i < s[k] + n[k]; ++i)
+        if (items[i].blk >= first_blk - 1 && items[i].blk < first_blk - 1 + nblk)
+          sel.push_back(items[i]);
+      if ((int)sel.size() > st) {
+        sg.push_back(g[k]);
+        ss.push_back(st);
+        sn.push_back((int)sel.size() - st);
+      }
+    }
+    WorkItem* d_sel = nullptr;
+    if ((rc = upload(&d_sel, sel))) return rc;
+    rc = launch_tangent(M, c->blocks, d_sel, sg, ss, sn, precision,
+                        c->stream);
+    hipStreamSynchronize(c->stream);
+    dfree(d_sel);
+  }
+  if (rc) return fail(rc, "w3d_tangent: kernel launch failed");
+  c->tangent_precision = precision;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  prof_collect(c);
+  return W3D_OK;
+}
+
+int w3d_tangent(w3d_ctx* c, int32_t step, int32_t iter, int32_t precision) {
+  if (!c) return fail(W3D_ERR_ARG, "w3d_tangent: null context");
+  return w3d_tangent_blocks(c, 1, c->nelblk, step, iter, precision);
+}
+
+int w3d_pattern(w3d_ctx* c, const int32_t* cstmap, int32_t* neqns_out,
+                int64_t* ncoeff_out) {
+  if (!c || !cstmap) return fail(W3D_ERR_ARG, "w3d_pattern: null argument");
+  const int nd = 3 * c->nonode;
+  // dof_map (assemble_code.f:561)
+  c->dof_eqn_map.assign(nd, 0);
+  c->eqn_node_map.assign(nd, 0);
+  int neq = 0;
+  for (int d = 1; d <= nd; ++d) {
+    if (cstmap[d - 1] != 0) continue;
+    neq = neq + 1;
+    c->dof_eqn_map[d - 1] = neq;
+    c->eqn_node_map[neq - 1] = (d - 1) / 3 + 1;
+  }
+  c->eqn_node_map.resize(neq);
+  c->neqns = neq;
+  // rows of the upper triangle (build_col_sparse_symm_srow): columns of the
+  // row's node's elements right of the diagonal, sorted, unique
+  std::vector<std::vector<int32_t>> rows(neq > 0 ? neq : 1);
+  c->k_ptrs.assign(neq, 0);
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int srow = 1; srow <= neq - 1; ++srow) {
+    int snode = c->eqn_node_map[srow - 1] - 1;
+    std::vector<int32_t>& list = rows[srow - 1];
+    for (int64_t p = c->inv_ptr[snode]; p < c->inv_ptr[snode + 1]; ++p) {
+      int e = c->inv_elem[p];
+      const BlockDesc& B = c->blocks[c->elem_blk[e]];
+      const int32_t* nd_ = &c->conn[B.conn_off + (int64_t)(e - (B.felem - 1)) * B.nnode];
+      for (int ecol = 0; ecol < B.totdof; ++ecol) {
+        int cmp = ecol / B.nnode, j = ecol % B.nnode;
+        int scol = c->dof_eqn_map[3 * nd_[j] + cmp];
+        if (scol <= srow) continue;
+        list.push_back(scol);
+      }
+    }
+    std::sort(list.begin(), list.end());
+    list.erase(std::unique(list.begin(), list.end()), list.end());
+    c->k_ptrs[srow - 1] = (int32_t)list.size();
+  }
+  c->row_start.assign(neq + 1, 0);
+  for (int r = 0; r < neq; ++r) c->row_start[r + 1] = c->row_start[r] + c->k_ptrs[r];
+  c->ncoeff = c->row_start[neq];
+  c->k_indexes.resize(c->ncoeff);
+  for (int r = 0; r < neq; ++r)
+    if (!rows[r].empty())
+      std::memcpy(&c->k_indexes[c->row_start[r]], rows[r].data(),
+                  rows[r].size() * sizeof(int32_t));
+  rows.clear();
+  // assembly map: output t (diagonal of row r at t = r, coefficient k of
+  // the upper triangle at t = neqns + k) sums the Ke words listed in
+  // src[ptr[t] .. ptr[t+1]) in the order of the elements on the row's node
+  // (assem_a_row).
+  const int64_t nout = (int64_t)neq + c->ncoeff;
+  std::vector<int64_t> cnt(nout + 1, 0);
+  auto for_row = [&](int srow, auto&& emit) {
+    int snode = c->eqn_node_map[srow - 1] - 1;
+    const int32_t* kidx = c->k_indexes.data() + c->row_start[srow - 1];
+    int nk = c->k_ptrs[srow - 1];
+    for (int64_t p = c->inv_ptr[snode]; p < c->inv_ptr[snode + 1]; ++p) {
+      int e = c->inv_elem[p];
+      const BlockDesc& B = c->blocks[c->elem_blk[e]];
+      int i = e - (B.felem - 1);
+      const int32_t* nd_ = &c->conn[B.conn_off + (int64_t)i * B.nnode];
+      int erow = -1;
+      for (int k = 0; k < B.totdof && erow < 0; ++k)
+        if (c->dof_eqn_map[3 * nd_[k % B.nnode] + k / B.nnode] == srow) erow = k;
+      for (int ecol = 0; ecol < B.totdof; ++ecol) {
+        int scol = c->dof_eqn_map[3 * nd_[ecol % B.nnode] + ecol / B.nnode];
+        if (scol < srow) continue;
+        int mx = std::max(ecol, erow), mn = std::min(ecol, erow);
+        int64_t src = B.ek_off + (int64_t)i * B.utsz +
+                      (int64_t)mx * (mx + 1) / 2 + mn;
+        int64_t t;
+        if (scol == srow) {
+          t = srow - 1;
+        } else {
+          const int32_t* f = std::lower_bound(kidx, kidx + nk, scol);
+          t = neq + c->row_start[srow - 1] + (f - kidx);
+        }
+        emit(t, src);
+      }
+    }
+  };
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int srow = 1; srow <= neq; ++srow)
+    for_row(srow, [&](int64_t t, int64_t) { cnt[t + 1] += 1; });
+  for (int64_t t = 0; t < nout; ++t) cnt[t + 1] += cnt[t];
+  std::vector<uint32_t> src(cnt[nout]);
+  {
+    std::vector<int64_t> cur(cnt.begin(), cnt.end() - 1);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int srow = 1; srow <= neq; ++srow)
+      for_row(srow, [&](int64_t t, int64_t s) { src[cur[t]++] = (uint32_t)s; });
+  }
+  HIPCHK(hipSetDevice(c->device));
+  dfree(c->d_asm_ptr);
+  dfree(c->d_asm_src);
+  dfree(c->d_kout);
+  int rc;
+  if ((rc = upload(&c->d_asm_ptr, cnt))) return rc;
+  if ((rc = upload(&c->d_asm_src, src))) return rc;
+  if ((rc = dalloc(&c->d_kout, nout))) return rc;
+  if (neqns_out) *neqns_out = neq;
+  if (ncoeff_out) *ncoeff_out = c->ncoeff;
+  return W3D_OK;
+}
+
+int w3d_get_pattern(const w3d_ctx* c, int32_t* dof_eqn_map,
+                    int32_t* eqn_node_map, int32_t* k_ptrs,
+                    int32_t* k_indexes) {
+  if (!c || c->dof_eqn_map.empty())
+    return fail(W3D_ERR_ARG, "w3d_get_pattern: no pattern");
+  if (dof_eqn_map)
+    std::memcpy(dof_eqn_map, c->dof_eqn_map.data(), 12 * (size_t)c->nonode);
+  if (eqn_node_map)
+    std::memcpy(eqn_node_map, c->eqn_node_map.data(), 4 * (size_t)c->neqns);
+  if (k_ptrs) std::memcpy(k_ptrs, c->k_ptrs.data(), 4 * (size_t)c->neqns);
+  if (k_indexes)
+    std::memcpy(k_indexes, c->k_indexes.data(), 4 * (size_t)c->ncoeff);
+  return W3D_OK;
+}
+
+int w3d_assemble_vss_dev(w3d_ctx* c, double* d_k_diag, double* d_k_coeffs) {
+  if (!c || !c->d_asm_ptr) return fail(W3D_ERR_ARG, "w3d_assemble: no pattern");
+  if (c->tangent_precision < 0)
+    return fail(W3D_ERR_ARG, "w3d_assemble: no tangent computed");
+  HIPCHK(hipSetDevice(c->device));
+  int64_t nout = (int64_t)c->neqns + c->ncoeff;
+  prof_begin(c, W3D_PHASE_ASSEMBLE);
+  int rc = launch_assemble(c->d_ek64, c->d_ek32,
+                           c->tangent_precision == W3D_FP32, c->d_asm_ptr,
+                           c->d_asm_src, nout, c->d_kout, c->stream);
+  prof_end(c, W3D_PHASE_ASSEMBLE);
+  if (rc) return fail(rc, "w3d_assemble: kernel launch failed");
+  if (d_k_diag)
+    HIPCHK(hipMemcpyAsync(d_k_diag, c->d_kout, c->neqns * sizeof(double),
+                          hipMemcpyDeviceToDevice, c->stream));
+  if (d_k_coeffs && c->ncoeff)
+    HIPCHK(hipMemcpyAsync(d_k_coeffs, c->d_kout + c->neqns,
+                          c->ncoeff * sizeof(double), hipMemcpyDeviceToDevice,
+                          c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  prof_collect(c);
+  return W3D_OK;
+}
+
+int w3d_assemble_vss(w3d_ctx* c, double* k_diag, double* k_coeffs) {
+  if (!c || !c->d_asm_ptr) return fail(W3D_ERR_ARG, "w3d_assemble: no pattern");
+  if (c->tangent_precision < 0)
+    return fail(W3D_ERR_ARG, "w3d_assemble: no tangent computed");
+  HIPCHK(hipSetDevice(c->device));
+  int64_t nout = (int64_t)c->neqns + c->ncoeff;
+  prof_begin(c, W3D_PHASE_ASSEMBLE);
+  int rc = launch_assemble(c->d_ek64, c->d_ek32,
+                           c->tangent_precision == W3D_FP32, c->d_asm_ptr,
+                           c->d_asm_src, nout, c->d_kout, c->stream);
+  prof_end(c, W3D_PHASE_ASSEMBLE);
+  if (rc) return fail(rc, "w3d_assemble: kernel launch failed");
+  if (k_diag)
+    HIPCHK(hipMemcpyAsync(k_diag, c->d_kout, c->neqns * sizeof(double),
+                          hipMemcpyDeviceToHost, c->stream));
+  if (k_coeffs && c->ncoeff)
+    HIPCHK(hipMemcpyAsync(k_coeffs, c->d_kout + c->neqns,
+                          c->ncoeff * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  prof_collect(c);
+  return W3D_OK;
+}
+
+int w3d_commit_step(w3d_ctx* c) {
+  if (!c) return fail(W3D_ERR_ARG, "null context");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemcpyAsync(c->d_stress_n, c->d_stress_n1, 9 * c->nq * 8,
+                        hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_strain_n, c->d_strain_n1, 6 * c->nq * 8,
+                        hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_hist_n, c->d_hist_n1, c->nhist * 8,
+                        hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return W3D_OK;
+}
+
+int w3d_rollback(w3d_ctx* c) {
+  if (!c) return fail(W3D_ERR_ARG, "null context");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemcpyAsync(c->d_stress_n1, c->d_stress_n, 9 * c->nq * 8,
+                        hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_strain_n1, c->d_strain_n, 6 * c->nq * 8,
+                        hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_hist_n1, c->d_hist_n, c->nhist * 8,
+                        hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return W3D_OK;
+}
+
+static int state_size(const BlockDesc& B, int which, int64_t* n) {
+  int64_t nq = (int64_t)B.span * B.ngp;
+  switch (which) {
+    case W3D_STRESS_N: case W3D_STRESS_N1: *n = 9 * nq; return 9;
+    case W3D_STRAIN_N: case W3D_STRAIN_N1: *n = 6 * nq; return 6;
+    case W3D_HIST_N: case W3D_HIST_N1: *n = B.hist * nq; return B.hist;
+    case W3D_CEP: *n = 21 * nq; return 21;
+    case W3D_ESTIFF: *n = (int64_t)B.utsz * B.span; return 0;
+    case W3D_EINFVEC: *n = (int64_t)B.totdof * B.span; return 0;
+    case W3D_ELEM_VOL: *n = B.span; return 0;
+    default: *n = 0; return -1;
+  }
+}
+
+int w3d_get_block_state(const w3d_ctx* c, int32_t blk, int32_t which,
+                        double* out, int64_t n_out) {
+  if (!c || !out || blk < 1 || blk > c->nelblk)
+    return fail(W3D_ERR_ARG, "w3d_get_block_state: bad block");
+  const BlockDesc& B = c->blocks[blk - 1];
+  int64_t n;
+  int nw = state_size(B, which, &n);
+  if (nw < 0 || n_out < n)
+    return fail(W3D_ERR_ARG, "w3d_get_block_state: selector or size");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const int64_t S = B.span, G = B.ngp, SG = S * G;
+  std::vector<double> tmp(n);
+  const double* src = nullptr;
+  switch (which) {
+    case W3D_STRESS_N: src = c->d_stress_n + 9 * B.q_off; break;
+    case W3D_STRESS_N1: src = c->d_stress_n1 + 9 * B.q_off; break;
+    case W3D_STRAIN_N: src = c->d_strain_n + 6 * B.q_off; break;
+    case W3D_STRAIN_N1: src = c->d_strain_n1 + 6 * B.q_off; break;
+    case W3D_HIST_N: src = c->d_hist_n + B.hist_off; break;
+    case W3D_HIST_N1: src = c->d_hist_n1 + B.hist_off; break;
+    case W3D_CEP: src = c->d_cep + 21 * B.q_off; break;
+    case W3D_EINFVEC: src = c->d_einfvec + B.ifv_off; break;
+    case W3D_ELEM_VOL: src = c->d_evol + (B.felem - 1); break;
+    default: break;
+  }
+  if (which == W3D_ESTIFF) {
+    if (c->tangent_precision == W3D_FP32) {
+      std::vector<float> f(n);
+      HIPCHK(hipMemcpy(f.data(), c->d_ek32 + B.ek_off, n * 4,
+                       hipMemcpyDeviceToHost));
+      for (int64_t k = 0; k < n; ++k) out[k] = f[k];
+    } else {
+      HIPCHK(hipMemcpy(out, c->d_ek64 + B.ek_off, n * 8,
+                       hipMemcpyDeviceToHost));
+    }
+    return W3D_OK;
+  }
+  HIPCHK(hipMemcpy(tmp.data(), src, n * 8, hipMemcpyDeviceToHost));
+  if (which == W3D_ELEM_VOL) {
+    std::memcpy(out, tmp.data(), n * 8);
+  } else if (which == W3D_EINFVEC) {  // device (i, dof) -> (span, totdof)
+    for (int64_t i = 0; i < S; ++i)
+      for (int64_t j = 0; j < B.totdof; ++j)
+        out[j * S + i] = tmp[i * B.totdof + j];
+  } else if (which == W3D_CEP) {  // device (k, i, g) -> (21, span, ngp)
+    for (int64_t k = 0; k < 21; ++k)
+      for (int64_t i = 0; i < S; ++i)
+        for (int64_t g = 0; g < G; ++g)
+          out[(g * S + i) * 21 + k] = tmp[k * SG + i * G + g];
+  } else {  // device (k, i, g) -> (nw, ngp, span)
+    for (int64_t k = 0; k < nw; ++k)
+      for (int64_t i = 0; i < S; ++i)
+        for (int64_t g = 0; g < G; ++g)
+          out[(i * G + g) * nw + k] = tmp[k * SG + i * G + g];
+  }
+  return W3D_OK;
+}
+
+int w3d_set_block_state(w3d_ctx* c, int32_t blk, int32_t which,
+                        const double* in, int64_t n_in) {
+  if (!c || !in || blk < 1 || blk > c->nelblk)
+    return fail(W3D_ERR_ARG, "w3d_set_block_state: bad block");
+  if (which != W3D_STRESS_N && which != W3D_STRAIN_N && which != W3D_HIST_N)
+    return fail(W3D_ERR_ARG, "w3d_set_block_state: selector");
+  const BlockDesc& B = c->blocks[blk - 1];
+  int64_t n;
+  int nw = state_size(B, which, &n);
+  if (n_in < n) return fail(W3D_ERR_ARG, "w3d_set_block_state: size");
+  const int64_t S = B.span, G = B.ngp, SG = S * G;
+  std::vector<double> tmp(n);
+  for (int64_t k = 0; k < nw; ++k)
+    for (int64_t i = 0; i < S; ++i)
+      for (int64_t g = 0; g < G; ++g)
+        tmp[k * SG + i * G + g] = in[(i * G + g) * nw + k];
+  double* dst = which == W3D_STRESS_N   ? c->d_stress_n + 9 * B.q_off
+                : which == W3D_STRAIN_N ? c->d_strain_n + 6 * B.q_off
+                                        : c->d_hist_n + B.hist_off;
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(dst, tmp.data(), n * 8, hipMemcpyHostToDevice));
+  return W3D_OK;
+}
+
+int w3d_set_profiling(w3d_ctx* c, int32_t on) {
+  if (!c) return fail(W3D_ERR_ARG, "null context");
+  HIPCHK(hipSetDevice(c->device));
+  if (on && !c->ev_a[0])
+    for (int ph = 0; ph < W3D_NUM_PHASES; ++ph) {
+      HIPCHK(hipEventCreate(&c->ev_a[ph]));
+      HIPCHK(hipEventCreate(&c->ev_b[ph]));
+    }
+  c->profiling = on != 0;
+  return W3D_OK;
+}
+
+int w3d_get_phase_time(const w3d_ctx* c, int32_t phase, int64_t* launches,
+                       double* total_ms) {
+  if (!c || phase < 0 || phase >= W3D_NUM_PHASES)
+    return fail(W3D_ERR_ARG, "w3d_get_phase_time: phase");
+  if (launches) *launches = c->prof_n[phase];
+  if (total_ms) *total_ms = c->prof_ms[phase];
+  return W3D_OK;
+}
+
+int w3d_reset_phase_times(w3d_ctx* c) {
+  if (!c) return fail(W3D_ERR_ARG, "null context");
+  for (int ph = 0; ph < W3D_NUM_PHASES; ++ph) {
+    c->prof_n[ph] = 0;
+    c->prof_ms[ph] = 0.0;
+  }
+  return W3D_OK;
+}
+
+int w3d_synchronize(w3d_ctx* c) {
+  if (!c) return fail(W3D_ERR_ARG, "null context");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return W3D_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,185 @@
+// w3d_elements.cpp -- host-side Gauss rules and parametric shape-function
+// derivatives; they fill the GaussTable that the kernels stage into LDS.
+//
+// Rules and node numbering (file:line under the reference's src/):
+//   hex20 (q3disop) quad1 getgpts.f:177 / deriv1 derivs.f:182
+//   hex8  (l3disop) quad2 getgpts.f:642 / deriv2 derivs.f:283
+//   tet10           quad6 getgpts.f:771 / deriv6 derivs.f:606
+// The constants are the reference's, including its truncated abscissae.
+#include <cstring>
+
+#include "w3d_internal.h"
+
+namespace w3d {
+
+int ngp_of(int etype, int order) {
+  if (etype == W3D_L3DISOP) return order == 1 ? 8 : 0;
+  if (etype == W3D_Q3DISOP)
+    return order == 1 ? 27 : order == 2 ? 9 : order == 8 ? 8 : order == 9 ? 14
+                                                                          : 0;
+  if (etype == W3D_TET10)
+    return (order == 1 || order == 4 || order == 5) ? order : 0;
+  return 0;
+}
+
+int nnode_of(int etype) {
+  return etype == W3D_Q3DISOP ? 20 : etype == W3D_L3DISOP ? 8
+                                   : etype == W3D_TET10   ? 10
+                                                          : 0;
+}
+
+static const int kCube[8][3] = {{-1, -1, -1}, {-1, 1, -1}, {1, -1, -1},
+                                {1, 1, -1},   {-1, -1, 1}, {-1, 1, 1},
+                                {1, -1, 1},   {1, 1, 1}};
+
+static void point(int etype, int order, int g, double p[3], double* w) {
+  if (etype == W3D_L3DISOP) {
+    const double l = 0.5773502691896257645;
+    for (int k = 0; k < 3; ++k) p[k] = kCube[g][k] * l;
+    *w = 1.0;
+    return;
+  }
+  if (etype == W3D_Q3DISOP) {
+    if (order == 1) {  // 3x3x3: eta fastest, then xi, then zeta
+      const double l1 = 0.774596669241483;
+      const double lv[3] = {-l1, 0.0, l1};
+      const double wv[4] = {0.1714677640603571, 0.2743484224965712,
+                            0.4389574759945135, 0.7023319615912210};
+      int ie = g % 3, ix = (g / 3) % 3, iz = g / 9;
+      p[0] = lv[ix];
+      p[1] = lv[ie];
+      p[2] = lv[iz];
+      *w = wv[(ix == 1) + (ie == 1) + (iz == 1)];
+    } else if (order == 8) {
+      const double l = 0.577350269189626;
+      for (int k = 0; k < 3; ++k) p[k] = kCube[g][k] * l;
+      *w = 1.0;
+    } else if (order == 2) {
+      const double l = 0.774596669241483;
+      if (g == 8) {
+        p[0] = p[1] = p[2] = 0.0;
+        *w = 3.555555555555556;
+      } else {
+        for (int k = 0; k < 3; ++k) p[k] = kCube[g][k] * l;
+        *w = 0.555555555555556;
+      }
+    } else {  // order 9, 14 points
+      static const int s[14][3] = {
+          {-1, 0, 0},  {1, 0, 0},   {0, -1, 0}, {0, 1, 0},  {0, 0, -1},
+          {0, 0, 1},   {-1, -1, 1}, {-1, -1, -1}, {-1, 1, -1}, {-1, 1, 1},
+          {1, -1, 1},  {1, -1, -1}, {1, 1, -1}, {1, 1, 1}};
+      const double l = g < 6 ? 0.795822426 : 0.758786911;
+      for (int k = 0; k < 3; ++k) p[k] = s[g][k] == 0 ? 0.0 : s[g][k] * l;
+      *w = g < 6 ? 0.886426593 : 0.335180055;
+    }
+    return;
+  }
+  // tet10: volume coordinates (s2, s3, s4)
+  if (order == 1) {
+    p[0] = p[1] = p[2] = 0.25;
+    *w = 1.0;
+    return;
+  }
+  double a, b;
+  int pt = g + 1;
+  if (order == 4) {
+    a = 0.58541020;
+    b = 0.13819660;
+    *w = 0.25;
+  } else {
+    if (pt == 1) {
+      p[0] = p[1] = p[2] = 0.25;
+      *w = -4.0 / 5.0;
+      return;
+    }
+    a = 1.0 / 2.0;
+    b = 1.0 / 6.0;
+    *w = 9.0 / 20.0;
+    pt -= 1;
+  }
+  p[0] = pt == 2 ? a : b;
+  p[1] = pt == 3 ? a : b;
+  p[2] = pt == 4 ? a : b;
+}
+
+static void derivs(int etype, const double p[3], double* nx, double* ny,
+                   double* nz) {
+  if (etype == W3D_L3DISOP) {
+    // 8-node brick: node natural coordinates (derivs.f:283 numbering)
+    static const int nat[8][3] = {{-1, -1, 1}, {-1, -1, -1}, {-1, 1, -1},
+                                  {-1, 1, 1},  {1, -1, 1},   {1, -1, -1},
+                                  {1, 1, -1},  {1, 1, 1}};
+    for (int j = 0; j < 8; ++j) {
+      double a = 1.0 + nat[j][0] * p[0], b = 1.0 + nat[j][1] * p[1],
+             c = 1.0 + nat[j][2] * p[2];
+      nx[j] = nat[j][0] * b * c * 0.125;
+      ny[j] = nat[j][1] * a * c * 0.125;
+      nz[j] = nat[j][2] * a * b * 0.125;
+    }
+    return;
+  }
+  if (etype == W3D_Q3DISOP) {
+    // 20-node serendipity brick, derivs.f:182 numbering
+    static const int nat[20][3] = {
+        {-1, -1, 1}, {-1, -1, -1}, {-1, 1, -1}, {-1, 1, 1}, {1, -1, 1},
+        {1, -1, -1}, {1, 1, -1},   {1, 1, 1},   {-1, -1, 0}, {-1, 0, -1},
+        {-1, 1, 0},  {-1, 0, 1},   {1, -1, 0},  {1, 0, -1}, {1, 1, 0},
+        {1, 0, 1},   {0, -1, 1},   {0, -1, -1}, {0, 1, -1}, {0, 1, 1}};
+    const double r = p[0], s = p[1], t = p[2];
+    for (int j = 0; j < 20; ++j) {
+      int ri = nat[j][0], si = nat[j][1], ti = nat[j][2];
+      if (ri != 0 && si != 0 && ti != 0) {
+        double a = 1.0 + ri * r, b = 1.0 + si * s, c = 1.0 + ti * t;
+        double q = ri * r + si * s + ti * t - 2.0;
+        nx[j] = 0.125 * ri * b * c * (q + a);
+        ny[j] = 0.125 * si * a * c * (q + b);
+        nz[j] = 0.125 * ti * a * b * (q + c);
+      } else if (ri == 0) {
+        double b = 1.0 + si * s, c = 1.0 + ti * t;
+        nx[j] = -0.5 * r * b * c;
+        ny[j] = 0.25 * si * (1.0 - r * r) * c;
+        nz[j] = 0.25 * ti * (1.0 - r * r) * b;
+      } else if (si == 0) {
+        double a = 1.0 + ri * r, c = 1.0 + ti * t;
+        nx[j] = 0.25 * ri * (1.0 - s * s) * c;
+        ny[j] = -0.5 * s * a * c;
+        nz[j] = 0.25 * ti * (1.0 - s * s) * a;
+      } else {
+        double a = 1.0 + ri * r, b = 1.0 + si * s;
+        nx[j] = 0.25 * ri * (1.0 - t * t) * b;
+        ny[j] = 0.25 * si * (1.0 - t * t) * a;
+        nz[j] = -0.5 * t * a * b;
+      }
+    }
+    return;
+  }
+  // tet10 in (s2, s3, s4) with s1 = 1 - s2 - s3 - s4
+  const double s2 = p[0], s3 = p[1], s4 = p[2], s1 = 1.0 - s2 - s3 - s4;
+  const double d2[10] = {1 - 4 * s1, 4 * s2 - 1, 0, 0, 4 * (s1 - s2),
+                         4 * s3,     -4 * s3,    -4 * s4, 4 * s4, 0};
+  const double d3[10] = {1 - 4 * s1, 0, 4 * s3 - 1, 0, -4 * s2,
+                         4 * s2,     4 * (s1 - s3), -4 * s4, 0, 4 * s4};
+  const double d4[10] = {1 - 4 * s1, 0, 0, 4 * s4 - 1, -4 * s2,
+                         0,          -4 * s3, 4 * (s1 - s4), 4 * s2, 4 * s3};
+  std::memcpy(nx, d2, sizeof d2);
+  std::memcpy(ny, d3, sizeof d3);
+  std::memcpy(nz, d4, sizeof d4);
+}
+
+bool fill_gauss_table(int etype, int order, GaussTable* t) {
+  std::memset(t, 0, sizeof(*t));
+  int g = ngp_of(etype, order), n = nnode_of(etype);
+  if (g == 0 || n == 0) return false;
+  t->etype = etype;
+  t->order = order;
+  t->ngp = g;
+  t->nnode = n;
+  for (int k = 0; k < g; ++k) {
+    double p[3];
+    point(etype, order, k, p, &t->w[k]);
+    derivs(etype, p, t->dn[0][k], t->dn[1][k], t->dn[2][k]);
+  }
+  return true;
+}
+
+}  // namespace w3d

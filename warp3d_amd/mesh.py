@@ -1,0 +1,189 @@
+"""Synthetic structured meshes in WARP3D's element conventions.
+
+Node numbering inside each element follows the reference's shape functions:
+  * 8-node brick (l3disop, derivs.f:283): node k at natural coordinates
+    (-,-,+) (-,-,-) (-,+,-) (-,+,+) (+,-,+) (+,-,-) (+,+,-) (+,+,+);
+  * 20-node brick (q3disop, derivs.f:182): the same corners, then mid-edge
+    nodes 9..20 at (-,-,0) (-,0,-) (-,+,0) (-,0,+) (+,-,0) (+,0,-) (+,+,0)
+    (+,0,+) (0,-,+) (0,-,-) (0,+,-) (0,+,+);
+  * 10-node tetrahedron (tet10, derivs.f:606): corners 1..4, then mid-edge
+    nodes on edges 1-2, 2-3, 1-3, 1-4, 2-4, 3-4.
+The natural axes map onto +x, +y, +z so every element has det J > 0.
+"""
+import numpy as np
+
+from ._abi import (W3D_L3DISOP, W3D_Q3DISOP, W3D_TET10, ModelArrays)
+
+HEX8_NAT = np.array([(-1, -1, 1), (-1, -1, -1), (-1, 1, -1), (-1, 1, 1),
+                     (1, -1, 1), (1, -1, -1), (1, 1, -1), (1, 1, 1)])
+HEX20_NAT = np.vstack([HEX8_NAT, np.array([
+    (-1, -1, 0), (-1, 0, -1), (-1, 1, 0), (-1, 0, 1),
+    (1, -1, 0), (1, 0, -1), (1, 1, 0), (1, 0, 1),
+    (0, -1, 1), (0, -1, -1), (0, 1, -1), (0, 1, 1)])])
+TET10_EDGES = [(0, 1), (1, 2), (0, 2), (0, 3), (1, 3), (2, 3)]
+
+
+class Mesh:
+    """coords (nonode, 3); conn (noelem, nn) 1-based; etype per element."""
+
+    def __init__(self, coords, conn, etype, lattice=None, h=None):
+        self.coords = np.ascontiguousarray(coords, dtype=np.float64)
+        self.conn = np.ascontiguousarray(conn, dtype=np.int32)
+        self.etype = etype
+        self.lattice = lattice  # integer lattice position of every node
+        self.h = h
+
+    @property
+    def nonode(self):
+        return self.coords.shape[0]
+
+    @property
+    def noelem(self):
+        return self.conn.shape[0]
+
+    def model(self, material, int_order=None, bbar=None, elblks=None,
+              auto_size=128, eps_bbar=0.0, angles=None):
+        """ModelArrays for this mesh with one material for all elements."""
+        ne, nn = self.conn.shape
+        if int_order is None:
+            int_order = {W3D_L3DISOP: 1, W3D_Q3DISOP: 8, W3D_TET10: 4}[
+                self.etype]
+        if bbar is None:
+            bbar = 1 if self.etype == W3D_L3DISOP else 0
+        mats = material if isinstance(material, (list, tuple)) else [material]
+        return ModelArrays(
+            self.coords, self.conn.ravel(),
+            np.arange(ne, dtype=np.int32) * nn + 1,
+            np.full(ne, self.etype, np.int32),
+            np.full(ne, int_order, np.int32),
+            np.ones(ne, np.int32), np.full(ne, bbar, np.int32), mats,
+            elblks=elblks, auto_size=auto_size, eps_bbar=eps_bbar,
+            angles=angles)
+
+
+def _jitter(coords, lattice, top, h, frac, seed):
+    """Move nodes strictly inside the box by U(-frac, frac) * h per axis."""
+    if frac <= 0:
+        return coords
+    rng = np.random.default_rng(seed)
+    inner = np.all((lattice > 0) & (lattice < top), axis=1)
+    d = rng.uniform(-frac, frac, size=coords.shape) * h
+    out = coords.copy()
+    out[inner] += d[inner]
+    return out
+
+
+def box_hex8(nx, ny=None, nz=None, size=(1.0, 1.0, 1.0), jitter=0.0,
+             seed=0):
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    n = np.array([nx, ny, nz])
+    h = np.asarray(size, float) / n
+    i, j, k = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1),
+                          np.arange(nz + 1), indexing="ij")
+    lat = np.stack([i.ravel(order="F"), j.ravel(order="F"),
+                    k.ravel(order="F")], axis=1)
+    coords = lat * h
+    coords = _jitter(coords, lat, n, h, jitter, seed)
+
+    def nid(a, b, c):
+        return 1 + a + (nx + 1) * (b + (ny + 1) * c)
+
+    ex, ey, ez = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz),
+                             indexing="ij")
+    ex, ey, ez = (v.ravel(order="F") for v in (ex, ey, ez))
+    conn = np.empty((ex.size, 8), np.int32)
+    for k_, (r, s, t) in enumerate(HEX8_NAT):
+        conn[:, k_] = nid(ex + (r + 1) // 2, ey + (s + 1) // 2,
+                          ez + (t + 1) // 2)
+    return Mesh(coords, conn, W3D_L3DISOP, lattice=lat, h=h)
+
+
+def box_hex20(nx, ny=None, nz=None, size=(1.0, 1.0, 1.0), jitter=0.0,
+              seed=0):
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    n = np.array([nx, ny, nz])
+    h = np.asarray(size, float) / n
+    # half lattice; keep corners and mid-edge points (at most one odd index)
+    i, j, k = np.meshgrid(np.arange(2 * nx + 1), np.arange(2 * ny + 1),
+                          np.arange(2 * nz + 1), indexing="ij")
+    lat2 = np.stack([i.ravel(order="F"), j.ravel(order="F"),
+                     k.ravel(order="F")], axis=1)
+    keep = (lat2 % 2).sum(axis=1) <= 1
+    ids = np.full(lat2.shape[0], -1, np.int64)
+    ids[keep] = np.arange(keep.sum())
+    lat2 = lat2[keep]
+    coords = lat2 * (h / 2)
+    coords = _jitter(coords, lat2, 2 * n, h / 2, jitter, seed)
+
+    def nid(a, b, c):
+        return 1 + ids[a + (2 * nx + 1) * (b + (2 * ny + 1) * c)]
+
+    ex, ey, ez = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz),
+                             indexing="ij")
+    ex, ey, ez = (v.ravel(order="F") for v in (ex, ey, ez))
+    conn = np.empty((ex.size, 20), np.int32)
+    for k_, (r, s, t) in enumerate(HEX20_NAT):
+        conn[:, k_] = nid(2 * ex + r + 1, 2 * ey + s + 1, 2 * ez + t + 1)
+    return Mesh(coords, conn, W3D_Q3DISOP, lattice=lat2, h=h / 2)
+
+
+def box_tet10(nx, ny=None, nz=None, size=(1.0, 1.0, 1.0), jitter=0.0,
+              seed=0):
+    """Each cube cell split into the 6 Kuhn tetrahedra (conforming)."""
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    n = np.array([nx, ny, nz])
+    h = np.asarray(size, float) / n
+    i, j, k = np.meshgrid(np.arange(2 * nx + 1), np.arange(2 * ny + 1),
+                          np.arange(2 * nz + 1), indexing="ij")
+    lat2 = np.stack([i.ravel(order="F"), j.ravel(order="F"),
+                     k.ravel(order="F")], axis=1)
+    coords = lat2 * (h / 2)
+    coords = _jitter(coords, lat2, 2 * n, h / 2, jitter, seed)
+
+    def nid(p):
+        return 1 + p[..., 0] + (2 * nx + 1) * (p[..., 1] + (2 * ny + 1) *
+                                               p[..., 2])
+
+    perms = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1),
+             (2, 1, 0)]
+    ex, ey, ez = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz),
+                             indexing="ij")
+    base = 2 * np.stack([ex.ravel(order="F"), ey.ravel(order="F"),
+                         ez.ravel(order="F")], axis=1)
+    conns = []
+    for p in perms:
+        v = [np.zeros(3, int)]
+        for ax in p:
+            e = np.zeros(3, int)
+            e[ax] = 2
+            v.append(v[-1] + e)
+        v = np.array(v)
+        d = np.linalg.det(np.array([v[1] - v[0], v[2] - v[0], v[3] - v[0]],
+                                   float))
+        if d < 0:
+            v[[1, 2]] = v[[2, 1]]
+        corners = base[:, None, :] + v[None, :, :]
+        mids = [(corners[:, a] + corners[:, b]) // 2 for a, b in TET10_EDGES]
+        allp = np.concatenate([corners, np.stack(mids, axis=1)], axis=1)
+        conns.append(nid(allp))
+    conn = np.stack(conns, axis=1).reshape(-1, 10).astype(np.int32)
+    return Mesh(coords, conn, W3D_TET10, lattice=lat2, h=h / 2)
+
+
+def uniaxial_bc(mesh, strain, size=1.0):
+    """Symmetry planes x=0, y=0, z=0 and an imposed z-displacement
+    strain*size on z=size.  Returns (cstmap, u_top_dofs mask, value)."""
+    nd = 3 * mesh.nonode
+    cst = np.zeros(nd, np.int32)
+    x = mesh.coords
+    tol = 1e-9 * size
+    cst[0::3][np.abs(x[:, 0]) < tol] = 1
+    cst[1::3][np.abs(x[:, 1]) < tol] = 1
+    cst[2::3][np.abs(x[:, 2]) < tol] = 1
+    top = np.zeros(nd, bool)
+    top[2::3] = np.abs(x[:, 2] - size) < tol
+    cst[top] = 1
+    return cst, top, strain * size
