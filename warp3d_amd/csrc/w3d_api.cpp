@@ -95,15 +95,14 @@ struct w3d_ctx {
   double* d_cep = nullptr;
   double* d_ek64 = nullptr;
   float* d_ek32 = nullptr;
-  double *d_einfvec = nullptr, *d_evol = nullptr, *d_energy_q = nullptr;
+  double *d_einfvec = nullptr, *d_evol = nullptr, *d_en_wg = nullptr;
   int32_t* d_flags = nullptr;
   WorkItem *d_work_rec = nullptr, *d_work_tan = nullptr;
   std::vector<int> rec_groups, rec_start, rec_count;
   std::vector<int> tan_groups, tan_start, tan_count;
   int64_t *d_inv_ptr = nullptr, *d_inv_off = nullptr;
   int32_t* d_inv_nn = nullptr;
-  int64_t* d_blk_q_off = nullptr;
-  int32_t *d_blk_ngp = nullptr, *d_blk_span = nullptr;
+  int32_t *d_blk_wfirst = nullptr, *d_blk_wcount = nullptr;
   double* d_blk_sums = nullptr;
   double *d_u = nullptr, *d_du = nullptr, *d_ifv = nullptr;
   // sparsity + assembly map
@@ -141,7 +140,6 @@ struct w3d_ctx {
     M.ek32 = d_ek32;
     M.einfvec = d_einfvec;
     M.evol = d_evol;
-    M.energy_q = d_energy_q;
     M.flags = d_flags;
     M.eps_bbar = eps_bbar;
     return M;
@@ -188,7 +186,7 @@ void build_work(const w3d_ctx* c, bool tangent, std::vector<WorkItem>& items,
   std::vector<std::pair<int, WorkItem>> all;
   for (int b = 0; b < c->nelblk; ++b) {
     const BlockDesc& B = c->blocks[b];
-    int E = tangent ? tan_elems(B.nnode) : rec_elems(B.ngp);
+    int E = tangent ? tan_mfma_chunk(B.nnode) : rec_elems(B.ngp);
     int key = group_key(B.nnode, B.ngp, B.bbar);
     for (int f = 0; f < B.span; f += E) {
       WorkItem w;
@@ -223,10 +221,10 @@ void free_ctx(w3d_ctx* c) {
   dfree(c->d_strain_n); dfree(c->d_strain_n1);
   dfree(c->d_hist_n); dfree(c->d_hist_n1); dfree(c->d_cep);
   dfree(c->d_ek64); dfree(c->d_ek32); dfree(c->d_einfvec);
-  dfree(c->d_evol); dfree(c->d_energy_q); dfree(c->d_flags);
+  dfree(c->d_evol); dfree(c->d_en_wg); dfree(c->d_flags);
   dfree(c->d_work_rec); dfree(c->d_work_tan);
   dfree(c->d_inv_ptr); dfree(c->d_inv_off); dfree(c->d_inv_nn);
-  dfree(c->d_blk_q_off); dfree(c->d_blk_ngp); dfree(c->d_blk_span);
+  dfree(c->d_blk_wfirst); dfree(c->d_blk_wcount);
   dfree(c->d_blk_sums); dfree(c->d_u); dfree(c->d_du); dfree(c->d_ifv);
   dfree(c->d_asm_ptr); dfree(c->d_asm_src); dfree(c->d_kout);
   for (int ph = 0; ph < W3D_NUM_PHASES; ++ph) {
@@ -440,7 +438,7 @@ int create_impl(w3d_ctx* c, const w3d_model* m) {
               {&c->d_hist_n, h},        {&c->d_hist_n1, h},
               {&c->d_cep, 21 * q},      {&c->d_ek64, ek},
               {&c->d_einfvec, fv},      {&c->d_evol, c->noelem},
-              {&c->d_energy_q, 2 * q},  {&c->d_u, 3 * (int64_t)c->nonode},
+              {&c->d_u, 3 * (int64_t)c->nonode},
               {&c->d_du, 3 * (int64_t)c->nonode},
               {&c->d_ifv, 3 * (int64_t)c->nonode},
               {&c->d_blk_sums, 2 * (int64_t)c->nelblk}};
@@ -456,6 +454,17 @@ int create_impl(w3d_ctx* c, const w3d_model* m) {
     std::vector<WorkItem> items;
     build_work(c, false, items, c->rec_groups, c->rec_start, c->rec_count);
     if ((rc = upload(&c->d_work_rec, items))) return rc;
+    // chunk energy partials, and each block's run of chunks (a block's
+    // chunks are consecutive in the work list, in element order)
+    if ((rc = dalloc(&c->d_en_wg, 2 * items.size()))) return rc;
+    std::vector<int32_t> wf(c->nelblk, -1), wc(c->nelblk, 0);
+    for (size_t k = 0; k < items.size(); ++k) {
+      int b = items[k].blk;
+      if (wf[b] < 0) wf[b] = (int32_t)k;
+      wc[b] += 1;
+    }
+    if ((rc = upload(&c->d_blk_wfirst, wf))) return rc;
+    if ((rc = upload(&c->d_blk_wcount, wc))) return rc;
     build_work(c, true, items, c->tan_groups, c->tan_start, c->tan_count);
     if ((rc = upload(&c->d_work_tan, items))) return rc;
   }
@@ -472,18 +481,6 @@ int create_impl(w3d_ctx* c, const w3d_model* m) {
     if ((rc = upload(&c->d_inv_ptr, c->inv_ptr))) return rc;
     if ((rc = upload(&c->d_inv_off, off))) return rc;
     if ((rc = upload(&c->d_inv_nn, nn))) return rc;
-  }
-  {
-    std::vector<int64_t> qo(c->nelblk);
-    std::vector<int32_t> ng(c->nelblk), sp(c->nelblk);
-    for (int b = 0; b < c->nelblk; ++b) {
-      qo[b] = c->blocks[b].q_off;
-      ng[b] = c->blocks[b].ngp;
-      sp[b] = c->blocks[b].span;
-    }
-    if ((rc = upload(&c->d_blk_q_off, qo))) return rc;
-    if ((rc = upload(&c->d_blk_ngp, ng))) return rc;
-    if ((rc = upload(&c->d_blk_span, sp))) return rc;
   }
   return W3D_OK;
 }
@@ -535,7 +532,7 @@ int recover_impl(w3d_ctx* c, int step, int iter, int extrapolated,
     prof_begin(c, W3D_PHASE_RECOVER);
     rc = launch_recover(M, c->blocks, c->d_work_rec, c->rec_groups,
                         c->rec_start, c->rec_count, step, iter, nonlinear,
-                        d_du, want_energy, c->stream);
+                        d_du, want_energy, c->d_en_wg, c->stream);
     prof_end(c, W3D_PHASE_RECOVER);
   } else {
     // restrict the work list to the requested blocks
@@ -558,7 +555,7 @@ int recover_impl(w3d_ctx* c, int step, int iter, int extrapolated,
     WorkItem* d_sel = nullptr;
     if ((rc = upload(&d_sel, sel))) return rc;
     rc = launch_recover(M, c->blocks, d_sel, sg, ss, sn, step, iter,
-                        nonlinear, d_du, want_energy, c->stream);
+                        nonlinear, d_du, want_energy, c->d_en_wg, c->stream);
     hipStreamSynchronize(c->stream);
     dfree(d_sel);
   }
@@ -573,9 +570,8 @@ int recover_impl(w3d_ctx* c, int step, int iter, int extrapolated,
   double en = 0.0, pw = 0.0;
   if ((energy || plastic_work) && want_energy) {
     prof_begin(c, W3D_PHASE_ENERGY);
-    rc = launch_energy_reduce(c->d_energy_q, c->d_blk_q_off, c->d_blk_ngp,
-                              c->d_blk_span, c->nelblk, c->d_blk_sums,
-                              c->stream);
+    rc = launch_energy_reduce(c->d_en_wg, c->d_blk_wfirst, c->d_blk_wcount,
+                              c->nelblk, c->d_blk_sums, c->stream);
     prof_end(c, W3D_PHASE_ENERGY);
     if (rc) return fail(rc, "recover: energy reduction launch failed");
     std::vector<double> sums(2 * c->nelblk);
@@ -721,8 +717,8 @@ int w3d_tangent_blocks(w3d_ctx* c, int32_t first_blk, int32_t nblk,
   int rc;
   if (first_blk == 1 && nblk == c->nelblk) {
     prof_begin(c, W3D_PHASE_TANGENT);
-    rc = launch_tangent(M, c->blocks, c->d_work_tan, c->tan_groups,
-                        c->tan_start, c->tan_count, precision, c->stream);
+    rc = launch_tangent_mfma(M, c->d_work_tan, c->tan_groups, c->tan_start,
+                             c->tan_count, precision, c->stream);
     prof_end(c, W3D_PHASE_TANGENT);
   } else {
     std::vector<WorkItem> items;
@@ -743,8 +739,7 @@ int w3d_tangent_blocks(w3d_ctx* c, int32_t first_blk, int32_t nblk,
     }
     WorkItem* d_sel = nullptr;
     if ((rc = upload(&d_sel, sel))) return rc;
-    rc = launch_tangent(M, c->blocks, d_sel, sg, ss, sn, precision,
-                        c->stream);
+    rc = launch_tangent_mfma(M, d_sel, sg, ss, sn, precision, c->stream);
     hipStreamSynchronize(c->stream);
     dfree(d_sel);
   }
@@ -886,21 +881,17 @@ int w3d_assemble_vss_dev(w3d_ctx* c, double* d_k_diag, double* d_k_coeffs) {
   if (!c || !c->d_asm_ptr) return fail(W3D_ERR_ARG, "w3d_assemble: no pattern");
   if (c->tangent_precision < 0)
     return fail(W3D_ERR_ARG, "w3d_assemble: no tangent computed");
+  if (!d_k_diag || (!d_k_coeffs && c->ncoeff))
+    return fail(W3D_ERR_ARG, "w3d_assemble_vss_dev: null output");
   HIPCHK(hipSetDevice(c->device));
   int64_t nout = (int64_t)c->neqns + c->ncoeff;
   prof_begin(c, W3D_PHASE_ASSEMBLE);
   int rc = launch_assemble(c->d_ek64, c->d_ek32,
                            c->tangent_precision == W3D_FP32, c->d_asm_ptr,
-                           c->d_asm_src, nout, c->d_kout, c->stream);
+                           c->d_asm_src, nout, c->neqns, d_k_diag, d_k_coeffs,
+                           c->stream);
   prof_end(c, W3D_PHASE_ASSEMBLE);
   if (rc) return fail(rc, "w3d_assemble: kernel launch failed");
-  if (d_k_diag)
-    HIPCHK(hipMemcpyAsync(d_k_diag, c->d_kout, c->neqns * sizeof(double),
-                          hipMemcpyDeviceToDevice, c->stream));
-  if (d_k_coeffs && c->ncoeff)
-    HIPCHK(hipMemcpyAsync(d_k_coeffs, c->d_kout + c->neqns,
-                          c->ncoeff * sizeof(double), hipMemcpyDeviceToDevice,
-                          c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   prof_collect(c);
   return W3D_OK;
@@ -915,7 +906,8 @@ int w3d_assemble_vss(w3d_ctx* c, double* k_diag, double* k_coeffs) {
   prof_begin(c, W3D_PHASE_ASSEMBLE);
   int rc = launch_assemble(c->d_ek64, c->d_ek32,
                            c->tangent_precision == W3D_FP32, c->d_asm_ptr,
-                           c->d_asm_src, nout, c->d_kout, c->stream);
+                           c->d_asm_src, nout, c->neqns, c->d_kout,
+                           c->d_kout + c->neqns, c->stream);
   prof_end(c, W3D_PHASE_ASSEMBLE);
   if (rc) return fail(rc, "w3d_assemble: kernel launch failed");
   if (k_diag)

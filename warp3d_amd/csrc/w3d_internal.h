@@ -80,43 +80,61 @@ struct DevModel {
   float* ek32;
   double* einfvec;
   double* evol;
-  double* energy_q;       // per-point energy terms (2 per point)
   int32_t* flags;         // [0] fatal numeric, [1] cut step, [2] det J <= 0
   double eps_bbar;
 };
 
-// Kernel launchers (w3d_kernels.hip / w3d_tangent.hip).
+// Kernel launchers (w3d_kernels.hip, w3d_tangent.hip).
 int launch_recover(const DevModel& M, const std::vector<BlockDesc>& hblocks,
                    const WorkItem* d_work, const std::vector<int>& groups,
                    const std::vector<int>& group_start,
                    const std::vector<int>& group_count, int step, int iter,
                    int nonlinear, const double* d_du, int want_energy,
-                   hipStream_t s);
-int launch_tangent(const DevModel& M, const std::vector<BlockDesc>& hblocks,
-                   const WorkItem* d_work, const std::vector<int>& groups,
-                   const std::vector<int>& group_start,
-                   const std::vector<int>& group_count, int precision,
-                   hipStream_t s);
+                   double* en_wg, hipStream_t s);
 int launch_ifv_gather(const double* einfvec, const int64_t* inv_ptr,
                       const int64_t* inv_off, const int32_t* inv_nn,
                       int nonode, double* ifv, hipStream_t s);
-int launch_energy_reduce(const double* energy_q, const int64_t* blk_q_off,
-                         const int32_t* blk_nq, const int32_t* blk_span,
-                         int nblk, double* blk_sums, hipStream_t s);
+int launch_energy_reduce(const double* en_wg, const int32_t* blk_wfirst,
+                         const int32_t* blk_wcount, int nblk,
+                         double* blk_sums, hipStream_t s);
 int launch_assemble(const double* ek64, const float* ek32, int fp32,
                     const int64_t* row_ptr, const uint32_t* src,
-                    int64_t nout, double* out, hipStream_t s);
+                    int64_t nout, int64_t neq, double* diag, double* coef,
+                    hipStream_t s);
+int launch_tangent_mfma(const DevModel& M, const WorkItem* d_work,
+                        const std::vector<int>& groups,
+                        const std::vector<int>& group_start,
+                        const std::vector<int>& group_count, int precision,
+                        hipStream_t s);
+// elements per workgroup of the MFMA tangent (one wave per element)
+__host__ __device__ constexpr int tan_mfma_waves(int nnode) {
+  return nnode == 20 ? 2 : 4;
+}
+__host__ __device__ constexpr int tan_mfma_chunk(int nnode) {
+  return 8 * tan_mfma_waves(nnode);
+}
 
 // Kernel-group keys: one instantiation per (nnode, ngp, bbar).
 int group_key(int nnode, int ngp, int bbar);
 
-// Elements per workgroup of the recovery (lanes = element x point) and of
-// the vector tangent (lanes = element x 4x4 tile of the upper triangle).
+// Lower-triangle index of (row, col) in the 21-word [D] storage
+// (rstgp1_store_cep order (1,1),(2,1),(2,2),(3,1),...).
+__host__ __device__ constexpr int tri(int r, int c) {
+  return r >= c ? r * (r + 1) / 2 + c : c * (c + 1) / 2 + r;
+}
+
+// Upper 16x16 tiles (I <= J) of a tile grid, enumerated column by column.
+__host__ __device__ constexpr int tile_i(int t) {
+  return t < 1 ? 0 : t < 3 ? t - 1 : t < 6 ? t - 3 : t - 6;
+}
+__host__ __device__ constexpr int tile_j(int t) {
+  return t < 1 ? 0 : t < 3 ? 1 : t < 6 ? 2 : 3;
+}
+
+// Elements per workgroup of the recovery (lanes = element x point).
 __host__ __device__ constexpr int rec_elems(int ngp) {
   return 256 / ngp < 40 ? 256 / ngp : 40;
 }
-__host__ __device__ constexpr int tan_elems(int nnode) {
-  return 256 / (((3 * nnode + 3) / 4) * ((3 * nnode + 3) / 4 + 1) / 2);
-}
+
 
 }  // namespace w3d
