@@ -162,6 +162,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first run 3 device-to-device copies of MIB MiB "
+                         "(calibrates HBM byte counters against a copy of "
+                         "known size)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not "
@@ -172,6 +176,16 @@ def main():
     if hip.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         return 1
+    if args.copy_probe:
+        nb = args.copy_probe << 20
+        a = hip.DeviceArray(nb, np.uint8)
+        b = hip.DeviceArray(nb, np.uint8)
+        for _ in range(3):
+            hip.copy_d2d(b.ptr, a.ptr, nb)
+        hip.synchronize()
+        a.free()
+        b.free()
+        log(f"copy probe: 3 x {args.copy_probe} MiB device-to-device")
     t = time.perf_counter()
     m, arr, cst, du = build_workload(args.n)
     log(f"workload: {m.noelem} elements, {m.noelem * 8} points, "

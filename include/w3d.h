@@ -39,7 +39,7 @@
 extern "C" {
 #endif
 
-#define W3D_ABI_VERSION 1
+#define W3D_ABI_VERSION 2
 
 /* ---- status codes ---------------------------------------------------- */
 enum {
@@ -91,22 +91,35 @@ enum {
  * (mod_main.f:373): e (1), nu (2), beta (3), tan_e (4), yld_pt (5); they are
  * single precision on purpose, because the reference rounds them through
  * REAL and forms H' = e*tan_e/(e - tan_e) in REAL (elprp.f:722,872).
- * The mm10 parameters are double precision (mod_crystals.f:149-221).
+ * The mm10 fields are one crystal definition, the reference's double
+ * precision c_array entry (mod_crystals.f:149-221, defaults at :238-417),
+ * shared by every crystal of the material; the orientation of each crystal
+ * comes from w3d_model.angles.
  */
 typedef struct w3d_material {
   int32_t model;            /* W3D_MM01 or W3D_MM10                        */
   /* mm01 */
   float e, nu, beta, tan_e, yld_pt;
   int32_t linear_elastic;   /* lprops(17): response always linear elastic  */
-  /* mm10 (FCC slip, Voce hardening); ignored for mm01 */
-  double c11, c12, c44;     /* cubic crystal elastic constants            */
-  double tau_y, tau_v;      /* Voce initial slip resistance, saturation gap */
-  double voce_m, theta_0;   /* Voce exponent m and initial hardening rate  */
-  double rate_n;            /* slip-rate sensitivity exponent (harden_n)   */
-  double iD_v;              /* Voce diffusion term (0 in the configs)      */
+  /* mm10 (Voce hardening); ignored for mm01 */
+  int32_t slip_type;        /* 1 = fcc, 2 = bcc: 12 systems each           */
+  int32_t elastic_type;     /* 1 = isotropic (cp_e, cp_nu),
+                               2 = cubic (cp_e, cp_nu, cp_mu)              */
+  double cp_e, cp_nu, cp_mu;/* crystal elastic constants                   */
+  double harden_n;          /* slip-rate exponent (rate_n)                 */
+  double theta_0;           /* Voce initial hardening rate                 */
+  double tau_y, tau_v;      /* initial slip resistance, saturation gap     */
+  double voce_m;            /* Voce exponent                               */
+  double iD_v;              /* diffusion term                              */
+  double eps_dot_0_y;       /* fixed rate prefactor when alter_mode = 1    */
   double atol, atol1, rtol, rtol1;  /* local Newton tolerances             */
   int32_t miter;            /* local Newton iteration limit                */
-  int32_t angle_convention; /* 1 = Kocks (mm10_rotation_matrix)            */
+  int32_t alter_mode;       /* Voce alternative rate form                  */
+  int32_t angle_type;       /* 1 = degrees, 2 = radians                    */
+  int32_t angle_convention; /* 1 = Kocks (the only one the reference
+                               accepts, rknstr.f:1927)                     */
+  int32_t ncrystals;        /* crystals per point, Taylor-averaged         */
+  int32_t pad_;
 } w3d_material;
 
 /*
@@ -124,8 +137,10 @@ typedef struct w3d_model {
   const int32_t* int_order; /* (noelem) integration order, iprops(5)        */
   const int32_t* matnum;    /* (noelem) 1-based index into mats             */
   const int32_t* bbar;      /* (noelem) mean-dilatation B-bar (hex8 only)   */
-  const double* angles;     /* (3*noelem) crystal Euler angles, degrees;
+  const double* angles;     /* crystal Euler angles: crystal c of element e
+                               at angles[3*(angle_stride*(e-1) + c) + k];
                                may be NULL when no element uses mm10        */
+  int32_t angle_stride;     /* angle triplets per element (0 means 1)       */
   int32_t nmat;
   const w3d_material* mats;
   int32_t nelblk;           /* >0: user blocking table below; 0: automatic  */
@@ -158,13 +173,21 @@ int w3d_get_blocking(const w3d_ctx* ctx, int32_t* nelblk, int32_t* elblks);
 int w3d_block_info(const w3d_ctx* ctx, int32_t blk, int32_t* info5);
 
 /*
+ * Time increment of the current load step (the reference's dt, main_data),
+ * which mm10 uses for its rates and its hardening-rate predictor.  The
+ * default is 1.0.
+ */
+int w3d_set_time_increment(w3d_ctx* ctx, double dt);
+
+/*
  * Strain/stress recovery, material update and internal forces for every
  * block, then the deterministic assembly of the element forces into the
  * structure vector ifv (length 3*nonode).  u is the displacement at n, du the
  * increment n -> n+1 (both 3*nonode, structure dof order).
  *   iter = 0 is the reference's pseudo-update (linear-elastic [D] unless
- *   extrapolated != 0, rstgp1.f:835); history is scattered only for iter > 0
- *   (rplstr.f).
+ *   extrapolated != 0, rstgp1.f:835; for mm10 unless extrapolated != 0 and
+ *   step > 1, rstgp1.f:4453); history is scattered for iter > 0, and for
+ *   mm10 always, since its [D] lives in the history (rplstr.f:64-66).
  * energy / plastic_work receive the block-summed internal energy and plastic
  * work (rstgp1_b); either may be NULL.
  * Returns W3D_CUT_STEP when any Gauss point asks for a step cut.
@@ -240,8 +263,9 @@ int w3d_synchronize(w3d_ctx* ctx);
 /*
  * Per-phase device timing, the counterpart of the reference's thyme()
  * accumulators (thyme.f:15): when enabled, every kernel launch of a phase is
- * bracketed by HIP events on the context's stream.  Phases: 0 recovery,
- * 1 element-force gather, 2 energy reduction, 3 tangent, 4 assembly.
+ * bracketed by HIP events on the context's stream.  Phases: 0 recovery
+ * (kinematics, mm01 and element forces), 1 element-force gather, 2 energy
+ * reduction, 3 tangent, 4 assembly, 5 the mm10 material update.
  */
 enum {
   W3D_PHASE_RECOVER = 0,
@@ -249,7 +273,8 @@ enum {
   W3D_PHASE_ENERGY = 2,
   W3D_PHASE_TANGENT = 3,
   W3D_PHASE_ASSEMBLE = 4,
-  W3D_NUM_PHASES = 5
+  W3D_PHASE_MM10 = 5,
+  W3D_NUM_PHASES = 6
 };
 int w3d_set_profiling(w3d_ctx* ctx, int32_t on);
 /* launches and total device milliseconds of one phase since the last reset */

@@ -8,7 +8,8 @@ import os
 
 import numpy as np
 
-from warp3d_amd._abi import (PROTOS, W3DModel, dptr, iptr, lptr)
+from warp3d_amd._abi import (PROTOS, W3DMaterial, W3DModel, dptr, iptr,
+                             lptr)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
@@ -47,12 +48,45 @@ def lib():
             [C.POINTER(C.c_double)] * 5 + [C.c_double, C.POINTER(C.c_double)]
         L.orc_hprime.restype = C.c_float
         L.orc_hprime.argtypes = [C.c_float, C.c_float]
+        _D = C.POINTER(C.c_double)
+        L.orc_set_time_increment.argtypes = [C.c_void_p, C.c_double]
+        L.orc_mm10_crystal.restype = C.c_int
+        L.orc_mm10_crystal.argtypes = [C.POINTER(W3DMaterial), _D,
+                                       C.POINTER(Crystal)]
+        L.orc_mm10_gp.restype = C.c_int
+        L.orc_mm10_gp.argtypes = [C.POINTER(W3DMaterial), C.POINTER(Crystal),
+                                  C.c_int, C.c_int, C.c_double, _D, _D, _D,
+                                  _D, _D]
+        L.orc_mm10_residual.argtypes = [C.POINTER(W3DMaterial),
+                                        C.POINTER(Crystal), _D, _D,
+                                        C.c_double, _D, _D, _D]
+        L.orc_mm10_rotation_matrix.argtypes = [_D, C.c_int, _D]
+        L.orc_mm10_rt2rve.argtypes = [_D, _D]
+        L.orc_mm10_rt2rvw.argtypes = [_D, _D]
+        L.orc_mm10_expmw.argtypes = [_D, _D]
         _LIB = L
     return _LIB
 
 
 class OracleError(RuntimeError):
     pass
+
+
+class Crystal(C.Structure):
+    """orc_crystal: one crystal's constants in the reference frame."""
+    _fields_ = [("C", C.c_double * 36), ("ms", C.c_double * 72),
+                ("qs", C.c_double * 36), ("g", C.c_double * 9),
+                ("angles", C.c_double * 3)]
+
+    def stiffness(self):
+        return np.array(self.C).reshape(6, 6, order="F")
+
+    def schmid(self):
+        return (np.array(self.ms).reshape(12, 6),
+                np.array(self.qs).reshape(12, 3))
+
+    def rotation(self):
+        return np.array(self.g).reshape(3, 3, order="F")
 
 
 def _chk(rc, what):
@@ -125,6 +159,61 @@ def mm01_gp(step, e, nu, beta, hp, yld, cgn, deps, hist, hist1=None,
     return cgn1, hist1, cep.reshape(6, 6)
 
 
+# ---------------------------------------------------------------- mm10
+def mm10_crystal(mat, angles):
+    cr = Crystal()
+    a = np.ascontiguousarray(angles, dtype=np.float64)
+    if lib().orc_mm10_crystal(C.byref(mat), dptr(a), C.byref(cr)) != 0:
+        raise OracleError("mm10 crystal set-up")
+    return cr
+
+
+def mm10_gp(mat, crystals, step, iter0_off, dt, uddt, urcs_n, hist_n):
+    """One mm10 point: returns (rc, urcs_n1, hist_n1, hist_n as used)."""
+    ncry = len(crystals)
+    arr = (Crystal * ncry)(*crystals)
+    hn = np.array(hist_n, dtype=np.float64)
+    hn1 = np.zeros(87 + 71 * ncry)
+    u1 = np.zeros(9)
+    rc = lib().orc_mm10_gp(C.byref(mat), C.cast(arr, C.POINTER(Crystal)),
+                           step, int(iter0_off), dt,
+                           dptr(np.ascontiguousarray(uddt, dtype=float)),
+                           dptr(np.ascontiguousarray(urcs_n, dtype=float)),
+                           dptr(u1), dptr(hn), dptr(hn1))
+    return rc, u1, hn1, hn
+
+
+def mm10_residual(mat, crystal, hist_crystal, uddt, dt, x):
+    R = np.zeros(7)
+    J = np.zeros(49)
+    lib().orc_mm10_residual(C.byref(mat), C.byref(crystal),
+                            dptr(np.ascontiguousarray(hist_crystal, float)),
+                            dptr(np.ascontiguousarray(uddt, float)), dt,
+                            dptr(np.ascontiguousarray(x, float)), dptr(R),
+                            dptr(J))
+    return R, J.reshape(7, 7, order="F")
+
+
+def mm10_rotation_matrix(angles, angle_type=1):
+    r = np.zeros(9)
+    lib().orc_mm10_rotation_matrix(
+        dptr(np.ascontiguousarray(angles, float)), angle_type, dptr(r))
+    return r.reshape(3, 3, order="F")
+
+
+def mm10_rt2rve(rt):
+    rv = np.zeros(36)
+    lib().orc_mm10_rt2rve(dptr(np.asfortranarray(rt, float).ravel(order="F")),
+                          dptr(rv))
+    return rv.reshape(6, 6, order="F")
+
+
+def mm10_expmw(wv):
+    a = np.zeros(9)
+    lib().orc_mm10_expmw(dptr(np.ascontiguousarray(wv, float)), dptr(a))
+    return a.reshape(3, 3, order="F")
+
+
 # ---------------------------------------------------------------- model
 class OracleModel:
     """The oracle behind the same calls as warp3d_amd.BlockModel."""
@@ -150,6 +239,9 @@ class OracleModel:
         if getattr(self, "h", None):
             lib().orc_destroy(self.h)
             self.h = None
+
+    def set_time_increment(self, dt):
+        lib().orc_set_time_increment(self.h, dt)
 
     def block_info(self, blk):
         info = np.zeros(5, np.int32)

@@ -56,6 +56,8 @@ typedef struct {
   double *einfvec;            /* (span, totdof) */
   double *evol;               /* (span) */
   int32_t* edest;             /* (totdof, span) 1-based structure dofs */
+  int ncry;                   /* mm10: crystals per point                */
+  orc_crystal* crys;          /* mm10: (ncry, span) crystal constants    */
 } orc_block;
 
 struct orc_model {
@@ -65,6 +67,7 @@ struct orc_model {
   w3d_material* mats;
   double eps_bbar;
   int32_t* elblks; /* (4, nelblk) */
+  double dt;       /* time increment of the load step */
   orc_block* blk;
   int32_t* elem_blk; /* (noelem) 0-based block of each element */
   /* inverse incidences: per node, elements in ascending order */
@@ -79,7 +82,14 @@ struct orc_model {
 static int g_threads = 0;
 void orc_set_threads(int n) { g_threads = n; }
 
-static int hist_size(int model) { return model == W3D_MM01 ? 11 : 158; }
+void orc_set_time_increment(orc_model* om, double dt) { om->dt = dt; }
+
+static int ncry_of(const w3d_material* m) {
+  return m->ncrystals > 0 ? m->ncrystals : 1;
+}
+static int hist_size(const w3d_material* m) {
+  return m->model == W3D_MM01 ? 11 : orc_mm10_hist_size(ncry_of(m));
+}
 
 /* ------------------------------------------------------------------ */
 /* blocking: inelbk_simple_blocking / inelbk_chk_match                 */
@@ -138,6 +148,7 @@ orc_model* orc_create(const w3d_model* m) {
   om->noelem = m->noelem;
   om->nmat = m->nmat;
   om->eps_bbar = m->eps_bbar;
+  om->dt = 1.0;
   om->c = (double*)malloc(sizeof(double) * 3 * m->nonode);
   memcpy(om->c, m->c, sizeof(double) * 3 * m->nonode);
   om->etype = (int32_t*)malloc(sizeof(int32_t) * m->noelem);
@@ -176,8 +187,16 @@ orc_model* orc_create(const w3d_model* m) {
       orc_destroy(om);
       return NULL;
     }
-    int mod = om->mats[om->matnum[e] - 1].model;
-    if (mod != W3D_MM01) {
+    const w3d_material* mt = &om->mats[om->matnum[e] - 1];
+    int ok = mt->model == W3D_MM01;
+    if (mt->model == W3D_MM10)
+      ok = (mt->slip_type == 1 || mt->slip_type == 2) &&
+           (mt->elastic_type == 1 || mt->elastic_type == 2) &&
+           mt->angle_convention == 1 &&
+           (mt->angle_type == 1 || mt->angle_type == 2) &&
+           ncry_of(mt) <= (m->angle_stride > 0 ? m->angle_stride : 1) &&
+           m->angles != NULL;
+    if (!ok) {
       fail(W3D_ERR_UNSUPPORTED, "orc_create: material model");
       orc_destroy(om);
       return NULL;
@@ -229,7 +248,7 @@ orc_model* orc_create(const w3d_model* m) {
     B->utsz = B->totdof * (B->totdof + 1) / 2;
     B->mat = om->matnum[fe] - 1;
     B->model = om->mats[B->mat].model;
-    B->hist = hist_size(B->model);
+    B->hist = hist_size(&om->mats[B->mat]);
     B->bbar = om->bbar[fe];
     int64_t nq = (int64_t)B->ngp * B->span;
     B->urcs_n = (double*)calloc(9 * nq, sizeof(double));
@@ -243,6 +262,16 @@ orc_model* orc_create(const w3d_model* m) {
     B->einfvec = (double*)calloc((int64_t)B->totdof * B->span, sizeof(double));
     B->evol = (double*)calloc(B->span, sizeof(double));
     B->edest = (int32_t*)malloc(sizeof(int32_t) * B->totdof * B->span);
+    if (B->model == W3D_MM10) {
+      const w3d_material* mt = &om->mats[B->mat];
+      int stride = m->angle_stride > 0 ? m->angle_stride : 1;
+      B->ncry = ncry_of(mt);
+      B->crys = (orc_crystal*)malloc(sizeof(orc_crystal) * B->ncry * B->span);
+      for (int i = 0; i < B->span; ++i)
+        for (int c = 0; c < B->ncry; ++c)
+          orc_mm10_crystal(mt, m->angles + 3 * ((int64_t)stride * (fe + i) + c),
+                           &B->crys[i * B->ncry + c]);
+    }
     for (int i = 0; i < B->span; ++i) {
       int e = fe + i;
       om->elem_blk[e] = b;
@@ -291,7 +320,7 @@ void orc_destroy(orc_model* om) {
       orc_block* B = &om->blk[b];
       free(B->urcs_n); free(B->urcs_n1); free(B->eps_n); free(B->eps_n1);
       free(B->hist_n); free(B->hist_n1); free(B->cep); free(B->ek);
-      free(B->einfvec); free(B->evol); free(B->edest);
+      free(B->einfvec); free(B->evol); free(B->edest); free(B->crys);
     }
   free(om->blk); free(om->elem_blk); free(om->c); free(om->incid);
   free(om->incmap); free(om->etype); free(om->order); free(om->matnum);
@@ -410,7 +439,7 @@ static int recover_block(orc_model* om, int b, int step, int iter,
   }
   double blk_energy = 0.0, blk_pwork = 0.0;
   double bmat[6 * MXDOF];
-  int rc = 0;
+  int rc = 0, cut = 0;
   /* mm01_set_history at step 1 resets the gathered n-history of every
      element before any element is updated: the first element's word 11
      read by mm01_general (mm01.f:845) is then zero. */
@@ -441,7 +470,20 @@ static int recover_block(orc_model* om, int b, int step, int iter,
       double* cgn = sloc + q * 9;
       double* cgn1 = B->urcs_n1 + q * 9;
       double cep[36];
-      if (nonlinear) {
+      if (B->model == W3D_MM10) {
+        /* drive_10_update: rotation I (small strain); iteration 0 is the
+           linear-elastic update unless the increment was extrapolated
+           after step 1 (rstgp1.f:4453-4457); history always scattered */
+        int iter0_off = iter == 0 && (!extrapolated || step == 1);
+        double* h1 = B->hist_n1 + q * H;
+        if (orc_mm10_gp(mat, B->crys + (int64_t)i * B->ncry, step, iter0_off,
+                        om->dt, deps, cgn, cgn1, hloc + q * H, h1))
+          cut = 1;
+        /* [D] as stored for the tangent: history words 1-36 (6x6,
+           column-major, drive_10_cnst) */
+        for (int r = 0; r < 6; ++r)
+          for (int cc = 0; cc < 6; ++cc) cep[6 * r + cc] = h1[6 * cc + r];
+      } else if (nonlinear) {
         if (step == 1) first_hist11 = 0.0;
         double h1[11];
         if (orc_mm01_gp(step, e, nu, beta, hp, lnelas, yld, cgn, cgn1, deps,
@@ -512,22 +554,25 @@ static int recover_block(orc_model* om, int b, int step, int iter,
   free(sloc);
   *energy = blk_energy;
   *pwork = blk_pwork;
-  return rc;
+  return rc ? rc : (cut ? W3D_CUT_STEP : 0);
 }
 
 int orc_recover_blocks(orc_model* om, int first, int n, int step, int iter,
                        int extrapolated, const double* u, const double* du) {
   if (first < 1 || n < 0 || first - 1 + n > om->nelblk)
     return fail(W3D_ERR_ARG, "orc_recover_blocks: block range");
-  int rc = 0;
+  int rc = 0, cut = 0;
 #pragma omp parallel for schedule(dynamic, 1) reduction(min : rc) \
+    reduction(max : cut) \
     num_threads(g_threads > 0 ? g_threads : omp_get_max_threads())
   for (int b = first - 1; b < first - 1 + n; ++b) {
     double en, pw;
     int r = recover_block(om, b, step, iter, extrapolated, u, du, &en, &pw);
-    if (r < rc) rc = r;
+    if (r == W3D_CUT_STEP) cut = 1;
+    else if (r < rc) rc = r;
   }
   if (rc) return fail(rc, "mm01: negative plastic multiplier");
+  if (cut) return fail(W3D_CUT_STEP, "mm10: local solve failed, cut step");
   return 0;
 }
 
@@ -537,13 +582,15 @@ int orc_recover(orc_model* om, int step, int iter, int extrapolated,
   int nb = om->nelblk;
   double* be = (double*)calloc(nb, sizeof(double));
   double* bp = (double*)calloc(nb, sizeof(double));
-  int rc = 0;
+  int rc = 0, cut = 0;
 #pragma omp parallel for schedule(dynamic, 1) reduction(min : rc) \
+    reduction(max : cut) \
     num_threads(g_threads > 0 ? g_threads : omp_get_max_threads())
   for (int b = 0; b < nb; ++b) {
     int r = recover_block(om, b, step, iter, extrapolated, u, du, &be[b],
                           &bp[b]);
-    if (r < rc) rc = r;
+    if (r == W3D_CUT_STEP) cut = 1;
+    else if (r < rc) rc = r;
   }
   double en = 0.0, pw = 0.0;
   for (int b = 0; b < nb; ++b) {
@@ -577,6 +624,7 @@ int orc_recover(orc_model* om, int step, int iter, int extrapolated,
       ifv[3 * n + 2] = f[2];
     }
   }
+  if (cut) return fail(W3D_CUT_STEP, "mm10: local solve failed, cut step");
   return 0;
 }
 
@@ -609,17 +657,25 @@ static void tangent_block(orc_model* om, int b) {
     for (int g = 0; g < G; ++g) {
       orc_bmatrix(N, gama[g], t.nxi[g], t.neta[g], t.nzeta[g],
                   B->bbar ? vol : NULL, om->eps_bbar, bmat);
-      /* extract_D_symmetric: f = w |J| times the stored lower triangle */
       double f = t.w[g] * dj[g];
-      const double* cs = B->cep + ((int64_t)g * S + i) * 21;
       double dm[36];
-      int k = 0;
-      for (int r = 0; r < 6; ++r)
-        for (int c = 0; c <= r; ++c) {
-          double v = f * cs[k++];
-          dm[6 * r + c] = v;
-          dm[6 * c + r] = v;
-        }
+      if (B->model == W3D_MM10) {
+        /* drive_10_cnst (gptns1.f:1371): d(m,k) = history1 word 6k+m
+           times w |J| */
+        const double* h1 = B->hist_n1 + ((int64_t)i * G + g) * B->hist;
+        for (int m = 0; m < 6; ++m)
+          for (int kk = 0; kk < 6; ++kk) dm[6 * m + kk] = h1[6 * kk + m] * f;
+      } else {
+        /* extract_D_symmetric: f = w |J| times the stored lower triangle */
+        const double* cs = B->cep + ((int64_t)g * S + i) * 21;
+        int k = 0;
+        for (int r = 0; r < 6; ++r)
+          for (int c = 0; c <= r; ++c) {
+            double v = f * cs[k++];
+            dm[6 * r + c] = v;
+            dm[6 * c + r] = v;
+          }
+      }
       if (scalar != 1.0) /* adjust_cnst */
         for (int q = 0; q < 36; ++q) dm[q] = scalar * dm[q];
       /* bdbtgen: bd(j,k) = sum_m d(m,k) b(j,m) */

@@ -50,6 +50,38 @@ int orc_mm01_gp(int step, double e, double nu, double beta, double hprime,
 /* H' = e*et/(e-et) in REAL arithmetic (elprp.f:872). */
 float orc_hprime(float e, float et);
 
+/* ---- mm10 (orc_mm10.c) ------------------------------------------------ */
+/* One crystal of one element: stiffness rotated into the reference frame
+   (i,j) at C[i + 6j], Schmid tensors ms (Voigt, engineering shear) and skew
+   vectors qs of the 12 slip systems, the crystal -> reference rotation g
+   ((i,j) at g[i + 3j]) and the initial Euler angles. */
+typedef struct {
+  double C[36];
+  double ms[12][6];
+  double qs[12][3];
+  double g[9];
+  double angles[3];
+} orc_crystal;
+int orc_mm10_crystal(const w3d_material* mat, const double angles[3],
+                     orc_crystal* cr);
+int orc_mm10_hist_size(int ncrystals);
+/* mm10 for one Gauss point (all its crystals).  hn is the point's gathered
+   n-history (re-initialised in place at step 1), hn1 receives the n+1
+   history, urcs 9 words.  Returns 0, or 1 when the local solve fails (the
+   reference's material_cut_step). */
+int orc_mm10_gp(const w3d_material* mat, const orc_crystal* crys, int step,
+                int iter0_off, double dt, const double* uddt,
+                const double* urcs_n, double* urcs_n1, double* hn,
+                double* hn1);
+void orc_mm10_residual(const w3d_material* mat, const orc_crystal* cr,
+                       const double* hist_crystal, const double* uddt,
+                       double dt, const double* x, double* R, double* J);
+void orc_mm10_rotation_matrix(const double angles[3], int angle_type,
+                              double r[9]);
+void orc_mm10_rt2rve(const double* rt, double* rv);
+void orc_mm10_rt2rvw(const double* rt, double* rv);
+void orc_mm10_expmw(const double wv[3], double a[9]);
+
 /* ---- block model ------------------------------------------------------ */
 orc_model* orc_create(const w3d_model* m);
 void orc_destroy(orc_model* om);
@@ -57,6 +89,7 @@ const char* orc_last_error(void);
 int orc_get_blocking(const orc_model* om, int32_t* nelblk, int32_t* elblks);
 int orc_block_info(const orc_model* om, int blk, int32_t* info5);
 void orc_set_threads(int n);
+void orc_set_time_increment(orc_model* om, double dt);
 
 int orc_recover(orc_model* om, int step, int iter, int extrapolated,
                 const double* u, const double* du, double* ifv,

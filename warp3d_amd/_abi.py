@@ -7,7 +7,7 @@ import ctypes as C
 
 import numpy as np
 
-W3D_ABI_VERSION = 1
+W3D_ABI_VERSION = 2
 
 W3D_OK = 0
 W3D_CUT_STEP = 1
@@ -34,13 +34,17 @@ class W3DMaterial(C.Structure):
         ("e", C.c_float), ("nu", C.c_float), ("beta", C.c_float),
         ("tan_e", C.c_float), ("yld_pt", C.c_float),
         ("linear_elastic", C.c_int32),
-        ("c11", C.c_double), ("c12", C.c_double), ("c44", C.c_double),
+        ("slip_type", C.c_int32), ("elastic_type", C.c_int32),
+        ("cp_e", C.c_double), ("cp_nu", C.c_double), ("cp_mu", C.c_double),
+        ("harden_n", C.c_double), ("theta_0", C.c_double),
         ("tau_y", C.c_double), ("tau_v", C.c_double),
-        ("voce_m", C.c_double), ("theta_0", C.c_double),
-        ("rate_n", C.c_double), ("iD_v", C.c_double),
+        ("voce_m", C.c_double), ("iD_v", C.c_double),
+        ("eps_dot_0_y", C.c_double),
         ("atol", C.c_double), ("atol1", C.c_double),
         ("rtol", C.c_double), ("rtol1", C.c_double),
-        ("miter", C.c_int32), ("angle_convention", C.c_int32),
+        ("miter", C.c_int32), ("alter_mode", C.c_int32),
+        ("angle_type", C.c_int32), ("angle_convention", C.c_int32),
+        ("ncrystals", C.c_int32), ("pad_", C.c_int32),
     ]
 
 
@@ -57,6 +61,7 @@ class W3DModel(C.Structure):
         ("matnum", C.POINTER(C.c_int32)),
         ("bbar", C.POINTER(C.c_int32)),
         ("angles", C.POINTER(C.c_double)),
+        ("angle_stride", C.c_int32),
         ("nmat", C.c_int32),
         ("mats", C.POINTER(W3DMaterial)),
         ("nelblk", C.c_int32),
@@ -96,12 +101,48 @@ def mm01_material(e, nu, yld_pt, tan_e=None, hprime=None, beta=1.0,
     return m
 
 
+def mm10_material(tau_y, tau_v, theta_0, voce_m=1.0, harden_n=20.0,
+                  slip_type="fcc", elastic="isotropic", e=69000.0, nu=0.33,
+                  mu=None, iD_v=0.0, ncrystals=1, angle_type="degrees",
+                  atol=1.0e-5, atol1=1.0e-5, rtol=5.0e-5, rtol1=1.0e-5,
+                  miter=30, alter_mode=False, eps_dot_0_y=1.0e10):
+    """An mm10 (crystal plasticity, Voce hardening) material.  Defaults are
+    the reference's crystal defaults (mod_crystals.f:238-417)."""
+    m = W3DMaterial()
+    m.model = W3D_MM10
+    m.slip_type = {"fcc": 1, "bcc": 2}[slip_type]
+    m.elastic_type = {"isotropic": 1, "cubic": 2}[elastic]
+    m.cp_e = e
+    m.cp_nu = nu
+    m.cp_mu = e / 2.0 / (1.0 + nu) if mu is None else mu
+    m.harden_n = harden_n
+    m.theta_0 = theta_0
+    m.tau_y = tau_y
+    m.tau_v = tau_v
+    m.voce_m = voce_m
+    m.iD_v = iD_v
+    m.eps_dot_0_y = eps_dot_0_y
+    m.atol, m.atol1, m.rtol, m.rtol1 = atol, atol1, rtol, rtol1
+    m.miter = miter
+    m.alter_mode = int(bool(alter_mode))
+    m.angle_type = {"degrees": 1, "radians": 2}[angle_type]
+    m.angle_convention = 1
+    m.ncrystals = ncrystals
+    return m
+
+
+def mm10_hist_size(ncrystals=1):
+    """History words of one mm10 point: 87 common + 71 per crystal
+    (mm10_d.f:25)."""
+    return 87 + 71 * ncrystals
+
+
 class ModelArrays:
     """Holds the numpy arrays a w3d_model points into (keeps them alive)."""
 
     def __init__(self, coords, incid, incmap, etype, int_order, matnum,
                  bbar, materials, elblks=None, auto_size=128, eps_bbar=0.0,
-                 angles=None):
+                 angles=None, angle_stride=1):
         self.coords = np.ascontiguousarray(coords, dtype=np.float64).ravel()
         self.incid = np.ascontiguousarray(incid, dtype=np.int32).ravel()
         self.incmap = np.ascontiguousarray(incmap, dtype=np.int32)
@@ -111,6 +152,7 @@ class ModelArrays:
         self.bbar = np.ascontiguousarray(bbar, dtype=np.int32)
         self.angles = (None if angles is None else
                        np.ascontiguousarray(angles, dtype=np.float64).ravel())
+        self.angle_stride = angle_stride
         self.materials = list(materials)
         self.mats = (W3DMaterial * len(self.materials))(*self.materials)
         self.elblks = (None if elblks is None else
@@ -134,6 +176,7 @@ class ModelArrays:
         m.bbar = iptr(self.bbar)
         m.angles = (dptr(self.angles) if self.angles is not None
                     else C.POINTER(C.c_double)())
+        m.angle_stride = self.angle_stride
         m.nmat = len(self.materials)
         m.mats = C.cast(self.mats, C.POINTER(W3DMaterial))
         if self.elblks is not None:

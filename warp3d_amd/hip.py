@@ -81,6 +81,12 @@ class DeviceArray:
             pass
 
 
+def copy_d2d(dst, src, nbytes):
+    """Device-to-device copy (a plain copy kernel on one device)."""
+    _chk(hip().hipMemcpy(C.c_void_p(dst), C.c_void_p(src), C.c_size_t(nbytes),
+                         D2D), "hipMemcpy D2D")
+
+
 def synchronize():
     _chk(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
 
