@@ -62,10 +62,13 @@ def test_struct_layouts_match_header():
 #define P(T, f) printf("%s %zu\n", #T "." #f, offsetof(T, f))
 int main(void) {
   P(w3d_material, model); P(w3d_material, e); P(w3d_material, yld_pt);
-  P(w3d_material, c11); P(w3d_material, rtol1); P(w3d_material, miter);
-  P(w3d_material, angle_convention);
+  P(w3d_material, slip_type); P(w3d_material, cp_e); P(w3d_material, cp_mu);
+  P(w3d_material, harden_n); P(w3d_material, eps_dot_0_y);
+  P(w3d_material, rtol1); P(w3d_material, miter);
+  P(w3d_material, angle_convention); P(w3d_material, ncrystals);
   printf("w3d_material.size %zu\n", sizeof(w3d_material));
   P(w3d_model, c); P(w3d_model, incid); P(w3d_model, angles);
+  P(w3d_model, angle_stride);
   P(w3d_model, nmat); P(w3d_model, mats); P(w3d_model, nelblk);
   P(w3d_model, elblks); P(w3d_model, auto_size); P(w3d_model, eps_bbar);
   printf("w3d_model.size %zu\n", sizeof(w3d_model));

@@ -15,8 +15,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libw3dblk.so")
 ARCH = "gfx950"
 
-SOURCES = ["w3d_api.cpp", "w3d_elements.cpp", "w3d_kernels.hip",
-           "w3d_tangent.hip"]
+SOURCES = ["w3d_api.cpp", "w3d_elements.cpp", "w3d_mm10_setup.cpp",
+           "w3d_kernels.hip", "w3d_tangent.hip", "w3d_mm10.hip"]
 
 
 def hipcc():

@@ -56,7 +56,7 @@ int dalloc(T** p, size_t n) {
 
 template <typename T>
 void dfree(T*& p) {
-  if (p) hipFree((void*)p);
+  if (p) (void)hipFree((void*)p);
   p = nullptr;
 }
 
@@ -97,6 +97,13 @@ struct w3d_ctx {
   float* d_ek32 = nullptr;
   double *d_einfvec = nullptr, *d_evol = nullptr, *d_en_wg = nullptr;
   int32_t* d_flags = nullptr;
+  // mm10: crystal records, strain increments, per-point work items
+  CrystalDev* d_crys = nullptr;
+  double* d_uddt = nullptr;
+  WorkItem* d_work_mm10 = nullptr;
+  int n_work_mm10 = 0;
+  bool has_mm10 = false;
+  double dt = 1.0;
   WorkItem *d_work_rec = nullptr, *d_work_tan = nullptr;
   std::vector<int> rec_groups, rec_start, rec_count;
   std::vector<int> tan_groups, tan_start, tan_count;
@@ -117,8 +124,9 @@ struct w3d_ctx {
   double eps_bbar = 0.0;
   // phase timing (w3d_set_profiling)
   bool profiling = false;
-  hipEvent_t ev_a[W3D_NUM_PHASES] = {}, ev_b[W3D_NUM_PHASES] = {};
-  bool pending[W3D_NUM_PHASES] = {};
+  // up to two timed segments per phase between collections
+  hipEvent_t ev_a[W3D_NUM_PHASES][2] = {}, ev_b[W3D_NUM_PHASES][2] = {};
+  int nseg[W3D_NUM_PHASES] = {};
   int64_t prof_n[W3D_NUM_PHASES] = {};
   double prof_ms[W3D_NUM_PHASES] = {};
 
@@ -141,7 +149,10 @@ struct w3d_ctx {
     M.einfvec = d_einfvec;
     M.evol = d_evol;
     M.flags = d_flags;
+    M.crys = d_crys;
+    M.uddt = d_uddt;
     M.eps_bbar = eps_bbar;
+    M.dt = dt;
     return M;
   }
 };
@@ -187,7 +198,8 @@ void build_work(const w3d_ctx* c, bool tangent, std::vector<WorkItem>& items,
   for (int b = 0; b < c->nelblk; ++b) {
     const BlockDesc& B = c->blocks[b];
     int E = tangent ? tan_mfma_chunk(B.nnode) : rec_elems(B.ngp);
-    int key = group_key(B.nnode, B.ngp, B.bbar);
+    int key = group_key(B.nnode, B.ngp, B.bbar,
+                        !tangent && B.model == W3D_MM10);
     for (int f = 0; f < B.span; f += E) {
       WorkItem w;
       w.blk = b;
@@ -227,10 +239,13 @@ void free_ctx(w3d_ctx* c) {
   dfree(c->d_blk_wfirst); dfree(c->d_blk_wcount);
   dfree(c->d_blk_sums); dfree(c->d_u); dfree(c->d_du); dfree(c->d_ifv);
   dfree(c->d_asm_ptr); dfree(c->d_asm_src); dfree(c->d_kout);
+  dfree(c->d_crys); dfree(c->d_uddt); dfree(c->d_work_mm10);
   for (int ph = 0; ph < W3D_NUM_PHASES; ++ph) {
-    if (c->ev_a[ph]) hipEventDestroy(c->ev_a[ph]);
-    if (c->ev_b[ph]) hipEventDestroy(c->ev_b[ph]);
-    c->ev_a[ph] = c->ev_b[ph] = nullptr;
+    for (int k = 0; k < 2; ++k) {
+      if (c->ev_a[ph][k]) hipEventDestroy(c->ev_a[ph][k]);
+      if (c->ev_b[ph][k]) hipEventDestroy(c->ev_b[ph][k]);
+      c->ev_a[ph][k] = c->ev_b[ph][k] = nullptr;
+    }
   }
   if (c->stream) hipStreamDestroy(c->stream);
   c->stream = nullptr;
@@ -288,6 +303,35 @@ int create_impl(w3d_ctx* c, const w3d_model* m) {
       d.beta = (double)s.beta;
       d.hprime = (double)hprime_real(s.e, s.tan_e);
       d.yld = (double)s.yld_pt;
+    } else if (s.model == W3D_MM10) {
+      // the implemented scope: Voce hardening, Kocks angles, 12-system
+      // fcc / bcc crystals with isotropic or cubic elasticity
+      if ((s.slip_type != 1 && s.slip_type != 2) ||
+          (s.elastic_type != 1 && s.elastic_type != 2) ||
+          s.angle_convention != 1 ||
+          (s.angle_type != 1 && s.angle_type != 2) || s.ncrystals < 1 ||
+          s.ncrystals > (m->angle_stride > 0 ? m->angle_stride : 1) ||
+          !(s.cp_e > 0.0) || !(s.tau_v > 0.0) || !(s.harden_n > 0.0) ||
+          s.miter < 1 || !m->angles)
+        return fail(W3D_ERR_UNSUPPORTED,
+                    "w3d_create: mm10 properties of material " +
+                        std::to_string(k + 1));
+      CpDesc& cp = d.cp;
+      cp.rate_n = s.harden_n;
+      cp.tau_y = s.tau_y;
+      cp.tau_v = s.tau_v;
+      cp.voce_m = s.voce_m;
+      cp.theta_0 = s.theta_0;
+      cp.iD_v = s.iD_v;
+      cp.eps_dot_0_y = s.eps_dot_0_y;
+      cp.atol = s.atol;
+      cp.atol1 = s.atol1;
+      cp.rtol = s.rtol;
+      cp.rtol1 = s.rtol1;
+      cp.miter = s.miter;
+      cp.alter_mode = s.alter_mode;
+      cp.angle_type = s.angle_type;
+      cp.ncry = s.ncrystals;
     } else {
       return fail(W3D_ERR_UNSUPPORTED,
                   "w3d_create: material model " + std::to_string(s.model) +
@@ -330,7 +374,7 @@ int create_impl(w3d_ctx* c, const w3d_model* m) {
   // blocks
   c->blocks.resize(c->nelblk);
   c->elem_blk.resize(c->noelem);
-  int64_t q = 0, h = 0, ek = 0, fv = 0, cn = 0;
+  int64_t q = 0, h = 0, ek = 0, fv = 0, cn = 0, ncr = 0;
   for (int b = 0; b < c->nelblk; ++b) {
     BlockDesc& B = c->blocks[b];
     std::memset(&B, 0, sizeof B);
@@ -345,7 +389,9 @@ int create_impl(w3d_ctx* c, const w3d_model* m) {
     B.utsz = B.totdof * (B.totdof + 1) / 2;
     B.mat = c->matnum[fe] - 1;
     B.model = c->mats[B.mat].model;
-    B.hist = B.model == W3D_MM01 ? kMM01Hist : kMM10Hist;
+    B.ncry = B.model == W3D_MM10 ? c->mats[B.mat].cp.ncry : 0;
+    B.hist = B.model == W3D_MM01 ? kMM01Hist
+                                 : kMM10Common + kMM10Crystal * B.ncry;
     B.bbar = c->bbar[fe];
     B.table = table_of(B.etype, B.order);
     B.q_off = q;
@@ -353,6 +399,8 @@ int create_impl(w3d_ctx* c, const w3d_model* m) {
     B.ek_off = ek;
     B.ifv_off = fv;
     B.conn_off = cn;
+    B.crys_off = ncr;
+    ncr += (int64_t)B.span * B.ncry;
     q += (int64_t)B.span * B.ngp;
     h += (int64_t)B.span * B.ngp * B.hist;
     ek += (int64_t)B.span * B.utsz;
@@ -418,6 +466,25 @@ int create_impl(w3d_ctx* c, const w3d_model* m) {
       return fail(W3D_ERR_ARG, "w3d_create: node " + std::to_string(n + 1) +
                                    " has no elements");
 
+  // mm10 crystal records of every element of the mm10 blocks
+  std::vector<CrystalDev> crys(ncr);
+  {
+    const int stride = m->angle_stride > 0 ? m->angle_stride : 1;
+    for (int b = 0; b < c->nelblk; ++b) {
+      const BlockDesc& B = c->blocks[b];
+      if (B.model != W3D_MM10) continue;
+      c->has_mm10 = true;
+      for (int i = 0; i < B.span; ++i)
+        for (int k = 0; k < B.ncry; ++k) {
+          const double* a =
+              m->angles + 3 * ((int64_t)stride * (B.felem - 1 + i) + k);
+          if (!mm10_crystal_setup(m->mats[B.mat], a,
+                                  &crys[B.crys_off + (int64_t)i * B.ncry + k]))
+            return fail(W3D_ERR_UNSUPPORTED, "w3d_create: mm10 crystal");
+        }
+    }
+  }
+
   // ---- device ----
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -426,6 +493,27 @@ int create_impl(w3d_ctx* c, const w3d_model* m) {
   if ((rc = upload(&c->d_mats, c->mats))) return rc;
   if ((rc = upload(&c->d_tables, c->tables))) return rc;
   if ((rc = upload(&c->d_conn, c->conn))) return rc;
+  if (c->has_mm10) {
+    if ((rc = upload(&c->d_crys, crys))) return rc;
+    if ((rc = dalloc(&c->d_uddt, 6 * q))) return rc;
+    // one work item per 64 consecutive points of an mm10 block
+    std::vector<WorkItem> mw;
+    for (int b = 0; b < c->nelblk; ++b) {
+      const BlockDesc& B = c->blocks[b];
+      if (B.model != W3D_MM10) continue;
+      const int np = B.span * B.ngp;
+      for (int f = 0; f < np; f += 64) {
+        WorkItem w;
+        w.blk = b;
+        w.first = f;
+        w.count = std::min(64, np - f);
+        w.pad_ = 0;
+        mw.push_back(w);
+      }
+    }
+    c->n_work_mm10 = (int)mw.size();
+    if ((rc = upload(&c->d_work_mm10, mw))) return rc;
+  }
   {
     std::vector<double> xyz(m->c, m->c + 3 * (int64_t)m->nonode);
     if ((rc = upload(&c->d_coords, xyz))) return rc;
@@ -486,25 +574,33 @@ int create_impl(w3d_ctx* c, const w3d_model* m) {
 }
 
 void prof_begin(w3d_ctx* c, int ph) {
-  if (!c->profiling) return;
-  hipEventRecord(c->ev_a[ph], c->stream);
+  if (!c->profiling || c->nseg[ph] >= 2) return;
+  hipEventRecord(c->ev_a[ph][c->nseg[ph]], c->stream);
 }
 void prof_end(w3d_ctx* c, int ph) {
-  if (!c->profiling) return;
-  hipEventRecord(c->ev_b[ph], c->stream);
-  c->pending[ph] = true;
+  if (!c->profiling || c->nseg[ph] >= 2) return;
+  hipEventRecord(c->ev_b[ph][c->nseg[ph]], c->stream);
+  c->nseg[ph] += 1;
 }
-// called after the stream has been synchronised
+// called after the stream has been synchronised: one phase call, the sum
+// of its segments
 void prof_collect(w3d_ctx* c) {
   if (!c->profiling) return;
   for (int ph = 0; ph < W3D_NUM_PHASES; ++ph) {
-    if (!c->pending[ph]) continue;
-    float ms = 0.0f;
-    if (hipEventElapsedTime(&ms, c->ev_a[ph], c->ev_b[ph]) == hipSuccess) {
-      c->prof_ms[ph] += ms;
+    if (!c->nseg[ph]) continue;
+    double tot = 0.0;
+    bool ok = true;
+    for (int k = 0; k < c->nseg[ph]; ++k) {
+      float ms = 0.0f;
+      ok = ok && hipEventElapsedTime(&ms, c->ev_a[ph][k], c->ev_b[ph][k]) ==
+                     hipSuccess;
+      tot += ms;
+    }
+    if (ok) {
+      c->prof_ms[ph] += tot;
       c->prof_n[ph] += 1;
     }
-    c->pending[ph] = false;
+    c->nseg[ph] = 0;
   }
 }
 
@@ -527,37 +623,72 @@ int recover_impl(w3d_ctx* c, int step, int iter, int extrapolated,
   const int nonlinear = (iter >= 1 || extrapolated) ? 1 : 0;
   const int want_energy = (iter > 0) ? 1 : 0;
   DevModel M = c->dev();
+  // mm10's iteration-0 linear-elastic update (rstgp1.f:4453-4457)
+  const int iter0_off = (iter == 0 && (!extrapolated || step == 1)) ? 1 : 0;
   int rc;
-  if (first_blk == 1 && nblk == c->nelblk) {
-    prof_begin(c, W3D_PHASE_RECOVER);
-    rc = launch_recover(M, c->blocks, c->d_work_rec, c->rec_groups,
-                        c->rec_start, c->rec_count, step, iter, nonlinear,
-                        d_du, want_energy, c->d_en_wg, c->stream);
-    prof_end(c, W3D_PHASE_RECOVER);
-  } else {
-    // restrict the work list to the requested blocks
+  const WorkItem* d_w = c->d_work_rec;
+  const WorkItem* d_wm = c->d_work_mm10;
+  int nwm = c->n_work_mm10;
+  std::vector<int> sg = c->rec_groups, ss = c->rec_start, sn = c->rec_count;
+  WorkItem *d_sel = nullptr, *d_selm = nullptr;
+  if (!(first_blk == 1 && nblk == c->nelblk)) {
+    // restrict the work lists to the requested blocks
+    auto in = [&](const WorkItem& w) {
+      return w.blk >= first_blk - 1 && w.blk < first_blk - 1 + nblk;
+    };
     std::vector<WorkItem> items;
     std::vector<int> g, s, n;
     build_work(c, false, items, g, s, n);
     std::vector<WorkItem> sel;
-    std::vector<int> sg, ss, sn;
+    sg.clear();
+    ss.clear();
+    sn.clear();
     for (size_t k = 0; k < g.size(); ++k) {
       int st = (int)sel.size();
       for (int i = s[k]; i < s[k] + n[k]; ++i)
-        if (items[i].blk >= first_blk - 1 && items[i].blk < first_blk - 1 + nblk)
-          sel.push_back(items[i]);
+        if (in(items[i])) sel.push_back(items[i]);
       if ((int)sel.size() > st) {
         sg.push_back(g[k]);
         ss.push_back(st);
         sn.push_back((int)sel.size() - st);
       }
     }
-    WorkItem* d_sel = nullptr;
     if ((rc = upload(&d_sel, sel))) return rc;
-    rc = launch_recover(M, c->blocks, d_sel, sg, ss, sn, step, iter,
-                        nonlinear, d_du, want_energy, c->d_en_wg, c->stream);
+    d_w = d_sel;
+    if (c->has_mm10) {
+      std::vector<WorkItem> all(c->n_work_mm10), selm;
+      HIPCHK(hipMemcpy(all.data(), c->d_work_mm10,
+                       all.size() * sizeof(WorkItem), hipMemcpyDeviceToHost));
+      for (const WorkItem& w : all)
+        if (in(w)) selm.push_back(w);
+      if ((rc = upload(&d_selm, selm))) {
+        dfree(d_sel);
+        return rc;
+      }
+      d_wm = d_selm;
+      nwm = (int)selm.size();
+    }
+  }
+  prof_begin(c, W3D_PHASE_RECOVER);
+  rc = launch_recover(M, c->blocks, d_w, sg, ss, sn, step, iter, nonlinear,
+                      d_du, want_energy, c->d_en_wg, 0, c->stream);
+  prof_end(c, W3D_PHASE_RECOVER);
+  if (!rc && nwm > 0) {
+    prof_begin(c, W3D_PHASE_MM10);
+    rc = launch_mm10(M, d_wm, nwm, step, iter0_off, c->stream);
+    prof_end(c, W3D_PHASE_MM10);
+    if (!rc) {
+      prof_begin(c, W3D_PHASE_RECOVER);
+      rc = launch_recover(M, c->blocks, d_w, sg, ss, sn, step, iter,
+                          nonlinear, d_du, want_energy, c->d_en_wg, 1,
+                          c->stream);
+      prof_end(c, W3D_PHASE_RECOVER);
+    }
+  }
+  if (d_sel || d_selm) {
     hipStreamSynchronize(c->stream);
     dfree(d_sel);
+    dfree(d_selm);
   }
   if (rc) return fail(rc, "recover: kernel launch failed");
   if (d_ifv) {
@@ -1046,13 +1177,23 @@ int w3d_set_block_state(w3d_ctx* c, int32_t blk, int32_t which,
   return W3D_OK;
 }
 
+int w3d_set_time_increment(w3d_ctx* c, double dt) {
+  if (!c) return fail(W3D_ERR_ARG, "w3d_set_time_increment: null context");
+  if (!(dt > 0.0))
+    return fail(W3D_ERR_ARG, "w3d_set_time_increment: dt must be > 0");
+  c->dt = dt;
+  return W3D_OK;
+}
+
 int w3d_set_profiling(w3d_ctx* c, int32_t on) {
   if (!c) return fail(W3D_ERR_ARG, "null context");
   HIPCHK(hipSetDevice(c->device));
-  if (on && !c->ev_a[0])
+  if (on && !c->ev_a[0][0])
     for (int ph = 0; ph < W3D_NUM_PHASES; ++ph) {
-      HIPCHK(hipEventCreate(&c->ev_a[ph]));
-      HIPCHK(hipEventCreate(&c->ev_b[ph]));
+      for (int k = 0; k < 2; ++k) {
+        HIPCHK(hipEventCreate(&c->ev_a[ph][k]));
+        HIPCHK(hipEventCreate(&c->ev_b[ph][k]));
+      }
     }
   c->profiling = on != 0;
   return W3D_OK;

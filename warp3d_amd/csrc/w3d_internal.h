@@ -24,27 +24,50 @@ constexpr int kMaxNodes = 20;
 constexpr int kMaxDof = 60;
 constexpr int kMaxGp = 27;
 constexpr int kMM01Hist = 11;
-constexpr int kMM10Hist = 158;
+// mm10 history of one point: 87 common words + 71 per crystal (mm10_d.f)
+constexpr int kMM10Common = 87;
+constexpr int kMM10Crystal = 71;
 
 // One element block as the kernels see it.
 struct BlockDesc {
   int32_t span, felem, etype, order, ngp, nnode, totdof, utsz;
   int32_t mat, model, hist, bbar;
   int32_t table;     // index of the Gauss/derivative table
-  int32_t pad_;
+  int32_t ncry;      // mm10: crystals per point
   int64_t q_off;     // first Gauss point of the block (global numbering)
   int64_t hist_off;  // first history word of the block
   int64_t ek_off;    // first Ke word of the block
   int64_t ifv_off;   // first einfvec word of the block
   int64_t conn_off;  // first connectivity entry of the block
+  int64_t crys_off;  // mm10: first crystal record of the block
 };
 
-// Material constants widened the way setup_mm01_rknstr does (REAL -> FP64).
+// mm10 (Voce) constants of a material (the c_array entry).
+struct CpDesc {
+  double rate_n, tau_y, tau_v, voce_m, theta_0, iD_v, eps_dot_0_y;
+  double atol, atol1, rtol, rtol1;
+  int32_t miter, alter_mode, angle_type, ncry;
+};
+
+// Material constants; mm01's widened the way setup_mm01_rknstr does
+// (REAL -> FP64).
 struct MatDesc {
   int32_t model;
   int32_t lnelas;
   double e, nu, beta, hprime, yld;
-  double mm10[24];
+  CpDesc cp;
+};
+
+// One crystal of one element (setup_mm10_rknstr): stiffness rotated into
+// the reference frame ((i,j) at C[i + 6j]), Schmid tensors (Voigt,
+// engineering shear) and skew vectors of the 12 slip systems, the crystal
+// -> reference rotation g ((i,j) at g[i + 3j]) and the initial angles.
+struct CrystalDev {
+  double C[36];
+  double ms[12][6];
+  double qs[12][3];
+  double g[9];
+  double angles[3];
 };
 
 // Gauss points and parametric derivatives of one (element type, order).
@@ -81,16 +104,25 @@ struct DevModel {
   double* einfvec;
   double* evol;
   int32_t* flags;         // [0] fatal numeric, [1] cut step, [2] det J <= 0
+  const CrystalDev* crys; // mm10 crystal records
+  double* uddt;           // mm10: strain increments (6, points) per block
   double eps_bbar;
+  double dt;              // time increment of the load step
 };
 
 // Kernel launchers (w3d_kernels.hip, w3d_tangent.hip).
+// Recovery passes: pass 0 runs mm01 blocks in full and the strain
+// increments of mm10 blocks; pass 1 forms the element forces of mm10
+// blocks from the stresses their material update left.
 int launch_recover(const DevModel& M, const std::vector<BlockDesc>& hblocks,
                    const WorkItem* d_work, const std::vector<int>& groups,
                    const std::vector<int>& group_start,
                    const std::vector<int>& group_count, int step, int iter,
                    int nonlinear, const double* d_du, int want_energy,
-                   double* en_wg, hipStream_t s);
+                   double* en_wg, int pass, hipStream_t s);
+// mm10 update, one thread per point over work items of <= 64 points.
+int launch_mm10(const DevModel& M, const WorkItem* d_work, int nwork,
+                int step, int iter0_off, hipStream_t s);
 int launch_ifv_gather(const double* einfvec, const int64_t* inv_ptr,
                       const int64_t* inv_off, const int32_t* inv_nn,
                       int nonode, double* ifv, hipStream_t s);
@@ -114,8 +146,14 @@ __host__ __device__ constexpr int tan_mfma_chunk(int nnode) {
   return 8 * tan_mfma_waves(nnode);
 }
 
-// Kernel-group keys: one instantiation per (nnode, ngp, bbar).
-int group_key(int nnode, int ngp, int bbar);
+// Host set-up of one crystal record from the material and its angles
+// (w3d_mm10_setup.cpp); false for an unsupported slip / elastic type.
+bool mm10_crystal_setup(const w3d_material& m, const double* angles,
+                        CrystalDev* cr);
+
+// Kernel-group keys: one instantiation per (nnode, ngp, bbar) and, for the
+// recovery, per material family (mm10 blocks run in two passes).
+int group_key(int nnode, int ngp, int bbar, int mm10 = 0);
 
 // Lower-triangle index of (row, col) in the 21-word [D] storage
 // (rstgp1_store_cep order (1,1),(2,1),(2,2),(3,1),...).

@@ -244,8 +244,11 @@ struct BCol {
 // ---------------------------------------------------------------------
 // Recovery: one workgroup = E consecutive elements of one block, lanes are
 // (element, Gauss point) in phases 1-3 and (element, dof) in phase 4.
+// MODE 0: mm01 blocks, everything fused.  MODE 1: mm10 blocks, phases 1-3
+// up to the strain increment (left in uddt for the mm10 kernel).  MODE 2:
+// mm10 blocks, energy and element forces from the updated stresses.
 // ---------------------------------------------------------------------
-template <int NN, int G, bool BBAR>
+template <int NN, int G, bool BBAR, int MODE>
 __global__ void __launch_bounds__(256)
     k_recover(DevModel M, const WorkItem* __restrict__ work, int step,
               int iter, int nonlinear, const double* __restrict__ du,
@@ -283,9 +286,11 @@ __global__ void __launch_bounds__(256)
     s_ce[e][j] = M.coords[3 * node];
     s_ce[e][NN + j] = M.coords[3 * node + 1];
     s_ce[e][2 * NN + j] = M.coords[3 * node + 2];
-    s_du[e][j] = du[3 * node];
-    s_du[e][NN + j] = du[3 * node + 1];
-    s_du[e][2 * NN + j] = du[3 * node + 2];
+    if (MODE != 2) {
+      s_du[e][j] = du[3 * node];
+      s_du[e][NN + j] = du[3 * node + 1];
+      s_du[e][2 * NN + j] = du[3 * node + 2];
+    }
   }
   __syncthreads();
 
@@ -349,7 +354,14 @@ __global__ void __launch_bounds__(256)
 
   // phase 3: strain increment, material update, state write-back
   double sig[8];
-  if (active) {
+  if (MODE == 2 && active) {  // stresses left by the mm10 update
+    const int64_t SG = (int64_t)bd.span * G;
+    const int64_t q = (int64_t)(wi.first + e) * G + g;
+    const double* sn1 = M.stress_n1 + 9 * bd.q_off + q;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) sig[k] = sn1[k * SG];
+  }
+  if (MODE != 2 && active) {
     const double* ga = s_gam[e][g];
     double deps[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll 2
@@ -387,11 +399,18 @@ __global__ void __launch_bounds__(256)
     double* cp = M.cep + 21 * bd.q_off + q;
 #pragma unroll
     for (int k = 0; k < 6; ++k) en1[k * SG] = en[k * SG] + deps[k];
+    if (MODE == 1) {
+      double* ud = M.uddt + 6 * bd.q_off + q;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) ud[k * SG] = deps[k];
+    }
     double cgn[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) cgn[k] = sn[k * SG];
     const MatDesc md = M.mats[bd.mat];
-    if (nonlinear) {
+    if (MODE == 1) {
+      // the material update is the mm10 kernel's
+    } else if (nonlinear) {
       double h[kMM01Hist];
       double* hn = M.hist_n + bd.hist_off + q;
       double* hn1 = M.hist_n1 + bd.hist_off + q;
@@ -431,6 +450,7 @@ __global__ void __launch_bounds__(256)
         for (int c = 0; c <= r; ++c) cp[tri(r, c) * SG] = dm[r][c];
     }
   }
+  if (MODE == 1) return;  // uniform across the workgroup
   __syncthreads();  // increments no longer read: their LDS takes stresses
   if (active) {
 #pragma unroll
@@ -582,8 +602,8 @@ __global__ void k_assemble(const KT* __restrict__ ek,
 // ---------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------
-int group_key(int nnode, int ngp, int bbar) {
-  return (nnode * 100 + ngp) * 2 + (bbar ? 1 : 0);
+int group_key(int nnode, int ngp, int bbar, int mm10) {
+  return ((nnode * 100 + ngp) * 2 + (bbar ? 1 : 0)) * 2 + (mm10 ? 1 : 0);
 }
 
 #define W3D_GROUPS(X)    \
@@ -602,18 +622,30 @@ int launch_recover(const DevModel& M, const std::vector<BlockDesc>& hblocks,
                    const std::vector<int>& gstart,
                    const std::vector<int>& gcount, int step, int iter,
                    int nonlinear, const double* d_du, int want_energy,
-                   double* en_wg, hipStream_t s) {
+                   double* en_wg, int pass, hipStream_t s) {
   (void)hblocks;
   for (size_t k = 0; k < groups.size(); ++k) {
     if (gcount[k] == 0) continue;
+    const bool mm10 = groups[k] & 1;
+    if (pass == 1 && !mm10) continue;
+    const int mode = mm10 ? (pass == 0 ? 1 : 2) : 0;
     const WorkItem* w = d_work + gstart[k];
     bool done = false;
-#define X(NN, G, BB)                                                       \
-  if (!done && groups[k] == group_key(NN, G, BB)) {                         \
-    hipLaunchKernelGGL((k_recover<NN, G, BB>), dim3(gcount[k]), dim3(256), \
-                       0, s, M, w, step, iter, nonlinear, d_du,            \
-                       want_energy, en_wg, gstart[k]);                     \
-    done = true;                                                           \
+#define X(NN, G, BB)                                                        \
+  if (!done && (groups[k] >> 1) == (group_key(NN, G, BB) >> 1)) {          \
+    if (mode == 0)                                                          \
+      hipLaunchKernelGGL((k_recover<NN, G, BB, 0>), dim3(gcount[k]),        \
+                         dim3(256), 0, s, M, w, step, iter, nonlinear, d_du, \
+                         want_energy, en_wg, gstart[k]);                    \
+    else if (mode == 1)                                                     \
+      hipLaunchKernelGGL((k_recover<NN, G, BB, 1>), dim3(gcount[k]),        \
+                         dim3(256), 0, s, M, w, step, iter, nonlinear, d_du, \
+                         want_energy, en_wg, gstart[k]);                    \
+    else                                                                    \
+      hipLaunchKernelGGL((k_recover<NN, G, BB, 2>), dim3(gcount[k]),        \
+                         dim3(256), 0, s, M, w, step, iter, nonlinear, d_du, \
+                         want_energy, en_wg, gstart[k]);                    \
+    done = true;                                                            \
   }
     W3D_GROUPS(X)
 #undef X

@@ -1,0 +1,1118 @@
+// w3d_mm10.hip -- gfx950 kernel of the mm10 crystal-plasticity update
+// (Voce hardening, 12 fcc or bcc slip systems), one thread per Gauss point.
+//
+// Reference routines restated (file:line under the reference's src/):
+//   rstgp1.f:4327 drive_10_update (small strain: R = I; the iteration-0
+//   "extrapolate off" linear-elastic update), mm10_a.f:29 mm10 (crystal
+//   loop, averages, store), :480 mm10_init_cc_hist0, :1777/:1808/:2605
+//   step-1 history initialisation, :2493 mm10_copy_cc_hist, :1689
+//   mm10_setup_np1, :852 mm10_setup, :2086 mm10_setup_voche, :1102
+//   mm10_solve_crystal, :2657 mm10_solve_strup (sub-stepping), :2901
+//   mm10_solve (predictor :3032, corrector :3224), :664 mm10_tangent,
+//   :3991 symmetrisation, :3375 mm10_update_rotation, :3447 expmw3x3,
+//   :3493 mm10_output, :1195 mm10_update_euler_angles, :998
+//   mm10_store_cryhist; mm10_b.f:63 formR2, :177 formJ11, :281 formJ12,
+//   :377 formJ21, :439 formJ22, :1073 formR1, :1169 form_dbarp, :1239
+//   form_wbarp, :1312 form_wp, :1513 symsw, :1613 iw, :1813 slipinc,
+//   :1845 rs, :1896-2074 the Voce functions (with ehard's extra
+//   sign(slip) factor on its rate term, :1978-1983, kept as written).
+//
+// Every array of the local problem has compile-time indices, so the whole
+// solve stays in registers; the pivoting of the small LU factorisations is
+// done with selects instead of indexed row swaps.  Operation order follows
+// the reference loop by loop (LAPACK/BLAS calls in their unblocked
+// reference-BLAS order), except that each slip system's resolved shear is
+// formed once per residual evaluation and reused (bitwise the value the
+// reference recomputes).
+#include <hip/hip_runtime.h>
+
+#include "w3d_internal.h"
+
+namespace w3d {
+namespace {
+
+constexpr int NS = 12;
+
+struct Props {
+  const CrystalDev* cr;
+  double rate_n, tau_y, tau_v, voce_m, theta_0, iD_v, eps_dot_0_y;
+  double atol, atol1, rtol, rtol1;
+  int miter, alter_mode, angle_type;
+};
+
+// crystal_state of one crystal (the Voce subset).  The slip tensors
+// (mm10_setup's ms, qs; qc equals qs bitwise for R = I) depend only on the
+// plastic rotation at n, so every sub-step shares one copy.
+// The tensors live in LDS, one column per lane (word w of lane t at
+// g[w * kLanes + t]), so that the local problem's registers stay free.
+constexpr int kLanes = 64;
+struct CState {
+  double D[6];
+  double tt_rate, tinc, dg;
+  const double* g;  // 72 ms words then 36 qs words, lane-strided
+  __device__ double ms(int i, int k) const { return g[(6 * i + k) * kLanes]; }
+  __device__ double qs(int i, int k) const {
+    return g[(72 + 3 * i + k) * kLanes];
+  }
+};
+
+__device__ inline double sgn(double x) { return x >= 0.0 ? 1.0 : -1.0; }
+
+// [b] c with b 3x3 column-major
+__device__ inline void mv3(double* a, const double* b, const double* c) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    a[i] = b[i] * c[0] + b[i + 3] * c[1] + b[i + 6] * c[2];
+}
+
+// mm10_rt2rve of rt (3x3 column-major) -> rv (6x6 column-major)
+__device__ inline void rt2rve(const double* rt, double* rv) {
+#define RT(i, j) rt[((i)-1) + 3 * ((j)-1)]
+#define RV(i, j) rv[((i)-1) + 6 * ((j)-1)]
+  RV(1, 1) = RT(1, 1) * RT(1, 1);
+  RV(1, 2) = RT(1, 2) * RT(1, 2);
+  RV(1, 3) = RT(1, 3) * RT(1, 3);
+  RV(1, 4) = 2.0 * RT(1, 1) * RT(1, 2);
+  RV(1, 5) = 2.0 * RT(1, 3) * RT(1, 2);
+  RV(1, 6) = 2.0 * RT(1, 1) * RT(1, 3);
+  RV(2, 1) = RT(2, 1) * RT(2, 1);
+  RV(2, 2) = RT(2, 2) * RT(2, 2);
+  RV(2, 3) = RT(2, 3) * RT(2, 3);
+  RV(2, 4) = 2.0 * RT(2, 1) * RT(2, 2);
+  RV(2, 5) = 2.0 * RT(2, 3) * RT(2, 2);
+  RV(2, 6) = 2.0 * RT(2, 1) * RT(2, 3);
+  RV(3, 1) = RT(3, 1) * RT(3, 1);
+  RV(3, 2) = RT(3, 2) * RT(3, 2);
+  RV(3, 3) = RT(3, 3) * RT(3, 3);
+  RV(3, 4) = 2.0 * RT(3, 1) * RT(3, 2);
+  RV(3, 5) = 2.0 * RT(3, 3) * RT(3, 2);
+  RV(3, 6) = 2.0 * RT(3, 1) * RT(3, 3);
+  RV(4, 1) = RT(1, 1) * RT(2, 1);
+  RV(4, 2) = RT(1, 2) * RT(2, 2);
+  RV(4, 3) = RT(1, 3) * RT(2, 3);
+  RV(4, 4) = RT(1, 1) * RT(2, 2) + RT(2, 1) * RT(1, 2);
+  RV(4, 5) = RT(1, 2) * RT(2, 3) + RT(1, 3) * RT(2, 2);
+  RV(4, 6) = RT(1, 1) * RT(2, 3) + RT(1, 3) * RT(2, 1);
+  RV(5, 1) = RT(2, 1) * RT(3, 1);
+  RV(5, 2) = RT(3, 2) * RT(2, 2);
+  RV(5, 3) = RT(2, 3) * RT(3, 3);
+  RV(5, 4) = RT(2, 1) * RT(3, 2) + RT(2, 2) * RT(3, 1);
+  RV(5, 5) = RT(2, 2) * RT(3, 3) + RT(3, 2) * RT(2, 3);
+  RV(5, 6) = RT(2, 1) * RT(3, 3) + RT(2, 3) * RT(3, 1);
+  RV(6, 1) = RT(1, 1) * RT(3, 1);
+  RV(6, 2) = RT(1, 2) * RT(3, 2);
+  RV(6, 3) = RT(1, 3) * RT(3, 3);
+  RV(6, 4) = RT(1, 1) * RT(3, 2) + RT(1, 2) * RT(3, 1);
+  RV(6, 5) = RT(1, 2) * RT(3, 3) + RT(1, 3) * RT(3, 2);
+  RV(6, 6) = RT(1, 1) * RT(3, 3) + RT(3, 1) * RT(1, 3);
+#undef RV
+}
+
+// mm10_rt2rvw
+__device__ inline void rt2rvw(const double* rt, double* rv) {
+#define RW(i, j) rv[((i)-1) + 3 * ((j)-1)]
+  RW(1, 1) = RT(2, 2) * RT(3, 3) - RT(2, 3) * RT(3, 2);
+  RW(1, 2) = RT(2, 1) * RT(3, 3) - RT(2, 3) * RT(3, 1);
+  RW(1, 3) = RT(2, 1) * RT(3, 2) - RT(2, 2) * RT(3, 1);
+  RW(2, 1) = RT(1, 2) * RT(3, 3) - RT(1, 3) * RT(3, 2);
+  RW(2, 2) = RT(1, 1) * RT(3, 3) - RT(1, 3) * RT(3, 1);
+  RW(2, 3) = RT(1, 1) * RT(3, 2) - RT(1, 2) * RT(3, 1);
+  RW(3, 1) = RT(1, 2) * RT(2, 3) - RT(1, 3) * RT(2, 2);
+  RW(3, 2) = RT(1, 1) * RT(2, 3) - RT(1, 3) * RT(2, 1);
+  RW(3, 3) = RT(1, 1) * RT(2, 2) - RT(1, 2) * RT(2, 1);
+#undef RW
+#undef RT
+}
+
+// LU with partial pivoting (first largest |pivot|) and the solve of
+// nrhs right-hand sides, column-major with leading dimension N.
+template <int N, int NRHS>
+__device__ inline void lu_solve(double* a, double* b) {
+  int piv[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    int p = j;
+    double amax = fabs(a[j + N * j]);
+#pragma unroll
+    for (int i = j + 1; i < N; ++i) {
+      double v = fabs(a[i + N * j]);
+      if (v > amax) {
+        amax = v;
+        p = i;
+      }
+    }
+    piv[j] = p;
+#pragma unroll
+    for (int i = j + 1; i < N; ++i) {
+      if (i == p) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          double t = a[j + N * k];
+          a[j + N * k] = a[i + N * k];
+          a[i + N * k] = t;
+        }
+      }
+    }
+    double r = 1.0 / a[j + N * j];
+#pragma unroll
+    for (int i = j + 1; i < N; ++i) a[i + N * j] = a[i + N * j] * r;
+#pragma unroll
+    for (int k = j + 1; k < N; ++k) {
+      double t = -a[j + N * k];
+      if (t != 0.0) {
+#pragma unroll
+        for (int i = j + 1; i < N; ++i)
+          a[i + N * k] = a[i + N * k] + a[i + N * j] * t;
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < NRHS; ++c) {
+    double* x = b + N * c;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+#pragma unroll
+      for (int i = j + 1; i < N; ++i) {
+        if (i == piv[j]) {
+          double t = x[j];
+          x[j] = x[i];
+          x[i] = t;
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < N; ++k)
+      if (x[k] != 0.0) {
+#pragma unroll
+        for (int i = k + 1; i < N; ++i) x[i] = x[i] - x[k] * a[i + N * k];
+      }
+#pragma unroll
+    for (int k = N - 1; k >= 0; --k)
+      if (x[k] != 0.0) {
+        x[k] = x[k] / a[k + N * k];
+#pragma unroll
+        for (int i = 0; i < k; ++i) x[i] = x[i] - x[k] * a[i + N * k];
+      }
+  }
+}
+
+// DPOSV('U', 6, 1) with the stiffness
+__device__ inline void chol_solve6(const double* __restrict__ ain, double* b) {
+  double a[36];
+#pragma unroll
+  for (int k = 0; k < 36; ++k) a[k] = ain[k];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < j; ++k) s = s + a[k + 6 * j] * a[k + 6 * j];
+    double ajj = sqrt(a[j + 6 * j] - s);
+    a[j + 6 * j] = ajj;
+#pragma unroll
+    for (int c = j + 1; c < 6; ++c) {
+      double t = 0.0;
+#pragma unroll
+      for (int k = 0; k < j; ++k) t = t + a[k + 6 * j] * a[k + 6 * c];
+      a[j + 6 * c] = (a[j + 6 * c] - t) / ajj;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    double t = b[j];
+#pragma unroll
+    for (int k = 0; k < j; ++k) t = t - a[k + 6 * j] * b[k];
+    b[j] = t / a[j + 6 * j];
+  }
+#pragma unroll
+  for (int j = 5; j >= 0; --j) {
+    b[j] = b[j] / a[j + 6 * j];
+#pragma unroll
+    for (int i = 0; i < j; ++i) b[i] = b[i] - b[j] * a[i + 6 * j];
+  }
+}
+
+// mm10_setup: effective strain increment (k_0 = 0: no geometric hardening)
+__device__ inline void setup(const Props& p, CState& s) {
+  const double twothirds = 2.0 / 3.0;
+  double t1 = s.D[0] * s.D[0] + s.D[1] * s.D[1] + s.D[2] * s.D[2];
+  double t2 = s.D[3] * s.D[3] + s.D[4] * s.D[4] + s.D[5] * s.D[5];
+  s.dg = sqrt(twothirds * (t1 + 0.5 * t2));
+  if (p.alter_mode) s.dg = p.eps_dot_0_y * s.tinc;
+}
+
+// mm10_setup: the slip tensors rotated by the plastic rotation at n
+__device__ inline void slip_geometry(const Props& p, const double* Rpn,
+                                     double* gl) {
+  double tn[9], RE[36], RW[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) tn[i + 3 * j] = Rpn[j + 3 * i];
+  rt2rve(tn, RE);
+  rt2rvw(tn, RW);
+#pragma unroll
+  for (int i = 0; i < NS; ++i) {
+    const double* m0 = p.cr->ms[i];
+    double m[6], q[3];
+#pragma unroll
+    for (int r = 0; r < 6; ++r) m[r] = RE[r] * m0[0];
+#pragma unroll
+    for (int j = 1; j < 6; ++j)
+#pragma unroll
+      for (int r = 0; r < 6; ++r) m[r] = m[r] + RE[r + 6 * j] * m0[j];
+    mv3(q, RW, p.cr->qs[i]);
+#pragma unroll
+    for (int r = 0; r < 6; ++r) gl[(6 * i + r) * kLanes] = m[r];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) gl[(72 + 3 * i + r) * kLanes] = q[r];
+  }
+}
+
+__device__ inline double rs_of(const CState& s, const double* x, int i) {
+  return x[0] * s.ms(i, 0) + x[1] * s.ms(i, 1) + x[2] * s.ms(i, 2) +
+         x[3] * s.ms(i, 3) + x[4] * s.ms(i, 4) + x[5] * s.ms(i, 5);
+}
+
+__device__ inline void symsw(const double* s, const double* w, double* sw) {
+  sw[0] = s[3] * w[2] - s[5] * w[1];
+  sw[1] = s[3] * w[2] - s[4] * w[0];
+  sw[2] = s[5] * w[1] + s[4] * w[0];
+  sw[3] = 0.5 * (w[2] * (s[0] - s[1]) + w[0] * s[5] - w[1] * s[4]);
+  sw[4] = 0.5 * (w[0] * (s[1] - s[2]) + w[1] * s[3] + w[2] * s[5]);
+  sw[5] = 0.5 * (w[1] * (s[0] - s[2]) + w[0] * s[3] - w[2] * s[4]);
+}
+
+// resolved shears and slip increments at x (stress) and tau
+__device__ inline void slips(const Props& p, const CState& s, const double* x,
+                             double tt, double* rs, double* gam) {
+#pragma unroll
+  for (int i = 0; i < NS; ++i) {
+    rs[i] = rs_of(s, x, i);
+    gam[i] = s.dg / tt * pow(fabs(rs[i] / tt), p.rate_n - 1.0) * rs[i];
+  }
+}
+
+// form_wp / form_wbarp (qc == qs)
+__device__ inline void form_w(const Props& p, const CState& s,
+                              const double* rs, const double* gam, double* w) {
+  w[0] = w[1] = w[2] = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; ++i) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) w[k] = w[k] + gam[i] * s.qs(i, k);
+    double c = rs[i] * s.tinc * p.iD_v;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) w[k] = w[k] + c * s.qs(i, k);
+  }
+}
+
+// formR1 at x with tau tt
+__device__ inline void formR1(const Props& p, const CState& s,
+                              const double* sn, const double* x, double tt,
+                              double* R1) {
+  double rs[NS], gam[NS], dbar[6], wp[3], sw[6], w1[6];
+  slips(p, s, x, tt, rs, gam);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) dbar[k] = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; ++i) {
+    double c = rs[i] * s.tinc * p.iD_v + gam[i];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) dbar[k] = dbar[k] + c * s.ms(i, k);
+  }
+  form_w(p, s, rs, gam, wp);
+  symsw(x, wp, sw);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) w1[k] = s.D[k] - dbar[k];
+  const double* C = p.cr->C;
+  double w2[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) w2[i] = C[i] * w1[0];
+#pragma unroll
+  for (int j = 1; j < 6; ++j)
+#pragma unroll
+    for (int i = 0; i < 6; ++i) w2[i] = w2[i] + C[i + 6 * j] * w1[j];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) R1[k] = x[k] - sn[k] - w2[k] + 2.0 * sw[k];
+}
+
+// h_voche (tau_l = 0)
+__device__ inline double h_voche(const Props& p, const double* gam, double tt,
+                                 double tau_n) {
+  double h = 0.0;
+  double ht = 1.0 - (tt - p.tau_y) / p.tau_v + 0.0 / (tt - p.tau_y);
+  double f = pow(fabs(ht), p.voce_m) * sgn(ht);
+#pragma unroll
+  for (int i = 0; i < NS; ++i) h = h + f * fabs(gam[i]);
+  return tau_n + p.theta_0 * h;
+}
+
+// formR (7 residuals); records the hardening rate
+__device__ inline void formR(const Props& p, CState& s, const double* sn,
+                             double tau_n, const double* x, double* R) {
+  formR1(p, s, sn, x, x[6], R);
+  double rs[NS], gam[NS];
+  slips(p, s, x, x[6], rs, gam);
+  double h = h_voche(p, gam, x[6], tau_n);
+  R[6] = x[6] - h;
+  s.tt_rate = (h - tau_n) / s.tinc;
+}
+
+// formJ11 (6x6, column-major)
+__device__ inline void formJ11(const Props& p, const CState& s,
+                               const double* x, double tt, double* J11) {
+  double rs[NS], gam[NS];
+  slips(p, s, x, tt, rs, gam);
+  const double* C = p.cr->C;
+#pragma unroll
+  for (int k = 0; k < 36; ++k) J11[k] = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; ++i) {
+    double d = pow(fabs(rs[i]), p.rate_n - 1.0);
+    d = s.dg * p.rate_n / pow(tt, p.rate_n) * d;
+    double dgdt = d + s.tinc * p.iD_v;
+    double st[6], wv[6];
+    double qv[3] = {s.qs(i, 0), s.qs(i, 1), s.qs(i, 2)};
+    symsw(x, qv, st);
+#pragma unroll
+    for (int r = 0; r < 6; ++r) wv[r] = 2.0 * st[r];
+#pragma unroll
+    for (int j = 0; j < 6; ++j)
+#pragma unroll
+      for (int r = 0; r < 6; ++r) wv[r] = wv[r] + C[r + 6 * j] * s.ms(i, j);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      double t = dgdt * s.ms(i, c);
+      if (s.ms(i, c) != 0.0) {
+#pragma unroll
+        for (int r = 0; r < 6; ++r) J11[r + 6 * c] = J11[r + 6 * c] + wv[r] * t;
+      }
+    }
+  }
+  double wp[3];
+  form_w(p, s, rs, gam, wp);
+  // + mm10_iw(wp), then + I
+  J11[0 + 6 * 3] = J11[0 + 6 * 3] + 2.0 * wp[2];
+  J11[0 + 6 * 5] = J11[0 + 6 * 5] + -2.0 * wp[1];
+  J11[1 + 6 * 3] = J11[1 + 6 * 3] + 2.0 * wp[2];
+  J11[1 + 6 * 4] = J11[1 + 6 * 4] + -2.0 * wp[0];
+  J11[2 + 6 * 4] = J11[2 + 6 * 4] + 2.0 * wp[0];
+  J11[2 + 6 * 5] = J11[2 + 6 * 5] + 2.0 * wp[1];
+  J11[3 + 6 * 0] = J11[3 + 6 * 0] + wp[2];
+  J11[3 + 6 * 1] = J11[3 + 6 * 1] + -wp[2];
+  J11[3 + 6 * 4] = J11[3 + 6 * 4] + -wp[1];
+  J11[3 + 6 * 5] = J11[3 + 6 * 5] + wp[0];
+  J11[4 + 6 * 1] = J11[4 + 6 * 1] + wp[0];
+  J11[4 + 6 * 2] = J11[4 + 6 * 2] + -wp[0];
+  J11[4 + 6 * 3] = J11[4 + 6 * 3] + wp[1];
+  J11[4 + 6 * 5] = J11[4 + 6 * 5] + wp[2];
+  J11[5 + 6 * 0] = J11[5 + 6 * 0] + wp[1];
+  J11[5 + 6 * 2] = J11[5 + 6 * 2] + -wp[1];
+  J11[5 + 6 * 3] = J11[5 + 6 * 3] + wp[0];
+  J11[5 + 6 * 4] = J11[5 + 6 * 4] + -wp[2];
+  // the zero entries of iw are added too: x + 0.0 == x
+#pragma unroll
+  for (int i = 0; i < 6; ++i) J11[i + 6 * i] = J11[i + 6 * i] + 1.0;
+}
+
+// formJ: the 7x7 Jacobian (column-major, leading dimension 7)
+__device__ inline void formJ(const Props& p, const CState& s, const double* x,
+                             double* J) {
+  const double tt = x[6];
+  double J11[36];
+  formJ11(p, s, x, tt, J11);
+  double rs[NS], gam[NS];
+  slips(p, s, x, tt, rs, gam);
+  const double* C = p.cr->C;
+  double J12[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int i = 0; i < NS; ++i) {
+    double st[6], tm[6];
+    double qv[3] = {s.qs(i, 0), s.qs(i, 1), s.qs(i, 2)};
+    symsw(x, qv, st);
+    double dgdh = -p.rate_n / tt * gam[i];
+#pragma unroll
+    for (int r = 0; r < 6; ++r) tm[r] = 2.0 * st[r];
+#pragma unroll
+    for (int j = 0; j < 6; ++j)
+#pragma unroll
+      for (int r = 0; r < 6; ++r) tm[r] = tm[r] + C[r + 6 * j] * s.ms(i, j);
+    if (dgdh != 0.0) {
+#pragma unroll
+      for (int r = 0; r < 6; ++r) J12[r] = J12[r] + tm[r] * (1.0 * dgdh);
+    }
+  }
+  const double ht = 1.0 - (tt - p.tau_y) / p.tau_v + 0.0 / (tt - p.tau_y);
+  const double hm = pow(fabs(ht), p.voce_m);
+  double et[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int i = 0; i < NS; ++i) {
+    double c = hm * sgn(ht) * pow(fabs(rs[i]), p.rate_n - 2.0) * rs[i];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) et[k] = et[k] + c * s.ms(i, k);
+  }
+  const double sc = p.theta_0 * s.dg * p.rate_n / pow(tt, p.rate_n);
+  double etau = 0.0;
+  const double dty = tt - p.tau_y;
+#pragma unroll
+  for (int i = 0; i < NS; ++i)
+    etau = etau + (p.voce_m * (-1.0 / p.tau_v - 0.0 / (dty * dty)) *
+                       fabs(gam[i]) / fabs(ht) -
+                   fabs(gam[i]) * p.rate_n / tt * sgn(ht) * sgn(gam[i])) *
+                      hm;
+  etau = p.theta_0 * etau;
+  etau = 1.0 - etau;
+#pragma unroll
+  for (int c = 0; c < 6; ++c)
+#pragma unroll
+    for (int r = 0; r < 6; ++r) J[r + 7 * c] = J11[r + 6 * c];
+#pragma unroll
+  for (int r = 0; r < 6; ++r) J[r + 42] = J12[r];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) J[6 + 7 * c] = -(sc * et[c]);
+  J[48] = etau;
+}
+
+__device__ inline bool any_nan(const double* x, int n) {
+  bool b = false;
+  for (int i = 0; i < n; ++i) b |= isnan(x[i]);
+  return b;
+}
+
+__device__ inline double dot6(const double* a, const double* b) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3] + a[4] * b[4] +
+         a[5] * b[5];
+}
+
+// mm10_solve: predictor (6x6 Newton at fixed extrapolated tau) then the
+// 7x7 corrector; J receives the Jacobian of the last corrector iteration
+__device__ void solve(const Props& p, CState& s, const double* sn,
+                      double tau_n, double tt_rate_n, const double* Dn,
+                      double* stress, double* tt, bool& fail, double dtinc,
+                      double* J) {
+  const double c = 1.0e-4, red = 0.5;
+  const int mls = 10, mmin = 1;
+  double x[7], inR1;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) x[k] = stress[k];
+  x[6] = *tt;
+  {  // predictor
+    double x1[6], R1[6], d1[6], d2[6], t1, t2, dtrace;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) x1[k] = x[k];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) d1[k] = s.D[k];
+    dtrace = (d1[0] + d1[1] + d1[2]) / 3.0;
+    d1[0] = d1[0] - dtrace;
+    d1[1] = d1[1] - dtrace;
+    d1[2] = d1[2] - dtrace;
+    t1 = d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2];
+    t2 = d1[3] * d1[3] + d1[4] * d1[4] + d1[5] * d1[5];
+    if (t1 + t2 == 0.0) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) d1[k] = 0.0;
+    } else {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) d1[k] = d1[k] / sqrt(t1 + t2);
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) d2[k] = Dn[k];
+    dtrace = (d2[0] + d2[1] + d2[2]) / 3.0;
+    d2[0] = d2[0] - dtrace;
+    d2[1] = d2[1] - dtrace;
+    d2[2] = d2[2] - dtrace;
+    t1 = d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2];
+    t2 = d2[3] * d2[3] + d2[4] * d2[4] + d2[5] * d2[5];
+    if (t1 + t2 == 0.0) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) d2[k] = 0.0;
+    } else {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) d2[k] = d2[k] / sqrt(t1 + t2);
+    }
+    t1 = d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2];
+    t2 = d1[3] * d2[3] + d1[4] * d2[4] + d1[5] * d2[5];
+    double cos_ang = t1 + t2 > 0.0 ? t1 + t2 : 0.0;
+    double x2 = x[6] + cos_ang * tt_rate_n * dtinc;
+    formR1(p, s, sn, x1, x2, R1);
+    double nR1 = sqrt(R1[0] * R1[0] + R1[1] * R1[1] + R1[2] * R1[2] +
+                      R1[3] * R1[3] + R1[4] * R1[4] + R1[5] * R1[5]);
+    inR1 = nR1;
+    bool ok = true;
+    int iter = 0;
+    while (nR1 > p.atol1 && nR1 / inR1 > p.rtol1) {
+      double J11[36], mJ[36], dx1[6], wv[6];
+      formJ11(p, s, x1, x2, J11);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) dx1[k] = R1[k];
+#pragma unroll
+      for (int k = 0; k < 36; ++k) mJ[k] = -J11[k];
+      lu_solve<6, 1>(mJ, dx1);
+      double alpha = 1.0;
+      double dotR1 = R1[0] * R1[0] + R1[1] * R1[1] + R1[2] * R1[2] +
+                     R1[3] * R1[3] + R1[4] * R1[4] + R1[5] * R1[5];
+      double ls1 = 0.5 * dotR1;
+#pragma unroll
+      for (int j = 0; j < 6; ++j)
+        wv[j] = J11[0 + 6 * j] * R1[0] + J11[1 + 6 * j] * R1[1] +
+                J11[2 + 6 * j] * R1[2] + J11[3 + 6 * j] * R1[3] +
+                J11[4 + 6 * j] * R1[4] + J11[5 + 6 * j] * R1[5];
+      double ls2 = c * dot6(dx1, wv);
+      int ls = 0;
+      for (;;) {
+        double nlsx = ls1 + ls2 * alpha, xn[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) xn[k] = x1[k] + alpha * dx1[k];
+        formR1(p, s, sn, xn, x2, R1);
+        dotR1 = R1[0] * R1[0] + R1[1] * R1[1] + R1[2] * R1[2] +
+                R1[3] * R1[3] + R1[4] * R1[4] + R1[5] * R1[5];
+        nR1 = sqrt(dotR1);
+        double nRs = 0.5 * dotR1;
+        if (nRs <= nlsx || ls > mls) {
+#pragma unroll
+          for (int k = 0; k < 6; ++k) x1[k] = xn[k];
+          break;
+        }
+        alpha = red * alpha;
+        ls = ls + 1;
+      }
+      iter = iter + 1;
+      if (iter > p.miter || any_nan(x1, 6)) {
+        fail = true;
+        ok = false;
+        break;
+      }
+    }
+    if (ok) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) x[k] = x1[k];
+      x[6] = x2;
+    }
+  }
+  {  // corrector
+    int iter = 0;
+    double R[7];
+    formR(p, s, sn, tau_n, x, R);
+    double dotR = 0.0;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) dotR = dotR + R[k] * R[k];
+    double nR = sqrt(dotR), inR = nR;
+    if (inR == 0.0) inR = inR1;
+    while ((nR > p.atol && nR / inR > p.rtol) || iter < mmin) {
+      double mJ[49], dx[7], wv[7];
+      formJ(p, s, x, J);
+#pragma unroll
+      for (int k = 0; k < 7; ++k) dx[k] = R[k];
+#pragma unroll
+      for (int k = 0; k < 49; ++k) mJ[k] = -J[k];
+      lu_solve<7, 1>(mJ, dx);
+      double alpha = 1.0;
+      dotR = 0.0;
+#pragma unroll
+      for (int k = 0; k < 7; ++k) dotR = dotR + R[k] * R[k];
+      double ls1 = 0.5 * dotR;
+#pragma unroll
+      for (int j = 0; j < 7; ++j) {
+        double t = 0.0;
+#pragma unroll
+        for (int l = 0; l < 7; ++l) t = t + J[l + 7 * j] * R[l];
+        wv[j] = t;
+      }
+      double dd = 0.0;
+#pragma unroll
+      for (int k = 0; k < 7; ++k) dd = dd + dx[k] * wv[k];
+      double ls2 = c * dd;
+      int ls = 0;
+      for (;;) {
+        double nlsx = ls1 + ls2 * alpha, xn[7];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) xn[k] = x[k] + alpha * dx[k];
+        formR(p, s, sn, tau_n, xn, R);
+        dotR = 0.0;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) dotR = dotR + R[k] * R[k];
+        nR = sqrt(dotR);
+        double nRs = 0.5 * dotR;
+        if (nRs <= nlsx || ls > mls) {
+#pragma unroll
+          for (int k = 0; k < 7; ++k) x[k] = xn[k];
+          break;
+        }
+        alpha = red * alpha;
+        ls = ls + 1;
+      }
+      iter = iter + 1;
+      if (iter > p.miter || any_nan(x, 7)) {
+        fail = true;
+        break;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) stress[k] = x[k];
+  *tt = x[6];
+}
+
+// mm10_update_euler_angles (Kocks): angles of g Rp (R = I)
+__device__ inline void euler_angles(const Props& p, const double* Rp,
+                                    double* eul) {
+  const double pi = 3.141592653589793, one_eighty = 180.0;
+  const double* g = p.cr->g;
+  // g (Rp R^T) with R = I: Rp R^T is Rp exactly
+  double full[9];
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      full[i + 3 * j] = g[i] * Rp[0 + 3 * j] + g[i + 3] * Rp[1 + 3 * j] +
+                        g[i + 6] * Rp[2 + 3 * j];
+  double psiK = atan2(full[2 + 3 * 1], full[2 + 3 * 0]);
+  if (psiK < 0.0) psiK = psiK + 2.0 * pi;
+  double phiK = atan2(full[1 + 3 * 2], full[0 + 3 * 2]);
+  if (phiK < 0.0) phiK = phiK + 2.0 * pi;
+  double f33 = full[8] > 1.0 ? 1.0 : full[8];
+  double thetaK = acos(f33);
+  if (p.angle_type == 1) {
+    eul[0] = one_eighty / pi * psiK;
+    eul[1] = one_eighty / pi * thetaK;
+    eul[2] = one_eighty / pi * phiK;
+  } else {
+    eul[0] = psiK;
+    eul[1] = thetaK;
+    eul[2] = phiK;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------
+// One thread per Gauss point of an mm10 block (all its crystals).
+// ---------------------------------------------------------------------
+__global__ void __launch_bounds__(64)
+    k_mm10(DevModel M, const WorkItem* __restrict__ work, int step,
+           int iter0_off) {
+  __shared__ double s_geo[108 * kLanes];
+  const WorkItem wi = work[blockIdx.x];
+  if ((int)threadIdx.x >= wi.count) return;
+  const BlockDesc bd = M.blocks[wi.blk];
+  const MatDesc md = M.mats[bd.mat];
+  const int G = bd.ngp;
+  const int64_t SG = (int64_t)bd.span * G;
+  const int64_t q = wi.first + threadIdx.x;  // point inside the block
+  const int elem = (int)(q / G);
+  const int ncry = bd.ncry;
+  const double* hn = M.hist_n + bd.hist_off + q;
+  double* hn1 = M.hist_n1 + bd.hist_off + q;
+  const double* sn9 = M.stress_n + 9 * bd.q_off + q;
+  double* sn1 = M.stress_n1 + 9 * bd.q_off + q;
+  double* cp = M.cep + 21 * bd.q_off + q;
+  const double* ud = M.uddt + 6 * bd.q_off + q;
+  double uddt[6], urcs_n[9];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) uddt[k] = ud[k * SG];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) urcs_n[k] = sn9[k * SG];
+  const double dt = M.dt;
+
+  Props p;
+  p.rate_n = md.cp.rate_n;
+  p.tau_y = md.cp.tau_y;
+  p.tau_v = md.cp.tau_v;
+  p.voce_m = md.cp.voce_m;
+  p.theta_0 = md.cp.theta_0;
+  p.iD_v = md.cp.iD_v;
+  p.eps_dot_0_y = md.cp.eps_dot_0_y;
+  p.atol = md.cp.atol;
+  p.atol1 = md.cp.atol1;
+  p.rtol = md.cp.rtol;
+  p.rtol1 = md.cp.rtol1;
+  p.miter = md.cp.miter;
+  p.alter_mode = md.cp.alter_mode;
+  p.angle_type = md.cp.angle_type;
+
+  // crystal sums of the tangent and the slips accumulate in their output
+  // words (history 1-36 and the cep slots), read back only by this thread
+  double sig_avg[6] = {0, 0, 0, 0, 0, 0};
+  double t_work = 0.0, p_work = 0.0, p_strain = 0.0;
+  double slip_avg[NS];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) slip_avg[k] = 0.0;
+  bool cut = false;
+
+  for (int c = 0; c < ncry; ++c) {
+    p.cr = M.crys + bd.crys_off + (int64_t)elem * ncry + c;
+    const int64_t cb = (int64_t)(87 + 71 * c) * SG;
+    // n-state of the crystal (mm10_copy_cc_hist; at step 1 the
+    // mm10_init_cc_hist0 values)
+    double sn[6], Rpn[9], Dn[6], tau_n, tt_rate_n;
+    if (step == 1) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) sn[k] = urcs_n[k];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) Rpn[k] = (k % 4 == 0) ? 1.0 : 0.0;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) Dn[k] = 0.0;
+      tau_n = p.tau_y + 1.0e-5;
+    } else {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) sn[k] = hn[cb + k * SG];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) Rpn[k] = hn[cb + (9 + k) * SG];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) Dn[k] = hn[cb + (18 + k) * SG];
+      tau_n = hn[cb + 42 * SG];
+    }
+    tt_rate_n = hn[cb + 58 * SG];
+
+    // np1 (mm10_setup_np1 + mm10_setup)
+    double* gl = s_geo + threadIdx.x;
+    slip_geometry(p, Rpn, gl);
+    CState np1;
+    np1.g = gl;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) np1.D[k] = uddt[k];
+    np1.tinc = dt;
+    np1.tt_rate = 0.0;
+    setup(p, np1);
+    // mm10_solve_strup
+    double stress[6], ostress[6], tt = tau_n, ott = tau_n;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) stress[k] = ostress[k] = sn[k];
+    double temp1 = dot6(np1.D, np1.D);
+    double temp2 = dot6(stress, stress);
+    const bool no_load = temp2 == 0.0 && temp1 == 0.0;
+    double J[49];
+#pragma unroll
+    for (int k = 0; k < 49; ++k) J[k] = 0.0;
+    bool fail = false;
+    double curr_tt_rate = 0.0;
+    double tangent[36];
+    if (iter0_off || no_load) {
+      const double* C = p.cr->C;
+#pragma unroll
+      for (int k = 0; k < 36; ++k) tangent[k] = C[k];
+      if (!no_load) {
+        double R1[6];
+        formR1(p, np1, sn, stress, tt, R1);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) stress[k] = stress[k] - R1[k];
+      }
+    } else {
+      double frac = 0.0, sstep = 1.0;
+      int cuts = 0;
+      CState curr = np1;
+      while (frac < 1.0) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) curr.D[k] = np1.D[k] * (sstep + frac);
+        curr.tinc = np1.tinc * (sstep + frac);
+        curr.tt_rate = 0.0;
+        setup(p, curr);
+        tt = tau_n;
+        solve(p, curr, sn, tau_n, tt_rate_n, Dn, stress, &tt, fail,
+              np1.tinc * sstep, J);
+        if (fail) {
+#pragma unroll
+          for (int k = 0; k < 6; ++k) stress[k] = ostress[k];
+          tt = ott;
+          sstep = sstep * 0.5;
+          cuts = cuts + 1;
+          if (cuts > 4) break;
+          fail = false;
+        } else {
+#pragma unroll
+          for (int k = 0; k < 6; ++k) ostress[k] = stress[k];
+          ott = tt;
+          frac = frac + sstep;
+        }
+      }
+      if (fail || isnan(tt) || any_nan(stress, 6)) {
+        cut = true;
+        break;
+      }
+      curr_tt_rate = curr.tt_rate;
+    }
+    np1.tt_rate = curr_tt_rate;
+    double eul[3] = {0, 0, 0}, Rp1[9], ee[6], slip[NS], u[15], ep6[6],
+           ed6[6];
+    double work_inc = 0.0, p_work_inc = 0.0, p_strain_inc = 0.0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Rp1[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) ee[k] = ep6[k] = ed6[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) slip[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 15; ++k) u[k] = 0.0;
+    if (!(iter0_off || no_load)) {
+      // ---- mm10_tangent: T = (J11 - J12 J21 / J22)^-1 C, symmetrised
+      double JJ[36], beta[12];
+#pragma unroll
+      for (int cc = 0; cc < 6; ++cc)
+#pragma unroll
+        for (int r = 0; r < 6; ++r) JJ[r + 6 * cc] = J[r + 7 * cc];
+      double a1 = J[48];
+#pragma unroll
+      for (int cc = 0; cc < 6; ++cc) {
+        beta[cc] = J[6 + 7 * cc];
+        beta[6 + cc] = 0.0;
+      }
+      lu_solve<1, 12>(&a1, beta);
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        double t = -1.0 * beta[j];
+        if (t != 0.0) {
+#pragma unroll
+          for (int i = 0; i < 6; ++i) JJ[i + 6 * j] = JJ[i + 6 * j] + t * J[i + 42];
+        }
+      }
+      const double* C = p.cr->C;
+#pragma unroll
+      for (int k = 0; k < 36; ++k) tangent[k] = C[k] - 0.0 - 0.0;
+      lu_solve<6, 6>(JJ, tangent);
+      double w[36];
+#pragma unroll
+      for (int j = 0; j < 6; ++j)
+#pragma unroll
+        for (int i = 0; i < 6; ++i)
+          w[i + 6 * j] = (tangent[i + 6 * j] + tangent[j + 6 * i]) * 0.5;
+#pragma unroll
+      for (int k = 0; k < 36; ++k) tangent[k] = w[k];
+      // ---- slips at the converged state (np1: full-step dg)
+      double rs[NS], gam[NS];
+      slips(p, np1, stress, tt, rs, gam);
+      // ---- mm10_update_rotation: Rp = expm(W(wbarp)) Rp_n
+      {
+        double wb[3];
+        form_w(p, np1, rs, gam, wb);
+        double W[9] = {0.0, -wb[2], -wb[1], wb[2], 0.0, -wb[0],
+                       wb[1], wb[0], 0.0};  // column-major of wv2wt
+        double alpha = sqrt(W[1 + 3 * 2] * W[1 + 3 * 2] +
+                            W[0 + 3 * 2] * W[0 + 3 * 2] +
+                            W[0 + 3 * 1] * W[0 + 3 * 1]);
+        double A[9];
+        if (alpha < 1.0e-16) {
+#pragma unroll
+          for (int k = 0; k < 9; ++k) A[k] = 0.0;
+        } else {
+          double ca = (1.0 - cos(alpha)) / (alpha * alpha);
+          double sa = sin(alpha) / alpha;
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) A[i + 3 * j] = sa * W[i + 3 * j];
+#pragma unroll
+            for (int l = 0; l < 3; ++l) {
+              double t = ca * W[l + 3 * j];
+#pragma unroll
+              for (int i = 0; i < 3; ++i)
+                A[i + 3 * j] = A[i + 3 * j] + t * W[i + 3 * l];
+            }
+          }
+        }
+        A[0] = A[0] + 1.0;
+        A[4] = A[4] + 1.0;
+        A[8] = A[8] + 1.0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+            Rp1[i + 3 * j] = A[i] * Rpn[0 + 3 * j] + A[i + 3] * Rpn[1 + 3 * j] +
+                             A[i + 6] * Rpn[2 + 3 * j];
+      }
+      // ---- mm10_output
+#pragma unroll
+      for (int i = 0; i < NS; ++i) slip[i] = gam[i];
+      euler_angles(p, Rp1, eul);
+      double dif[NS], t1, t2;
+#pragma unroll
+      for (int i = 0; i < NS; ++i) dif[i] = rs[i] * np1.tinc * p.iD_v;
+#pragma unroll
+      for (int i = 0; i < NS; ++i)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) ed6[k] = ed6[k] + dif[i] * np1.ms(i, k);
+      double edt[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) edt[k] = ed6[k];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) ed6[k] = edt[k] / np1.tinc;
+      t1 = edt[0] * edt[0] + edt[1] * edt[1] + edt[2] * edt[2];
+      t2 = edt[3] * edt[3] + edt[4] * edt[4] + edt[5] * edt[5];
+      double ed_dot = sqrt(2.0 / 3.0 * (t1 + 0.5 * t2));
+      u[14] = ed_dot / np1.tinc;
+      work_inc = dot6(stress, np1.D);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) ee[k] = stress[k];
+      chol_solve6(p.cr->C, ee);  // R = I: the rotation of ee is exact
+      double dbarp[6] = {0, 0, 0, 0, 0, 0}, wp[3], ewwe[6], ep[6];
+#pragma unroll
+      for (int i = 0; i < NS; ++i)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) dbarp[k] = dbarp[k] + slip[i] * np1.ms(i, k);
+      form_w(p, np1, rs, gam, wp);
+      symsw(ee, wp, ewwe);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) ep[k] = dbarp[k] + ewwe[k];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) ep6[k] = ep[k] / np1.tinc;
+      t1 = ep[0] * ep[0] + ep[1] * ep[1] + ep[2] * ep[2];
+      t2 = ep[3] * ep[3] + ep[4] * ep[4] + ep[5] * ep[5];
+      u[10] = sqrt(2.0 / 3.0 * (t1 + 0.5 * t2)) / np1.tinc;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) ep[k] = ep[k] + edt[k];
+      t1 = ep[0] * ep[0] + ep[1] * ep[1] + ep[2] * ep[2];
+      t2 = ep[3] * ep[3] + ep[4] * ep[4] + ep[5] * ep[5];
+      p_strain_inc = sqrt(2.0 / 3.0 * (t1 + 0.5 * t2));
+      p_work_inc = dot6(stress, ep);
+      double ec_dot = p_strain_inc / np1.tinc, n_eff;
+      if (ec_dot > 0.0) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) ep[k] = ep[k] / np1.tinc;
+        n_eff = 0.0;
+#pragma unroll
+        for (int i = 0; i < NS; ++i) {
+          double d = pow(fabs(rs[i]), p.rate_n - 1.0);
+          d = np1.dg * p.rate_n / pow(tt, p.rate_n) * d;
+          double dgdt = d + np1.tinc * p.iD_v;
+          double a = np1.ms(i, 0) * ep[0] + np1.ms(i, 1) * ep[1] +
+                     np1.ms(i, 2) * ep[2];
+          double b = np1.ms(i, 3) * ep[3] + np1.ms(i, 4) * ep[4] +
+                     np1.ms(i, 5) * ep[5];
+          double ec_slip = a + 0.5 * b;
+          double q1 = rs[i] * dgdt / ec_dot;
+          double q2 = ec_slip / ec_dot / np1.tinc;
+          n_eff = n_eff + (2.0 / 3.0) * q1 * q2;
+        }
+      } else {
+        n_eff = 1.0e10;
+      }
+      u[11] = n_eff;
+      double s_tr = (stress[0] + stress[1] + stress[2]) / 3.0;
+      double se0 = stress[0] - s_tr, se1 = stress[1] - s_tr,
+             se2 = stress[2] - s_tr;
+      t1 = se0 * se0 + se1 * se1 + se2 * se2;
+      t2 = stress[3] * stress[3] + stress[4] * stress[4] +
+           stress[5] * stress[5];
+      s_tr = sqrt(1.5 * (t1 + 2.0 * t2));
+      u[12] = s_tr;
+      double B_eff;
+      if (ec_dot < 1.0e-100)
+        B_eff = 0.0;
+      else if (n_eff > 100.0)
+        B_eff = -1.0;
+      else
+        B_eff = ec_dot / pow(s_tr, n_eff);
+      u[13] = B_eff;
+#pragma unroll
+      for (int i = 0; i < NS; ++i) slip[i] = slip[i] + dif[i];
+      double maxslip = 0.0;
+      int sysID = 0;
+#pragma unroll
+      for (int i = 0; i < NS; ++i) {
+        double cs = fabs(slip[i]);
+        if (cs > maxslip) {
+          maxslip = cs;
+          sysID = i + 1;
+        }
+      }
+      u[5] = maxslip / np1.tinc;
+      u[6] = (double)sysID;
+      int numAct = 0;
+      maxslip = 0.1 * maxslip;
+#pragma unroll
+      for (int i = 0; i < NS; ++i)
+        if (fabs(slip[i]) >= maxslip) numAct = numAct + 1;
+      u[7] = (double)numAct;
+    }
+    // accumulate for the averages
+#pragma unroll
+    for (int k = 0; k < 6; ++k) sig_avg[k] = sig_avg[k] + stress[k];
+    if (c == 0) {
+#pragma unroll
+      for (int k = 0; k < 36; ++k) hn1[k * SG] = tangent[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 36; ++k) hn1[k * SG] = hn1[k * SG] + tangent[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NS; ++k) slip_avg[k] = slip_avg[k] + slip[k];
+    t_work = t_work + work_inc;
+    p_work = p_work + p_work_inc;
+    p_strain = p_strain + p_strain_inc;
+    // mm10_store_cryhist
+    double* h1 = hn1 + cb;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) h1[k * SG] = stress[k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) h1[(6 + k) * SG] = eul[k];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) h1[(9 + k) * SG] = Rp1[k];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) h1[(18 + k) * SG] = np1.D[k];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) h1[(24 + k) * SG] = ee[k];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) h1[(30 + k) * SG] = slip[k];
+    h1[42 * SG] = tt;
+#pragma unroll
+    for (int k = 0; k < 15; ++k) h1[(43 + k) * SG] = u[k];
+    h1[58 * SG] = np1.tt_rate;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) h1[(59 + k) * SG] = ep6[k];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) h1[(65 + k) * SG] = ed6[k];
+  }
+  if (cut) {
+    atomicOr(&M.flags[1], 1);
+    return;
+  }
+  const double rn = (double)ncry;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) sig_avg[k] = sig_avg[k] / rn;
+  double tang_avg[36];
+#pragma unroll
+  for (int k = 0; k < 36; ++k) tang_avg[k] = hn1[k * SG] / rn;
+#pragma unroll
+  for (int k = 0; k < NS; ++k) slip_avg[k] = slip_avg[k] / rn;
+  t_work = t_work / rn;
+  p_work = p_work / rn;
+  p_strain = p_strain / rn;
+  // mm10_a_store_crystal
+#pragma unroll
+  for (int k = 0; k < 6; ++k) sn1[k * SG] = sig_avg[k];
+  sn1[6 * SG] = urcs_n[6] + t_work;
+  sn1[7 * SG] = urcs_n[7] + p_work;
+  sn1[8 * SG] = urcs_n[8] + p_strain;
+#pragma unroll
+  for (int k = 0; k < 36; ++k) hn1[k * SG] = tang_avg[k];
+#pragma unroll
+  for (int k = 36; k < 63; ++k) hn1[k * SG] = 0.0;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) hn1[(63 + k) * SG] = (k % 4 == 0) ? 1.0 : 0.0;
+  double w72 = step == 1 ? 0.0 : hn[72 * SG];
+  double w73 = step == 1 ? 0.0 : hn[73 * SG];
+  double w74 = step == 1 ? 0.0 : hn[74 * SG];
+  hn1[72 * SG] = w72 + t_work;
+  hn1[73 * SG] = w73 + p_work;
+  hn1[74 * SG] = w74 + p_strain;
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    double base = step == 1 ? 0.0 : hn[(75 + k) * SG];
+    hn1[(75 + k) * SG] = base + slip_avg[k];
+  }
+  // [D] for the tangent kernel: the lower triangle of history words 1-36
+#pragma unroll
+  for (int r = 0; r < 6; ++r)
+#pragma unroll
+    for (int cc = 0; cc <= r; ++cc)
+      cp[tri(r, cc) * SG] = tang_avg[r + 6 * cc];
+}
+
+int launch_mm10(const DevModel& M, const WorkItem* d_work, int nwork,
+                int step, int iter0_off, hipStream_t s) {
+  if (nwork > 0)
+    hipLaunchKernelGGL(k_mm10, dim3(nwork), dim3(64), 0, s, M, d_work, step,
+                       iter0_off);
+  return hipGetLastError() == hipSuccess ? W3D_OK : W3D_ERR_DEVICE;
+}
+
+}  // namespace w3d

@@ -42,7 +42,7 @@ class Mesh:
         return self.conn.shape[0]
 
     def model(self, material, int_order=None, bbar=None, elblks=None,
-              auto_size=128, eps_bbar=0.0, angles=None):
+              auto_size=128, eps_bbar=0.0, angles=None, angle_stride=1):
         """ModelArrays for this mesh with one material for all elements."""
         ne, nn = self.conn.shape
         if int_order is None:
@@ -58,7 +58,32 @@ class Mesh:
             np.full(ne, int_order, np.int32),
             np.ones(ne, np.int32), np.full(ne, bbar, np.int32), mats,
             elblks=elblks, auto_size=auto_size, eps_bbar=eps_bbar,
-            angles=angles)
+            angles=angles, angle_stride=angle_stride)
+
+
+def random_orientations(n, seed=0, per_element=1):
+    """Seeded SO(3)-uniform crystal orientations as Kocks Euler angles in
+    degrees, (n * per_element, 3): uniform unit quaternions turned into
+    rotations r, then psi = atan2(r32, r31), theta = acos(r33),
+    phi = atan2(r23, r13) (the inverse of mm10_rotation_matrix)."""
+    rng = np.random.default_rng(seed)
+    q = rng.normal(size=(n * per_element, 4))
+    q /= np.linalg.norm(q, axis=1)[:, None]
+    w, x, y, z = q.T
+    r = np.empty((q.shape[0], 3, 3))
+    r[:, 0, 0] = 1 - 2 * (y * y + z * z)
+    r[:, 0, 1] = 2 * (x * y - z * w)
+    r[:, 0, 2] = 2 * (x * z + y * w)
+    r[:, 1, 0] = 2 * (x * y + z * w)
+    r[:, 1, 1] = 1 - 2 * (x * x + z * z)
+    r[:, 1, 2] = 2 * (y * z - x * w)
+    r[:, 2, 0] = 2 * (x * z - y * w)
+    r[:, 2, 1] = 2 * (y * z + x * w)
+    r[:, 2, 2] = 1 - 2 * (x * x + y * y)
+    psi = np.arctan2(r[:, 2, 1], r[:, 2, 0])
+    theta = np.arccos(np.clip(r[:, 2, 2], -1.0, 1.0))
+    phi = np.arctan2(r[:, 1, 2], r[:, 0, 2])
+    return np.degrees(np.stack([psi, theta, phi], axis=1)) % 360.0
 
 
 def _jitter(coords, lattice, top, h, frac, seed):

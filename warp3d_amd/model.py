@@ -61,6 +61,8 @@ def lib():
         L.w3d_get_phase_time.restype = C.c_int32
         L.w3d_reset_phase_times.argtypes = [C.c_void_p]
         L.w3d_reset_phase_times.restype = C.c_int32
+        L.w3d_set_time_increment.argtypes = [C.c_void_p, C.c_double]
+        L.w3d_set_time_increment.restype = C.c_int32
         _LIB = L
     return _LIB
 
@@ -98,6 +100,10 @@ class BlockModel:
 
     def __del__(self):
         self.close()
+
+    def set_time_increment(self, dt):
+        _chk(lib().w3d_set_time_increment(self.h, dt),
+             "w3d_set_time_increment")
 
     def block_info(self, blk):
         info = np.zeros(5, np.int32)
@@ -184,7 +190,7 @@ class BlockModel:
 
     # -- phase timing (HIP events on the library's stream) ----------------
     PHASES = ("recover", "ifv_gather", "energy_reduce", "tangent",
-              "assemble")
+              "assemble", "mm10")
 
     def set_profiling(self, on=True):
         _chk(lib().w3d_set_profiling(self.h, int(bool(on))),
