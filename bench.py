@@ -1,22 +1,29 @@
 #!/usr/bin/env python3
 """Element-block hot-path benchmark on one MI355X.
 
-Workload (default): 60x60x60 8-node bricks (216,000 elements, 2x2x2 Gauss
-points with B-bar: 1,728,000 points), interior nodes jittered +-5 % of the
-edge (seeded), mm01 bilinear Mises (E = 200 GPa, nu = 0.3, sigma_y =
-350 MPa, H' = E/100), a seeded displacement increment at 0.75 % mean axial
-strain (the mid-point of a 10-step ramp to 1.5 %) that puts most points
-in active plastic flow.
+Workloads (all 1,728,000 Gauss points):
+  hex8  (default): 60x60x60 8-node bricks (216,000 elements, 2x2x2 Gauss
+        points with B-bar), interior nodes jittered +-5 % of the edge
+        (seeded), mm01 bilinear Mises (E = 200 GPa, nu = 0.3, sigma_y =
+        350 MPa, H' = E/100), fixed base, a seeded top-face displacement
+        field spread through the height, 0.15 % mean axial strain per step
+        of a 10-step ramp to 1.5 %;
+  hex20: 40x40x40 20-node bricks (64,000 elements, 3x3x3 Gauss points),
+        mm01 (E = 70 GPa, nu = 0.33, sigma_y = 300 MPa, H' = E/50), a
+        bending-style field (axial strain varying linearly through the
+        thickness, 1 % peak over 15 steps).
 
-One step = one Newton iteration of the element-block hot path, all
-device-resident: strain/stress recovery + mm01 + element internal forces,
-the assembly of the element forces, the block tangents and their assembly
-into the VSS sparse matrix.  The metric is Gauss-point stress+tangent
-updates per second; `value` is the path with the FP32 tangent (the library's
-mixed-precision path), reported beside the plain FP64 path measured in the
-same run.
+One step = one Newton iteration of the element-block hot path of load step
+2 (after a step-1 pseudo-update, update and commit, so every history word is
+read), all device-resident: strain/stress recovery + material update +
+element internal forces, the assembly of the element forces, the block
+tangents and their assembly into the VSS sparse matrix.  The metric is
+Gauss-point stress+tangent updates per second; `value` is the path with the
+FP32 tangent (the library's mixed-precision path), reported beside the plain
+FP64 path measured in the same run.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+                    [--workload hex8|hex20]
 """
 import argparse
 import json
@@ -30,6 +37,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0         # MI355X HBM3E
+
+
+def kernel_name(phase, workload, fp32=True):
+    """The kernel (as the rocprofv3 summaries name it) of a phase of the
+    reported FP32-tangent path."""
+    tmpl = {"hex8": "8, 8, true", "hex20": "20, 27, false"}[workload]
+    prec = "float" if fp32 else "double"
+    return {"recover": f"k_recover<{tmpl}, 0>",
+            "ifv_gather": "k_ifv_gather",
+            "tangent": f"k_tangent_mfma<{tmpl}, {prec}>",
+            "assemble": f"k_assemble<{prec}>",
+            "energy_reduce": "k_energy_reduce",
+            "mm10": "k_mm10"}[phase]
 FP64_PEAK_TFS = 78.6          # MI355X FP64 (vector and matrix)
 FP32_MFMA_PEAK_TFS = 157.3    # MI355X dense FP32 matrix
 
@@ -38,49 +58,102 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_workload(n, seed=2024):
+WORKLOADS = {
+    "hex8": "hex8 60x60x60 box, 216000 elements, 1728000 Gauss points "
+            "(2x2x2, B-bar), mm01, blocks of 128, one Newton iteration of "
+            "the element-block path per step",
+    "hex20": "hex20 40x40x40 box, 64000 elements, 1728000 Gauss points "
+             "(3x3x3), mm01, blocks of 128, one Newton iteration of the "
+             "element-block path per step",
+}
+
+
+def build_workload(name, seed=2024):
+    """Mesh, model arrays, constraints and the step increment du (the
+    displacement increment of one load step)."""
     from warp3d_amd import _abi
     from warp3d_amd import mesh as M
-    m = M.box_hex8(n, jitter=0.05, seed=seed)
-    mat = _abi.mm01_material(200e3, 0.3, 350.0, hprime=2000.0)
-    arr = m.model(mat, auto_size=128)
+    rng = np.random.default_rng(seed)
+    if name == "hex8":
+        m = M.box_hex8(60, jitter=0.05, seed=seed)
+        mat = _abi.mm01_material(200e3, 0.3, 350.0, hprime=2000.0)
+        arr = m.model(mat, auto_size=128)
+        eps = 0.015 / 10
+    else:
+        m = M.box_hex20(40, jitter=0.05, seed=seed)
+        mat = _abi.mm01_material(70e3, 0.33, 300.0, hprime=70e3 / 50)
+        arr = m.model(mat, int_order=1, auto_size=128)
+        eps = 0.01 / 15
     x = m.coords
     nd = 3 * m.nonode
     base = np.abs(x[:, 2]) < 1e-9
     cst = np.zeros(nd, np.int32)
     for k in range(3):
         cst[k::3][base] = 1
-    rng = np.random.default_rng(seed)
-    # seeded top-face field, mean axial strain 0.75 %, spread through the
-    # height as the linear-elastic solution would
-    top = rng.uniform(0.5, 1.5, size=m.nonode)
     du = np.zeros_like(x)
-    du[:, 2] = 0.0075 * x[:, 2] * top
-    du[:, 0] = -0.3 * 0.0075 * (x[:, 0] - 0.5) * x[:, 2]
-    du[:, 1] = -0.3 * 0.0075 * (x[:, 1] - 0.5) * x[:, 2]
+    if name == "hex8":
+        # seeded top-face field spread through the height as the
+        # linear-elastic solution would
+        top = rng.uniform(0.5, 1.5, size=m.nonode)
+        du[:, 2] = eps * x[:, 2] * top
+        du[:, 0] = -0.3 * eps * (x[:, 0] - 0.5) * x[:, 2]
+        du[:, 1] = -0.3 * eps * (x[:, 1] - 0.5) * x[:, 2]
+    else:
+        # bending about the y axis: axial strain linear through x
+        du[:, 2] = eps * 2.0 * (x[:, 0] - 0.5) * x[:, 2] * 2.0
+        du[:, 0] = -eps * x[:, 2] ** 2
+        du[:, 1] = -0.33 * eps * (x[:, 1] - 0.5) * x[:, 2]
+        du += 0.05 * eps * rng.uniform(-1, 1, size=du.shape) * \
+            (x[:, 2:3] > 0)
     du[base] = 0.0
     return m, arr, cst, du.ravel()
 
 
-def algorithmic(m, ncontrib, nout, ninv, ke_bytes):
+def algorithmic(m, ngp_e, ncontrib, nout, ninv, ke_bytes):
     """Algorithmic work per launch of each phase (bytes or flops)."""
-    ngp = m.noelem * 8
     ne = m.noelem
+    nn = m.conn.shape[1]
+    T = 3 * nn
+    utsz = T * (T + 1) // 2
+    ngp = ne * ngp_e
     return {
         # per point: sigma_n 9 + eps_n 6 + hist_n 11 read, sigma 9 + eps 6 +
         # hist 11 + [D] 21 written = 73 words; per element: coordinates and
-        # increments gathered (2 x 24 words), connectivity 8 ints, element
-        # forces 24 words and volume 1 word written
-        "recover": ("hbm", ngp * 73 * 8 + ne * ((24 + 24 + 24 + 1) * 8 + 32)),
-        # per structure node: its element list (8-byte offset + 4-byte
-        # stride, 3 words read each) and 3 words written
-        "ifv_gather": ("hbm", ninv * (12 + 24) + m.nonode * 24),
+        # increments gathered (2 x T words), connectivity (NN ints),
+        # element forces T words and volume 1 word written
+        "recover": ("hbm", ngp * 73 * 8 +
+                    ne * ((3 * T + 1) * 8 + 4 * nn)),
+        # per (node, element) pair: its 8-byte offset and 3 force words
+        # read; per structure node 3 words written
+        "ifv_gather": ("hbm", ninv * (8 + 24) + m.nonode * 24),
         # upper-triangle B^T D B as bdbtgen forms it: 2(36 T + 6 utsz)
-        "tangent": ("flop", ngp * 2 * (36 * 24 + 6 * 300)),
+        "tangent": ("flop", ngp * 2 * (36 * T + 6 * utsz)),
         # one index + one Ke word per contribution, the output pointer and
         # the assembled word per output
         "assemble": ("hbm", ncontrib * (4 + ke_bytes) + nout * (8 + 8)),
     }
+
+
+def profiled_traffic(kernel, workload):
+    """HBM bytes per launch of a kernel from the committed rocprofv3
+    counter summary (profiles/hbm_traffic.md): FETCH_SIZE scaled by the
+    copy-kernel calibration factor stated there, plus WRITE_SIZE.  None
+    when the summary has no row for the kernel and workload."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    fetch_scale = None
+    for line in open(path):
+        if line.startswith("fetch_scale:"):
+            fetch_scale = float(line.split(":")[1])
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if (fetch_scale and len(cells) >= 4 and cells[0] == workload and
+                cells[1] == kernel):
+            try:
+                return (float(cells[2]) * fetch_scale + float(cells[3])) * 1024
+            except ValueError:
+                continue
+    return None
 
 
 def run_device(arr, cst, du, steps, warmup, precision):
@@ -93,14 +166,19 @@ def run_device(arr, cst, du, steps, warmup, precision):
     d_ifv = hip.DeviceArray(du.size)
     d_kd = hip.DeviceArray(neq)
     d_kc = hip.DeviceArray(max(nco, 1))
-    # state at the start of the timed region: one pseudo-update and one
-    # nonlinear update already done (iterations of the current step)
+    # load step 1 (pseudo-update, tangent, one update) committed, then the
+    # timed iterations are those of load step 2 from u = du
     dev.recover_dev(1, 0, d_u.ptr, d_du.ptr, d_ifv.ptr)
     dev.tangent(1, 0, precision)
+    dev.recover_dev(1, 1, d_u.ptr, d_du.ptr, d_ifv.ptr)
+    dev.commit_step()
+    d_u.upload(du)
+    dev.recover_dev(2, 0, d_u.ptr, d_du.ptr, d_ifv.ptr)
+    dev.tangent(2, 0, precision)
 
     def one(it):
-        dev.recover_dev(1, it, d_u.ptr, d_du.ptr, d_ifv.ptr)
-        dev.tangent(1, it, precision)
+        dev.recover_dev(2, it, d_u.ptr, d_du.ptr, d_ifv.ptr)
+        dev.tangent(2, it, precision)
         dev.assemble_vss_dev(d_kd.ptr, d_kc.ptr)
 
     for w in range(warmup):
@@ -131,28 +209,32 @@ def contrib_counts(m, cst):
     return int(np.sum(nf * (nf + 1) // 2))
 
 
-def cpu_baseline(arr, du, budget_s=10.0):
+def cpu_baseline(arr, du, ngp, budget_s=10.0):
     from oracle import oracle as O
     threads = len(os.sched_getaffinity(0))
     om = O.OracleModel(arr, threads=threads)
     u = np.zeros_like(du)
     om.recover(1, 0, u, du)
-    om.tangent(1, 0)
+    om.recover(1, 1, u, du)
+    om.commit_step()
+    u = du.copy()
+    om.recover(2, 0, u, du)
+    om.tangent(2, 0)
     it = 0
     t0 = time.perf_counter()
     while True:
         it += 1
-        om.recover(1, it, u, du)
-        om.tangent(1, it)
+        om.recover(2, it, u, du)
+        om.tangent(2, it)
         if time.perf_counter() - t0 > budget_s and it >= 2:
             break
     t1 = time.perf_counter()
-    ngp = arr.noelem * 8
     return {"value": ngp * it / (t1 - t0), "unit": "GP updates/s",
             "cores": threads, "kind": "port",
-            "sample": f"oracle (C, OpenMP over blocks of 128) recovery + mm01 "
-                      f"+ element forces + FP64 tangent on the full workload, "
-                      f"{it} iterations in {t1 - t0:.1f} s"}
+            "sample": f"oracle (C, OpenMP over blocks of 128) recovery + "
+                      f"material update + element forces + FP64 tangent on "
+                      f"the full workload at load step 2, {it} Newton "
+                      f"iterations in {t1 - t0:.1f} s"}
 
 
 def main():
@@ -160,7 +242,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=60)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="hex8")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run 3 device-to-device copies of MIB MiB "
@@ -187,8 +269,10 @@ def main():
         b.free()
         log(f"copy probe: 3 x {args.copy_probe} MiB device-to-device")
     t = time.perf_counter()
-    m, arr, cst, du = build_workload(args.n)
-    log(f"workload: {m.noelem} elements, {m.noelem * 8} points, "
+    m, arr, cst, du = build_workload(args.workload)
+    ngp_e = 8 if args.workload == "hex8" else 27
+    ngp = m.noelem * ngp_e
+    log(f"workload {args.workload}: {m.noelem} elements, {ngp} points, "
         f"{m.nonode} nodes ({time.perf_counter() - t:.1f} s)")
     res = {}
     for prec, name in ((_abi.W3D_FP64, "fp64"), (_abi.W3D_FP32, "fp32")):
@@ -200,13 +284,12 @@ def main():
             + ", ".join(f"{k} {v[1] / max(v[0], 1):.3f} ms" for k, v in
                         phases.items() if v[0]) +
             f"  ({time.perf_counter() - t:.1f} s)")
-    ngp = m.noelem * 8
     ms32, ph32, info = res["fp32"]
     ms64, ph64, _ = res["fp64"]
     ninv = int(m.conn.size)
     ncontrib = contrib_counts(m, cst)
     nout = info["neqns"] + info["ncoeff"]
-    alg = algorithmic(m, ncontrib, nout, ninv, 4)
+    alg = algorithmic(m, ngp_e, ncontrib, nout, ninv, 4)
     # dominant kernel of the reported path
     avg = {k: v[1] / v[0] for k, v in ph32.items() if v[0]}
     dom = max(avg, key=avg.get)
@@ -216,17 +299,20 @@ def main():
         achieved = work / dur_s / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None}
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": profiled_traffic(kernel_name(dom, args.workload),
+                                            args.workload)}
     else:
         achieved = work / dur_s / 1e12
         roof = {"bound": "mfma", "achieved": round(achieved, 2),
                 "peak": FP32_MFMA_PEAK_TFS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP32_MFMA_PEAK_TFS, 4),
-                "traffic": None}
-    roof["kernel"] = dom
+                "traffic": profiled_traffic(kernel_name(dom, args.workload),
+                                            args.workload)}
+    roof["kernel"] = kernel_name(dom, args.workload)
     phase_report = {}
     for tag, ph, kb in (("fp32", ph32, 4), ("fp64", ph64, 8)):
-        a = algorithmic(m, ncontrib, nout, ninv, kb)
+        a = algorithmic(m, ngp_e, ncontrib, nout, ninv, kb)
         rep = {}
         for k, (n, tot) in ph.items():
             if not n:
@@ -254,9 +340,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64 (recovery, forces, assembly) + f32 (tangent)",
         "data": "synthetic (seeded jittered mesh and displacement field)",
-        "config": {"workload": "hex8 60x60x60 box, 216000 elements, 1728000 "
-                               "Gauss points, mm01, blocks of 128, one Newton "
-                               "iteration of the element-block path per step",
+        "config": {"workload": WORKLOADS[args.workload],
                    "elements": m.noelem, "gauss_points": ngp,
                    "equations": info["neqns"], "matrix_terms": info["ncoeff"]},
         "plain_fp64": {"value": round(ngp / (ms64 * 1e-3), 1),
@@ -267,7 +351,7 @@ def main():
     }
     if not args.no_cpu:
         t = time.perf_counter()
-        out["cpu_baseline"] = cpu_baseline(arr, du)
+        out["cpu_baseline"] = cpu_baseline(arr, du, ngp)
         log(f"cpu baseline done ({time.perf_counter() - t:.1f} s)")
     else:
         out["cpu_baseline"] = None

@@ -108,7 +108,6 @@ struct w3d_ctx {
   std::vector<int> rec_groups, rec_start, rec_count;
   std::vector<int> tan_groups, tan_start, tan_count;
   int64_t *d_inv_ptr = nullptr, *d_inv_off = nullptr;
-  int32_t* d_inv_nn = nullptr;
   int32_t *d_blk_wfirst = nullptr, *d_blk_wcount = nullptr;
   double* d_blk_sums = nullptr;
   double *d_u = nullptr, *d_du = nullptr, *d_ifv = nullptr;
@@ -235,7 +234,7 @@ void free_ctx(w3d_ctx* c) {
   dfree(c->d_ek64); dfree(c->d_ek32); dfree(c->d_einfvec);
   dfree(c->d_evol); dfree(c->d_en_wg); dfree(c->d_flags);
   dfree(c->d_work_rec); dfree(c->d_work_tan);
-  dfree(c->d_inv_ptr); dfree(c->d_inv_off); dfree(c->d_inv_nn);
+  dfree(c->d_inv_ptr); dfree(c->d_inv_off);
   dfree(c->d_blk_wfirst); dfree(c->d_blk_wcount);
   dfree(c->d_blk_sums); dfree(c->d_u); dfree(c->d_du); dfree(c->d_ifv);
   dfree(c->d_asm_ptr); dfree(c->d_asm_src); dfree(c->d_kout);
@@ -558,17 +557,14 @@ int create_impl(w3d_ctx* c, const w3d_model* m) {
   }
   {
     std::vector<int64_t> off(c->inv_elem.size());
-    std::vector<int32_t> nn(c->inv_elem.size());
     for (size_t p = 0; p < off.size(); ++p) {
       int e = c->inv_elem[p];
       const BlockDesc& B = c->blocks[c->elem_blk[e]];
       int i = e - (B.felem - 1);
-      off[p] = B.ifv_off + (int64_t)i * B.totdof + c->inv_ln[p];
-      nn[p] = B.nnode;
+      off[p] = B.ifv_off + (int64_t)i * B.totdof + 3 * c->inv_ln[p];
     }
     if ((rc = upload(&c->d_inv_ptr, c->inv_ptr))) return rc;
     if ((rc = upload(&c->d_inv_off, off))) return rc;
-    if ((rc = upload(&c->d_inv_nn, nn))) return rc;
   }
   return W3D_OK;
 }
@@ -694,7 +690,7 @@ int recover_impl(w3d_ctx* c, int step, int iter, int extrapolated,
   if (d_ifv) {
     prof_begin(c, W3D_PHASE_IFV);
     rc = launch_ifv_gather(c->d_einfvec, c->d_inv_ptr, c->d_inv_off,
-                           c->d_inv_nn, c->nonode, d_ifv, c->stream);
+                           c->nonode, d_ifv, c->stream);
     prof_end(c, W3D_PHASE_IFV);
     if (rc) return fail(rc, "recover: ifv gather launch failed");
   }
@@ -1134,10 +1130,11 @@ int w3d_get_block_state(const w3d_ctx* c, int32_t blk, int32_t which,
   HIPCHK(hipMemcpy(tmp.data(), src, n * 8, hipMemcpyDeviceToHost));
   if (which == W3D_ELEM_VOL) {
     std::memcpy(out, tmp.data(), n * 8);
-  } else if (which == W3D_EINFVEC) {  // device (i, dof) -> (span, totdof)
+  } else if (which == W3D_EINFVEC) {  // device (i, 3j+c) -> (span, totdof)
     for (int64_t i = 0; i < S; ++i)
-      for (int64_t j = 0; j < B.totdof; ++j)
-        out[j * S + i] = tmp[i * B.totdof + j];
+      for (int64_t j = 0; j < B.nnode; ++j)
+        for (int64_t cc = 0; cc < 3; ++cc)
+          out[(cc * B.nnode + j) * S + i] = tmp[i * B.totdof + 3 * j + cc];
   } else if (which == W3D_CEP) {  // device (k, i, g) -> (21, span, ngp)
     for (int64_t k = 0; k < 21; ++k)
       for (int64_t i = 0; i < S; ++i)

@@ -5,7 +5,8 @@
 //     point (element i, point g) sits at base_b + k*S*G + i*G + g, so the
 //     lanes of a wavefront that walk consecutive (i, g) read and write
 //     consecutive addresses.
-//   * element forces einfvec are element-contiguous: base + i*T + dof.
+//   * element forces einfvec are element-contiguous and node-major inside
+//     the element: base + i*T + 3*j + c (component c of local node j).
 //   * element tangents are element-contiguous packed upper triangles, the
 //     reference's estiff_blocks (utsz, span) layout: base + i*utsz + pos.
 #pragma once
@@ -124,8 +125,8 @@ int launch_recover(const DevModel& M, const std::vector<BlockDesc>& hblocks,
 int launch_mm10(const DevModel& M, const WorkItem* d_work, int nwork,
                 int step, int iter0_off, hipStream_t s);
 int launch_ifv_gather(const double* einfvec, const int64_t* inv_ptr,
-                      const int64_t* inv_off, const int32_t* inv_nn,
-                      int nonode, double* ifv, hipStream_t s);
+                      const int64_t* inv_off, int nonode, double* ifv,
+                      hipStream_t s);
 int launch_energy_reduce(const double* en_wg, const int32_t* blk_wfirst,
                          const int32_t* blk_wcount, int nblk,
                          double* blk_sums, hipStream_t s);

@@ -487,11 +487,13 @@ __global__ void __launch_bounds__(256)
     }
   }
 
-  // phase 4: element internal forces f = sum_gp B^T sigma w |J| (gpifv1)
+  // phase 4: element internal forces f = sum_gp B^T sigma w |J| (gpifv1),
+  // stored node-major inside the element: word 3 j + c is the force of
+  // component c at local node j (the reference's dof c*NN + j)
   const double scalar = bd.etype == W3D_TET10 ? 1.0 / 6.0 : 1.0;
   for (int k = tid; k < cnt * T; k += 256) {
-    int ee = k / T, dof = k % T;
-    int c = dof / NN, j = dof % NN;
+    int ee = k / T, r = k % T;
+    int j = r / 3, c = r % 3;
     double f = 0.0;
     for (int gg = 0; gg < G; ++gg) {
       const double* ga = s_gam[ee][gg];
@@ -530,7 +532,7 @@ __global__ void __launch_bounds__(256)
             by * s[4] * w * d + bx * s[5] * w * d;
       }
     }
-    M.einfvec[bd.ifv_off + (int64_t)(wi.first + ee) * T + dof] = f;
+    M.einfvec[bd.ifv_off + (int64_t)(wi.first + ee) * T + r] = f;
   }
   for (int k = tid; k < cnt; k += 256) {
     double v = 0.0;
@@ -544,18 +546,16 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------
 __global__ void k_ifv_gather(const double* __restrict__ einfvec,
                              const int64_t* __restrict__ inv_ptr,
-                             const int64_t* __restrict__ inv_off,
-                             const int32_t* __restrict__ inv_nn, int nonode,
+                             const int64_t* __restrict__ inv_off, int nonode,
                              double* __restrict__ ifv) {
   int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= nonode) return;
   double fx = 0.0, fy = 0.0, fz = 0.0;
   for (int64_t p = inv_ptr[n]; p < inv_ptr[n + 1]; ++p) {
-    int64_t o = inv_off[p];
-    int nn = inv_nn[p];
+    int64_t o = inv_off[p];  // the node's 3 contiguous force words
     fx = fx + einfvec[o];
-    fy = fy + einfvec[o + nn];
-    fz = fz + einfvec[o + 2 * nn];
+    fy = fy + einfvec[o + 1];
+    fz = fz + einfvec[o + 2];
   }
   ifv[3 * n] = fx;
   ifv[3 * n + 1] = fy;
@@ -655,12 +655,12 @@ int launch_recover(const DevModel& M, const std::vector<BlockDesc>& hblocks,
 }
 
 int launch_ifv_gather(const double* einfvec, const int64_t* inv_ptr,
-                      const int64_t* inv_off, const int32_t* inv_nn,
-                      int nonode, double* ifv, hipStream_t s) {
+                      const int64_t* inv_off, int nonode, double* ifv,
+                      hipStream_t s) {
   int nb = (nonode + 255) / 256;
   if (nb > 0)
     hipLaunchKernelGGL(k_ifv_gather, dim3(nb), dim3(256), 0, s, einfvec,
-                       inv_ptr, inv_off, inv_nn, nonode, ifv);
+                       inv_ptr, inv_off, nonode, ifv);
   return hipGetLastError() == hipSuccess ? W3D_OK : W3D_ERR_DEVICE;
 }
 
