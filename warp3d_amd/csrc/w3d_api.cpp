@@ -60,6 +60,15 @@ void dfree(T*& p) {
   p = nullptr;
 }
 
+// ranks of an element's nodes by ascending global number
+void node_ranks(const int32_t* nodes, int nn, int* rank) {
+  for (int a = 0; a < nn; ++a) {
+    int r = 0;
+    for (int b = 0; b < nn; ++b) r += nodes[b] < nodes[a] ? 1 : 0;
+    rank[a] = r;
+  }
+}
+
 float hprime_real(float e, float et) {
   volatile float num = e * et;  // REAL arithmetic as elprp.f:872
   volatile float den = e - et;
@@ -101,6 +110,7 @@ struct w3d_ctx {
   CrystalDev* d_crys = nullptr;
   double* d_uddt = nullptr;
   WorkItem* d_work_mm10 = nullptr;
+  uint16_t* d_npb_tab = nullptr;
   int n_work_mm10 = 0;
   bool has_mm10 = false;
   double dt = 1.0;
@@ -149,6 +159,7 @@ struct w3d_ctx {
     M.evol = d_evol;
     M.flags = d_flags;
     M.crys = d_crys;
+    M.npb_tab = d_npb_tab;
     M.uddt = d_uddt;
     M.eps_bbar = eps_bbar;
     M.dt = dt;
@@ -239,6 +250,7 @@ void free_ctx(w3d_ctx* c) {
   dfree(c->d_blk_sums); dfree(c->d_u); dfree(c->d_du); dfree(c->d_ifv);
   dfree(c->d_asm_ptr); dfree(c->d_asm_src); dfree(c->d_kout);
   dfree(c->d_crys); dfree(c->d_uddt); dfree(c->d_work_mm10);
+  dfree(c->d_npb_tab);
   for (int ph = 0; ph < W3D_NUM_PHASES; ++ph) {
     for (int k = 0; k < 2; ++k) {
       if (c->ev_a[ph][k]) hipEventDestroy(c->ev_a[ph][k]);
@@ -492,6 +504,24 @@ int create_impl(w3d_ctx* c, const w3d_model* m) {
   if ((rc = upload(&c->d_mats, c->mats))) return rc;
   if ((rc = upload(&c->d_tables, c->tables))) return rc;
   if ((rc = upload(&c->d_conn, c->conn))) return rc;
+  {
+    // decode tables of the node-pair block layout (see npb_pos)
+    std::vector<uint16_t> tab(kNpbTabSize);
+    for (int nn : {8, 10, 20}) {
+      uint16_t* t = tab.data() + npb_tab_off(nn);
+      for (int r = 0; r < nn; ++r) {
+        int w = npb_row_start(nn, r);
+        const int dg[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+        for (int k = 0; k < 6; ++k)
+          t[w++] = (uint16_t)(r | r << 5 | dg[k][0] << 10 | dg[k][1] << 12);
+        for (int s = r + 1; s < nn; ++s)
+          for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+              t[w++] = (uint16_t)(r | s << 5 | i << 10 | j << 12);
+      }
+    }
+    if ((rc = upload(&c->d_npb_tab, tab))) return rc;
+  }
   if (c->has_mm10) {
     if ((rc = upload(&c->d_crys, crys))) return rc;
     if ((rc = dalloc(&c->d_uddt, 6 * q))) return rc;
@@ -945,15 +975,16 @@ int w3d_pattern(w3d_ctx* c, const int32_t* cstmap, int32_t* neqns_out,
       const BlockDesc& B = c->blocks[c->elem_blk[e]];
       int i = e - (B.felem - 1);
       const int32_t* nd_ = &c->conn[B.conn_off + (int64_t)i * B.nnode];
+      int rank[kMaxNodes];
+      node_ranks(nd_, B.nnode, rank);
       int erow = -1;
       for (int k = 0; k < B.totdof && erow < 0; ++k)
         if (c->dof_eqn_map[3 * nd_[k % B.nnode] + k / B.nnode] == srow) erow = k;
       for (int ecol = 0; ecol < B.totdof; ++ecol) {
         int scol = c->dof_eqn_map[3 * nd_[ecol % B.nnode] + ecol / B.nnode];
         if (scol < srow) continue;
-        int mx = std::max(ecol, erow), mn = std::min(ecol, erow);
         int64_t src = B.ek_off + (int64_t)i * B.utsz +
-                      (int64_t)mx * (mx + 1) / 2 + mn;
+                      npb_pos(B.nnode, rank, erow, ecol);
         int64_t t;
         if (scol == srow) {
           t = srow - 1;
@@ -1120,10 +1151,20 @@ int w3d_get_block_state(const w3d_ctx* c, int32_t blk, int32_t which,
       std::vector<float> f(n);
       HIPCHK(hipMemcpy(f.data(), c->d_ek32 + B.ek_off, n * 4,
                        hipMemcpyDeviceToHost));
-      for (int64_t k = 0; k < n; ++k) out[k] = f[k];
+      for (int64_t k = 0; k < n; ++k) tmp[k] = f[k];
     } else {
-      HIPCHK(hipMemcpy(out, c->d_ek64 + B.ek_off, n * 8,
+      HIPCHK(hipMemcpy(tmp.data(), c->d_ek64 + B.ek_off, n * 8,
                        hipMemcpyDeviceToHost));
+    }
+    // node-pair block order -> packed upper column-major (pos = cp(q) + p)
+    for (int64_t i = 0; i < S; ++i) {
+      int rank[kMaxNodes];
+      node_ranks(&c->conn[B.conn_off + i * B.nnode], B.nnode, rank);
+      const double* src = tmp.data() + i * B.utsz;
+      double* dst = out + i * B.utsz;
+      for (int q = 0; q < B.totdof; ++q)
+        for (int p = 0; p <= q; ++p)
+          dst[q * (q + 1) / 2 + p] = src[npb_pos(B.nnode, rank, p, q)];
     }
     return W3D_OK;
   }

@@ -7,8 +7,10 @@
 //     consecutive addresses.
 //   * element forces einfvec are element-contiguous and node-major inside
 //     the element: base + i*T + 3*j + c (component c of local node j).
-//   * element tangents are element-contiguous packed upper triangles, the
-//     reference's estiff_blocks (utsz, span) layout: base + i*utsz + pos.
+//   * element tangents are element-contiguous, utsz words each, in the
+//     node-pair block order described at npb_pos (base + i*utsz + word);
+//     w3d_get_block_state converts them to the reference's packed upper
+//     triangle (estiff_blocks (utsz, span)).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -109,7 +111,16 @@ struct DevModel {
   double* uddt;           // mm10: strain increments (6, points) per block
   double eps_bbar;
   double dt;              // time increment of the load step
+  // word -> (rank r, rank s, component i, component j) of the node-pair
+  // block layout, r | s << 5 | i << 10 | j << 12, for 8, 10 and 20 nodes
+  // at npb_tab_off(NN)
+  const uint16_t* npb_tab;
 };
+
+__host__ __device__ constexpr int npb_tab_off(int NN) {
+  return NN == 8 ? 0 : NN == 10 ? 300 : 765;
+}
+constexpr int kNpbTabSize = 300 + 465 + 1830;
 
 // Kernel launchers (w3d_kernels.hip, w3d_tangent.hip).
 // Recovery passes: pass 0 runs mm01 blocks in full and the strain
@@ -160,6 +171,33 @@ int group_key(int nnode, int ngp, int bbar, int mm10 = 0);
 // (rstgp1_store_cep order (1,1),(2,1),(2,2),(3,1),...).
 __host__ __device__ constexpr int tri(int r, int c) {
   return r >= c ? r * (r + 1) / 2 + c : c * (c + 1) / 2 + r;
+}
+
+// Element tangents are stored in a node-pair block layout: the element's
+// nodes ranked by ascending global number; block row r (the node of rank r)
+// holds the upper triangle of its 3x3 diagonal block (6 words, row-major:
+// (0,0) (0,1) (0,2) (1,1) (1,2) (2,2)) then the full 3x3 blocks with every
+// node of higher rank s = r+1 .. NN-1 (9 words each, row-major, rows the
+// components of the rank-r node).  The words of one element are exactly the
+// contributions it makes to the rows of each of its nodes in the assembled
+// upper triangle, so the row-by-row assembly reads them in runs.  Same size
+// as the packed upper triangle: 6 NN + 9 NN (NN-1)/2 = 3NN (3NN+1)/2.
+__host__ __device__ constexpr int npb_row_start(int NN, int r) {
+  return 6 * r + 9 * (r * (NN - 1) - r * (r - 1) / 2);
+}
+// Word of Ke(p, q) (local dofs, dof = component*NN + node) given the ranks.
+__host__ __device__ inline int npb_pos(int NN, const int* rank, int p, int q) {
+  int a = p % NN, i = p / NN, b = q % NN, j = q / NN;
+  int ra = rank[a], rb = rank[b];
+  if (ra == rb) {
+    int lo = i < j ? i : j, hi = i < j ? j : i;
+    return npb_row_start(NN, ra) + lo * (5 - lo) / 2 + hi;
+  }
+  if (ra > rb) {
+    int t = ra; ra = rb; rb = t;
+    t = i; i = j; j = t;
+  }
+  return npb_row_start(NN, ra) + 6 + 9 * (rb - ra - 1) + 3 * i + j;
 }
 
 // Upper 16x16 tiles (I <= J) of a tile grid, enumerated column by column.

@@ -68,6 +68,8 @@ __global__ void __launch_bounds__(64 * tan_mfma_waves(NN))
   __shared__ double s_gam[W][G][10];
   __shared__ double s_vol[W][24];
   __shared__ double s_d[W][G][21];
+  __shared__ int s_node[W][NN];
+  __shared__ int s_inv[W][NN];
   // per wave: B(r, dof) and BD(r, dof) = (D B)(r, dof) of the chunk's two
   // points; after the contraction the same slab stages the packed Ke
   __shared__ T s_bd[W][2][12][LDP];
@@ -98,10 +100,12 @@ __global__ void __launch_bounds__(64 * tan_mfma_waves(NN))
   // element data in flight: coordinates of node l (lanes < NN) and this
   // lane's share of the stored [D]
   double pc[3], pd[ND];
+  int pnode = 0;
   auto fetch = [&](int e) {
     const int ie = wi.first + e;
     if (l < NN) {
       int node = M.conn[bd.conn_off + (int64_t)ie * NN + l];
+      pnode = node;
       pc[0] = M.coords[3 * node];
       pc[1] = M.coords[3 * node + 1];
       pc[2] = M.coords[3 * node + 2];
@@ -114,6 +118,7 @@ __global__ void __launch_bounds__(64 * tan_mfma_waves(NN))
   };
   auto stage = [&]() {
     if (l < NN) {
+      s_node[wv][l] = pnode;
       s_ce[wv][l] = pc[0];
       s_ce[wv][NN + l] = pc[1];
       s_ce[wv][2 * NN + l] = pc[2];
@@ -258,8 +263,16 @@ __global__ void __launch_bounds__(64 * tan_mfma_waves(NN))
       }
       wave_sync();
     }
-    // P4: the upper triangle in the packed column-major layout, gathered
-    // in the wave's slab and stored contiguously
+    // P4: the upper triangle staged packed (column-major) in the wave's
+    // slab, then stored contiguously in the node-pair block order
+    // (npb_pos): word k gathers Ke(p, q) of the nodes of ranks r and s
+    if (l < NN) {
+      int r = 0;
+      const int mine = s_node[wv][l];
+#pragma unroll
+      for (int b = 0; b < NN; ++b) r += s_node[wv][b] < mine ? 1 : 0;
+      s_inv[wv][r] = l;
+    }
     T* stg = &s_bd[wv][0][0][0];
 #pragma unroll
     for (int t = 0; t < NUP; ++t) {
@@ -272,7 +285,18 @@ __global__ void __launch_bounds__(64 * tan_mfma_waves(NN))
     }
     wave_sync();
     T* out = ek_out + bd.ek_off + (int64_t)ie * (TD * (TD + 1) / 2);
-    for (int k = l; k < TD * (TD + 1) / 2; k += 64) out[k] = stg[k];
+    const uint16_t* tab = M.npb_tab + npb_tab_off(NN);
+    for (int k = l; k < TD * (TD + 1) / 2; k += 64) {
+      const int code = tab[k];
+      const int a = s_inv[wv][code & 31], b = s_inv[wv][(code >> 5) & 31];
+      int p = ((code >> 10) & 3) * NN + a, q = (code >> 12) * NN + b;
+      if (p > q) {
+        int x = p;
+        p = q;
+        q = x;
+      }
+      out[k] = stg[q * (q + 1) / 2 + p];
+    }
     wave_sync();
     if (more) stage();
   }
