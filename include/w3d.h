@@ -240,6 +240,26 @@ int w3d_get_pattern(const w3d_ctx* ctx, int32_t* dof_eqn_map,
 int w3d_assemble_vss(w3d_ctx* ctx, double* k_diag, double* k_coeffs);
 int w3d_assemble_vss_dev(w3d_ctx* ctx, double* d_k_diag, double* d_k_coeffs);
 
+/*
+ * Jacobi-preconditioned conjugate gradients, FP64, on the device, for the
+ * matrix of the last w3d_assemble_vss call (its outputs may be NULL to
+ * assemble on the device only): solves K x = rhs over the neqns free
+ * equations.  x holds the initial guess on entry and the solution on
+ * return.  Stops when ||rhs - K x|| <= rtol ||rhs|| (W3D_OK) or after maxit
+ * iterations (W3D_ERR_NUMERIC, x the last iterate); iters and relres (either
+ * may be NULL) receive the iterations and the final relative residual.
+ * Replaces the reference's assembled-equation solve (drive_assemble_solve.f
+ * :654 -> drive_pardiso.f:11; eqn_solve in mnralg.f:503).
+ */
+int w3d_solve_pcg(w3d_ctx* ctx, const double* rhs, double* x, double rtol,
+                  int32_t maxit, int32_t* iters, double* relres);
+/* Device-pointer form: the matrix is the d_k_diag / d_k_coeffs pair of a
+   w3d_assemble_vss_dev call. */
+int w3d_solve_pcg_dev(w3d_ctx* ctx, const double* d_k_diag,
+                      const double* d_k_coeffs, const double* d_rhs,
+                      double* d_x, double rtol, int32_t maxit,
+                      int32_t* iters, double* relres);
+
 /* n+1 -> n roll-over of stresses, strains and history (update.f:15). */
 int w3d_commit_step(w3d_ctx* ctx);
 /* Discard the n+1 state (adaptive step restart): n+1 <- n. */

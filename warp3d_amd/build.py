@@ -16,7 +16,8 @@ LIB = os.path.join(HERE, "libw3dblk.so")
 ARCH = "gfx950"
 
 SOURCES = ["w3d_api.cpp", "w3d_elements.cpp", "w3d_mm10_setup.cpp",
-           "w3d_kernels.hip", "w3d_tangent.hip", "w3d_mm10.hip"]
+           "w3d_kernels.hip", "w3d_tangent.hip", "w3d_mm10.hip",
+           "w3d_solver.hip"]
 
 
 def hipcc():
@@ -65,7 +66,43 @@ def build(verbose=False, jobs=8):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_fortran(verbose)
     return LIB
+
+
+FORTRAN = [os.path.join(HERE, "..", "include", "w3d.f90"),
+           os.path.join(HERE, "fortran", "w3d_newton.f90"),
+           os.path.join(HERE, "fortran", "w3d_drive.f90")]
+DRIVE = os.path.join(HERE, "bin", "w3d_drive")
+
+
+def amdflang():
+    for cand in (os.environ.get("AMDFLANG"), "/opt/rocm/bin/amdflang",
+                 shutil.which("amdflang")):
+        if cand and os.path.exists(cand):
+            return cand
+    return None
+
+
+def build_fortran(verbose=False):
+    """The Fortran ISO_C_BINDING module, the Newton driver and the w3d_drive
+    program, linked against libw3dblk.so (rpath relative to the program).
+    Skipped, with a message, when no Fortran compiler is installed."""
+    fc = amdflang()
+    if fc is None:
+        print("amdflang not found: Fortran driver not built", file=sys.stderr)
+        return None
+    mod = os.path.join(HERE, "_build", "fmod")
+    os.makedirs(mod, exist_ok=True)
+    os.makedirs(os.path.dirname(DRIVE), exist_ok=True)
+    tmp = DRIVE + ".tmp"
+    cmd = [fc, "-O2", "-J", mod] + FORTRAN + [
+        "-L", HERE, "-lw3dblk", "-Wl,-rpath,$ORIGIN/..", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, DRIVE)
+    return DRIVE
 
 
 if __name__ == "__main__":

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -129,6 +130,15 @@ struct w3d_ctx {
   int64_t* d_asm_ptr = nullptr;
   uint32_t* d_asm_src = nullptr;
   double* d_kout = nullptr;  // neqns diagonal then ncoeff coefficients
+  int64_t assembled = 0;     // serial of the last assembly
+  // full symmetric CSR of the assembled matrix for the solver: row
+  // pointers, columns (0-based), the d_kout index of each entry, values
+  int64_t csr_nnz = 0, csr_serial = -1;
+  int64_t* d_csr_rp = nullptr;
+  int32_t* d_csr_col = nullptr;
+  uint32_t* d_csr_vidx = nullptr;
+  double* d_csr_val = nullptr;
+  double* d_pcg = nullptr;   // solver work vectors and scalars
   int tangent_precision = -1;
   double eps_bbar = 0.0;
   // phase timing (w3d_set_profiling)
@@ -249,6 +259,8 @@ void free_ctx(w3d_ctx* c) {
   dfree(c->d_blk_wfirst); dfree(c->d_blk_wcount);
   dfree(c->d_blk_sums); dfree(c->d_u); dfree(c->d_du); dfree(c->d_ifv);
   dfree(c->d_asm_ptr); dfree(c->d_asm_src); dfree(c->d_kout);
+  dfree(c->d_csr_rp); dfree(c->d_csr_col); dfree(c->d_csr_vidx);
+  dfree(c->d_csr_val); dfree(c->d_pcg);
   dfree(c->d_crys); dfree(c->d_uddt); dfree(c->d_work_mm10);
   dfree(c->d_npb_tab);
   for (int ph = 0; ph < W3D_NUM_PHASES; ++ph) {
@@ -746,6 +758,81 @@ int recover_impl(w3d_ctx* c, int step, int iter, int extrapolated,
   return check_flags(c);
 }
 
+// Jacobi-PCG on an assembled matrix (diag, coef device arrays); b and x
+// device arrays of neqns (x: initial guess in, solution out).
+int pcg_impl(w3d_ctx* c, const double* d_diag, const double* d_coef,
+             const double* d_b, double* d_x, double rtol, int maxit,
+             int32_t* iters, double* relres) {
+  const int n = c->neqns;
+  hipStream_t s = c->stream;
+  double* r = c->d_pcg;
+  double* z = r + n;
+  double* p = z + n;
+  double* q = p + n;
+  double* sc = q + 3 * (int64_t)n;  // rz_old, rz, pq, rr, bb
+  double* part = sc + 8;
+  int rc = solver_csr_values(d_diag, d_coef, n, c->d_csr_vidx, c->csr_nnz,
+                             c->d_csr_val, s);
+  auto dot = [&](const double* a, const double* b, double* out) {
+    return solver_dot(n, a, b, part, out, s);
+  };
+  auto read = [&](int k, double* v) -> int {
+    if (hipMemcpyAsync(v, sc + k, sizeof(double), hipMemcpyDeviceToHost,
+                       s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return W3D_ERR_DEVICE;
+    return W3D_OK;
+  };
+  // r = b - A x
+  if (!rc) rc = solver_spmv(n, c->d_csr_rp, c->d_csr_col, c->d_csr_val, d_x,
+                            r, s);
+  if (!rc) rc = solver_residual(n, d_b, r, s);
+  if (!rc) rc = dot(d_b, d_b, sc + 4);
+  if (!rc) rc = dot(r, r, sc + 3);
+  double bb = 0.0, rr = 0.0;
+  if (!rc) rc = read(4, &bb);
+  if (!rc) rc = read(3, &rr);
+  if (rc) return fail(rc, "w3d_solve_pcg: device failure");
+  const double bnorm = std::sqrt(bb);
+  int it = 0;
+  if (bnorm == 0.0) {
+    HIPCHK(hipMemsetAsync(d_x, 0, n * sizeof(double), s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (iters) *iters = 0;
+    if (relres) *relres = 0.0;
+    return W3D_OK;
+  }
+  if (std::sqrt(rr) > rtol * bnorm) {
+    rc = solver_jacobi(n, d_diag, r, z, s);
+    if (!rc) rc = dot(r, z, sc + 1);
+    if (!rc && hipMemcpyAsync(p, z, n * sizeof(double),
+                              hipMemcpyDeviceToDevice, s) != hipSuccess)
+      rc = W3D_ERR_DEVICE;
+    while (!rc && it < maxit) {
+      it = it + 1;
+      rc = solver_spmv(n, c->d_csr_rp, c->d_csr_col, c->d_csr_val, p, q, s);
+      if (!rc) rc = dot(p, q, sc + 2);
+      if (!rc) rc = solver_update_xr(n, p, q, sc, d_x, r, s);
+      if (!rc) rc = dot(r, r, sc + 3);
+      if (!rc) rc = read(3, &rr);
+      if (rc || !(std::sqrt(rr) > rtol * bnorm)) break;
+      rc = solver_jacobi(n, d_diag, r, z, s);
+      if (!rc) rc = solver_shift(sc, s);
+      if (!rc) rc = dot(r, z, sc + 1);
+      if (!rc) rc = solver_update_p(n, z, sc, p, s);
+    }
+  }
+  if (rc) return fail(rc, "w3d_solve_pcg: device failure");
+  if (iters) *iters = it;
+  if (relres) *relres = std::sqrt(rr) / bnorm;
+  if (!(std::sqrt(rr) <= rtol * bnorm))
+    return fail(W3D_ERR_NUMERIC,
+                "w3d_solve_pcg: no convergence in " + std::to_string(it) +
+                    " iterations (relative residual " +
+                    std::to_string(std::sqrt(rr) / bnorm) + ")");
+  return W3D_OK;
+}
+
 // state views ----------------------------------------------------------
 struct View {
   double* d = nullptr;
@@ -1015,6 +1102,54 @@ int w3d_pattern(w3d_ctx* c, const int32_t* cstmap, int32_t* neqns_out,
   if ((rc = upload(&c->d_asm_ptr, cnt))) return rc;
   if ((rc = upload(&c->d_asm_src, src))) return rc;
   if ((rc = dalloc(&c->d_kout, nout))) return rc;
+  // full symmetric CSR for the solver: row j = the transposed upper entries
+  // (i, j) of rows i < j (ascending i), the diagonal, then row j's upper
+  // entries (ascending columns)
+  {
+    std::vector<int64_t> low(neq + 1, 0);
+    for (int64_t k = 0; k < c->ncoeff; ++k) low[c->k_indexes[k] - 1 + 1] += 1;
+    std::vector<int64_t> rp(neq + 1, 0);
+    for (int r = 0; r < neq; ++r)
+      rp[r + 1] = rp[r] + low[r + 1] + 1 + c->k_ptrs[r];
+    const int64_t nnz = rp[neq];
+    std::vector<int32_t> col(nnz);
+    std::vector<uint32_t> vidx(nnz);
+    std::vector<int64_t> fill(neq);
+    for (int r = 0; r < neq; ++r) fill[r] = rp[r];
+    for (int r = 0; r < neq; ++r) {
+      for (int64_t k = c->row_start[r]; k < c->row_start[r + 1]; ++k) {
+        int j = c->k_indexes[k] - 1;
+        int64_t f = fill[j]++;
+        col[f] = r;
+        vidx[f] = (uint32_t)(neq + k);
+      }
+    }
+    for (int r = 0; r < neq; ++r) {
+      int64_t f = fill[r];
+      col[f] = r;
+      vidx[f] = (uint32_t)r;
+      ++f;
+      for (int64_t k = c->row_start[r]; k < c->row_start[r + 1]; ++k, ++f) {
+        col[f] = c->k_indexes[k] - 1;
+        vidx[f] = (uint32_t)(neq + k);
+      }
+    }
+    dfree(c->d_csr_rp);
+    dfree(c->d_csr_col);
+    dfree(c->d_csr_vidx);
+    dfree(c->d_csr_val);
+    dfree(c->d_pcg);
+    if ((rc = upload(&c->d_csr_rp, rp))) return rc;
+    if ((rc = upload(&c->d_csr_col, col))) return rc;
+    if ((rc = upload(&c->d_csr_vidx, vidx))) return rc;
+    if ((rc = dalloc(&c->d_csr_val, nnz))) return rc;
+    // r, z, p, q, b, x (6 neq) + partials + 4 scalars
+    if ((rc = dalloc(&c->d_pcg, 6 * (int64_t)neq + solver_npart(neq) + 8)))
+      return rc;
+    c->csr_nnz = nnz;
+    c->csr_serial = -1;
+    c->assembled = 0;
+  }
   if (neqns_out) *neqns_out = neq;
   if (ncoeff_out) *ncoeff_out = c->ncoeff;
   return W3D_OK;
@@ -1068,6 +1203,7 @@ int w3d_assemble_vss(w3d_ctx* c, double* k_diag, double* k_coeffs) {
                            c->d_kout + c->neqns, c->stream);
   prof_end(c, W3D_PHASE_ASSEMBLE);
   if (rc) return fail(rc, "w3d_assemble: kernel launch failed");
+  c->assembled += 1;
   if (k_diag)
     HIPCHK(hipMemcpyAsync(k_diag, c->d_kout, c->neqns * sizeof(double),
                           hipMemcpyDeviceToHost, c->stream));
@@ -1213,6 +1349,46 @@ int w3d_set_block_state(w3d_ctx* c, int32_t blk, int32_t which,
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipMemcpy(dst, tmp.data(), n * 8, hipMemcpyHostToDevice));
   return W3D_OK;
+}
+
+int w3d_solve_pcg_dev(w3d_ctx* c, const double* d_k_diag,
+                      const double* d_k_coeffs, const double* d_rhs,
+                      double* d_x, double rtol, int32_t maxit, int32_t* iters,
+                      double* relres) {
+  if (!c || !c->d_csr_rp) return fail(W3D_ERR_ARG, "w3d_solve_pcg: no pattern");
+  if (!d_k_diag || !d_rhs || !d_x || (!d_k_coeffs && c->ncoeff))
+    return fail(W3D_ERR_ARG, "w3d_solve_pcg_dev: null argument");
+  if (!(rtol > 0.0) || maxit < 1)
+    return fail(W3D_ERR_ARG, "w3d_solve_pcg: rtol > 0 and maxit >= 1");
+  HIPCHK(hipSetDevice(c->device));
+  return pcg_impl(c, d_k_diag, d_k_coeffs, d_rhs, d_x, rtol, maxit, iters,
+                  relres);
+}
+
+int w3d_solve_pcg(w3d_ctx* c, const double* rhs, double* x, double rtol,
+                  int32_t maxit, int32_t* iters, double* relres) {
+  if (!c || !c->d_csr_rp) return fail(W3D_ERR_ARG, "w3d_solve_pcg: no pattern");
+  if (!c->assembled)
+    return fail(W3D_ERR_ARG, "w3d_solve_pcg: no w3d_assemble_vss since the "
+                             "pattern");
+  if (!rhs || !x) return fail(W3D_ERR_ARG, "w3d_solve_pcg: null argument");
+  if (!(rtol > 0.0) || maxit < 1)
+    return fail(W3D_ERR_ARG, "w3d_solve_pcg: rtol > 0 and maxit >= 1");
+  HIPCHK(hipSetDevice(c->device));
+  const int n = c->neqns;
+  double* b = c->d_pcg + 4 * (int64_t)n;
+  double* xd = b + n;
+  HIPCHK(hipMemcpyAsync(b, rhs, n * sizeof(double), hipMemcpyHostToDevice,
+                        c->stream));
+  HIPCHK(hipMemcpyAsync(xd, x, n * sizeof(double), hipMemcpyHostToDevice,
+                        c->stream));
+  int rc = pcg_impl(c, c->d_kout, c->d_kout + n, b, xd, rtol, maxit, iters,
+                    relres);
+  if (rc < 0 && rc != W3D_ERR_NUMERIC) return rc;
+  HIPCHK(hipMemcpyAsync(x, xd, n * sizeof(double), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return rc;
 }
 
 int w3d_set_time_increment(w3d_ctx* c, double dt) {

@@ -158,6 +158,24 @@ __host__ __device__ constexpr int tan_mfma_chunk(int nnode) {
   return 8 * tan_mfma_waves(nnode);
 }
 
+// Jacobi-PCG building blocks (w3d_solver.hip)
+int solver_csr_values(const double* diag, const double* coef, int neq,
+                      const uint32_t* vidx, int64_t nnz, double* vals,
+                      hipStream_t s);
+int solver_spmv(int n, const int64_t* rp, const int32_t* col,
+                const double* val, const double* x, double* y, hipStream_t s);
+int solver_npart(int n);
+int solver_dot(int n, const double* a, const double* b, double* part,
+               double* out, hipStream_t s);
+int solver_jacobi(int n, const double* diag, const double* r, double* z,
+                  hipStream_t s);
+int solver_update_p(int n, const double* z, const double* sc, double* p,
+                    hipStream_t s);
+int solver_update_xr(int n, const double* p, const double* q,
+                     const double* sc, double* x, double* r, hipStream_t s);
+int solver_residual(int n, const double* b, double* r, hipStream_t s);
+int solver_shift(double* sc, hipStream_t s);
+
 // Host set-up of one crystal record from the material and its angles
 // (w3d_mm10_setup.cpp); false for an unsupported slip / elastic type.
 bool mm10_crystal_setup(const w3d_material& m, const double* angles,

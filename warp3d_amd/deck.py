@@ -1,0 +1,90 @@
+"""Binary decks and results of the Fortran load-history driver
+(warp3d_amd/bin/w3d_drive, built from warp3d_amd/fortran/w3d_drive.f90).
+
+A deck holds a model (the w3d_model arrays and materials), the absolute
+constraints and the imposed displacement increments of every load step; the
+driver runs each step with the Fortran Newton loop over libw3dblk on the GPU
+and writes, per step, its status, iteration count, PCG iterations, test-2
+residual ratios, energy, plastic work and the displacements after the step.
+"""
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DRIVE = os.path.join(HERE, "bin", "w3d_drive")
+
+
+def write_deck(path, arrays, cstmap, du_steps, precision=0, res_tol=0.01,
+               mniter=1, mxiter=15, pcg_rtol=1e-12, pcg_maxit=50000,
+               dt=1.0):
+    """arrays: warp3d_amd._abi.ModelArrays (single-type connectivity);
+    du_steps: (nsteps, 3*nonode) imposed increments (zero off the
+    constrained dofs)."""
+    nnpe = int(np.diff(arrays.incmap).max()) if arrays.noelem > 1 else \
+        arrays.incid.size
+    if not np.array_equal(arrays.incmap,
+                          np.arange(arrays.noelem) * nnpe + 1):
+        raise ValueError("write_deck: one element type per deck")
+    du_steps = np.ascontiguousarray(du_steps, dtype=np.float64)
+    nsteps = du_steps.shape[0]
+    with open(path, "wb") as f:
+        np.array([arrays.nonode, arrays.noelem, nnpe,
+                  len(arrays.materials), nsteps, precision, mniter, mxiter,
+                  arrays.auto_size, pcg_maxit], np.int32).tofile(f)
+        np.array([res_tol, pcg_rtol, dt], np.float64).tofile(f)
+        arrays.coords.astype(np.float64).tofile(f)
+        for a in (arrays.incid, arrays.etype, arrays.int_order, arrays.matnum,
+                  arrays.bbar):
+            np.asarray(a, np.int32).tofile(f)
+        for m in arrays.materials:
+            np.array([m.model, m.linear_elastic, m.slip_type, m.elastic_type,
+                      m.miter, m.alter_mode, m.angle_type, m.ncrystals],
+                     np.int32).tofile(f)
+            np.array([m.e, m.nu, m.beta, m.tan_e, m.yld_pt],
+                     np.float32).tofile(f)
+            np.array([m.cp_e, m.cp_nu, m.cp_mu, m.harden_n, m.theta_0,
+                      m.tau_y, m.tau_v, m.voce_m, m.iD_v, m.eps_dot_0_y,
+                      m.atol, m.atol1, m.rtol, m.rtol1],
+                     np.float64).tofile(f)
+        has = arrays.angles is not None
+        np.array([int(has), arrays.angle_stride], np.int32).tofile(f)
+        if has:
+            arrays.angles.astype(np.float64).tofile(f)
+        np.asarray(cstmap, np.int32).tofile(f)
+        du_steps.tofile(f)
+
+
+def read_result(path):
+    """List of per-step dicts (status, converged, iters, pcg_iters, ratio,
+    energy, plastic_work, u)."""
+    steps = []
+    with open(path, "rb") as f:
+        nsteps, nodof = np.fromfile(f, np.int32, 2)
+        for _ in range(nsteps):
+            head = np.fromfile(f, np.int32, 3)
+            if head.size < 3:
+                break
+            status, conv, iters = (int(x) for x in head)
+            pcg = np.fromfile(f, np.int32, iters)
+            ratio = np.fromfile(f, np.float64, iters)
+            en, pw = np.fromfile(f, np.float64, 2)
+            u = np.fromfile(f, np.float64, nodof)
+            steps.append(dict(status=status, converged=bool(conv),
+                              iters=iters, pcg_iters=pcg, ratio=ratio,
+                              energy=en, plastic_work=pw, u=u))
+    return steps
+
+
+def run(deck_path, result_path, device=0, timeout=600):
+    """Run the Fortran driver on a deck (a child process)."""
+    if not os.path.exists(DRIVE):
+        raise RuntimeError("w3d_drive is not built (python -m "
+                           "warp3d_amd.build)")
+    p = subprocess.run([DRIVE, deck_path, result_path, str(device)],
+                       capture_output=True, text=True, timeout=timeout)
+    if p.returncode != 0:
+        raise RuntimeError(f"w3d_drive failed ({p.returncode}):\n{p.stdout}"
+                           f"\n{p.stderr}")
+    return p.stdout, read_result(result_path)
