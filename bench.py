@@ -12,6 +12,14 @@ Workloads (all 1,728,000 Gauss points):
         mm01 (E = 70 GPa, nu = 0.33, sigma_y = 300 MPa, H' = E/50), a
         bending-style field (axial strain varying linearly through the
         thickness, 1 % peak over 15 steps).
+  mixed: 150,000 elements in blocks of seeded spans 32-256 (1,935,000 Gauss
+        points): 75,000 hex8 (2x2x2, B-bar), 45,000 hex20 (3x3x3), 30,000
+        tet10 (4 points), three boxes side by side; mm10 fcc crystal
+        plasticity with Voce hardening (tau_y = 50 MPa, tau_v = 100 MPa, so
+        the slip resistance saturates at 150 MPa; theta_0 = 250 MPa,
+        m = 1, n = 20; cubic elasticity of copper: e = 66.7 GPa, nu = 0.42,
+        mu = 75 GPa), seeded SO(3)-uniform orientations, 0.1 % axial strain
+        per step of a 10-step ramp to 1 %.
 
 One step = one Newton iteration of the element-block hot path of load step
 2 (after a step-1 pseudo-update, update and commit, so every history word is
@@ -23,7 +31,7 @@ FP32 tangent (the library's mixed-precision path), reported beside the plain
 FP64 path measured in the same run.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
-                    [--workload hex8|hex20]
+                    [--workload hex8|hex20|mixed]
 """
 import argparse
 import json
@@ -42,7 +50,8 @@ HBM_PEAK_GBS = 8000.0         # MI355X HBM3E
 def kernel_name(phase, workload, fp32=True):
     """The kernel (as the rocprofv3 summaries name it) of a phase of the
     reported FP32-tangent path."""
-    tmpl = {"hex8": "8, 8, true", "hex20": "20, 27, false"}[workload]
+    tmpl = {"hex8": "8, 8, true", "hex20": "20, 27, false",
+            "mixed": "8, 8, true"}[workload]
     prec = "float" if fp32 else "double"
     return {"recover": f"k_recover<{tmpl}, 0>",
             "ifv_gather": "k_ifv_gather",
@@ -65,15 +74,50 @@ WORKLOADS = {
     "hex20": "hex20 40x40x40 box, 64000 elements, 1728000 Gauss points "
              "(3x3x3), mm01, blocks of 128, one Newton iteration of the "
              "element-block path per step",
+    "mixed": "150000 elements (75000 hex8 2x2x2 B-bar, 45000 hex20 3x3x3, "
+             "30000 tet10 4-point), 1935000 Gauss points, blocks of seeded "
+             "spans 32-256, mm10 fcc Voce crystal plasticity with seeded "
+             "SO(3)-uniform orientations, one Newton iteration of the "
+             "element-block path per step",
 }
 
 
 def build_workload(name, seed=2024):
-    """Mesh, model arrays, constraints and the step increment du (the
-    displacement increment of one load step)."""
+    """Mesh, model arrays, constraints, the step increment du (the
+    displacement increment of one load step) and the element parts
+    (count, nodes, Gauss points, mm10)."""
     from warp3d_amd import _abi
     from warp3d_amd import mesh as M
     rng = np.random.default_rng(seed)
+    if name == "mixed":
+        h8 = M.box_hex8(50, 50, 30, size=(1.0, 1.0, 0.6), jitter=0.05,
+                        seed=seed)
+        h20 = M.box_hex20(30, 30, 50, size=(0.6, 0.6, 1.0), jitter=0.05,
+                          seed=seed + 1)
+        t10 = M.box_tet10(20, 25, 10, size=(0.4, 0.5, 0.2), jitter=0.05,
+                          seed=seed + 2)
+        mm = M.MixedMesh([(h8, 1, 1), (h20, 1, 0), (t10, 4, 0)])
+        mat = _abi.mm10_material(tau_y=50.0, tau_v=100.0, theta_0=250.0,
+                                 voce_m=1.0, harden_n=20.0, elastic="cubic",
+                                 e=66.7e3, nu=0.42, mu=75.0e3)
+        ang = M.random_orientations(mm.noelem, seed=seed)
+        arr = mm.model(mat, elblks=mm.seeded_blocks(32, 256, seed=seed),
+                       angles=ang)
+        x = mm.coords
+        base = np.abs(x[:, 2]) < 1e-9
+        cst = np.zeros(3 * mm.nonode, np.int32)
+        for k in range(3):
+            cst[k::3][base] = 1
+        eps = 0.01 / 10
+        du = np.zeros_like(x)
+        du[:, 2] = eps * x[:, 2]
+        du[:, 0] = -0.3 * eps * x[:, 2] * (x[:, 0] - x[:, 0].mean()) * 0.1
+        du += 0.05 * eps * rng.uniform(-1, 1, size=du.shape) * \
+            (x[:, 2:3] > 0) * 0.02
+        du[base] = 0.0
+        parts = [(h8.noelem, 8, 8, True), (h20.noelem, 20, 27, True),
+                 (t10.noelem, 10, 4, True)]
+        return mm, arr, cst, du.ravel(), parts
     if name == "hex8":
         m = M.box_hex8(60, jitter=0.05, seed=seed)
         mat = _abi.mm01_material(200e3, 0.3, 350.0, hprime=2000.0)
@@ -106,32 +150,49 @@ def build_workload(name, seed=2024):
         du += 0.05 * eps * rng.uniform(-1, 1, size=du.shape) * \
             (x[:, 2:3] > 0)
     du[base] = 0.0
-    return m, arr, cst, du.ravel()
+    parts = [(m.noelem, m.conn.shape[1], 8 if name == "hex8" else 27, False)]
+    return m, arr, cst, du.ravel(), parts
 
 
-def algorithmic(m, ngp_e, ncontrib, nout, ninv, ke_bytes):
-    """Algorithmic work per launch of each phase (bytes or flops)."""
-    ne = m.noelem
-    nn = m.conn.shape[1]
-    T = 3 * nn
-    utsz = T * (T + 1) // 2
-    ngp = ne * ngp_e
-    return {
-        # per point: sigma_n 9 + eps_n 6 + hist_n 11 read, sigma 9 + eps 6 +
-        # hist 11 + [D] 21 written = 73 words; per element: coordinates and
-        # increments gathered (2 x T words), connectivity (NN ints),
-        # element forces T words and volume 1 word written
-        "recover": ("hbm", ngp * 73 * 8 +
-                    ne * ((3 * T + 1) * 8 + 4 * nn)),
+def algorithmic(parts, nonode, ncontrib, nout, ninv, ke_bytes):
+    """Algorithmic work per launch of each phase (bytes or flops), summed
+    over the element parts (count, nodes, Gauss points, mm10)."""
+    rec = tan = mm10 = 0
+    for ne, nn, ngp_e, is_mm10 in parts:
+        T = 3 * nn
+        utsz = T * (T + 1) // 2
+        ngp = ne * ngp_e
+        # per element: coordinates and increments gathered (2 x T words),
+        # connectivity (NN ints), element forces T words and volume 1 word
+        elem = ne * ((3 * T + 1) * 8 + 4 * nn)
+        if is_mm10:
+            # two passes: eps_n read, eps_n1 and the increments written;
+            # then sigma_n1 (8 words) read for the forces (and the element
+            # gathers twice)
+            rec += ngp * (6 + 6 + 6 + 8) * 8 + 2 * elem
+            # history 158 words read and written, sigma_n and the
+            # increments read, sigma_n1 and [D] written
+            mm10 += ngp * (2 * 158 + 9 + 6 + 9 + 21) * 8
+        else:
+            # per point: sigma_n 9 + eps_n 6 + hist_n 11 read, sigma 9 +
+            # eps 6 + hist 11 + [D] 21 written = 73 words
+            rec += ngp * 73 * 8 + elem
+        # upper-triangle B^T D B as bdbtgen forms it: 2(36 T + 6 utsz)
+        tan += ngp * 2 * (36 * T + 6 * utsz)
+    out = {
+        "recover": ("hbm", rec),
         # per (node, element) pair: its 8-byte offset and 3 force words
         # read; per structure node 3 words written
-        "ifv_gather": ("hbm", ninv * (8 + 24) + m.nonode * 24),
-        # upper-triangle B^T D B as bdbtgen forms it: 2(36 T + 6 utsz)
-        "tangent": ("flop", ngp * 2 * (36 * T + 6 * utsz)),
-        # one index + one Ke word per contribution, the output pointer and
-        # the assembled word per output
-        "assemble": ("hbm", ncontrib * (4 + ke_bytes) + nout * (8 + 8)),
+        "ifv_gather": ("hbm", ninv * (8 + 24) + nonode * 24),
+        "tangent": ("flop", tan),
+        # the unavoidable traffic: every Ke word read once, every assembled
+        # word written once (the segment map's headers and indices, about
+        # 1.5 bytes per Ke word, count as extra traffic)
+        "assemble": ("hbm", ncontrib * ke_bytes + nout * 8),
     }
+    if mm10:
+        out["mm10"] = ("hbm", mm10)
+    return out
 
 
 def profiled_traffic(kernel, workload):
@@ -198,15 +259,20 @@ def run_device(arr, cst, du, steps, warmup, precision):
     return (t1 - t0) * 1e3 / steps, phases, info
 
 
-def contrib_counts(m, cst):
-    """Sizes of the assembly map (same rules as w3d_pattern)."""
-    # upper-triangle contributions per element: pairs (erow, ecol) with both
-    # dofs free and scol >= srow; for a brick every free pair counts once
-    conn = m.conn - 1
-    dofs = np.concatenate([conn * 3, conn * 3 + 1, conn * 3 + 2], axis=1)
-    free = cst[dofs] == 0
-    nf = free.sum(axis=1)
-    return int(np.sum(nf * (nf + 1) // 2))
+def contrib_counts(arr, cst):
+    """Ke words the assembly reads: per element, the pairs (erow, ecol)
+    with both dofs free and scol >= srow (every free pair once)."""
+    from warp3d_amd import _abi
+    total = 0
+    for et in np.unique(arr.etype):
+        nn = _abi.NNODE[int(et)]
+        sel = np.nonzero(arr.etype == et)[0]
+        idx = arr.incmap[sel][:, None] - 1 + np.arange(nn)[None, :]
+        conn = arr.incid[idx] - 1
+        dofs = np.concatenate([conn * 3, conn * 3 + 1, conn * 3 + 2], axis=1)
+        nf = (cst[dofs] == 0).sum(axis=1)
+        total += int(np.sum(nf * (nf + 1) // 2))
+    return total
 
 
 def cpu_baseline(arr, du, ngp, budget_s=10.0):
@@ -269,9 +335,8 @@ def main():
         b.free()
         log(f"copy probe: 3 x {args.copy_probe} MiB device-to-device")
     t = time.perf_counter()
-    m, arr, cst, du = build_workload(args.workload)
-    ngp_e = 8 if args.workload == "hex8" else 27
-    ngp = m.noelem * ngp_e
+    m, arr, cst, du, parts = build_workload(args.workload)
+    ngp = sum(ne * g for ne, _, g, _ in parts)
     log(f"workload {args.workload}: {m.noelem} elements, {ngp} points, "
         f"{m.nonode} nodes ({time.perf_counter() - t:.1f} s)")
     res = {}
@@ -286,10 +351,10 @@ def main():
             f"  ({time.perf_counter() - t:.1f} s)")
     ms32, ph32, info = res["fp32"]
     ms64, ph64, _ = res["fp64"]
-    ninv = int(m.conn.size)
-    ncontrib = contrib_counts(m, cst)
+    ninv = int(arr.incid.size)
+    ncontrib = contrib_counts(arr, cst)
     nout = info["neqns"] + info["ncoeff"]
-    alg = algorithmic(m, ngp_e, ncontrib, nout, ninv, 4)
+    alg = algorithmic(parts, m.nonode, ncontrib, nout, ninv, 4)
     # dominant kernel of the reported path
     avg = {k: v[1] / v[0] for k, v in ph32.items() if v[0]}
     dom = max(avg, key=avg.get)
@@ -312,7 +377,7 @@ def main():
     roof["kernel"] = kernel_name(dom, args.workload)
     phase_report = {}
     for tag, ph, kb in (("fp32", ph32, 4), ("fp64", ph64, 8)):
-        a = algorithmic(m, ngp_e, ncontrib, nout, ninv, kb)
+        a = algorithmic(parts, m.nonode, ncontrib, nout, ninv, kb)
         rep = {}
         for k, (n, tot) in ph.items():
             if not n:

@@ -23,7 +23,7 @@ module w3d
   integer(c_int32_t), parameter :: W3D_STRESS_N = 0, W3D_STRESS_N1 = 1, &
        W3D_STRAIN_N = 2, W3D_STRAIN_N1 = 3, W3D_HIST_N = 4, &
        W3D_HIST_N1 = 5, W3D_CEP = 6, W3D_ESTIFF = 7, W3D_EINFVEC = 8, &
-       W3D_ELEM_VOL = 9
+       W3D_ELEM_VOL = 9, W3D_ROT_N1 = 10
 
   ! struct w3d_material
   type, bind(C) :: w3d_material
@@ -45,7 +45,9 @@ module w3d
   type, bind(C) :: w3d_model
     integer(c_int32_t) :: abi_version = W3D_ABI_VERSION
     integer(c_int32_t) :: nonode = 0, noelem = 0
-    type(c_ptr) :: c, incid, incmap, etype, int_order, matnum, bbar, angles
+    type(c_ptr) :: c, incid, incmap, etype, int_order, matnum, bbar
+    type(c_ptr) :: geonl = c_null_ptr  ! NULL: every element small-strain
+    type(c_ptr) :: angles
     integer(c_int32_t) :: angle_stride = 1
     integer(c_int32_t) :: nmat = 0
     type(c_ptr) :: mats

@@ -39,7 +39,7 @@
 extern "C" {
 #endif
 
-#define W3D_ABI_VERSION 2
+#define W3D_ABI_VERSION 3
 
 /* ---- status codes ---------------------------------------------------- */
 enum {
@@ -83,7 +83,9 @@ enum {
   W3D_CEP = 6,        /* cep_blocks      (21, span, ngp)                  */
   W3D_ESTIFF = 7,     /* estiff_blocks   (utsz, span)                     */
   W3D_EINFVEC = 8,    /* einfvec_blocks  (span, totdof)                   */
-  W3D_ELEM_VOL = 9    /* element_vol_blocks (span)                        */
+  W3D_ELEM_VOL = 9,   /* element_vol_blocks (span)                        */
+  W3D_ROT_N1 = 10     /* rot_n1_blocks   (9, ngp, span): [R n+1] of F = RU,
+                         column-major; finite-strain blocks only         */
 };
 
 /*
@@ -137,6 +139,8 @@ typedef struct w3d_model {
   const int32_t* int_order; /* (noelem) integration order, iprops(5)        */
   const int32_t* matnum;    /* (noelem) 1-based index into mats             */
   const int32_t* bbar;      /* (noelem) mean-dilatation B-bar (hex8 only)   */
+  const int32_t* geonl;     /* (noelem) finite strains and rotations, the
+                               reference's lprops(18); NULL: all small     */
   const double* angles;     /* crystal Euler angles: crystal c of element e
                                at angles[3*(angle_stride*(e-1) + c) + k];
                                may be NULL when no element uses mm10        */

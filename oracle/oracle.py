@@ -64,6 +64,18 @@ def lib():
         L.orc_mm10_rt2rve.argtypes = [_D, _D]
         L.orc_mm10_rt2rvw.argtypes = [_D, _D]
         L.orc_mm10_expmw.argtypes = [_D, _D]
+        L.orc_lsq4.argtypes = [C.c_int, _D, _D, _D]
+        L.orc_tcomp1.argtypes = [C.c_int] + [_D] * 6
+        L.orc_fcomp1.argtypes = [_D, _D, _D]
+        L.orc_fcomp1.restype = C.c_int
+        L.orc_evcmp1.argtypes = [_D, _D]
+        L.orc_irscp1.argtypes = [_D, _D]
+        L.orc_rtcmp1.argtypes = [_D, _D]
+        L.orc_getrm1.argtypes = [_D, C.c_int, _D]
+        L.orc_ctran1.argtypes = [_D, _D, _D, C.c_int, C.c_double]
+        L.orc_kgstiff.argtypes = [C.c_int] + [_D] * 6 + \
+            [C.c_double, C.c_double, _D]
+        L.orc_mm10_calc_grads.argtypes = [C.c_int] * 5 + [_D, _D]
         _LIB = L
     return _LIB
 
@@ -208,6 +220,98 @@ def mm10_rt2rve(rt):
     return rv.reshape(6, 6, order="F")
 
 
+def lsq4(a, b):
+    """min |A x - b| for A (m x 4), as DGELS('N')."""
+    a = np.asfortranarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(4)
+    lib().orc_lsq4(a.shape[0], dptr(a.ravel(order="F")), dptr(b), dptr(x))
+    return x
+
+
+def mm10_calc_grads(etype, order, ngp_, span, hist, hloc):
+    """grad Fe words (27) of a small-strain single-crystal block from its
+    n-history hloc (hist, ngp, span) in Fortran order."""
+    h = np.asfortranarray(hloc, dtype=np.float64)
+    out = np.zeros(27)
+    lib().orc_mm10_calc_grads(etype, order, ngp_, span, hist,
+                              dptr(h.ravel(order="F")), dptr(out))
+    return out
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def tcomp1(gama, nxi, neta, nzeta, ue):
+    """Displacement gradient theta(9) at a point (gama row-major)."""
+    th = np.zeros(9)
+    g, a, b, c, u = (_f64(gama).ravel(), _f64(nxi), _f64(neta), _f64(nzeta),
+                     _f64(ue))
+    lib().orc_tcomp1(a.size, dptr(g), dptr(a), dptr(b), dptr(c), dptr(u),
+                     dptr(th))
+    return th
+
+
+def fcomp1(theta):
+    """(F row-major 3x3, det F, error flag)."""
+    f = np.zeros(9)
+    df = np.zeros(1)
+    t = _f64(theta)
+    err = lib().orc_fcomp1(dptr(t), dptr(f), dptr(df))
+    return f.reshape(3, 3), df[0], err
+
+
+def evcmp1(c6):
+    ev = np.zeros(3)
+    c = _f64(c6)
+    lib().orc_evcmp1(dptr(c), dptr(ev))
+    return ev
+
+
+def irscp1(f):
+    ui = np.zeros(6)
+    ff = _f64(f).ravel()
+    lib().orc_irscp1(dptr(ff), dptr(ui))
+    return ui
+
+
+def rtcmp1(f):
+    r = np.zeros(9)
+    ff = _f64(f).ravel()
+    lib().orc_rtcmp1(dptr(ff), dptr(r))
+    return r.reshape(3, 3)
+
+
+def getrm1(r, opt):
+    q = np.zeros(36)
+    rr = _f64(r).ravel()
+    lib().orc_getrm1(dptr(rr), opt, dptr(q))
+    return q.reshape(6, 6)
+
+
+def ctran1(d, q, cs, qbar, wdj):
+    dd = _f64(d).ravel().copy()
+    qq, c = _f64(q).ravel(), _f64(cs)
+    lib().orc_ctran1(dptr(dd), dptr(qq), dptr(c), int(qbar), wdj)
+    return dd.reshape(6, 6)
+
+
+def kgstiff(gama, nxi, neta, nzeta, sig, dj, w, vol=None):
+    """Geometric stiffness of one point, packed upper column-major."""
+    nn = len(nxi)
+    T = 3 * nn
+    ek = np.zeros(T * (T + 1) // 2)
+    g, a, b, c, s = (_f64(gama).ravel(), _f64(nxi), _f64(neta), _f64(nzeta),
+                     _f64(sig))
+    v = _f64(vol).ravel() if vol is not None else None
+    lib().orc_kgstiff(nn, dptr(g), dptr(a), dptr(b), dptr(c),
+                      dptr(v) if v is not None else C.POINTER(C.c_double)(),
+                      dptr(s), dj, w,
+                      dptr(ek))
+    return ek
+
+
 def mm10_expmw(wv):
     a = np.zeros(9)
     lib().orc_mm10_expmw(dptr(np.ascontiguousarray(wv, float)), dptr(a))
@@ -309,7 +413,7 @@ class OracleModel:
         nq = i["span"] * i["ngp"]
         return {0: 9 * nq, 1: 9 * nq, 2: 6 * nq, 3: 6 * nq, 4: i["hist"] * nq,
                 5: i["hist"] * nq, 6: 21 * nq, 7: i["utsz"] * i["span"],
-                8: i["totdof"] * i["span"], 9: i["span"]}[which]
+                8: i["totdof"] * i["span"], 9: i["span"], 10: 9 * nq}[which]
 
     def get_block_state(self, blk, which):
         n = self.state_size(blk, which)

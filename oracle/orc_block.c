@@ -58,12 +58,16 @@ typedef struct {
   int32_t* edest;             /* (totdof, span) 1-based structure dofs */
   int ncry;                   /* mm10: crystals per point                */
   orc_crystal* crys;          /* mm10: (ncry, span) crystal constants    */
+  int geonl;                  /* finite strains and rotations            */
+  double *rot_n, *rot_n1;     /* geonl: (9, ngp, span) [R] column-major  */
 } orc_block;
 
 struct orc_model {
   int nonode, noelem, nelblk, nmat;
   double* c;
-  int32_t *incid, *incmap, *etype, *order, *matnum, *bbar;
+  int32_t *incid, *incmap, *etype, *order, *matnum, *bbar, *geonl;
+  /* displacements of the last recovery (the tangent's configuration) */
+  double *u_last, *du_last;
   w3d_material* mats;
   double eps_bbar;
   int32_t* elblks; /* (4, nelblk) */
@@ -102,6 +106,7 @@ static int compatible(const orc_model* om, int a, int b) {
       orc_ngp(om->etype[b], om->order[b]))
     return 0;
   if (om->bbar[a] != om->bbar[b]) return 0;
+  if (om->geonl[a] != om->geonl[b]) return 0;
   return 1;
 }
 
@@ -155,6 +160,9 @@ orc_model* orc_create(const w3d_model* m) {
   om->order = (int32_t*)malloc(sizeof(int32_t) * m->noelem);
   om->matnum = (int32_t*)malloc(sizeof(int32_t) * m->noelem);
   om->bbar = (int32_t*)malloc(sizeof(int32_t) * m->noelem);
+  om->geonl = (int32_t*)malloc(sizeof(int32_t) * m->noelem);
+  om->u_last = (double*)calloc(3 * (size_t)m->nonode, sizeof(double));
+  om->du_last = (double*)calloc(3 * (size_t)m->nonode, sizeof(double));
   om->incmap = (int32_t*)malloc(sizeof(int32_t) * m->noelem);
   int64_t ninc = 0;
   for (int e = 0; e < m->noelem; ++e) {
@@ -174,6 +182,7 @@ orc_model* orc_create(const w3d_model* m) {
     om->order[e] = m->int_order[e];
     om->matnum[e] = m->matnum[e];
     om->bbar[e] = (m->etype[e] == W3D_L3DISOP) && m->bbar && m->bbar[e];
+    om->geonl[e] = m->geonl && m->geonl[e] ? 1 : 0;
     om->incmap[e] = (int32_t)pos + 1;
     for (int j = 0; j < nn; ++j)
       om->incid[pos + j] = m->incid[m->incmap[e] - 1 + j];
@@ -196,6 +205,7 @@ orc_model* orc_create(const w3d_model* m) {
            (mt->angle_type == 1 || mt->angle_type == 2) &&
            ncry_of(mt) <= (m->angle_stride > 0 ? m->angle_stride : 1) &&
            m->angles != NULL;
+    if (mt->model == W3D_MM10 && om->geonl[e]) ok = 0;
     if (!ok) {
       fail(W3D_ERR_UNSUPPORTED, "orc_create: material model");
       orc_destroy(om);
@@ -250,7 +260,16 @@ orc_model* orc_create(const w3d_model* m) {
     B->model = om->mats[B->mat].model;
     B->hist = hist_size(&om->mats[B->mat]);
     B->bbar = om->bbar[fe];
+    B->geonl = om->geonl[fe];
     int64_t nq = (int64_t)B->ngp * B->span;
+    if (B->geonl) { /* rotation_init (reopen.f:1840): unit matrices */
+      B->rot_n = (double*)calloc(9 * nq, sizeof(double));
+      B->rot_n1 = (double*)calloc(9 * nq, sizeof(double));
+      for (int64_t q = 0; q < nq; ++q) {
+        B->rot_n[9 * q] = B->rot_n[9 * q + 4] = B->rot_n[9 * q + 8] = 1.0;
+        B->rot_n1[9 * q] = B->rot_n1[9 * q + 4] = B->rot_n1[9 * q + 8] = 1.0;
+      }
+    }
     B->urcs_n = (double*)calloc(9 * nq, sizeof(double));
     B->urcs_n1 = (double*)calloc(9 * nq, sizeof(double));
     B->eps_n = (double*)calloc(6 * nq, sizeof(double));
@@ -321,10 +340,12 @@ void orc_destroy(orc_model* om) {
       free(B->urcs_n); free(B->urcs_n1); free(B->eps_n); free(B->eps_n1);
       free(B->hist_n); free(B->hist_n1); free(B->cep); free(B->ek);
       free(B->einfvec); free(B->evol); free(B->edest); free(B->crys);
+      free(B->rot_n); free(B->rot_n1);
     }
   free(om->blk); free(om->elem_blk); free(om->c); free(om->incid);
   free(om->incmap); free(om->etype); free(om->order); free(om->matnum);
-  free(om->bbar); free(om->mats); free(om->elblks); free(om->inv_ptr);
+  free(om->bbar); free(om->geonl); free(om->u_last); free(om->du_last);
+  free(om->mats); free(om->elblks); free(om->inv_ptr);
   free(om->inv_list); free(om->dof_eqn_map); free(om->eqn_node_map);
   free(om->k_ptrs); free(om->k_indexes); free(om->row_start);
   free(om);
@@ -426,6 +447,13 @@ static int recover_block(orc_model* om, int b, int step, int iter,
   double(*dj)[MXGP] = malloc(sizeof(double) * S * MXGP);
   double(*gama)[MXGP][9] = malloc(sizeof(double) * S * MXGP * 9);
   double(*vol)[24] = malloc(sizeof(double) * S * 24);
+  /* finite strains: displacements at n, [R n+1] and det F n+1 of every
+     point, det J at n = 0 (rstgp1, gtmat1) */
+  const int geonl = B->geonl;
+  double* ue = geonl ? (double*)malloc(sizeof(double) * S * T) : NULL;
+  double* rloc = geonl ? (double*)malloc(sizeof(double) * S * G * 9) : NULL;
+  double* dfn1 = geonl ? (double*)malloc(sizeof(double) * S * G) : NULL;
+  double* dj0 = geonl ? (double*)malloc(sizeof(double) * S * G) : NULL;
   /* local n-state copies (gathered, then possibly reset at step 1) */
   double* hloc = (double*)malloc(sizeof(double) * S * G * H);
   double* sloc = (double*)malloc(sizeof(double) * S * G * 9);
@@ -435,7 +463,18 @@ static int recover_block(orc_model* om, int b, int step, int iter,
     gather_coords(om, B, i, ce + i * T);
     const int32_t* ed = B->edest + (int64_t)i * T;
     for (int k = 0; k < T; ++k) due[i * T + k] = du[ed[k] - 1];
-    elem_kinematics(B, &t, ce + i * T, dj[i], gama[i], vol[i]);
+    if (geonl) {
+      /* rknstr_geonl: Jacobians and B-bar volume terms at the mid-step
+         configuration ce_0 + u + du/2 (dupstr_blocked) */
+      double cm[MXDOF];
+      for (int k = 0; k < T; ++k) {
+        ue[i * T + k] = u[ed[k] - 1];
+        cm[k] = ce[i * T + k] + ue[i * T + k] + 0.5 * due[i * T + k];
+      }
+      elem_kinematics(B, &t, cm, dj[i], gama[i], vol[i]);
+    } else {
+      elem_kinematics(B, &t, ce + i * T, dj[i], gama[i], vol[i]);
+    }
   }
   double blk_energy = 0.0, blk_pwork = 0.0;
   double bmat[6 * MXDOF];
@@ -452,6 +491,24 @@ static int recover_block(orc_model* om, int b, int step, int iter,
                   B->bbar ? vol[i] : NULL, om->eps_bbar, bmat);
       const double* d = due + i * T;
       double deps[6] = {0, 0, 0, 0, 0, 0};
+      double qnhalf[36];
+      if (geonl) {
+        /* gtmat1: F at n+1 and n+1/2 from the n = 0 Jacobian, polar
+           decompositions, [R n+1] kept, qnhalf from [R n+1/2] */
+        double jac0[9], g0[9], th[9], fm[9], df, rn[9];
+        orc_jacob1(N, t.nxi[g], t.neta[g], t.nzeta[g], ce + i * T, jac0,
+                   &dj0[q], g0);
+        double un[MXDOF];
+        for (int k = 0; k < T; ++k) un[k] = ue[i * T + k] + d[k];
+        orc_tcomp1(N, g0, t.nxi[g], t.neta[g], t.nzeta[g], un, th);
+        if (orc_fcomp1(th, fm, &dfn1[q])) cut = 2;
+        orc_rtcmp1(fm, rloc + 9 * q);
+        for (int k = 0; k < T; ++k) un[k] = ue[i * T + k] + 0.5 * d[k];
+        orc_tcomp1(N, g0, t.nxi[g], t.neta[g], t.nzeta[g], un, th);
+        if (orc_fcomp1(th, fm, &df)) cut = 2;
+        orc_rtcmp1(fm, rn);
+        orc_getrm1(rn, 1, qnhalf);
+      }
       for (int j = 0; j < N; ++j) {
         int j1 = j, j2 = N + j, j3 = 2 * N + j;
         deps[0] = deps[0] + bmat[0 * T + j1] * d[j1] + bmat[0 * T + j2] * d[j2] +
@@ -463,6 +520,11 @@ static int recover_block(orc_model* om, int b, int step, int iter,
         deps[3] = deps[3] + bmat[3 * T + j1] * d[j1] + bmat[3 * T + j2] * d[j2];
         deps[4] = deps[4] + bmat[4 * T + j2] * d[j2] + bmat[4 * T + j3] * d[j3];
         deps[5] = deps[5] + bmat[5 * T + j1] * d[j1] + bmat[5 * T + j3] * d[j3];
+      }
+      if (geonl) { /* uddt = [qnhalf] ddt (qmply1) */
+        double ddt[6];
+        memcpy(ddt, deps, sizeof ddt);
+        orc_qmply1(qnhalf, ddt, deps);
       }
       /* rstgp1_update_strains */
       for (int k = 0; k < 6; ++k)
@@ -516,21 +578,42 @@ static int recover_block(orc_model* om, int b, int step, int iter,
       for (int r = 0; r < 6; ++r)
         for (int cc = 0; cc <= r; ++cc) cs[k++] = cep[6 * r + cc];
     }
-    /* rstgp1_b (small strain), only for iter > 0 */
+    /* rstgp1_b, only for iter > 0: the deformed volume det F det J0 for
+       finite strains */
     if (iter > 0) {
       for (int i = 0; i < S; ++i) {
         int64_t q = (int64_t)i * G + g;
-        double factor = dj[i][g] * 1.0 * t.w[g];
+        double factor = geonl ? dfn1[q] * dj0[q] * 1.0 * t.w[g]
+                              : dj[i][g] * 1.0 * t.w[g];
         blk_energy = blk_energy + B->urcs_n1[q * 9 + 6] * factor;
         blk_pwork = blk_pwork + B->urcs_n1[q * 9 + 7] * factor;
       }
     }
+  }
+  /* rknstr_finish_cp: grad Fe (history words 37-63) of every element of
+     a single-crystal block from the gathered n-history as mm10 left it
+     (re-initialised in place at step 1); zero with several crystals */
+  if (B->model == W3D_MM10 && !cut) {
+    double gv[27];
+    if (B->ncry == 1)
+      orc_mm10_calc_grads(B->etype, B->order, G, S, H, hloc, gv);
+    else
+      memset(gv, 0, sizeof gv);
+    for (int64_t q = 0; q < (int64_t)S * G; ++q)
+      memcpy(B->hist_n1 + q * H + 36, gv, sizeof gv);
   }
   /* rknifv_std: element internal forces with the updated stresses (the
      coordinates at n+1 equal the initial ones in the small-displacement
      form, so the Jacobians above are reused) */
   memset(B->einfvec, 0, sizeof(double) * S * T);
   memset(B->evol, 0, sizeof(double) * S);
+  if (geonl) /* rknifv_std: kinematics at n+1 (ce_0 + u + du) */
+    for (int i = 0; i < S; ++i) {
+      double cn[MXDOF];
+      for (int k = 0; k < T; ++k)
+        cn[k] = ce[i * T + k] + ue[i * T + k] + due[i * T + k];
+      elem_kinematics(B, &t, cn, dj[i], gama[i], vol[i]);
+    }
   for (int g = 0; g < G; ++g) {
     double w = t.w[g] * scalar;
     for (int i = 0; i < S; ++i) {
@@ -538,6 +621,13 @@ static int recover_block(orc_model* om, int b, int step, int iter,
       orc_bmatrix(N, gama[i][g], t.nxi[g], t.neta[g], t.nzeta[g],
                   B->bbar ? vol[i] : NULL, om->eps_bbar, bmat);
       const double* s = B->urcs_n1 + q * 9;
+      double cs[6];
+      if (geonl) { /* gpifv1: Cauchy stress {T} = [qn1] {t} at n+1 */
+        double qn1[36];
+        orc_getrm1(rloc + 9 * q, 2, qn1);
+        orc_qmply1(qn1, s, cs);
+        s = cs;
+      }
       double d = dj[i][g];
       for (int j = 0; j < T; ++j) {
         double* f = &B->einfvec[(int64_t)j * S + i];
@@ -549,18 +639,33 @@ static int recover_block(orc_model* om, int b, int step, int iter,
     }
   }
   /* rplstr: the gathered history of iter <= 0 is not scattered back; the
-     n+1 history array keeps its previous contents */
+     n+1 history array keeps its previous contents.  [R n+1] (column-major
+     records) is scattered for iter > 0 only. */
+  if (geonl && iter > 0)
+    for (int64_t q = 0; q < (int64_t)S * G; ++q)
+      for (int a = 0; a < 3; ++a)
+        for (int c2 = 0; c2 < 3; ++c2)
+          B->rot_n1[9 * q + a + 3 * c2] = rloc[9 * q + 3 * a + c2];
   free(ce); free(due); free(dj); free(gama); free(vol); free(hloc);
-  free(sloc);
+  free(sloc); free(ue); free(rloc); free(dfn1); free(dj0);
+  if (cut == 2) /* det F <= 0: adaptive step reduction (rstgp1.f) */
+    return step > 1 ? W3D_CUT_STEP : W3D_ERR_NUMERIC;
   *energy = blk_energy;
   *pwork = blk_pwork;
   return rc ? rc : (cut ? W3D_CUT_STEP : 0);
+}
+
+static void keep_displacements(orc_model* om, const double* u,
+                               const double* du) {
+  memcpy(om->u_last, u, sizeof(double) * 3 * (size_t)om->nonode);
+  memcpy(om->du_last, du, sizeof(double) * 3 * (size_t)om->nonode);
 }
 
 int orc_recover_blocks(orc_model* om, int first, int n, int step, int iter,
                        int extrapolated, const double* u, const double* du) {
   if (first < 1 || n < 0 || first - 1 + n > om->nelblk)
     return fail(W3D_ERR_ARG, "orc_recover_blocks: block range");
+  keep_displacements(om, u, du);
   int rc = 0, cut = 0;
 #pragma omp parallel for schedule(dynamic, 1) reduction(min : rc) \
     reduction(max : cut) \
@@ -580,6 +685,7 @@ int orc_recover(orc_model* om, int step, int iter, int extrapolated,
                 const double* u, const double* du, double* ifv,
                 double* energy, double* plastic_work) {
   int nb = om->nelblk;
+  keep_displacements(om, u, du);
   double* be = (double*)calloc(nb, sizeof(double));
   double* bp = (double*)calloc(nb, sizeof(double));
   int rc = 0, cut = 0;
@@ -651,6 +757,11 @@ static void tangent_block(orc_model* om, int b) {
     }
   for (int i = 0; i < S; ++i) {
     gather_coords(om, B, i, ce);
+    if (B->geonl) { /* dptstf: coordinates updated to ce_0 + u + du */
+      const int32_t* ed = B->edest + (int64_t)i * T;
+      for (int k = 0; k < T; ++k)
+        ce[k] = ce[k] + om->u_last[ed[k] - 1] + om->du_last[ed[k] - 1];
+    }
     elem_kinematics(B, &t, ce, dj, gama, vol);
     double* ek = B->ek + (int64_t)i * U; /* transposed layout (utsz,span) */
     memset(ek, 0, sizeof(double) * U);
@@ -659,6 +770,15 @@ static void tangent_block(orc_model* om, int b) {
                   B->bbar ? vol : NULL, om->eps_bbar, bmat);
       double f = t.w[g] * dj[g];
       double dm[36];
+      double qn1[36], cs[6];
+      if (B->geonl) { /* gptns1_a: [qn1] from the stored [R n+1] */
+        const double* rc = B->rot_n1 + 9 * ((int64_t)i * G + g);
+        double r9[9];
+        for (int a = 0; a < 3; ++a)
+          for (int c2 = 0; c2 < 3; ++c2) r9[3 * a + c2] = rc[a + 3 * c2];
+        orc_getrm1(r9, 2, qn1);
+        orc_qmply1(qn1, B->urcs_n1 + 9 * ((int64_t)i * G + g), cs);
+      }
       if (B->model == W3D_MM10) {
         /* drive_10_cnst (gptns1.f:1371): d(m,k) = history1 word 6k+m
            times w |J| */
@@ -676,6 +796,8 @@ static void tangent_block(orc_model* om, int b) {
             dm[6 * c + r] = v;
           }
       }
+      if (B->geonl) /* ctran1 with the [Q-bar] correction (qbar_flag) */
+        orc_ctran1(dm, qn1, cs, 1, dj[g] * t.w[g]);
       if (scalar != 1.0) /* adjust_cnst */
         for (int q = 0; q < 36; ++q) dm[q] = scalar * dm[q];
       /* bdbtgen: bd(j,k) = sum_m d(m,k) b(j,m) */
@@ -696,6 +818,9 @@ static void tangent_block(orc_model* om, int b) {
                 bmat[4 * T + col] * bd[4 * T + row] +
                 bmat[5 * T + col] * bd[5 * T + row];
       }
+      if (B->geonl) /* kgstiff: initial-stress stiffness (no 1/6 for tets) */
+        orc_kgstiff(N, gama[g], t.nxi[g], t.neta[g], t.nzeta[g],
+                    B->bbar ? vol : NULL, cs, dj[g], t.w[g], ek);
     }
   }
   free(icp_r);
@@ -867,6 +992,7 @@ int orc_commit_step(orc_model* om) {
     memcpy(B->urcs_n, B->urcs_n1, sizeof(double) * 9 * nq);
     memcpy(B->eps_n, B->eps_n1, sizeof(double) * 6 * nq);
     memcpy(B->hist_n, B->hist_n1, sizeof(double) * B->hist * nq);
+    if (B->geonl) memcpy(B->rot_n, B->rot_n1, sizeof(double) * 9 * nq);
   }
   return 0;
 }
@@ -878,6 +1004,7 @@ int orc_rollback(orc_model* om) {
     memcpy(B->urcs_n1, B->urcs_n, sizeof(double) * 9 * nq);
     memcpy(B->eps_n1, B->eps_n, sizeof(double) * 6 * nq);
     memcpy(B->hist_n1, B->hist_n, sizeof(double) * B->hist * nq);
+    if (B->geonl) memcpy(B->rot_n1, B->rot_n, sizeof(double) * 9 * nq);
   }
   return 0;
 }
@@ -895,6 +1022,7 @@ static double* state_ptr(const orc_block* B, int which, int64_t* n) {
     case W3D_ESTIFF: *n = (int64_t)B->utsz * B->span; return B->ek;
     case W3D_EINFVEC: *n = (int64_t)B->totdof * B->span; return B->einfvec;
     case W3D_ELEM_VOL: *n = B->span; return B->evol;
+    case W3D_ROT_N1: *n = B->geonl ? 9 * nq : 0; return B->rot_n1;
     default: *n = 0; return NULL;
   }
 }

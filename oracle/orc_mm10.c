@@ -55,6 +55,9 @@
 
 #define NSL 12
 #define NCOMMON 87
+#ifndef MXGP
+#define MXGP 27
+#endif
 #define NCRYS 71
 
 /* mm10_constants (mod_crystals.f:122-129) */
@@ -1181,4 +1184,87 @@ void orc_mm10_residual(const w3d_material* mat, const orc_crystal* cr,
   memcpy(xx, x, sizeof xx);
   if (R) formR(&p, &np1, &n, xx, R);
   if (J) formJ(&p, &np1, xx, J);
+}
+
+/* ------------------------------------------------------------------ */
+/* grad Fe of single-crystal elements: rknstr_finish_cp (rknstr.f)     */
+/* ------------------------------------------------------------------ */
+/* Least-squares solution of min |A x - b| for A (m x 4, column-major,
+   leading dimension m), as LAPACK DGELS('N') takes it: Householder QR
+   (dgeqr2/dlarfg) and back substitution when m >= 4; for m < 4 the
+   minimum-norm solution A^T (A A^T)^-1 b (m = 1 here: the one-point
+   rule).  x receives 4 values. */
+void orc_lsq4(int m, const double* a_in, const double* b_in, double* x) {
+  double a[4 * MXGP], b[MXGP];
+  memcpy(a, a_in, sizeof(double) * 4 * m);
+  memcpy(b, b_in, sizeof(double) * m);
+  if (m < 4) { /* only the one-point rule reaches here */
+    double s = 0.0;
+    for (int j = 0; j < 4; ++j) s = s + a[j * m] * a[j * m];
+    for (int j = 0; j < 4; ++j) x[j] = s > 0.0 ? a[j * m] * b[0] / s : 0.0;
+    return;
+  }
+  for (int j = 0; j < 4; ++j) {
+    double* col = a + j * m;
+    double alpha = col[j], xn = 0.0;
+    for (int i = j + 1; i < m; ++i) xn = xn + col[i] * col[i];
+    xn = sqrt(xn);
+    double tau = 0.0;
+    if (xn != 0.0) {
+      double beta = -copysign(sqrt(alpha * alpha + xn * xn), alpha);
+      tau = (beta - alpha) / beta;
+      double sc = 1.0 / (alpha - beta);
+      for (int i = j + 1; i < m; ++i) col[i] = col[i] * sc;
+      col[j] = beta;
+    }
+    /* apply H = I - tau v v^T (v = [1; col(j+1:m)]) to the remaining
+       columns and to b */
+    for (int k = j + 1; k <= 4; ++k) {
+      double* y = k < 4 ? a + k * m : b;
+      double w = y[j];
+      for (int i = j + 1; i < m; ++i) w = w + col[i] * y[i];
+      w = tau * w;
+      y[j] = y[j] - w;
+      for (int i = j + 1; i < m; ++i) y[i] = y[i] - w * col[i];
+    }
+  }
+  for (int j = 3; j >= 0; --j) {
+    double s = b[j];
+    for (int k = j + 1; k < 4; ++k) s = s - a[k * m + j] * x[k];
+    x[j] = s / a[j * m + j];
+  }
+}
+
+/* mm10_calc_grads (mm10_a.f:1579) as rknstr_finish_cp calls it for small
+   strain: Rt3d(3,3,ngp) is a flat copy (mm10_a_copy_vector) of 9 ngp words
+   of the block's gathered n-history elem_hist(span, hist, ngp), starting
+   at (element 1, first Rp word of crystal 1, point 1) -- words of several
+   elements of the block, not element i's Rp.  For each (a, b) the
+   coefficients of a linear fit Rt3d(a,b,p) ~ c1 xi + c2 eta + c3 zeta + c4
+   over the Gauss points (DGELS) give grads(a,b,1:3).  hloc is the block's
+   n-history in the (hist, ngp, span) layout; grads_vec receives grads in
+   Fortran order (a fastest). */
+void orc_mm10_calc_grads(int etype, int order, int ngp, int span, int hist,
+                         const double* hloc, double* grads_vec) {
+  double amat[4 * MXGP];
+  for (int p = 0; p < ngp; ++p) {
+    double xi, eta, zeta, w;
+    orc_getgpts(etype, order, p + 1, &xi, &eta, &zeta, &w);
+    amat[p] = xi;
+    amat[ngp + p] = eta;
+    amat[2 * ngp + p] = zeta;
+    amat[3 * ngp + p] = 1.0;
+  }
+  const int64_t l0 = (int64_t)(NCOMMON + 9) * span;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+      double rhs[MXGP], x[4];
+      for (int p = 0; p < ngp; ++p) {
+        int64_t l = l0 + a + 3 * b + 9 * p;
+        int64_t ie = l % span, wd = (l / span) % hist, g = l / ((int64_t)span * hist);
+        rhs[p] = hloc[(ie * ngp + g) * hist + wd];
+      }
+      orc_lsq4(ngp, amat, rhs, x);
+      for (int d = 0; d < 3; ++d) grads_vec[a + 3 * b + 9 * d] = x[d];
+    }
 }

@@ -78,6 +78,28 @@ void orc_mm10_residual(const w3d_material* mat, const orc_crystal* cr,
                        double dt, const double* x, double* R, double* J);
 void orc_mm10_rotation_matrix(const double angles[3], int angle_type,
                               double r[9]);
+/* finite strains (orc_geonl.c); 3x3 matrices row-major */
+void orc_tcomp1(int nnode, const double gama[9], const double* nxi,
+                const double* neta, const double* nzeta, const double* ue,
+                double theta[9]);
+int orc_fcomp1(const double theta[9], double f[9], double* df);
+void orc_evcmp1(const double c[6], double ev[3]);
+void orc_irscp1(const double f[9], double ui[6]);
+void orc_rtcmp1(const double f[9], double r[9]);
+void orc_getrm1(const double r9[9], int opt, double q[36]);
+void orc_qmply1(const double q[36], const double m1[6], double m2[6]);
+void orc_ctran1(double d[36], const double q[36], const double cs[6],
+                int qbar, double wdj);
+void orc_kgstiff(int nnode, const double gama[9], const double* nxi,
+                 const double* neta, const double* nzeta, const double* vol,
+                 const double sig[6], double dj, double w, double* ek);
+/* Least squares min |A x - b|, A (m x 4) column-major (DGELS('N')). */
+void orc_lsq4(int m, const double* a, const double* b, double* x);
+/* grad Fe words (27) of single-crystal elements of a small-strain block
+   (rknstr_finish_cp / mm10_calc_grads) from the block's gathered
+   n-history hloc (hist, ngp, span). */
+void orc_mm10_calc_grads(int etype, int order, int ngp, int span, int hist,
+                         const double* hloc, double* grads_vec);
 void orc_mm10_rt2rve(const double* rt, double* rv);
 void orc_mm10_rt2rvw(const double* rt, double* rv);
 void orc_mm10_expmw(const double wv[3], double a[9]);

@@ -388,3 +388,68 @@ def test_iteration_zero_pseudo_update():
     assert np.allclose(u0[:6], h1[87:93] - R[:6], rtol=1e-14, atol=1e-12)
     assert np.array_equal(h0[87 + 9:87 + 18], np.zeros(9))
     assert h0[87 + 42] == h1[87 + 42]
+
+
+@pytest.mark.parametrize("m", [1, 4, 8, 14, 27])
+def test_least_squares_matches_numpy(m):
+    """The DGELS restatement: least-squares solution for m >= 4, minimum-norm
+    solution for the one-point rule (numpy's lstsq gives both)."""
+    rng = np.random.default_rng(m)
+    a = rng.uniform(-1, 1, (m, 4))
+    a[:, 3] = 1.0
+    b = rng.uniform(-2, 2, m)
+    ref = np.linalg.lstsq(a, b, rcond=None)[0]
+    assert np.allclose(O.lsq4(a, b), ref, rtol=1e-12, atol=1e-13)
+
+
+def gauss_coords(etype, order):
+    n = O.ngp(etype, order)
+    return np.array([O.getgpts(etype, order, p + 1)[:3] for p in range(n)])
+
+
+@pytest.mark.parametrize("etype,order,span", [(_abi.W3D_L3DISOP, 1, 128),
+                                              (_abi.W3D_L3DISOP, 1, 7),
+                                              (_abi.W3D_L3DISOP, 1, 1),
+                                              (_abi.W3D_Q3DISOP, 1, 3),
+                                              (_abi.W3D_TET10, 4, 5)])
+def test_grad_fe_flat_copy_and_fit(etype, order, span):
+    """rknstr_finish_cp for small strain: the 9 ngp words that
+    mm10_a_copy_vector copies out of elem_hist(span, hist, ngp) -- numpy's
+    own Fortran memory order here -- fitted linearly over the Gauss points
+    (numpy lstsq); a word of an element other than the first and words past
+    the first point's history are reached for small spans."""
+    hist = 158
+    xyz = gauss_coords(etype, order)
+    ngp_ = len(xyz)
+    rng = np.random.default_rng(span)
+    hloc = rng.uniform(-1, 1, (hist, ngp_, span))
+    elem_hist = np.transpose(hloc, (2, 0, 1)).copy(order="F")
+    flat = elem_hist.ravel(order="F")
+    start = np.ravel_multi_index((0, 96, 0), elem_hist.shape, order="F")
+    rt = flat[start:start + 9 * ngp_].reshape((3, 3, ngp_), order="F")
+    amat = np.column_stack([xyz, np.ones(ngp_)])
+    grads = np.zeros((3, 3, 3))
+    for a in range(3):
+        for b in range(3):
+            grads[a, b] = np.linalg.lstsq(amat, rt[a, b], rcond=None)[0][:3]
+    got = O.mm10_calc_grads(etype, order, ngp_, span, hist, hloc)
+    assert np.allclose(got, grads.ravel(order="F"), rtol=1e-12, atol=1e-13)
+
+
+def test_grad_fe_recovers_a_linear_field():
+    """Words that are an exact linear function of the Gauss-point
+    coordinates give back its slopes (span 1: the copied words are those of
+    one element, consecutive in the history)."""
+    etype, order, hist = _abi.W3D_Q3DISOP, 1, 158
+    xyz = gauss_coords(etype, order)
+    ngp_ = len(xyz)
+    slope = np.arange(27, dtype=float).reshape(3, 3, 3, order="F") * 0.1 - 1
+    icpt = np.linspace(-1, 1, 9).reshape(3, 3, order="F")
+    hloc = np.zeros((hist, ngp_, 1))
+    flat = np.zeros(hist * ngp_)
+    for p in range(ngp_):
+        val = icpt + slope @ xyz[p]
+        flat[96 + 9 * p:96 + 9 * p + 9] = val.ravel(order="F")
+    hloc[:, :, 0] = flat.reshape((hist, ngp_), order="F")
+    got = O.mm10_calc_grads(etype, order, ngp_, 1, hist, hloc)
+    assert np.allclose(got, slope.ravel(order="F"), rtol=1e-12, atol=1e-12)

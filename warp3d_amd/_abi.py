@@ -7,7 +7,7 @@ import ctypes as C
 
 import numpy as np
 
-W3D_ABI_VERSION = 2
+W3D_ABI_VERSION = 3
 
 W3D_OK = 0
 W3D_CUT_STEP = 1
@@ -23,7 +23,7 @@ W3D_FP64 = 0
 W3D_FP32 = 1
 
 STRESS_N, STRESS_N1, STRAIN_N, STRAIN_N1, HIST_N, HIST_N1, CEP, ESTIFF, \
-    EINFVEC, ELEM_VOL = range(10)
+    EINFVEC, ELEM_VOL, ROT_N1 = range(11)
 
 NNODE = {W3D_Q3DISOP: 20, W3D_L3DISOP: 8, W3D_TET10: 10}
 
@@ -60,6 +60,7 @@ class W3DModel(C.Structure):
         ("int_order", C.POINTER(C.c_int32)),
         ("matnum", C.POINTER(C.c_int32)),
         ("bbar", C.POINTER(C.c_int32)),
+        ("geonl", C.POINTER(C.c_int32)),
         ("angles", C.POINTER(C.c_double)),
         ("angle_stride", C.c_int32),
         ("nmat", C.c_int32),
@@ -142,7 +143,7 @@ class ModelArrays:
 
     def __init__(self, coords, incid, incmap, etype, int_order, matnum,
                  bbar, materials, elblks=None, auto_size=128, eps_bbar=0.0,
-                 angles=None, angle_stride=1):
+                 angles=None, angle_stride=1, geonl=None):
         self.coords = np.ascontiguousarray(coords, dtype=np.float64).ravel()
         self.incid = np.ascontiguousarray(incid, dtype=np.int32).ravel()
         self.incmap = np.ascontiguousarray(incmap, dtype=np.int32)
@@ -150,6 +151,8 @@ class ModelArrays:
         self.int_order = np.ascontiguousarray(int_order, dtype=np.int32)
         self.matnum = np.ascontiguousarray(matnum, dtype=np.int32)
         self.bbar = np.ascontiguousarray(bbar, dtype=np.int32)
+        self.geonl = (None if geonl is None else
+                      np.ascontiguousarray(geonl, dtype=np.int32))
         self.angles = (None if angles is None else
                        np.ascontiguousarray(angles, dtype=np.float64).ravel())
         self.angle_stride = angle_stride
@@ -174,6 +177,8 @@ class ModelArrays:
         m.int_order = iptr(self.int_order)
         m.matnum = iptr(self.matnum)
         m.bbar = iptr(self.bbar)
+        m.geonl = (iptr(self.geonl) if self.geonl is not None
+                   else C.POINTER(C.c_int32)())
         m.angles = (dptr(self.angles) if self.angles is not None
                     else C.POINTER(C.c_double)())
         m.angle_stride = self.angle_stride

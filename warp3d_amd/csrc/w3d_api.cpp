@@ -62,14 +62,6 @@ void dfree(T*& p) {
 }
 
 // ranks of an element's nodes by ascending global number
-void node_ranks(const int32_t* nodes, int nn, int* rank) {
-  for (int a = 0; a < nn; ++a) {
-    int r = 0;
-    for (int b = 0; b < nn; ++b) r += nodes[b] < nodes[a] ? 1 : 0;
-    rank[a] = r;
-  }
-}
-
 float hprime_real(float e, float et) {
   volatile float num = e * et;  // REAL arithmetic as elprp.f:872
   volatile float den = e - et;
@@ -82,7 +74,13 @@ struct w3d_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   int nonode = 0, noelem = 0, nelblk = 0;
-  std::vector<int32_t> etype, order, matnum, bbar;
+  std::vector<int32_t> etype, order, matnum, bbar, geonl;
+  bool has_geonl = false;
+  // finite strains: [R] at n and n+1 of every point (SoA like the
+  // stresses), and the displacements of the last recovery (the tangent's
+  // configuration)
+  double *d_rot_n = nullptr, *d_rot_n1 = nullptr;
+  double *d_u_t = nullptr, *d_du_t = nullptr;
   std::vector<int32_t> elblks;
   std::vector<BlockDesc> blocks;
   std::vector<MatDesc> mats;
@@ -111,8 +109,9 @@ struct w3d_ctx {
   CrystalDev* d_crys = nullptr;
   double* d_uddt = nullptr;
   WorkItem* d_work_mm10 = nullptr;
-  uint16_t* d_npb_tab = nullptr;
   int n_work_mm10 = 0;
+  int32_t* d_mm10_blks = nullptr;  // indices of the mm10 blocks
+  int n_mm10_blks = 0;
   bool has_mm10 = false;
   double dt = 1.0;
   WorkItem *d_work_rec = nullptr, *d_work_tan = nullptr;
@@ -127,8 +126,10 @@ struct w3d_ctx {
   int64_t ncoeff = 0;
   std::vector<int32_t> dof_eqn_map, eqn_node_map, k_ptrs, k_indexes;
   std::vector<int64_t> row_start;
-  int64_t* d_asm_ptr = nullptr;
+  uint16_t* d_asm_hdr = nullptr;   // assembly segments (w3d_pattern)
   uint32_t* d_asm_src = nullptr;
+  int64_t* d_asm_wbase = nullptr;
+  int64_t asm_nseg = 0;
   double* d_kout = nullptr;  // neqns diagonal then ncoeff coefficients
   int64_t assembled = 0;     // serial of the last assembly
   // full symmetric CSR of the assembled matrix for the solver: row
@@ -169,10 +170,12 @@ struct w3d_ctx {
     M.evol = d_evol;
     M.flags = d_flags;
     M.crys = d_crys;
-    M.npb_tab = d_npb_tab;
     M.uddt = d_uddt;
     M.eps_bbar = eps_bbar;
     M.dt = dt;
+    M.rot_n1 = d_rot_n1;
+    M.u_t = d_u_t;
+    M.du_t = d_du_t;
     return M;
   }
 };
@@ -183,7 +186,7 @@ bool compatible(const w3d_ctx* c, int a, int b) {
   return c->etype[a] == c->etype[b] && c->matnum[a] == c->matnum[b] &&
          c->order[a] == c->order[b] &&
          ngp_of(c->etype[a], c->order[a]) == ngp_of(c->etype[b], c->order[b]) &&
-         c->bbar[a] == c->bbar[b];
+         c->bbar[a] == c->bbar[b] && c->geonl[a] == c->geonl[b];
 }
 
 // inelbk_simple_blocking (inelbk.f:282): one sequential pass; a new block
@@ -219,7 +222,7 @@ void build_work(const w3d_ctx* c, bool tangent, std::vector<WorkItem>& items,
     const BlockDesc& B = c->blocks[b];
     int E = tangent ? tan_mfma_chunk(B.nnode) : rec_elems(B.ngp);
     int key = group_key(B.nnode, B.ngp, B.bbar,
-                        !tangent && B.model == W3D_MM10);
+                        !tangent && B.model == W3D_MM10, B.geonl);
     for (int f = 0; f < B.span; f += E) {
       WorkItem w;
       w.blk = b;
@@ -258,11 +261,11 @@ void free_ctx(w3d_ctx* c) {
   dfree(c->d_inv_ptr); dfree(c->d_inv_off);
   dfree(c->d_blk_wfirst); dfree(c->d_blk_wcount);
   dfree(c->d_blk_sums); dfree(c->d_u); dfree(c->d_du); dfree(c->d_ifv);
-  dfree(c->d_asm_ptr); dfree(c->d_asm_src); dfree(c->d_kout);
+  dfree(c->d_asm_hdr); dfree(c->d_asm_src); dfree(c->d_asm_wbase); dfree(c->d_kout);
   dfree(c->d_csr_rp); dfree(c->d_csr_col); dfree(c->d_csr_vidx);
   dfree(c->d_csr_val); dfree(c->d_pcg);
-  dfree(c->d_crys); dfree(c->d_uddt); dfree(c->d_work_mm10);
-  dfree(c->d_npb_tab);
+  dfree(c->d_crys); dfree(c->d_uddt); dfree(c->d_work_mm10); dfree(c->d_mm10_blks);
+  dfree(c->d_rot_n); dfree(c->d_rot_n1); dfree(c->d_u_t); dfree(c->d_du_t);
   for (int ph = 0; ph < W3D_NUM_PHASES; ++ph) {
     for (int k = 0; k < 2; ++k) {
       if (c->ev_a[ph][k]) hipEventDestroy(c->ev_a[ph][k]);
@@ -296,7 +299,10 @@ int create_impl(w3d_ctx* c, const w3d_model* m) {
   c->order.resize(m->noelem);
   c->matnum.resize(m->noelem);
   c->bbar.resize(m->noelem);
+  c->geonl.resize(m->noelem);
   for (int e = 0; e < m->noelem; ++e) {
+    c->geonl[e] = m->geonl && m->geonl[e] ? 1 : 0;
+    c->has_geonl = c->has_geonl || c->geonl[e];
     c->etype[e] = m->etype[e];
     c->order[e] = m->int_order[e];
     c->matnum[e] = m->matnum[e];
@@ -416,6 +422,11 @@ int create_impl(w3d_ctx* c, const w3d_model* m) {
     B.hist = B.model == W3D_MM01 ? kMM01Hist
                                  : kMM10Common + kMM10Crystal * B.ncry;
     B.bbar = c->bbar[fe];
+    B.geonl = c->geonl[fe];
+    if (B.geonl && B.model == W3D_MM10)
+      return fail(W3D_ERR_UNSUPPORTED,
+                  "w3d_create: finite strains with mm10 (block " +
+                      std::to_string(b + 1) + ")");
     B.table = table_of(B.etype, B.order);
     B.q_off = q;
     B.hist_off = h;
@@ -516,24 +527,6 @@ int create_impl(w3d_ctx* c, const w3d_model* m) {
   if ((rc = upload(&c->d_mats, c->mats))) return rc;
   if ((rc = upload(&c->d_tables, c->tables))) return rc;
   if ((rc = upload(&c->d_conn, c->conn))) return rc;
-  {
-    // decode tables of the node-pair block layout (see npb_pos)
-    std::vector<uint16_t> tab(kNpbTabSize);
-    for (int nn : {8, 10, 20}) {
-      uint16_t* t = tab.data() + npb_tab_off(nn);
-      for (int r = 0; r < nn; ++r) {
-        int w = npb_row_start(nn, r);
-        const int dg[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
-        for (int k = 0; k < 6; ++k)
-          t[w++] = (uint16_t)(r | r << 5 | dg[k][0] << 10 | dg[k][1] << 12);
-        for (int s = r + 1; s < nn; ++s)
-          for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j)
-              t[w++] = (uint16_t)(r | s << 5 | i << 10 | j << 12);
-      }
-    }
-    if ((rc = upload(&c->d_npb_tab, tab))) return rc;
-  }
   if (c->has_mm10) {
     if ((rc = upload(&c->d_crys, crys))) return rc;
     if ((rc = dalloc(&c->d_uddt, 6 * q))) return rc;
@@ -554,6 +547,11 @@ int create_impl(w3d_ctx* c, const w3d_model* m) {
     }
     c->n_work_mm10 = (int)mw.size();
     if ((rc = upload(&c->d_work_mm10, mw))) return rc;
+    std::vector<int32_t> mb;
+    for (int b = 0; b < c->nelblk; ++b)
+      if (c->blocks[b].model == W3D_MM10) mb.push_back(b);
+    c->n_mm10_blks = (int)mb.size();
+    if ((rc = upload(&c->d_mm10_blks, mb))) return rc;
   }
   {
     std::vector<double> xyz(m->c, m->c + 3 * (int64_t)m->nonode);
@@ -574,6 +572,20 @@ int create_impl(w3d_ctx* c, const w3d_model* m) {
   for (auto& a : arrs) {
     if ((rc = dalloc(a.p, a.n))) return rc;
     HIPCHK(hipMemset(*a.p, 0, std::max<int64_t>(a.n, 1) * sizeof(double)));
+  }
+  if (c->has_geonl) {  // rotation_init (reopen.f:1840): unit matrices
+    std::vector<double> rot(9 * (size_t)q, 0.0);
+    for (const BlockDesc& B : c->blocks) {
+      const int64_t SG = (int64_t)B.span * B.ngp;
+      for (int64_t p = 0; p < SG; ++p)
+        for (int k : {0, 4, 8}) rot[9 * B.q_off + k * SG + p] = 1.0;
+    }
+    if ((rc = upload(&c->d_rot_n, rot))) return rc;
+    if ((rc = upload(&c->d_rot_n1, rot))) return rc;
+    if ((rc = dalloc(&c->d_u_t, 3 * (int64_t)c->nonode))) return rc;
+    if ((rc = dalloc(&c->d_du_t, 3 * (int64_t)c->nonode))) return rc;
+    HIPCHK(hipMemset(c->d_u_t, 0, 3 * (size_t)c->nonode * sizeof(double)));
+    HIPCHK(hipMemset(c->d_du_t, 0, 3 * (size_t)c->nonode * sizeof(double)));
   }
   if ((rc = dalloc(&c->d_ek32, ek))) return rc;
   HIPCHK(hipMemset(c->d_ek32, 0, std::max<int64_t>(ek, 1) * sizeof(float)));
@@ -642,21 +654,35 @@ void prof_collect(w3d_ctx* c) {
   }
 }
 
-int check_flags(w3d_ctx* c) {
+int check_flags(w3d_ctx* c, int step) {
   int32_t f[4];
   HIPCHK(hipMemcpyAsync(f, c->d_flags, sizeof f, hipMemcpyDeviceToHost,
                         c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   prof_collect(c);
   if (f[0]) return fail(W3D_ERR_NUMERIC, "mm01: negative plastic multiplier");
-  if (f[1]) return W3D_CUT_STEP;
+  // det F <= 0 (fcomp1): a step reduction when one is possible (rstgp1.f)
+  if (f[3] && step == 1)
+    return fail(W3D_ERR_NUMERIC, "non-positive deformation Jacobian at step 1");
+  if (f[1] || f[3]) return W3D_CUT_STEP;
   return W3D_OK;
 }
 
 int recover_impl(w3d_ctx* c, int step, int iter, int extrapolated,
-                 const double* d_du, double* d_ifv, double* energy,
-                 double* plastic_work, int first_blk, int nblk) {
+                 const double* d_u, const double* d_du, double* d_ifv,
+                 double* energy, double* plastic_work, int first_blk,
+                 int nblk) {
   HIPCHK(hipSetDevice(c->device));
+  if (c->has_geonl) {
+    // the finite-strain update needs u; the tangent later uses X + u + du
+    if (!d_u)
+      return fail(W3D_ERR_ARG, "recover: finite strains need the displacements u");
+    const size_t nb = 3 * (size_t)c->nonode * sizeof(double);
+    HIPCHK(hipMemcpyAsync(c->d_u_t, d_u, nb, hipMemcpyDeviceToDevice,
+                          c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_du_t, d_du, nb, hipMemcpyDeviceToDevice,
+                          c->stream));
+  }
   HIPCHK(hipMemsetAsync(c->d_flags, 0, 4 * sizeof(int32_t), c->stream));
   const int nonlinear = (iter >= 1 || extrapolated) ? 1 : 0;
   const int want_energy = (iter > 0) ? 1 : 0;
@@ -709,16 +735,19 @@ int recover_impl(w3d_ctx* c, int step, int iter, int extrapolated,
   }
   prof_begin(c, W3D_PHASE_RECOVER);
   rc = launch_recover(M, c->blocks, d_w, sg, ss, sn, step, iter, nonlinear,
-                      d_du, want_energy, c->d_en_wg, 0, c->stream);
+                      d_u, d_du, want_energy, c->d_en_wg, 0, c->stream);
   prof_end(c, W3D_PHASE_RECOVER);
   if (!rc && nwm > 0) {
     prof_begin(c, W3D_PHASE_MM10);
     rc = launch_mm10(M, d_wm, nwm, step, iter0_off, c->stream);
+    if (!rc)
+      rc = launch_mm10_grads(M, c->d_mm10_blks, c->n_mm10_blks, first_blk - 1,
+                             first_blk - 1 + nblk, step, c->stream);
     prof_end(c, W3D_PHASE_MM10);
     if (!rc) {
       prof_begin(c, W3D_PHASE_RECOVER);
       rc = launch_recover(M, c->blocks, d_w, sg, ss, sn, step, iter,
-                          nonlinear, d_du, want_energy, c->d_en_wg, 1,
+                          nonlinear, d_u, d_du, want_energy, c->d_en_wg, 1,
                           c->stream);
       prof_end(c, W3D_PHASE_RECOVER);
     }
@@ -755,7 +784,7 @@ int recover_impl(w3d_ctx* c, int step, int iter, int extrapolated,
   }
   if (energy) *energy = en;
   if (plastic_work) *plastic_work = pw;
-  return check_flags(c);
+  return check_flags(c, step);
 }
 
 // Jacobi-PCG on an assembled matrix (diag, coef device arrays); b and x
@@ -904,10 +933,9 @@ int w3d_recover_dev(w3d_ctx* c, int32_t step, int32_t iter,
                     int32_t extrapolated, const double* d_u,
                     const double* d_du, double* d_ifv, double* energy,
                     double* plastic_work) {
-  (void)d_u;  // the small-displacement update uses only the increment
   if (!c || !d_du) return fail(W3D_ERR_ARG, "w3d_recover_dev: null argument");
   if (step < 1 || iter < 0) return fail(W3D_ERR_ARG, "w3d_recover: step/iter");
-  return recover_impl(c, step, iter, extrapolated, d_du, d_ifv, energy,
+  return recover_impl(c, step, iter, extrapolated, d_u, d_du, d_ifv, energy,
                       plastic_work, 1, c->nelblk);
 }
 
@@ -921,9 +949,9 @@ int w3d_recover(w3d_ctx* c, int32_t step, int32_t iter, int32_t extrapolated,
   if (u)
     HIPCHK(hipMemcpyAsync(c->d_u, u, nb, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->d_du, du, nb, hipMemcpyHostToDevice, c->stream));
-  int rc = recover_impl(c, step, iter, extrapolated, c->d_du,
-                        ifv ? c->d_ifv : nullptr, energy, plastic_work, 1,
-                        c->nelblk);
+  int rc = recover_impl(c, step, iter, extrapolated, u ? c->d_u : nullptr,
+                        c->d_du, ifv ? c->d_ifv : nullptr, energy,
+                        plastic_work, 1, c->nelblk);
   if (rc < 0) return rc;
   if (ifv) {
     HIPCHK(hipMemcpyAsync(ifv, c->d_ifv, nb, hipMemcpyDeviceToHost,
@@ -936,15 +964,16 @@ int w3d_recover(w3d_ctx* c, int32_t step, int32_t iter, int32_t extrapolated,
 int w3d_recover_blocks(w3d_ctx* c, int32_t first_blk, int32_t nblk,
                        int32_t step, int32_t iter, int32_t extrapolated,
                        const double* u, const double* du) {
-  (void)u;
   if (!c || !du) return fail(W3D_ERR_ARG, "w3d_recover_blocks: null argument");
   if (first_blk < 1 || nblk < 1 || first_blk - 1 + nblk > c->nelblk)
     return fail(W3D_ERR_ARG, "w3d_recover_blocks: block range");
   size_t nb = 3 * (size_t)c->nonode * sizeof(double);
   HIPCHK(hipSetDevice(c->device));
+  if (u)
+    HIPCHK(hipMemcpyAsync(c->d_u, u, nb, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->d_du, du, nb, hipMemcpyHostToDevice, c->stream));
-  return recover_impl(c, step, iter, extrapolated, c->d_du, nullptr, nullptr,
-                      nullptr, first_blk, nblk);
+  return recover_impl(c, step, iter, extrapolated, u ? c->d_u : nullptr,
+                      c->d_du, nullptr, nullptr, nullptr, first_blk, nblk);
 }
 
 int w3d_tangent_blocks(w3d_ctx* c, int32_t first_blk, int32_t nblk,
@@ -1047,60 +1076,211 @@ int w3d_pattern(w3d_ctx* c, const int32_t* cstmap, int32_t* neqns_out,
       std::memcpy(&c->k_indexes[c->row_start[r]], rows[r].data(),
                   rows[r].size() * sizeof(int32_t));
   rows.clear();
-  // assembly map: output t (diagonal of row r at t = r, coefficient k of
-  // the upper triangle at t = neqns + k) sums the Ke words listed in
-  // src[ptr[t] .. ptr[t+1]) in the order of the elements on the row's node
-  // (assem_a_row).
+  // assembly map (assem_by_row / assem_a_row, assemble_code.f:659,785).
+  // Outputs: the diagonal of row r at t = r, coefficient k of the upper
+  // triangle at t = neqns + k.  They are cut into segments: one per
+  // diagonal, then, row by row, one per further component of the row's own
+  // node and one per higher column node m covering m's components.  A
+  // segment lists one Ke word per contribution, in the reference's order:
+  // elements on the row's node in inverse-incidence order, then every local
+  // occurrence of the row's node (erow), then of the column node (ecol).
+  // Component c' of a column-node segment reads that word + c' (row-major
+  // ranked layout: the element's columns of one node are consecutive).
+  // Header: contributions | component mask << 8.  Per 64 segments: the
+  // index of their first contribution and of their first output.
+  struct Occ {
+    int m;          // global node (0-based) at a local position
+    int k;          // element's place in the row node's element list
+    int64_t base;   // first Ke word of the element
+    int td;         // element dofs
+    int rank;       // rank of the local position
+  };
   const int64_t nout = (int64_t)neq + c->ncoeff;
-  std::vector<int64_t> cnt(nout + 1, 0);
-  auto for_row = [&](int srow, auto&& emit) {
-    int snode = c->eqn_node_map[srow - 1] - 1;
-    const int32_t* kidx = c->k_indexes.data() + c->row_start[srow - 1];
-    int nk = c->k_ptrs[srow - 1];
-    for (int64_t p = c->inv_ptr[snode]; p < c->inv_ptr[snode + 1]; ++p) {
+  // occurrences of nodes >= n in the elements on n, sorted by (node,
+  // element) with local positions ascending; self = occurrences of n
+  auto node_occ = [&](int n, std::vector<Occ>& occ, std::vector<Occ>& self) {
+    occ.clear();
+    self.clear();
+    int k = 0;
+    for (int64_t p = c->inv_ptr[n]; p < c->inv_ptr[n + 1]; ++p, ++k) {
       int e = c->inv_elem[p];
       const BlockDesc& B = c->blocks[c->elem_blk[e]];
       int i = e - (B.felem - 1);
       const int32_t* nd_ = &c->conn[B.conn_off + (int64_t)i * B.nnode];
       int rank[kMaxNodes];
       node_ranks(nd_, B.nnode, rank);
-      int erow = -1;
-      for (int k = 0; k < B.totdof && erow < 0; ++k)
-        if (c->dof_eqn_map[3 * nd_[k % B.nnode] + k / B.nnode] == srow) erow = k;
-      for (int ecol = 0; ecol < B.totdof; ++ecol) {
-        int scol = c->dof_eqn_map[3 * nd_[ecol % B.nnode] + ecol / B.nnode];
-        if (scol < srow) continue;
-        int64_t src = B.ek_off + (int64_t)i * B.utsz +
-                      npb_pos(B.nnode, rank, erow, ecol);
-        int64_t t;
-        if (scol == srow) {
-          t = srow - 1;
-        } else {
-          const int32_t* f = std::lower_bound(kidx, kidx + nk, scol);
-          t = neq + c->row_start[srow - 1] + (f - kidx);
-        }
-        emit(t, src);
+      int64_t base = B.ek_off + (int64_t)i * B.utsz;
+      for (int b = 0; b < B.nnode; ++b) {
+        if (nd_[b] < n) continue;
+        Occ o{nd_[b], k, base, B.totdof, rank[b]};
+        if (nd_[b] == n) self.push_back(o);
+        else occ.push_back(o);
+      }
+    }
+    std::stable_sort(occ.begin(), occ.end(), [](const Occ& x, const Occ& y) {
+      return x.m != y.m ? x.m < y.m : x.k < y.k;
+    });
+  };
+  auto word = [](const Occ& a, int ca, const Occ& b, int cb) -> int64_t {
+    int p = 3 * a.rank + ca, q = 3 * b.rank + cb;
+    return a.base + (p <= q ? ek_word(a.td, p, q) : ek_word(a.td, q, p));
+  };
+  // contributions of row occurrences `self` against column occurrences
+  // col[y0, y1) (sorted by element), element by element
+  auto pairs = [](const std::vector<Occ>& self, const std::vector<Occ>& col,
+                  size_t y0, size_t y1, auto&& f) {
+    for (size_t x = 0; x < self.size();) {
+      size_t x1 = x;
+      while (x1 < self.size() && self[x1].k == self[x].k) ++x1;
+      size_t y = y0;
+      while (y < y1 && col[y].k < self[x].k) ++y;
+      for (size_t xx = x; xx < x1; ++xx)
+        for (size_t yy = y; yy < y1 && col[yy].k == self[x].k; ++yy)
+          f(self[xx], col[yy]);
+      x = x1;
+    }
+  };
+  // segments of the rows of node n, in output order:
+  // seg(row component, coefficient?, kernel mask, column node, column
+  // components, body(emit))
+  auto walk = [&](int n, const std::vector<Occ>& occ,
+                  const std::vector<Occ>& self, auto&& seg) {
+    for (int ca = 0; ca < 3; ++ca) {
+      if (c->dof_eqn_map[3 * n + ca] == 0) continue;
+      seg(ca, false, 1, n, 1 << ca, [&](auto&& emit) {
+        pairs(self, self, 0, self.size(),
+              [&](const Occ& a, const Occ& b) { emit(word(a, ca, b, ca)); });
+      });
+      for (int cb = ca + 1; cb < 3; ++cb) {
+        if (c->dof_eqn_map[3 * n + cb] == 0) continue;
+        seg(ca, true, 1, n, 1 << cb, [&](auto&& emit) {
+          pairs(self, self, 0, self.size(),
+                [&](const Occ& a, const Occ& b) { emit(word(a, ca, b, cb)); });
+        });
+      }
+      for (size_t g = 0; g < occ.size();) {
+        size_t h = g;
+        while (h < occ.size() && occ[h].m == occ[g].m) ++h;
+        const int m = occ[g].m;
+        int mask = 0;
+        for (int cb = 0; cb < 3; ++cb)
+          if (c->dof_eqn_map[3 * m + cb] != 0) mask |= 1 << cb;
+        if (mask)
+          seg(ca, true, mask, m, mask, [&](auto&& emit) {
+            pairs(self, occ, g, h,
+                  [&](const Occ& a, const Occ& b) { emit(word(a, ca, b, 0)); });
+          });
+        g = h;
       }
     }
   };
-#pragma omp parallel for schedule(dynamic, 256)
-  for (int srow = 1; srow <= neq; ++srow)
-    for_row(srow, [&](int64_t t, int64_t) { cnt[t + 1] += 1; });
-  for (int64_t t = 0; t < nout; ++t) cnt[t + 1] += cnt[t];
-  std::vector<uint32_t> src(cnt[nout]);
+  std::vector<int64_t> nseg_row(neq, 0), nsrc_diag(neq, 0), nsrc_coef(neq, 0);
+  int bad = 0;
+#pragma omp parallel
   {
-    std::vector<int64_t> cur(cnt.begin(), cnt.end() - 1);
-#pragma omp parallel for schedule(dynamic, 256)
-    for (int srow = 1; srow <= neq; ++srow)
-      for_row(srow, [&](int64_t t, int64_t s) { src[cur[t]++] = (uint32_t)s; });
+    std::vector<Occ> occ, self;
+#pragma omp for schedule(dynamic, 64) reduction(max : bad)
+    for (int n = 0; n < c->nonode; ++n) {
+      node_occ(n, occ, self);
+      walk(n, occ, self, [&](int ca, bool coef, int, int, int, auto&& body) {
+        const int srow = c->dof_eqn_map[3 * n + ca];
+        int64_t cnt = 0;
+        body([&](int64_t) { ++cnt; });
+        if (cnt > 255) bad = 1;
+        if (coef) {
+          nseg_row[srow - 1] += 1;
+          nsrc_coef[srow - 1] += cnt;
+        } else {
+          nsrc_diag[srow - 1] = cnt;
+        }
+      });
+    }
+  }
+  if (bad)
+    return fail(W3D_ERR_UNSUPPORTED,
+                "w3d_pattern: more than 255 contributions to one coefficient");
+  std::vector<int64_t> seg_off(neq + 1, 0), src_diag(neq + 1, 0),
+      src_coef(neq + 1, 0);
+  for (int r = 0; r < neq; ++r) {
+    seg_off[r + 1] = seg_off[r] + nseg_row[r];
+    src_diag[r + 1] = src_diag[r] + nsrc_diag[r];
+    src_coef[r + 1] = src_coef[r] + nsrc_coef[r];
+  }
+  const int64_t nseg = neq + seg_off[neq];
+  const int64_t nsrc = src_diag[neq] + src_coef[neq];
+  std::vector<uint16_t> hdr(nseg);
+  std::vector<uint32_t> src(nsrc);
+#pragma omp parallel
+  {
+    std::vector<Occ> occ, self;
+#pragma omp for schedule(dynamic, 64) reduction(max : bad)
+    for (int n = 0; n < c->nonode; ++n) {
+      node_occ(n, occ, self);
+      int64_t sg[3], sc[3], kp[3];
+      for (int ca = 0; ca < 3; ++ca) {
+        const int srow = c->dof_eqn_map[3 * n + ca];
+        if (srow == 0) continue;
+        sg[ca] = neq + seg_off[srow - 1];
+        sc[ca] = src_diag[neq] + src_coef[srow - 1];
+        kp[ca] = 0;
+      }
+      walk(n, occ, self, [&](int ca, bool coef, int kmask, int m, int cols,
+                             auto&& body) {
+        const int srow = c->dof_eqn_map[3 * n + ca];
+        int64_t pos = coef ? sc[ca] : src_diag[srow - 1];
+        const int64_t p0 = pos;
+        body([&](int64_t w) { src[pos++] = (uint32_t)w; });
+        const int cnt = (int)(pos - p0);
+        if (!coef) {
+          hdr[srow - 1] = (uint16_t)(cnt | 1 << 8);
+          return;
+        }
+        sc[ca] = pos;
+        hdr[sg[ca]++] = (uint16_t)(cnt | kmask << 8);
+        // the segment's outputs are the row's next coefficients
+        const int32_t* kidx = c->k_indexes.data() + c->row_start[srow - 1];
+        for (int cb = 0; cb < 3; ++cb) {
+          if (!(cols >> cb & 1)) continue;
+          if (kp[ca] >= c->k_ptrs[srow - 1] ||
+              kidx[kp[ca]] != c->dof_eqn_map[3 * m + cb])
+            bad = 1;
+          ++kp[ca];
+        }
+      });
+      for (int ca = 0; ca < 3; ++ca) {
+        const int srow = c->dof_eqn_map[3 * n + ca];
+        if (srow != 0 && kp[ca] != c->k_ptrs[srow - 1]) bad = 1;
+      }
+    }
+  }
+  if (bad)
+    return fail(W3D_ERR_NUMERIC, "w3d_pattern: assembly map does not match "
+                                 "the sparsity pattern");
+  const int64_t nwave = (nseg + 63) / 64;
+  std::vector<int64_t> wbase(2 * std::max<int64_t>(nwave, 1), 0);
+  {
+    int64_t so = 0, oo = 0;
+    for (int64_t t = 0; t < nseg; ++t) {
+      if (t % 64 == 0) {
+        wbase[2 * (t / 64)] = so;
+        wbase[2 * (t / 64) + 1] = oo;
+      }
+      so += hdr[t] & 255;
+      oo += __builtin_popcount(hdr[t] >> 8);
+    }
+    if (oo != nout)
+      return fail(W3D_ERR_NUMERIC, "w3d_pattern: assembly map output count");
   }
   HIPCHK(hipSetDevice(c->device));
-  dfree(c->d_asm_ptr);
+  dfree(c->d_asm_hdr);
   dfree(c->d_asm_src);
+  dfree(c->d_asm_wbase);
   dfree(c->d_kout);
+  c->asm_nseg = nseg;
   int rc;
-  if ((rc = upload(&c->d_asm_ptr, cnt))) return rc;
+  if ((rc = upload(&c->d_asm_hdr, hdr))) return rc;
   if ((rc = upload(&c->d_asm_src, src))) return rc;
+  if ((rc = upload(&c->d_asm_wbase, wbase))) return rc;
   if ((rc = dalloc(&c->d_kout, nout))) return rc;
   // full symmetric CSR for the solver: row j = the transposed upper entries
   // (i, j) of rows i < j (ascending i), the diagonal, then row j's upper
@@ -1171,18 +1351,17 @@ int w3d_get_pattern(const w3d_ctx* c, int32_t* dof_eqn_map,
 }
 
 int w3d_assemble_vss_dev(w3d_ctx* c, double* d_k_diag, double* d_k_coeffs) {
-  if (!c || !c->d_asm_ptr) return fail(W3D_ERR_ARG, "w3d_assemble: no pattern");
+  if (!c || !c->d_asm_hdr) return fail(W3D_ERR_ARG, "w3d_assemble: no pattern");
   if (c->tangent_precision < 0)
     return fail(W3D_ERR_ARG, "w3d_assemble: no tangent computed");
   if (!d_k_diag || (!d_k_coeffs && c->ncoeff))
     return fail(W3D_ERR_ARG, "w3d_assemble_vss_dev: null output");
   HIPCHK(hipSetDevice(c->device));
-  int64_t nout = (int64_t)c->neqns + c->ncoeff;
   prof_begin(c, W3D_PHASE_ASSEMBLE);
   int rc = launch_assemble(c->d_ek64, c->d_ek32,
-                           c->tangent_precision == W3D_FP32, c->d_asm_ptr,
-                           c->d_asm_src, nout, c->neqns, d_k_diag, d_k_coeffs,
-                           c->stream);
+                           c->tangent_precision == W3D_FP32, c->d_asm_hdr,
+                           c->d_asm_src, c->d_asm_wbase, c->asm_nseg,
+                           c->neqns, d_k_diag, d_k_coeffs, c->stream);
   prof_end(c, W3D_PHASE_ASSEMBLE);
   if (rc) return fail(rc, "w3d_assemble: kernel launch failed");
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1191,16 +1370,16 @@ int w3d_assemble_vss_dev(w3d_ctx* c, double* d_k_diag, double* d_k_coeffs) {
 }
 
 int w3d_assemble_vss(w3d_ctx* c, double* k_diag, double* k_coeffs) {
-  if (!c || !c->d_asm_ptr) return fail(W3D_ERR_ARG, "w3d_assemble: no pattern");
+  if (!c || !c->d_asm_hdr) return fail(W3D_ERR_ARG, "w3d_assemble: no pattern");
   if (c->tangent_precision < 0)
     return fail(W3D_ERR_ARG, "w3d_assemble: no tangent computed");
   HIPCHK(hipSetDevice(c->device));
-  int64_t nout = (int64_t)c->neqns + c->ncoeff;
   prof_begin(c, W3D_PHASE_ASSEMBLE);
   int rc = launch_assemble(c->d_ek64, c->d_ek32,
-                           c->tangent_precision == W3D_FP32, c->d_asm_ptr,
-                           c->d_asm_src, nout, c->neqns, c->d_kout,
-                           c->d_kout + c->neqns, c->stream);
+                           c->tangent_precision == W3D_FP32, c->d_asm_hdr,
+                           c->d_asm_src, c->d_asm_wbase, c->asm_nseg,
+                           c->neqns, c->d_kout, c->d_kout + c->neqns,
+                           c->stream);
   prof_end(c, W3D_PHASE_ASSEMBLE);
   if (rc) return fail(rc, "w3d_assemble: kernel launch failed");
   c->assembled += 1;
@@ -1225,6 +1404,9 @@ int w3d_commit_step(w3d_ctx* c) {
                         hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->d_hist_n, c->d_hist_n1, c->nhist * 8,
                         hipMemcpyDeviceToDevice, c->stream));
+  if (c->has_geonl)  // adaptive_saves (mnralg.f:1305)
+    HIPCHK(hipMemcpyAsync(c->d_rot_n, c->d_rot_n1, 9 * c->nq * 8,
+                          hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return W3D_OK;
 }
@@ -1238,6 +1420,9 @@ int w3d_rollback(w3d_ctx* c) {
                         hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(c->d_hist_n1, c->d_hist_n, c->nhist * 8,
                         hipMemcpyDeviceToDevice, c->stream));
+  if (c->has_geonl)  // adaptive restart (mnralg.f:1393)
+    HIPCHK(hipMemcpyAsync(c->d_rot_n1, c->d_rot_n, 9 * c->nq * 8,
+                          hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return W3D_OK;
 }
@@ -1252,8 +1437,14 @@ static int state_size(const BlockDesc& B, int which, int64_t* n) {
     case W3D_ESTIFF: *n = (int64_t)B.utsz * B.span; return 0;
     case W3D_EINFVEC: *n = (int64_t)B.totdof * B.span; return 0;
     case W3D_ELEM_VOL: *n = B.span; return 0;
-    default: *n = 0; return -1;
+    case W3D_ROT_N1:
+      if (!B.geonl) break;
+      *n = 9 * nq;
+      return 9;
+    default: break;
   }
+  *n = 0;
+  return -1;
 }
 
 int w3d_get_block_state(const w3d_ctx* c, int32_t blk, int32_t which,
@@ -1280,6 +1471,7 @@ int w3d_get_block_state(const w3d_ctx* c, int32_t blk, int32_t which,
     case W3D_CEP: src = c->d_cep + 21 * B.q_off; break;
     case W3D_EINFVEC: src = c->d_einfvec + B.ifv_off; break;
     case W3D_ELEM_VOL: src = c->d_evol + (B.felem - 1); break;
+    case W3D_ROT_N1: src = c->d_rot_n1 + 9 * B.q_off; break;
     default: break;
   }
   if (which == W3D_ESTIFF) {
@@ -1292,7 +1484,7 @@ int w3d_get_block_state(const w3d_ctx* c, int32_t blk, int32_t which,
       HIPCHK(hipMemcpy(tmp.data(), c->d_ek64 + B.ek_off, n * 8,
                        hipMemcpyDeviceToHost));
     }
-    // node-pair block order -> packed upper column-major (pos = cp(q) + p)
+    // ranked row-major order -> packed upper column-major (cp(q) + p)
     for (int64_t i = 0; i < S; ++i) {
       int rank[kMaxNodes];
       node_ranks(&c->conn[B.conn_off + i * B.nnode], B.nnode, rank);
@@ -1300,7 +1492,7 @@ int w3d_get_block_state(const w3d_ctx* c, int32_t blk, int32_t which,
       double* dst = out + i * B.utsz;
       for (int q = 0; q < B.totdof; ++q)
         for (int p = 0; p <= q; ++p)
-          dst[q * (q + 1) / 2 + p] = src[npb_pos(B.nnode, rank, p, q)];
+          dst[q * (q + 1) / 2 + p] = src[ek_pos(B.nnode, rank, p, q)];
     }
     return W3D_OK;
   }

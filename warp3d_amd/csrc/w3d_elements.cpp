@@ -6,6 +6,7 @@
 //   hex8  (l3disop) quad2 getgpts.f:642 / deriv2 derivs.f:283
 //   tet10           quad6 getgpts.f:771 / deriv6 derivs.f:606
 // The constants are the reference's, including its truncated abscissae.
+#include <cmath>
 #include <cstring>
 
 #include "w3d_internal.h"
@@ -166,6 +167,50 @@ static void derivs(int etype, const double p[3], double* nx, double* ny,
   std::memcpy(nz, d4, sizeof d4);
 }
 
+// Least-squares solution x (4) of min |A x - b|, A (m x 4) column-major,
+// by Householder QR and back substitution (the steps of LAPACK's DGELS for
+// a full-rank A); for m < 4 the minimum-norm solution A^T (A A^T)^-1 b of
+// the one-point rule (m = 1).
+static void lsq4(int m, const double* a_in, const double* b_in, double* x) {
+  double a[4 * kMaxGp], b[kMaxGp];
+  std::memcpy(a, a_in, sizeof(double) * 4 * m);
+  std::memcpy(b, b_in, sizeof(double) * m);
+  if (m < 4) {
+    double s = 0.0;
+    for (int j = 0; j < 4; ++j) s += a[j * m] * a[j * m];
+    for (int j = 0; j < 4; ++j) x[j] = s > 0.0 ? a[j * m] * b[0] / s : 0.0;
+    return;
+  }
+  for (int j = 0; j < 4; ++j) {
+    double* v = a + j * m;
+    double alpha = v[j], xn = 0.0;
+    for (int i = j + 1; i < m; ++i) xn += v[i] * v[i];
+    xn = std::sqrt(xn);
+    double tau = 0.0;
+    if (xn != 0.0) {
+      const double beta = -std::copysign(std::sqrt(alpha * alpha + xn * xn),
+                                         alpha);
+      tau = (beta - alpha) / beta;
+      const double sc = 1.0 / (alpha - beta);
+      for (int i = j + 1; i < m; ++i) v[i] *= sc;
+      v[j] = beta;
+    }
+    for (int k = j + 1; k <= 4; ++k) {
+      double* y = k < 4 ? a + k * m : b;
+      double w = y[j];
+      for (int i = j + 1; i < m; ++i) w += v[i] * y[i];
+      w *= tau;
+      y[j] -= w;
+      for (int i = j + 1; i < m; ++i) y[i] -= w * v[i];
+    }
+  }
+  for (int j = 3; j >= 0; --j) {
+    double s = b[j];
+    for (int k = j + 1; k < 4; ++k) s -= a[k * m + j] * x[k];
+    x[j] = s / a[j * m + j];
+  }
+}
+
 bool fill_gauss_table(int etype, int order, GaussTable* t) {
   std::memset(t, 0, sizeof(*t));
   int g = ngp_of(etype, order), n = nnode_of(etype);
@@ -174,10 +219,22 @@ bool fill_gauss_table(int etype, int order, GaussTable* t) {
   t->order = order;
   t->ngp = g;
   t->nnode = n;
+  double amat[4 * kMaxGp];
   for (int k = 0; k < g; ++k) {
     double p[3];
     point(etype, order, k, p, &t->w[k]);
     derivs(etype, p, t->dn[0][k], t->dn[1][k], t->dn[2][k]);
+    amat[k] = p[0];
+    amat[g + k] = p[1];
+    amat[2 * g + k] = p[2];
+    amat[3 * g + k] = 1.0;
+  }
+  // columns of the fit's pseudo-inverse: the solution for unit data
+  for (int k = 0; k < g; ++k) {
+    double e[kMaxGp] = {0}, x[4];
+    e[k] = 1.0;
+    lsq4(g, amat, e, x);
+    for (int d = 0; d < 3; ++d) t->lsq[d][k] = x[d];
   }
   return true;
 }

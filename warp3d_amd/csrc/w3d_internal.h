@@ -8,7 +8,7 @@
 //   * element forces einfvec are element-contiguous and node-major inside
 //     the element: base + i*T + 3*j + c (component c of local node j).
 //   * element tangents are element-contiguous, utsz words each, in the
-//     node-pair block order described at npb_pos (base + i*utsz + word);
+//     ranked row-major order described at ek_pos (base + i*utsz + word);
 //     w3d_get_block_state converts them to the reference's packed upper
 //     triangle (estiff_blocks (utsz, span)).
 #pragma once
@@ -37,6 +37,8 @@ struct BlockDesc {
   int32_t mat, model, hist, bbar;
   int32_t table;     // index of the Gauss/derivative table
   int32_t ncry;      // mm10: crystals per point
+  int32_t geonl;     // finite strains and rotations (lprops(18))
+  int32_t pad_;
   int64_t q_off;     // first Gauss point of the block (global numbering)
   int64_t hist_off;  // first history word of the block
   int64_t ek_off;    // first Ke word of the block
@@ -78,6 +80,9 @@ struct GaussTable {
   int32_t etype, order, ngp, nnode;
   double w[kMaxGp];
   double dn[3][kMaxGp][kMaxNodes];  // d/dxi, d/deta, d/dzeta
+  // slopes of the least-squares linear fit c1 xi + c2 eta + c3 zeta + c4
+  // of values at the points: c_d = sum_p lsq[d][p] v_p (d = 0..2)
+  double lsq[3][kMaxGp];
 };
 
 // A chunk of consecutive elements of one block handled by one workgroup.
@@ -111,16 +116,14 @@ struct DevModel {
   double* uddt;           // mm10: strain increments (6, points) per block
   double eps_bbar;
   double dt;              // time increment of the load step
-  // word -> (rank r, rank s, component i, component j) of the node-pair
-  // block layout, r | s << 5 | i << 10 | j << 12, for 8, 10 and 20 nodes
-  // at npb_tab_off(NN)
-  const uint16_t* npb_tab;
+  // finite strains: [R n+1] of every point (9 words, column-major, SoA
+  // like the stresses: word k of point q of block b at
+  // 9 q_off + k S G + q), and the displacements of the last recovery,
+  // which place the tangent's configuration at X + u + du
+  double* rot_n1;
+  const double* u_t;
+  const double* du_t;
 };
-
-__host__ __device__ constexpr int npb_tab_off(int NN) {
-  return NN == 8 ? 0 : NN == 10 ? 300 : 765;
-}
-constexpr int kNpbTabSize = 300 + 465 + 1830;
 
 // Kernel launchers (w3d_kernels.hip, w3d_tangent.hip).
 // Recovery passes: pass 0 runs mm01 blocks in full and the strain
@@ -130,32 +133,41 @@ int launch_recover(const DevModel& M, const std::vector<BlockDesc>& hblocks,
                    const WorkItem* d_work, const std::vector<int>& groups,
                    const std::vector<int>& group_start,
                    const std::vector<int>& group_count, int step, int iter,
-                   int nonlinear, const double* d_du, int want_energy,
-                   double* en_wg, int pass, hipStream_t s);
+                   int nonlinear, const double* d_u, const double* d_du,
+                   int want_energy, double* en_wg, int pass, hipStream_t s);
 // mm10 update, one thread per point over work items of <= 64 points.
 int launch_mm10(const DevModel& M, const WorkItem* d_work, int nwork,
                 int step, int iter0_off, hipStream_t s);
+// grad Fe words of the small-strain mm10 blocks listed in d_blks whose
+// index lies in [lo, hi) (rknstr_finish_cp), one workgroup per block.
+int launch_mm10_grads(const DevModel& M, const int32_t* d_blks, int nblks,
+                      int lo, int hi, int step, hipStream_t s);
 int launch_ifv_gather(const double* einfvec, const int64_t* inv_ptr,
                       const int64_t* inv_off, int nonode, double* ifv,
                       hipStream_t s);
 int launch_energy_reduce(const double* en_wg, const int32_t* blk_wfirst,
                          const int32_t* blk_wcount, int nblk,
                          double* blk_sums, hipStream_t s);
+// Assembly over the segment map built by w3d_pattern: per segment a
+// header (contributions | component mask << 8), its contributions' first
+// Ke words in src, and per 64 segments the index of the first contribution
+// and of the first output (diagonal outputs first, then coefficients).
 int launch_assemble(const double* ek64, const float* ek32, int fp32,
-                    const int64_t* row_ptr, const uint32_t* src,
-                    int64_t nout, int64_t neq, double* diag, double* coef,
-                    hipStream_t s);
+                    const uint16_t* hdr, const uint32_t* src,
+                    const int64_t* wbase, int64_t nseg, int64_t neq,
+                    double* diag, double* coef, hipStream_t s);
 int launch_tangent_mfma(const DevModel& M, const WorkItem* d_work,
                         const std::vector<int>& groups,
                         const std::vector<int>& group_start,
                         const std::vector<int>& group_count, int precision,
                         hipStream_t s);
-// elements per workgroup of the MFMA tangent (one wave per element)
+// MFMA tangent: waves per workgroup (one element per wave at a time) and
+// elements per workgroup (each wave takes every tan_mfma_waves-th one)
 __host__ __device__ constexpr int tan_mfma_waves(int nnode) {
-  return nnode == 20 ? 2 : 4;
+  return (void)nnode, 4;
 }
 __host__ __device__ constexpr int tan_mfma_chunk(int nnode) {
-  return 8 * tan_mfma_waves(nnode);
+  return nnode == 20 ? 16 : 32;
 }
 
 // Jacobi-PCG building blocks (w3d_solver.hip)
@@ -181,9 +193,10 @@ int solver_shift(double* sc, hipStream_t s);
 bool mm10_crystal_setup(const w3d_material& m, const double* angles,
                         CrystalDev* cr);
 
-// Kernel-group keys: one instantiation per (nnode, ngp, bbar) and, for the
-// recovery, per material family (mm10 blocks run in two passes).
-int group_key(int nnode, int ngp, int bbar, int mm10 = 0);
+// Kernel-group keys: one instantiation per (nnode, ngp, bbar), per
+// material family for the recovery (mm10 blocks run in two passes) and
+// per kinematics (finite strains).  Bit 0: geonl; bit 1: mm10.
+int group_key(int nnode, int ngp, int bbar, int mm10 = 0, int geonl = 0);
 
 // Lower-triangle index of (row, col) in the 21-word [D] storage
 // (rstgp1_store_cep order (1,1),(2,1),(2,2),(3,1),...).
@@ -191,31 +204,32 @@ __host__ __device__ constexpr int tri(int r, int c) {
   return r >= c ? r * (r + 1) / 2 + c : c * (c + 1) / 2 + r;
 }
 
-// Element tangents are stored in a node-pair block layout: the element's
-// nodes ranked by ascending global number; block row r (the node of rank r)
-// holds the upper triangle of its 3x3 diagonal block (6 words, row-major:
-// (0,0) (0,1) (0,2) (1,1) (1,2) (2,2)) then the full 3x3 blocks with every
-// node of higher rank s = r+1 .. NN-1 (9 words each, row-major, rows the
-// components of the rank-r node).  The words of one element are exactly the
-// contributions it makes to the rows of each of its nodes in the assembled
-// upper triangle, so the row-by-row assembly reads them in runs.  Same size
-// as the packed upper triangle: 6 NN + 9 NN (NN-1)/2 = 3NN (3NN+1)/2.
-__host__ __device__ constexpr int npb_row_start(int NN, int r) {
-  return 6 * r + 9 * (r * (NN - 1) - r * (r - 1) / 2);
+// Element tangents are stored in a ranked row-major layout: the element's
+// nodes are ranked by ascending global number (ties by local index), its
+// dofs renumbered d' = 3*rank + component, and the upper triangle of Ke in
+// that numbering stored row by row: word(p', q') = p' T - p'(p'-1)/2 +
+// q' - p' for p' <= q' (T = 3 NN).  The three rows of the node of rank r
+// are then the contiguous run of words the element adds to that node's
+// rows of the assembled upper triangle, each row ordered like the global
+// columns, so the row-by-row assembly reads them in runs.
+__host__ __device__ constexpr int ek_word(int T, int p, int q) {
+  return p * T - p * (p - 1) / 2 + q - p;
+}
+// Rank of every local node: the number of local nodes with a smaller
+// global number, or an equal one at a smaller local index.
+__host__ __device__ inline void node_ranks(const int32_t* nodes, int nn,
+                                           int* rank) {
+  for (int a = 0; a < nn; ++a) {
+    int r = 0;
+    for (int b = 0; b < nn; ++b)
+      r += (nodes[b] < nodes[a] || (nodes[b] == nodes[a] && b < a)) ? 1 : 0;
+    rank[a] = r;
+  }
 }
 // Word of Ke(p, q) (local dofs, dof = component*NN + node) given the ranks.
-__host__ __device__ inline int npb_pos(int NN, const int* rank, int p, int q) {
-  int a = p % NN, i = p / NN, b = q % NN, j = q / NN;
-  int ra = rank[a], rb = rank[b];
-  if (ra == rb) {
-    int lo = i < j ? i : j, hi = i < j ? j : i;
-    return npb_row_start(NN, ra) + lo * (5 - lo) / 2 + hi;
-  }
-  if (ra > rb) {
-    int t = ra; ra = rb; rb = t;
-    t = i; i = j; j = t;
-  }
-  return npb_row_start(NN, ra) + 6 + 9 * (rb - ra - 1) + 3 * i + j;
+__host__ __device__ inline int ek_pos(int NN, const int* rank, int p, int q) {
+  int a = 3 * rank[p % NN] + p / NN, b = 3 * rank[q % NN] + q / NN;
+  return a <= b ? ek_word(3 * NN, a, b) : ek_word(3 * NN, b, a);
 }
 
 // Upper 16x16 tiles (I <= J) of a tile grid, enumerated column by column.

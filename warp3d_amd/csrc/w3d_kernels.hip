@@ -12,6 +12,7 @@
 //   assem_by_row.
 #include <hip/hip_runtime.h>
 
+#include "w3d_geonl.h"
 #include "w3d_internal.h"
 
 namespace w3d {
@@ -247,17 +248,23 @@ struct BCol {
 // MODE 0: mm01 blocks, everything fused.  MODE 1: mm10 blocks, phases 1-3
 // up to the strain increment (left in uddt for the mm10 kernel).  MODE 2:
 // mm10 blocks, energy and element forces from the updated stresses.
+// GEONL (MODE 0, finite strains; rstgp1.f:19, gtmat1.f:17, rknifv.f:141):
+// the strain increment is formed with the mid-step configuration
+// X + u + du/2 and rotated to the unrotated frame by [R n+1/2]; [R n+1]
+// and det F n+1 come from F of the n = 0 configuration; the forces use the
+// n+1 configuration and the Cauchy stress [R n+1] t [R n+1]^T.
 // ---------------------------------------------------------------------
-template <int NN, int G, bool BBAR, int MODE>
+template <int NN, int G, bool BBAR, int MODE, bool GEONL = false>
 __global__ void __launch_bounds__(256)
     k_recover(DevModel M, const WorkItem* __restrict__ work, int step,
-              int iter, int nonlinear, const double* __restrict__ du,
-              int want_energy, double* __restrict__ en_wg, int slot0) {
+              int iter, int nonlinear, const double* __restrict__ u,
+              const double* __restrict__ du, int want_energy,
+              double* __restrict__ en_wg, int slot0) {
   constexpr int T = 3 * NN;
   constexpr int E = rec_elems(G);
-  // coordinates and increments (phases 0-3) share their LDS with the
-  // updated stresses and energy terms (phases 3-4)
-  constexpr int NA = 2 * E * T, NB = 8 * E * G;
+  // coordinates, displacements and increments (phases 0-3) share their
+  // LDS with the updated stresses and energy terms (phases 3-4)
+  constexpr int NA = (GEONL ? 3 : 2) * E * T, NB = 8 * E * G;
   __shared__ double s_dn[3][G][NN];
   __shared__ double s_w[G];
   __shared__ double s_buf[NA > NB ? NA : NB];
@@ -265,6 +272,7 @@ __global__ void __launch_bounds__(256)
   __shared__ double s_vol[BBAR ? E : 1][25];
   double(*s_ce)[T] = reinterpret_cast<double(*)[T]>(s_buf);
   double(*s_du)[T] = reinterpret_cast<double(*)[T]>(s_buf + E * T);
+  double(*s_u)[T] = reinterpret_cast<double(*)[T]>(s_buf + 2 * E * T);
   double(*s_sig)[G][6] = reinterpret_cast<double(*)[G][6]>(s_buf);
   double* s_en0 = s_buf + 6 * E * G;
   double* s_en1 = s_buf + 7 * E * G;
@@ -291,18 +299,28 @@ __global__ void __launch_bounds__(256)
       s_du[e][NN + j] = du[3 * node + 1];
       s_du[e][2 * NN + j] = du[3 * node + 2];
     }
+    if (GEONL) {
+      s_u[e][j] = u[3 * node];
+      s_u[e][NN + j] = u[3 * node + 1];
+      s_u[e][2 * NN + j] = u[3 * node + 2];
+    }
   }
   __syncthreads();
 
   const int e = tid / G, g = tid % G;
   const bool active = e < cnt;
-  // phase 1: coordinate Jacobian, determinant and inverse (jacob1)
-  if (active) {
+  // jacob1 of lane (e, g) on the configuration X + a u + b du
+  auto jacobian = [&](double a_u, double b_du) {
     double j11 = 0, j12 = 0, j13 = 0, j21 = 0, j22 = 0, j23 = 0, j31 = 0,
            j32 = 0, j33 = 0;
 #pragma unroll 2
     for (int j = 0; j < NN; ++j) {
       double x = s_ce[e][j], y = s_ce[e][NN + j], z = s_ce[e][2 * NN + j];
+      if (GEONL && a_u != 0.0) {
+        x = x + s_u[e][j] + b_du * s_du[e][j];
+        y = y + s_u[e][NN + j] + b_du * s_du[e][NN + j];
+        z = z + s_u[e][2 * NN + j] + b_du * s_du[e][2 * NN + j];
+      }
       double a = s_dn[0][g][j], b = s_dn[1][g][j], c = s_dn[2][g][j];
       j11 = j11 + a * x; j12 = j12 + a * y; j13 = j13 + a * z;
       j21 = j21 + b * x; j22 = j22 + b * y; j23 = j23 + b * z;
@@ -324,10 +342,35 @@ __global__ void __launch_bounds__(256)
     ga[8] = (j11 * j22 - j12 * j21) / dj;
     ga[9] = dj;
     if (dj <= 1.0e-20) atomicOr(&M.flags[2], 1);
+  };
+  // finite strains: [R n+1], [R n+1/2], det F n+1 and det J at n = 0 of
+  // the lane's point (gtmat1: F from the n = 0 Jacobian)
+  double rn1[9], rnh[9], dfn1 = 0.0, dj0 = 0.0;
+  if (GEONL && active) {
+    jacobian(0.0, 0.0);
+    const double* ga = s_gam[e][g];
+    dj0 = ga[9];
+    double g0[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) g0[k] = ga[k];
+    double f[9];
+    dfn1 = geonl_f<NN>(g0, s_dn[0][g], s_dn[1][g], s_dn[2][g], s_u[e],
+                       s_du[e], 1.0, f);
+    bool bad = dfn1 <= 1.0e-20;
+    geonl_rotation(f, rn1);
+    double dfh = geonl_f<NN>(g0, s_dn[0][g], s_dn[1][g], s_dn[2][g], s_u[e],
+                             s_du[e], 0.5, f);
+    bad = bad || dfh <= 1.0e-20;
+    geonl_rotation(f, rnh);
+    if (bad) atomicOr(&M.flags[3], 1);
   }
+  if (GEONL) __syncthreads();  // the n = 0 Jacobians are replaced below
+  // phase 1: coordinate Jacobian, determinant and inverse (jacob1), on the
+  // mid-step configuration for finite strains (rknstr_geonl)
+  if (active) jacobian(GEONL ? 1.0 : 0.0, 0.5);
   __syncthreads();
   // phase 2: mean-dilatation volume terms (vol_terms + vol_avg)
-  if (BBAR) {
+  auto vol_terms = [&]() {
     if (tid < cnt * 8) {
       int ee = tid >> 3, j = tid & 7;
       double vx = 0, vy = 0, vz = 0, volume = 0;
@@ -346,6 +389,9 @@ __global__ void __launch_bounds__(256)
       s_vol[ee][3 * j + 1] = vy * voli;
       s_vol[ee][3 * j + 2] = vz * voli;
     }
+  };
+  if (BBAR) {
+    vol_terms();
     __syncthreads();
   }
   const double third = 1.0 / 3.0;
@@ -389,6 +435,13 @@ __global__ void __launch_bounds__(256)
       deps[3] = deps[3] + by * du_u + bx * du_v;
       deps[4] = deps[4] + bz * du_v + by * du_w;
       deps[5] = deps[5] + bz * du_u + bx * du_w;
+    }
+    if (GEONL) {  // uddt = [qnhalf] ddt (qmply1)
+      double qh[36], ddt[6];
+      geonl_q(rnh, 1, qh);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) ddt[k] = deps[k];
+      geonl_qmul(qh, ddt, deps);
     }
     const int64_t SG = (int64_t)bd.span * G;
     const int64_t q = (int64_t)(wi.first + e) * G + g;
@@ -451,12 +504,41 @@ __global__ void __launch_bounds__(256)
     }
   }
   if (MODE == 1) return;  // uniform across the workgroup
-  __syncthreads();  // increments no longer read: their LDS takes stresses
-  if (active) {
+  __syncthreads();
+  if (GEONL) {
+    // rknifv_std: Jacobians and B-bar terms of the n+1 configuration
+    if (active) jacobian(1.0, 1.0);
+    __syncthreads();
+    if (BBAR) {
+      vol_terms();
+      __syncthreads();
+    }
+    if (active && iter > 0) {  // rplstr: [R n+1] kept for iter > 0
+      const int64_t SG = (int64_t)bd.span * G;
+      const int64_t q = (int64_t)(wi.first + e) * G + g;
+      double* rr = M.rot_n1 + 9 * bd.q_off + q;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) s_sig[e][g][k] = sig[k];
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) rr[(a + 3 * c) * SG] = rn1[3 * a + c];
+    }
+  }
+  // increments no longer read: their LDS takes the stresses (the Cauchy
+  // stresses for finite strains, gpifv1)
+  if (active) {
+    if (GEONL) {
+      double q2[36], cs[6];
+      geonl_q(rn1, 2, q2);
+      geonl_qmul(q2, sig, cs);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) s_sig[e][g][k] = cs[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) s_sig[e][g][k] = sig[k];
+    }
     if (want_energy) {  // rstgp1_b terms, point-major like its loops
-      double factor = s_gam[e][g][9] * 1.0 * s_w[g];
+      double factor = GEONL ? dfn1 * dj0 * 1.0 * s_w[g]
+                            : s_gam[e][g][9] * 1.0 * s_w[g];
       s_en0[g * cnt + e] = sig[6] * factor;
       s_en1[g * cnt + e] = sig[7] * factor;
     }
@@ -579,31 +661,65 @@ __global__ void k_energy_reduce(const double* __restrict__ en_wg,
   sums[2 * b + 1] = pw;
 }
 
-// Row-by-row assembly (assem_a_row): output t sums its Ke words in the
-// order of the elements on the row's node.  Outputs 0 .. neq-1 are the
+// Row-by-row assembly (assem_a_row, assemble_code.f:785) over the segment
+// map: one thread per segment (a diagonal, or a run of one row's
+// coefficients in the components of one column node).  The wave finds its
+// segments' first contribution and first output by prefix sums of the
+// headers from the wave's two base offsets.  Each output sums its
+// contributions in FP64 in the reference's order (elements on the row's
+// node, then erow, then ecol occurrences).  Outputs 0 .. neq-1 are the
 // diagonal, the rest the coefficients right of the diagonal.
+__device__ inline int wave_incl_scan(int x, int lane) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  return x;
+}
+
 template <typename KT>
-__global__ void k_assemble(const KT* __restrict__ ek,
-                           const int64_t* __restrict__ row_ptr,
-                           const uint32_t* __restrict__ src, int64_t nout,
-                           int64_t neq, double* __restrict__ diag,
-                           double* __restrict__ coef) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nout) return;
-  double s = 0.0;
-  for (int64_t p = row_ptr[t]; p < row_ptr[t + 1]; ++p)
-    s = s + (double)ek[src[p]];
-  if (t < neq)
-    diag[t] = s;
-  else
-    coef[t - neq] = s;
+__global__ void __launch_bounds__(256)
+    k_assemble(const KT* __restrict__ ek, const uint16_t* __restrict__ hdr,
+               const uint32_t* __restrict__ src,
+               const int64_t* __restrict__ wbase, int64_t nseg, int64_t neq,
+               double* __restrict__ diag, double* __restrict__ coef) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int h = t < nseg ? hdr[t] : 0;
+  const int n = h & 255, mask = (h >> 8) & 7;
+  const int len = __popc(mask);
+  const int n_ex = wave_incl_scan(n, lane) - n;
+  const int l_ex = wave_incl_scan(len, lane) - len;
+  if (t >= nseg) return;
+  const int64_t w = t >> 6;
+  const int64_t so = wbase[2 * w] + n_ex;
+  int64_t oo = wbase[2 * w + 1] + l_ex;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int k = 0; k < n; ++k) {
+    const uint32_t b = src[so + k];
+    if (mask & 1) s0 = s0 + (double)ek[b];
+    if (mask & 2) s1 = s1 + (double)ek[b + 1];
+    if (mask & 4) s2 = s2 + (double)ek[b + 2];
+  }
+  const double v[3] = {s0, s1, s2};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (!(mask >> c & 1)) continue;
+    if (oo < neq)
+      diag[oo] = v[c];
+    else
+      coef[oo - neq] = v[c];
+    ++oo;
+  }
 }
 
 // ---------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------
-int group_key(int nnode, int ngp, int bbar, int mm10) {
-  return ((nnode * 100 + ngp) * 2 + (bbar ? 1 : 0)) * 2 + (mm10 ? 1 : 0);
+int group_key(int nnode, int ngp, int bbar, int mm10, int geonl) {
+  return (((nnode * 100 + ngp) * 2 + (bbar ? 1 : 0)) * 2 + (mm10 ? 1 : 0)) *
+             2 + (geonl ? 1 : 0);
 }
 
 #define W3D_GROUPS(X)    \
@@ -621,30 +737,36 @@ int launch_recover(const DevModel& M, const std::vector<BlockDesc>& hblocks,
                    const WorkItem* d_work, const std::vector<int>& groups,
                    const std::vector<int>& gstart,
                    const std::vector<int>& gcount, int step, int iter,
-                   int nonlinear, const double* d_du, int want_energy,
-                   double* en_wg, int pass, hipStream_t s) {
+                   int nonlinear, const double* d_u, const double* d_du,
+                   int want_energy, double* en_wg, int pass, hipStream_t s) {
   (void)hblocks;
   for (size_t k = 0; k < groups.size(); ++k) {
     if (gcount[k] == 0) continue;
-    const bool mm10 = groups[k] & 1;
+    const bool mm10 = (groups[k] >> 1) & 1;
+    const bool geonl = groups[k] & 1;
     if (pass == 1 && !mm10) continue;
     const int mode = mm10 ? (pass == 0 ? 1 : 2) : 0;
+    if (geonl && mode != 0) return W3D_ERR_UNSUPPORTED;
     const WorkItem* w = d_work + gstart[k];
     bool done = false;
 #define X(NN, G, BB)                                                        \
-  if (!done && (groups[k] >> 1) == (group_key(NN, G, BB) >> 1)) {          \
-    if (mode == 0)                                                          \
+  if (!done && (groups[k] >> 2) == (group_key(NN, G, BB) >> 2)) {          \
+    if (geonl)                                                              \
+      hipLaunchKernelGGL((k_recover<NN, G, BB, 0, true>), dim3(gcount[k]),  \
+                         dim3(256), 0, s, M, w, step, iter, nonlinear, d_u,  \
+                         d_du, want_energy, en_wg, gstart[k]);              \
+    else if (mode == 0)                                                     \
       hipLaunchKernelGGL((k_recover<NN, G, BB, 0>), dim3(gcount[k]),        \
-                         dim3(256), 0, s, M, w, step, iter, nonlinear, d_du, \
-                         want_energy, en_wg, gstart[k]);                    \
+                         dim3(256), 0, s, M, w, step, iter, nonlinear, d_u,  \
+                         d_du, want_energy, en_wg, gstart[k]);              \
     else if (mode == 1)                                                     \
       hipLaunchKernelGGL((k_recover<NN, G, BB, 1>), dim3(gcount[k]),        \
-                         dim3(256), 0, s, M, w, step, iter, nonlinear, d_du, \
-                         want_energy, en_wg, gstart[k]);                    \
+                         dim3(256), 0, s, M, w, step, iter, nonlinear, d_u,  \
+                         d_du, want_energy, en_wg, gstart[k]);              \
     else                                                                    \
       hipLaunchKernelGGL((k_recover<NN, G, BB, 2>), dim3(gcount[k]),        \
-                         dim3(256), 0, s, M, w, step, iter, nonlinear, d_du, \
-                         want_energy, en_wg, gstart[k]);                    \
+                         dim3(256), 0, s, M, w, step, iter, nonlinear, d_u,  \
+                         d_du, want_energy, en_wg, gstart[k]);              \
     done = true;                                                            \
   }
     W3D_GROUPS(X)
@@ -675,16 +797,17 @@ int launch_energy_reduce(const double* en_wg, const int32_t* wfirst,
 }
 
 int launch_assemble(const double* ek64, const float* ek32, int fp32,
-                    const int64_t* row_ptr, const uint32_t* src, int64_t nout,
-                    int64_t neq, double* diag, double* coef, hipStream_t s) {
-  int64_t nb = (nout + 255) / 256;
+                    const uint16_t* hdr, const uint32_t* src,
+                    const int64_t* wbase, int64_t nseg, int64_t neq,
+                    double* diag, double* coef, hipStream_t s) {
+  const int64_t nb = (nseg + 255) / 256;
   if (nb > 0) {
     if (fp32)
       hipLaunchKernelGGL(k_assemble<float>, dim3((unsigned)nb), dim3(256), 0,
-                         s, ek32, row_ptr, src, nout, neq, diag, coef);
+                         s, ek32, hdr, src, wbase, nseg, neq, diag, coef);
     else
       hipLaunchKernelGGL(k_assemble<double>, dim3((unsigned)nb), dim3(256), 0,
-                         s, ek64, row_ptr, src, nout, neq, diag, coef);
+                         s, ek64, hdr, src, wbase, nseg, neq, diag, coef);
   }
   return hipGetLastError() == hipSuccess ? W3D_OK : W3D_ERR_DEVICE;
 }

@@ -1107,6 +1107,81 @@ __global__ void __launch_bounds__(64)
       cp[tri(r, cc) * SG] = tang_avg[r + 6 * cc];
 }
 
+// ---------------------------------------------------------------------
+// grad Fe of small-strain mm10 blocks: rknstr_finish_cp (rknstr.f) with
+// mm10_calc_grads (mm10_a.f:1579).  For single-crystal elements the
+// reference fits each component of Rt3d(3,3,ngp) linearly over the Gauss
+// points (DGELS) and stores the three slopes of every component, the same
+// 27 words at every point; Rt3d is a flat copy (mm10_a_copy_vector) of
+// 9 ngp consecutive words of the block's gathered n-history
+// elem_hist(span, hist, ngp) from (element 1, first Rp word of crystal 1,
+// point 1), so words of several elements of the block enter it.  At step
+// 1 those words are the values mm10 initialised in place.  Elements with
+// several crystals get zeros.  One 64-lane workgroup per mm10 block.
+// ---------------------------------------------------------------------
+__device__ double mm10_init_word(const DevModel& M, const BlockDesc& bd,
+                                 int w, int i, int64_t q, int64_t SG) {
+  const double* hn = M.hist_n + bd.hist_off;
+  if (w < kMM10Common)  // mm10_init_general_hist, _uout_hist, _slip_hist
+    return (w >= 63 && w < 72) ? ((w - 63) % 4 == 0 ? 1.0 : 0.0) : 0.0;
+  const int c = (w - kMM10Common) / kMM10Crystal;
+  const int k = (w - kMM10Common) % kMM10Crystal;
+  if (k < 6) return M.stress_n[9 * bd.q_off + k * SG + q];  // stress
+  if (k < 9)                                                // angles
+    return M.crys[bd.crys_off + (int64_t)i * bd.ncry + c].angles[k - 6];
+  if (k < 18) return (k - 9) % 4 == 0 ? 1.0 : 0.0;          // Rp
+  if (k == 42) return M.mats[bd.mat].cp.tau_y + 1.0e-5;     // tau tilde
+  if (k < 58) return 0.0;
+  return hn[w * SG + q];  // words mm10 does not initialise
+}
+
+__global__ void __launch_bounds__(64)
+    k_mm10_grads(DevModel M, const int32_t* __restrict__ blks, int lo,
+                 int hi, int step) {
+  const int b = blks[blockIdx.x];
+  if (b < lo || b >= hi) return;
+  const BlockDesc bd = M.blocks[b];
+  const GaussTable* gt = M.tables + bd.table;
+  const int S = bd.span, G = bd.ngp, H = bd.hist;
+  const int64_t SG = (int64_t)S * G;
+  const int l = threadIdx.x;
+  __shared__ double s_v[9 * kMaxGp];
+  __shared__ double s_g[27];
+  const double* hn = M.hist_n + bd.hist_off;
+  if (bd.ncry == 1) {
+    for (int f = l; f < 9 * G; f += 64) {
+      const int64_t L = (int64_t)(kMM10Common + 9) * S + f;
+      const int i = (int)(L % S);
+      const int w = (int)((L / S) % H);
+      const int g = (int)(L / ((int64_t)S * H));
+      const int64_t q = (int64_t)i * G + g;
+      s_v[f] = step == 1 ? mm10_init_word(M, bd, w, i, q, SG) : hn[w * SG + q];
+    }
+  }
+  __syncthreads();
+  if (l < 27) {
+    // grads(a, b, d) at a + 3b + 9d from Rt3d(a, b, p) at a + 3b + 9p
+    double x = 0.0;
+    if (bd.ncry == 1) {
+      const int ab = l % 9, d = l / 9;
+      for (int p = 0; p < G; ++p) x = x + gt->lsq[d][p] * s_v[ab + 9 * p];
+    }
+    s_g[l] = x;
+  }
+  __syncthreads();
+  double* hn1 = M.hist_n1 + bd.hist_off;
+  for (int64_t k = l; k < 27 * SG; k += 64)
+    hn1[(36 + k / SG) * SG + k % SG] = s_g[k / SG];
+}
+
+int launch_mm10_grads(const DevModel& M, const int32_t* d_blks, int nblks,
+                      int lo, int hi, int step, hipStream_t s) {
+  if (nblks > 0)
+    hipLaunchKernelGGL(k_mm10_grads, dim3(nblks), dim3(64), 0, s, M, d_blks,
+                       lo, hi, step);
+  return hipGetLastError() == hipSuccess ? W3D_OK : W3D_ERR_DEVICE;
+}
+
 int launch_mm10(const DevModel& M, const WorkItem* d_work, int nwork,
                 int step, int iter0_off, hipStream_t s) {
   if (nwork > 0)

@@ -6,15 +6,31 @@
 // gptns1.f:1174 extract_D_symmetric, gptns1.f:1628 bdbtgen, gpmas1.f:399
 // adjust_cnst.  bdbtgen forms, per point, BD = D B and then the upper
 // triangle Ke(row,col) += sum_r B(r,col) BD(r,row).  Here the sum over
-// points and strain components is one contraction over k = (point, r):
+// points and strain components is one contraction over k = (r, point):
 //   Ke = A B   with  A(row, k) = BD(r, row) at point p,  B(k, col) = B(r, col)
 // taken 4 k at a time by the 16x16x4 MFMA on 16x16 tiles of the upper
-// triangle.  One wave owns one element: per chunk of two points its lanes
-// build the B and BD columns in FP64 (one (point, dof) pair per lane) from
-// the element geometry and the stored [D], round them to the MFMA type into
-// a wave-private LDS slab, and then feed the 16x16x4 steps from it.
+// triangle.  A 4-k step is one strain component r at four points, so each
+// lane (point q = lane/16 of the step, dof column lane%16 of every tile)
+// builds its own A and B operands in registers: the three gradient
+// components of its node at its point, the B column (with the mean
+// dilatation terms for B-bar) and the six D B values, reused over the six
+// strain components.  Columns are the element dofs renumbered by node rank
+// (3*rank + component), so the accumulators store directly into the ranked
+// row-major element layout (ek_word) in runs of 16 words.
+//
+// One wave owns one element at a time; the next element's coordinates and
+// [D] are fetched while the current one is contracted.
+//
+// Finite strains (GEONL; gptns1.f:307 gptns1_a, :1735 ctran1, kg1.f:16
+// kgstiff): the configuration is X + u + du of the last recovery; [D] is
+// rotated to the Cauchy frame with the stored [R n+1] and corrected by the
+// initial-stress matrix (FP64, per point); the geometric stiffness
+// gtg(a, b) = sum_p (g_a^T S g_b) w |J| of every node pair is a second,
+// smaller contraction on the matrix cores (k = (point, component)) whose
+// NN x NN result is added to the three component blocks as Ke is stored.
 #include <hip/hip_runtime.h>
 
+#include "w3d_geonl.h"
 #include "w3d_internal.h"
 
 namespace w3d {
@@ -47,33 +63,35 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int NN, int G, bool BBAR, typename T>
-__global__ void __launch_bounds__(64 * tan_mfma_waves(NN))
+template <int NN, int G, bool BBAR, typename T, bool GEONL = false>
+__global__ void __launch_bounds__(64 * tan_mfma_waves(NN)) __attribute__((amdgpu_waves_per_eu(2)))
     k_tangent_mfma(DevModel M, const WorkItem* __restrict__ work,
                    T* __restrict__ ek_out) {
   constexpr int W = tan_mfma_waves(NN);   // waves per workgroup
   constexpr int TD = 3 * NN;              // element dofs
   constexpr int NTL = (TD + 15) / 16;     // 16-wide tiles per side
-  constexpr int TP = 16 * NTL;            // padded dof count
   constexpr int NUP = NTL * (NTL + 1) / 2;  // upper tiles
-  constexpr int NCH = (G + 1) / 2;        // chunks of two points (12 k)
-  constexpr int LDP = TP + 16;            // padded LDS row
+  constexpr int NPG = (G + 3) / 4;        // 4-point groups (one k step each)
   constexpr int ND = (21 * G + 63) / 64;  // [D] words per lane
   using MF = Mfma<T>;
   using v4 = typename MF::v4;
 
-  __shared__ double s_dn[3][G][NN];
+  __shared__ double s_dn64[3][G][NN];     // Jacobians (FP64)
+  __shared__ T s_dn[3][G][NN];            // gradients (MFMA type)
   __shared__ double s_w[G];
-  __shared__ double s_ce[W][TD];
-  __shared__ double s_gam[W][G][10];
-  __shared__ double s_vol[W][24];
-  __shared__ double s_d[W][G][21];
+  __shared__ double s_ce[W][3][NN];
+  __shared__ double s_detw[W][G];
+  __shared__ T s_gam[W][G][9];
+  __shared__ T s_vol[W][3 * NN];
+  __shared__ T s_d[W][G][21];
   __shared__ int s_node[W][NN];
   __shared__ int s_inv[W][NN];
-  // per wave: B(r, dof) and BD(r, dof) = (D B)(r, dof) of the chunk's two
-  // points; after the contraction the same slab stages the packed Ke
-  __shared__ T s_bd[W][2][12][LDP];
-  static_assert(24 * LDP >= TD * (TD + 1) / 2, "Ke staging fits the slab");
+  // finite strains: w|J|-scaled [D] in FP64 (then the geometric stiffness
+  // of the node pairs in the same words) and the Cauchy stresses
+  constexpr int GW = GEONL ? W : 1;
+  constexpr int GX = GEONL ? (21 * G > NN * NN ? 21 * G : NN * NN) : 1;
+  __shared__ double s_gx[GW][GX];
+  __shared__ T s_cs[GW][GEONL ? G : 1][6];
 
   const WorkItem wi = work[blockIdx.x];
   const BlockDesc bd = M.blocks[wi.blk];
@@ -81,224 +99,332 @@ __global__ void __launch_bounds__(64 * tan_mfma_waves(NN))
   const int tid = threadIdx.x;
   const int wv = tid >> 6;   // wave of this lane
   const int l = tid & 63;    // lane in the wave
-  const int q = l >> 4;      // k row inside a 4-wide k step
-  const int cres = l & 15;   // column residue
-  const double scalar = bd.etype == W3D_TET10 ? 1.0 / 6.0 : 1.0;
-  const double third = 1.0 / 3.0;
-  const double alpha = (2.0 + M.eps_bbar) * third;
-  const double beta_b = (1.0 - M.eps_bbar) * third;
+  const int q = l >> 4;      // point of the lane inside a 4-point group
+  const int cres = l & 15;   // column residue inside a tile
+  const bool tet = bd.etype == W3D_TET10;
+  const T third = (T)(1.0 / 3.0);
+  const T alpha = (T)((2.0 + M.eps_bbar) * (1.0 / 3.0));
+  const T beta_b = (T)((1.0 - M.eps_bbar) * (1.0 / 3.0));
+  (void)third;
   const int64_t SG = (int64_t)bd.span * G;
   const double* cep_b = M.cep + 21 * bd.q_off;
 
   for (int k = tid; k < 3 * G * NN; k += 64 * W) {
-    int d = k / (G * NN), rem = k % (G * NN);
-    s_dn[d][rem / NN][rem % NN] = gt->dn[d][rem / NN][rem % NN];
+    const int d = k / (G * NN), rem = k % (G * NN);
+    const double v = gt->dn[d][rem / NN][rem % NN];
+    s_dn64[d][rem / NN][rem % NN] = v;
+    s_dn[d][rem / NN][rem % NN] = (T)v;
   }
   if (tid < G) s_w[tid] = gt->w[tid];
   __syncthreads();
 
   // element data in flight: coordinates of node l (lanes < NN) and this
-  // lane's share of the stored [D]
-  double pc[3], pd[ND];
+  // lane's share of the stored [D] (word k = l + 64 i is component k / G
+  // of point k % G)
+  double pc[3] = {0, 0, 0}, pd[ND];
   int pnode = 0;
   auto fetch = [&](int e) {
     const int ie = wi.first + e;
     if (l < NN) {
-      int node = M.conn[bd.conn_off + (int64_t)ie * NN + l];
+      const int node = M.conn[bd.conn_off + (int64_t)ie * NN + l];
       pnode = node;
       pc[0] = M.coords[3 * node];
       pc[1] = M.coords[3 * node + 1];
       pc[2] = M.coords[3 * node + 2];
+      if (GEONL) {  // dptstf: X + u + du
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+          pc[k] = pc[k] + M.u_t[3 * node + k] + M.du_t[3 * node + k];
+      }
     }
 #pragma unroll
     for (int i = 0; i < ND; ++i) {
-      int k = l + 64 * i;
-      if (k < 21 * G) pd[i] = cep_b[(k / G) * SG + (int64_t)ie * G + k % G];
-    }
-  };
-  auto stage = [&]() {
-    if (l < NN) {
-      s_node[wv][l] = pnode;
-      s_ce[wv][l] = pc[0];
-      s_ce[wv][NN + l] = pc[1];
-      s_ce[wv][2 * NN + l] = pc[2];
-    }
-#pragma unroll
-    for (int i = 0; i < ND; ++i) {
-      int k = l + 64 * i;
-      if (k < 21 * G) s_d[wv][k % G][k / G] = pd[i];
+      const int k = l + 64 * i;
+      pd[i] = k < 21 * G ? cep_b[(k / G) * SG + (int64_t)ie * G + k % G] : 0.0;
     }
   };
 
   int e = wv;
-  if (e < wi.count) {
-    fetch(e);
-    stage();
-  }
+  if (e < wi.count) fetch(e);
   for (; e < wi.count; e += W) {
     const int ie = wi.first + e;
     wave_sync();
-    // P1: Jacobian at every point (jacob1)
+    if (l < NN) {
+      s_node[wv][l] = pnode;
+      s_ce[wv][0][l] = pc[0];
+      s_ce[wv][1][l] = pc[1];
+      s_ce[wv][2][l] = pc[2];
+    }
+    wave_sync();
+    // P1: node ranks; Jacobian at every point (jacob1), its inverse
+    if (l < NN) {
+      const int mine = s_node[wv][l];
+      int r = 0;
+#pragma unroll
+      for (int b = 0; b < NN; ++b) {
+        const int o = s_node[wv][b];
+        r += (o < mine || (o == mine && b < l)) ? 1 : 0;
+      }
+      s_inv[wv][r] = l;
+    }
     if (l < G) {
       const int g = l;
       double j11 = 0, j12 = 0, j13 = 0, j21 = 0, j22 = 0, j23 = 0, j31 = 0,
              j32 = 0, j33 = 0;
       for (int j = 0; j < NN; ++j) {
-        double x = s_ce[wv][j], y = s_ce[wv][NN + j], z = s_ce[wv][2 * NN + j];
-        double a = s_dn[0][g][j], b = s_dn[1][g][j], c = s_dn[2][g][j];
+        const double x = s_ce[wv][0][j], y = s_ce[wv][1][j],
+                     z = s_ce[wv][2][j];
+        const double a = s_dn64[0][g][j], b = s_dn64[1][g][j],
+                     c = s_dn64[2][g][j];
         j11 = j11 + a * x; j12 = j12 + a * y; j13 = j13 + a * z;
         j21 = j21 + b * x; j22 = j22 + b * y; j23 = j23 + b * z;
         j31 = j31 + c * x; j32 = j32 + c * y; j33 = j33 + c * z;
       }
-      double c1 = j22 * j33 - j23 * j32;
-      double c2 = j21 * j33 - j23 * j31;
-      double c3 = j21 * j32 - j22 * j31;
-      double dj = j11 * c1 - j12 * c2 + j13 * c3;
-      double* ga = s_gam[wv][g];
-      ga[0] = c1 / dj;
-      ga[3] = -c2 / dj;
-      ga[6] = c3 / dj;
-      ga[1] = (j32 * j13 - j12 * j33) / dj;
-      ga[4] = (j11 * j33 - j31 * j13) / dj;
-      ga[7] = (j12 * j31 - j11 * j32) / dj;
-      ga[2] = (j12 * j23 - j13 * j22) / dj;
-      ga[5] = (j13 * j21 - j11 * j23) / dj;
-      ga[8] = (j11 * j22 - j12 * j21) / dj;
-      ga[9] = dj;
+      const double c1 = j22 * j33 - j23 * j32;
+      const double c2 = j21 * j33 - j23 * j31;
+      const double c3 = j21 * j32 - j22 * j31;
+      const double dj = j11 * c1 - j12 * c2 + j13 * c3;
+      T* ga = s_gam[wv][g];
+      ga[0] = (T)(c1 / dj);
+      ga[3] = (T)(-c2 / dj);
+      ga[6] = (T)(c3 / dj);
+      ga[1] = (T)((j32 * j13 - j12 * j33) / dj);
+      ga[4] = (T)((j11 * j33 - j31 * j13) / dj);
+      ga[7] = (T)((j12 * j31 - j11 * j32) / dj);
+      ga[2] = (T)((j12 * j23 - j13 * j22) / dj);
+      ga[5] = (T)((j13 * j21 - j11 * j23) / dj);
+      ga[8] = (T)((j11 * j22 - j12 * j21) / dj);
+      s_detw[wv][g] = dj;
     }
     wave_sync();
     // P2: B-bar volume terms; [D] scaled by w |J| (extract_D_symmetric)
     // and by 1/6 for tetrahedra (adjust_cnst)
-    if (BBAR && l < 8) {
+    if (BBAR && l < NN) {
       const int j = l;
-      double vx = 0, vy = 0, vz = 0, volume = 0;
+      T vx = 0, vy = 0, vz = 0;
+      double volume = 0;
       for (int g = 0; g < G; ++g) {
-        const double* a = s_gam[wv][g];
-        double d = a[9];
-        double nx = s_dn[0][g][j], ny = s_dn[1][g][j], nz = s_dn[2][g][j];
+        const T* a = s_gam[wv][g];
+        const T d = (T)s_detw[wv][g];
+        const T nx = s_dn[0][g][j], ny = s_dn[1][g][j], nz = s_dn[2][g][j];
         vx = vx + (a[0] * nx + a[1] * ny + a[2] * nz) * d;
         vy = vy + (a[3] * nx + a[4] * ny + a[5] * nz) * d;
         vz = vz + (a[6] * nx + a[7] * ny + a[8] * nz) * d;
-        volume = volume + d;
+        volume = volume + s_detw[wv][g];
       }
-      double voli = 1.0 / volume;
+      const T voli = (T)(1.0 / volume);
       s_vol[wv][3 * j] = vx * voli;
       s_vol[wv][3 * j + 1] = vy * voli;
       s_vol[wv][3 * j + 2] = vz * voli;
     }
-    for (int k = l; k < 21 * G; k += 64) {
-      int g = k / 21, c = k % 21;
-      double v = (s_w[g] * s_gam[wv][g][9]) * s_d[wv][g][c];
-      if (bd.etype == W3D_TET10) v = scalar * v;
-      s_d[wv][g][c] = v;
+#pragma unroll
+    for (int i = 0; i < ND; ++i) {
+      const int k = l + 64 * i;
+      if (k < 21 * G) {
+        const int g = k % G;
+        double v = (s_w[g] * s_detw[wv][g]) * pd[i];
+        if (GEONL) {
+          s_gx[wv][21 * g + k / G] = v;
+        } else {
+          if (tet) v = (1.0 / 6.0) * v;
+          s_d[wv][g][k / G] = (T)v;
+        }
+      }
+    }
+    if (GEONL) {
+      // ctran1 (+ Q-bar) with [qn1] of the stored [R n+1] and the Cauchy
+      // stress {T} = [qn1] {t n+1}; then adjust_cnst
+      wave_sync();
+      if (l < G) {
+        const int g = l;
+        const int64_t SGb = (int64_t)bd.span * G;
+        const int64_t qp = (int64_t)ie * G + g;
+        const double* rr = M.rot_n1 + 9 * bd.q_off + qp;
+        const double* sn = M.stress_n1 + 9 * bd.q_off + qp;
+        double r9[9], sg[6], q2[36], cs[6], dm[36];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) r9[3 * a + c] = rr[(a + 3 * c) * SGb];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) sg[k] = sn[k * SGb];
+        geonl_q(r9, 2, q2);
+        geonl_qmul(q2, sg, cs);
+#pragma unroll
+        for (int r = 0; r < 6; ++r)
+#pragma unroll
+          for (int c = 0; c < 6; ++c) dm[6 * r + c] = s_gx[wv][21 * g + tri(r, c)];
+        geonl_ctran(dm, q2, cs, s_detw[wv][g] * s_w[g]);
+#pragma unroll
+        for (int r = 0; r < 6; ++r)
+#pragma unroll
+          for (int c = 0; c <= r; ++c) {
+            double v = dm[6 * r + c];
+            if (tet) v = (1.0 / 6.0) * v;
+            s_d[wv][g][tri(r, c)] = (T)v;
+          }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) s_cs[wv][g][k] = (T)cs[k];
+      }
     }
     // the next element's data travels while this one is contracted
-    const bool more = e + W < wi.count;
-    if (more) fetch(e + W);
+    if (e + W < wi.count) fetch(e + W);
     wave_sync();
-    // P3: the contraction on the matrix cores, two points per chunk
+    // the lane's dof column of every tile: node (local index) and component
+    int jn[NTL], cc[NTL];
+#pragma unroll
+    for (int t = 0; t < NTL; ++t) {
+      const int dp = 16 * t + cres;
+      jn[t] = dp < TD ? s_inv[wv][dp / 3] : -1;
+      cc[t] = dp % 3;
+    }
+    // P3: the contraction on the matrix cores, four points per k step
     v4 acc[NUP];
 #pragma unroll
-    for (int t = 0; t < NUP; ++t) acc[t] = v4{0, 0, 0, 0};
-    for (int ch = 0; ch < NCH; ++ch) {
-      // (a) B and BD columns of (point, dof) pairs into the wave's slab
-      for (int k = l; k < 2 * TP; k += 64) {
-        const int p = k / TP, dof = k % TP, g = 2 * ch + p;
-        double col[6] = {0, 0, 0, 0, 0, 0};
-        if (g < G && dof < TD) {
-          const int c = dof / NN, j = dof % NN;
-          const double* ga = s_gam[wv][g];
-          double nx = s_dn[0][g][j], ny = s_dn[1][g][j], nz = s_dn[2][g][j];
-          double bx = ga[0] * nx + ga[1] * ny + ga[2] * nz;
-          double by = ga[3] * nx + ga[4] * ny + ga[5] * nz;
-          double bz = ga[6] * nx + ga[7] * ny + ga[8] * nz;
-          if (c == 0) {
-            col[0] = bx; col[3] = by; col[5] = bz;
-            if (BBAR) {
-              double vx = s_vol[wv][3 * j];
-              col[0] = alpha * bx + beta_b * vx;
-              col[1] = col[2] = beta_b * (vx - bx);
-            }
-          } else if (c == 1) {
-            col[1] = by; col[3] = bx; col[4] = bz;
-            if (BBAR) {
-              double vy = s_vol[wv][3 * j + 1];
-              col[0] = col[2] = beta_b * (vy - by);
-              col[1] = alpha * by + beta_b * vy;
-            }
-          } else {
-            col[2] = bz; col[4] = by; col[5] = bx;
-            if (BBAR) {
-              double vz = s_vol[wv][3 * j + 2];
-              col[0] = col[1] = beta_b * (vz - bz);
-              col[2] = alpha * bz + beta_b * vz;
-            }
+    for (int u = 0; u < NUP; ++u) acc[u] = v4{0, 0, 0, 0};
+#pragma unroll 1
+    for (int pg = 0; pg < NPG; ++pg) {
+      const int p = 4 * pg + q;
+      const int pp = p < G ? p : G - 1;
+      const T* ga = s_gam[wv][pp];
+      const T g0 = ga[0], g1 = ga[1], g2 = ga[2], g3 = ga[3], g4 = ga[4],
+              g5 = ga[5], g6 = ga[6], g7 = ga[7], g8 = ga[8];
+      T col[NTL][6];
+#pragma unroll
+      for (int t = 0; t < NTL; ++t) {
+        const int j = jn[t] < 0 ? 0 : jn[t];
+        const T nx = s_dn[0][pp][j], ny = s_dn[1][pp][j], nz = s_dn[2][pp][j];
+        const T bx = g0 * nx + g1 * ny + g2 * nz;
+        const T by = g3 * nx + g4 * ny + g5 * nz;
+        const T bz = g6 * nx + g7 * ny + g8 * nz;
+        T c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0, c5 = 0;
+        if (cc[t] == 0) {
+          c0 = bx; c3 = by; c5 = bz;
+          if (BBAR) {
+            const T vx = s_vol[wv][3 * j];
+            c0 = alpha * bx + beta_b * vx;
+            c1 = c2 = beta_b * (vx - bx);
+          }
+        } else if (cc[t] == 1) {
+          c1 = by; c3 = bx; c4 = bz;
+          if (BBAR) {
+            const T vy = s_vol[wv][3 * j + 1];
+            c0 = c2 = beta_b * (vy - by);
+            c1 = alpha * by + beta_b * vy;
+          }
+        } else {
+          c2 = bz; c4 = by; c5 = bx;
+          if (BBAR) {
+            const T vz = s_vol[wv][3 * j + 2];
+            c0 = c1 = beta_b * (vz - bz);
+            c2 = alpha * bz + beta_b * vz;
           }
         }
-        const double* dm = s_d[wv][g < G ? g : 0];
+        const bool live = p < G && jn[t] >= 0;
+        col[t][0] = live ? c0 : (T)0;
+        col[t][1] = live ? c1 : (T)0;
+        col[t][2] = live ? c2 : (T)0;
+        col[t][3] = live ? c3 : (T)0;
+        col[t][4] = live ? c4 : (T)0;
+        col[t][5] = live ? c5 : (T)0;
+      }
+      const T* dm = s_d[wv][pp];
 #pragma unroll
-        for (int r = 0; r < 6; ++r) {
-          // BD(r, dof) = sum_m D(m, r) B(m, dof), bdbtgen order
-          double db = dm[tri(0, r)] * col[0] + dm[tri(1, r)] * col[1] +
-                      dm[tri(2, r)] * col[2] + dm[tri(3, r)] * col[3] +
-                      dm[tri(4, r)] * col[4] + dm[tri(5, r)] * col[5];
-          s_bd[wv][0][6 * p + r][dof] = (T)col[r];
-          s_bd[wv][1][6 * p + r][dof] = (T)db;
+      for (int r = 0; r < 6; ++r) {
+        // BD(r, dof) = sum_m D(m, r) B(m, dof), bdbtgen order
+        const T d0 = dm[tri(0, r)], d1 = dm[tri(1, r)], d2 = dm[tri(2, r)],
+                d3 = dm[tri(3, r)], d4 = dm[tri(4, r)], d5 = dm[tri(5, r)];
+        T bdv[NTL];
+#pragma unroll
+        for (int t = 0; t < NTL; ++t)
+          bdv[t] = d0 * col[t][0] + d1 * col[t][1] + d2 * col[t][2] +
+                   d3 * col[t][3] + d4 * col[t][4] + d5 * col[t][5];
+#pragma unroll
+        for (int u = 0; u < NUP; ++u)
+          acc[u] = MF::mma(bdv[tile_i(u)], col[tile_j(u)][r], acc[u]);
+      }
+    }
+    if (GEONL) {
+      // kgstiff on the matrix cores: one 4-k step per point, k = the
+      // three gradient components (and a zero row); lane (component
+      // q, node 16 t + cres)
+      constexpr int NTG = (NN + 15) / 16;
+      constexpr int NUG = NTG * (NTG + 1) / 2;
+      v4 accg[NUG];
+#pragma unroll
+      for (int u = 0; u < NUG; ++u) accg[u] = v4{0, 0, 0, 0};
+#pragma unroll 1
+      for (int p = 0; p < G; ++p) {
+        const T* ga = s_gam[wv][p];
+        const T* sp = s_cs[wv][p];
+        const T wdj = (T)(s_w[p] * s_detw[wv][p]);
+        T av[NTG], bv[NTG];
+#pragma unroll
+        for (int t = 0; t < NTG; ++t) {
+          const int a = 16 * t + cres;
+          T gx = 0, gy = 0, gz = 0;
+          if (a < NN) {
+            if (BBAR) {
+              gx = s_vol[wv][3 * a];
+              gy = s_vol[wv][3 * a + 1];
+              gz = s_vol[wv][3 * a + 2];
+            } else {
+              const T nx = s_dn[0][p][a], ny = s_dn[1][p][a], nz = s_dn[2][p][a];
+              gx = ga[0] * nx + ga[1] * ny + ga[2] * nz;
+              gy = ga[3] * nx + ga[4] * ny + ga[5] * nz;
+              gz = ga[6] * nx + ga[7] * ny + ga[8] * nz;
+            }
+          }
+          // (S g)_q with S = (xx, yy, zz, xy, yz, xz)
+          const T s0 = q == 0 ? sp[0] : q == 1 ? sp[3] : sp[5];
+          const T s1 = q == 0 ? sp[3] : q == 1 ? sp[1] : sp[4];
+          const T s2 = q == 0 ? sp[5] : q == 1 ? sp[4] : sp[2];
+          const T gq = q == 0 ? gx : q == 1 ? gy : gz;
+          av[t] = q < 3 ? (s0 * gx + s1 * gy + s2 * gz) * wdj : (T)0;
+          bv[t] = q < 3 ? gq : (T)0;
+        }
+#pragma unroll
+        for (int u = 0; u < NUG; ++u)
+          accg[u] = MF::mma(av[tile_i(u)], bv[tile_j(u)], accg[u]);
+      }
+      // the symmetric NN x NN result into the wave's FP64 slab
+#pragma unroll
+      for (int u = 0; u < NUG; ++u) {
+        const int I = tile_i(u), J = tile_j(u);
+        const int b = 16 * J + cres;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int a = 16 * I + MF::row(l, v);
+          if (a < NN && b < NN) {
+            s_gx[wv][a * NN + b] = (double)accg[u][v];
+            s_gx[wv][b * NN + a] = (double)accg[u][v];
+          }
         }
       }
       wave_sync();
-      // (b) three k steps of 4 rows: A from BD (row dofs), B from B (col)
-#pragma unroll
-      for (int ks = 0; ks < 3; ++ks) {
-        const int kr = 4 * ks + q;
-        T av[NTL], bv[NTL];
-#pragma unroll
-        for (int t = 0; t < NTL; ++t) {
-          av[t] = s_bd[wv][1][kr][16 * t + cres];
-          bv[t] = s_bd[wv][0][kr][16 * t + cres];
-        }
-#pragma unroll
-        for (int t = 0; t < NUP; ++t)
-          acc[t] = MF::mma(av[tile_i(t)], bv[tile_j(t)], acc[t]);
-      }
-      wave_sync();
     }
-    // P4: the upper triangle staged packed (column-major) in the wave's
-    // slab, then stored contiguously in the node-pair block order
-    // (npb_pos): word k gathers Ke(p, q) of the nodes of ranks r and s
-    if (l < NN) {
-      int r = 0;
-      const int mine = s_node[wv][l];
+    // P4: the upper triangle straight from the accumulators, ranked
+    // row-major: each store writes four runs of 16 consecutive words
+    T* out = ek_out + bd.ek_off + (int64_t)ie * (TD * (TD + 1) / 2);
+    // the store offsets depend on the lane only; formed here, per element,
+    // rather than held in registers across the element loop
+    int ls = l;
+    asm volatile("" : "+v"(ls));
 #pragma unroll
-      for (int b = 0; b < NN; ++b) r += s_node[wv][b] < mine ? 1 : 0;
-      s_inv[wv][r] = l;
-    }
-    T* stg = &s_bd[wv][0][0][0];
-#pragma unroll
-    for (int t = 0; t < NUP; ++t) {
-      const int I = tile_i(t), J = tile_j(t);
+    for (int u = 0; u < NUP; ++u) {
+      const int I = tile_i(u), J = tile_j(u);
+      const int cl = 16 * J + (ls & 15);
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int row = 16 * I + MF::row(l, v), col = 16 * J + cres;
-        if (row <= col && col < TD) stg[col * (col + 1) / 2 + row] = acc[t][v];
+        const int rw = 16 * I + MF::row(ls, v);
+        if (rw <= cl && cl < TD) {
+          T val = acc[u][v];
+          if (GEONL && rw % 3 == cl % 3)
+            val = val + (T)s_gx[wv][s_inv[wv][rw / 3] * NN + s_inv[wv][cl / 3]];
+          out[ek_word(TD, rw, cl)] = val;
+        }
       }
     }
-    wave_sync();
-    T* out = ek_out + bd.ek_off + (int64_t)ie * (TD * (TD + 1) / 2);
-    const uint16_t* tab = M.npb_tab + npb_tab_off(NN);
-    for (int k = l; k < TD * (TD + 1) / 2; k += 64) {
-      const int code = tab[k];
-      const int a = s_inv[wv][code & 31], b = s_inv[wv][(code >> 5) & 31];
-      int p = ((code >> 10) & 3) * NN + a, q = (code >> 12) * NN + b;
-      if (p > q) {
-        int x = p;
-        p = q;
-        q = x;
-      }
-      out[k] = stg[q * (q + 1) / 2 + p];
-    }
-    wave_sync();
-    if (more) stage();
   }
 }
 
@@ -321,18 +447,25 @@ int launch_tangent_mfma(const DevModel& M, const WorkItem* d_work,
   for (size_t k = 0; k < groups.size(); ++k) {
     if (gcount[k] == 0) continue;
     const WorkItem* w = d_work + gstart[k];
+    const bool geonl = groups[k] & 1;
+    const dim3 grid(gcount[k]);
     bool done = false;
-#define X(NN, G, BB)                                                       \
-  if (!done && groups[k] == group_key(NN, G, BB)) {                         \
-    if (precision == W3D_FP32)                                             \
-      hipLaunchKernelGGL((k_tangent_mfma<NN, G, BB, float>),               \
-                         dim3(gcount[k]), dim3(64 * tan_mfma_waves(NN)), 0, \
-                         s, M, w, M.ek32);                                 \
-    else                                                                   \
-      hipLaunchKernelGGL((k_tangent_mfma<NN, G, BB, double>),              \
-                         dim3(gcount[k]), dim3(64 * tan_mfma_waves(NN)), 0, \
-                         s, M, w, M.ek64);                                 \
-    done = true;                                                           \
+#define X(NN, G, BB)                                                         \
+  if (!done && (groups[k] >> 2) == (group_key(NN, G, BB) >> 2)) {           \
+    const dim3 blk(64 * tan_mfma_waves(NN));                                 \
+    if (precision == W3D_FP32 && geonl)                                      \
+      hipLaunchKernelGGL((k_tangent_mfma<NN, G, BB, float, true>), grid, blk, \
+                         0, s, M, w, M.ek32);                                \
+    else if (precision == W3D_FP32)                                          \
+      hipLaunchKernelGGL((k_tangent_mfma<NN, G, BB, float>), grid, blk, 0, s, \
+                         M, w, M.ek32);                                      \
+    else if (geonl)                                                          \
+      hipLaunchKernelGGL((k_tangent_mfma<NN, G, BB, double, true>), grid,    \
+                         blk, 0, s, M, w, M.ek64);                           \
+    else                                                                     \
+      hipLaunchKernelGGL((k_tangent_mfma<NN, G, BB, double>), grid, blk, 0,  \
+                         s, M, w, M.ek64);                                   \
+    done = true;                                                             \
   }
     W3D_TAN_GROUPS(X)
 #undef X

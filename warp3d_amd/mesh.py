@@ -42,8 +42,10 @@ class Mesh:
         return self.conn.shape[0]
 
     def model(self, material, int_order=None, bbar=None, elblks=None,
-              auto_size=128, eps_bbar=0.0, angles=None, angle_stride=1):
-        """ModelArrays for this mesh with one material for all elements."""
+              auto_size=128, eps_bbar=0.0, angles=None, angle_stride=1,
+              geonl=False):
+        """ModelArrays for this mesh with one material for all elements
+        (geonl: finite strains and rotations for every element)."""
         ne, nn = self.conn.shape
         if int_order is None:
             int_order = {W3D_L3DISOP: 1, W3D_Q3DISOP: 8, W3D_TET10: 4}[
@@ -58,7 +60,8 @@ class Mesh:
             np.full(ne, int_order, np.int32),
             np.ones(ne, np.int32), np.full(ne, bbar, np.int32), mats,
             elblks=elblks, auto_size=auto_size, eps_bbar=eps_bbar,
-            angles=angles, angle_stride=angle_stride)
+            angles=angles, angle_stride=angle_stride,
+            geonl=np.full(ne, 1 if geonl else 0, np.int32))
 
 
 def random_orientations(n, seed=0, per_element=1):
@@ -212,3 +215,65 @@ def uniaxial_bc(mesh, strain, size=1.0):
     top[2::3] = np.abs(x[:, 2] - size) < tol
     cst[top] = 1
     return cst, top, strain * size
+
+
+class MixedMesh:
+    """Meshes of different element types side by side along x with disjoint
+    node sets, elements numbered part after part (blocks are contiguous
+    element ranges of one type).  parts: (mesh, int_order, bbar)."""
+
+    def __init__(self, parts, gap=0.1):
+        coords, incid, incmap, etype, order, bbar = [], [], [], [], [], []
+        noff, x0, pos = 0, 0.0, 1
+        for mesh, int_order, bb in parts:
+            c = mesh.coords.copy()
+            c[:, 0] += x0 - c[:, 0].min()
+            x0 = c[:, 0].max() + gap
+            coords.append(c)
+            ne, nn = mesh.conn.shape
+            incid.append((mesh.conn + noff).ravel())
+            incmap.append(pos + nn * np.arange(ne, dtype=np.int32))
+            pos += ne * nn
+            etype.append(np.full(ne, mesh.etype, np.int32))
+            order.append(np.full(ne, int_order, np.int32))
+            bbar.append(np.full(ne, bb, np.int32))
+            noff += mesh.nonode
+        self.coords = np.concatenate(coords)
+        self.incid = np.concatenate(incid).astype(np.int32)
+        self.incmap = np.concatenate(incmap).astype(np.int32)
+        self.etype = np.concatenate(etype)
+        self.int_order = np.concatenate(order)
+        self.bbar = np.concatenate(bbar)
+        self.parts = parts
+
+    @property
+    def nonode(self):
+        return self.coords.shape[0]
+
+    @property
+    def noelem(self):
+        return self.etype.size
+
+    def seeded_blocks(self, lo=32, hi=256, seed=0):
+        """Blocking table (span, felem, 0, 0) with seeded spans uniform in
+        [lo, hi], each block inside one element type."""
+        rng = np.random.default_rng(seed)
+        rows, e = [], 0
+        while e < self.noelem:
+            end = e + 1
+            while end < self.noelem and self.etype[end] == self.etype[e]:
+                end += 1
+            while e < end:
+                span = min(int(rng.integers(lo, hi + 1)), end - e)
+                rows.append((span, e + 1, 0, 0))
+                e += span
+        return np.array(rows, np.int32)
+
+    def model(self, material, elblks=None, auto_size=128, angles=None,
+              angle_stride=1, geonl=False):
+        mats = material if isinstance(material, (list, tuple)) else [material]
+        return ModelArrays(
+            self.coords, self.incid, self.incmap, self.etype, self.int_order,
+            np.ones(self.noelem, np.int32), self.bbar, mats, elblks=elblks,
+            auto_size=auto_size, angles=angles, angle_stride=angle_stride,
+            geonl=np.full(self.noelem, 1 if geonl else 0, np.int32))

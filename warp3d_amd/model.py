@@ -214,7 +214,7 @@ class BlockModel:
         nq = i["span"] * i["ngp"]
         return {0: 9 * nq, 1: 9 * nq, 2: 6 * nq, 3: 6 * nq, 4: i["hist"] * nq,
                 5: i["hist"] * nq, 6: 21 * nq, 7: i["utsz"] * i["span"],
-                8: i["totdof"] * i["span"], 9: i["span"]}[which]
+                8: i["totdof"] * i["span"], 9: i["span"], 10: 9 * nq}[which]
 
     def get_block_state(self, blk, which):
         n = self.state_size(blk, which)
