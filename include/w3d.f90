@@ -9,7 +9,7 @@ module w3d
   implicit none
   private :: c_int32_t, c_int64_t, c_double, c_float, c_ptr, c_char
 
-  integer(c_int32_t), parameter :: W3D_ABI_VERSION = 2
+  integer(c_int32_t), parameter :: W3D_ABI_VERSION = 3
   ! status codes
   integer(c_int32_t), parameter :: W3D_OK = 0, W3D_CUT_STEP = 1, &
        W3D_ERR_ARG = -1, W3D_ERR_UNSUPPORTED = -2, W3D_ERR_DEVICE = -3, &
@@ -140,6 +140,23 @@ module w3d
       integer(c_int32_t), value :: maxit
       integer(c_int32_t), intent(out) :: iters
       real(c_double), intent(out) :: relres
+      integer(c_int) :: rc
+    end function
+    function w3d_iteration_dev(ctx, step, iter, extrapolated, d_u, d_du, &
+         d_ifv, precision, d_k_diag, d_k_coeffs, energy, plastic_work) &
+         bind(C, name="w3d_iteration_dev") result(rc)
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx, d_u, d_du, d_ifv, d_k_diag, d_k_coeffs
+      integer(c_int32_t), value :: step, iter, extrapolated, precision
+      real(c_double), intent(out) :: energy, plastic_work
+      integer(c_int) :: rc
+    end function
+    function w3d_autotune(ctx, precision, choices, capacity, n_choices) &
+         bind(C, name="w3d_autotune") result(rc)
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: precision, capacity
+      integer(c_int32_t), intent(out) :: choices(*), n_choices
       integer(c_int) :: rc
     end function
     function w3d_commit_step(ctx) bind(C, name="w3d_commit_step") result(rc)

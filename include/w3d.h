@@ -285,6 +285,36 @@ int w3d_set_block_state(w3d_ctx* ctx, int32_t blk, int32_t which,
 int w3d_synchronize(w3d_ctx* ctx);
 
 /*
+ * One Newton iteration of the element-block path on device buffers, as one
+ * HIP graph launch: w3d_recover_dev (all blocks, into d_ifv), w3d_tangent
+ * (all blocks) and w3d_assemble_vss_dev into (d_k_diag, d_k_coeffs), with
+ * the same results.  Replaces the call sequence of a Newton iteration of
+ * mnralg.f (drive_eps_sig_internal_forces :358/:1102, tanstf :367/:439,
+ * assem_by_row via eqn_solve :503).  The graph is captured on first use for
+ * each (step 1 or later, iteration 0 or later, extrapolated, precision,
+ * energy wanted, caller buffers, time increment) and replayed afterwards;
+ * up to 8 are kept.  Needs w3d_pattern first.  Returns like w3d_recover.
+ */
+int w3d_iteration_dev(w3d_ctx* ctx, int32_t step, int32_t iter,
+                      int32_t extrapolated, const double* d_u,
+                      const double* d_du, double* d_ifv, int32_t precision,
+                      double* d_k_diag, double* d_k_coeffs, double* energy,
+                      double* plastic_work);
+
+/*
+ * Autotuner (no reference counterpart: the reference's block size is the
+ * fixed mxvl): times the block tangent of every kernel group with work
+ * items of 8, 16, 32 and 64 elements and keeps the fastest, and the
+ * persistent mm10 kernel with 2, 4 and 8 workgroups per compute unit.  Run
+ * it after a recovery of a representative iteration (mm10 is timed on the
+ * increments in place).  choices receives (group key, elements) pairs, then
+ * (-1, mm10 workgroups) when the model has mm10 blocks; *n_choices their
+ * number of words.  Captured iterations are dropped.
+ */
+int w3d_autotune(w3d_ctx* ctx, int32_t precision, int32_t* choices,
+                 int32_t capacity, int32_t* n_choices);
+
+/*
  * Per-phase device timing, the counterpart of the reference's thyme()
  * accumulators (thyme.f:15): when enabled, every kernel launch of a phase is
  * bracketed by HIP events on the context's stream.  Phases: 0 recovery

@@ -92,3 +92,26 @@ int main(void) {
             assert C.sizeof(cls) == int(val), key
         else:
             assert getattr(cls, field).offset == int(val), key
+
+
+def test_fortran_module_constants_match_the_c_header():
+    """The ISO_C_BINDING module repeats the header's ABI version and the
+    integer constants a Fortran caller passes; both files must agree."""
+    import re
+    with open(os.path.join(ROOT, "include", "w3d.h")) as f:
+        h = f.read()
+    with open(os.path.join(ROOT, "include", "w3d.f90")) as f:
+        fm = f.read()
+    cdefs = dict(re.findall(r"#define\s+(W3D_\w+)\s+(-?\d+)\b", h))
+    cenum = dict(re.findall(r"\b(W3D_\w+)\s*=\s*(-?\d+)\s*[,}]", h))
+    # parameter statements, with their continuation lines joined
+    stmts = re.findall(r"parameter\s*::((?:[^\n]*&\s*\n)*[^\n]*)", fm)
+    fdefs = {}
+    for st in stmts:
+        fdefs.update(re.findall(r"(W3D_\w+)\s*=\s*(-?\d+)", st))
+    assert len(fdefs) >= 20, "constants of w3d.f90 not found"
+    assert fdefs["W3D_ABI_VERSION"] == cdefs["W3D_ABI_VERSION"]
+    cvals = {**cenum, **cdefs}
+    for name, val in fdefs.items():
+        if name in cvals:
+            assert cvals[name] == val, (name, val, cvals[name])

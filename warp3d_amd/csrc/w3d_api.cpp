@@ -112,6 +112,21 @@ struct w3d_ctx {
   int n_work_mm10 = 0;
   int32_t* d_mm10_blks = nullptr;  // indices of the mm10 blocks
   int n_mm10_blks = 0;
+  int32_t* d_counter = nullptr;     // work counter of persistent kernels
+  int mm10_grid = 0;                // workgroups of the persistent mm10
+  int n_cu = 0;                     // compute units of the device
+  // tangent work-item size (elements) per kernel group, autotuned
+  std::vector<std::pair<int, int>> tan_chunk;
+  // captured Newton iterations (w3d_iteration_dev), newest last
+  struct GraphEntry {
+    int key[5];
+    const void* ptr[5];
+    double dt;
+    hipGraphExec_t exec;
+  };
+  std::vector<GraphEntry> graphs;
+  bool capturing = false;
+  int last_step = 1;                // step of the last recovery
   bool has_mm10 = false;
   double dt = 1.0;
   WorkItem *d_work_rec = nullptr, *d_work_tan = nullptr;
@@ -220,9 +235,12 @@ void build_work(const w3d_ctx* c, bool tangent, std::vector<WorkItem>& items,
   std::vector<std::pair<int, WorkItem>> all;
   for (int b = 0; b < c->nelblk; ++b) {
     const BlockDesc& B = c->blocks[b];
-    int E = tangent ? tan_mfma_chunk(B.nnode) : rec_elems(B.ngp);
     int key = group_key(B.nnode, B.ngp, B.bbar,
                         !tangent && B.model == W3D_MM10, B.geonl);
+    int E = tangent ? tan_mfma_chunk(B.nnode) : rec_elems(B.ngp);
+    if (tangent)
+      for (const auto& kc : c->tan_chunk)
+        if (kc.first == key) E = kc.second;
     for (int f = 0; f < B.span; f += E) {
       WorkItem w;
       w.blk = b;
@@ -266,6 +284,9 @@ void free_ctx(w3d_ctx* c) {
   dfree(c->d_csr_val); dfree(c->d_pcg);
   dfree(c->d_crys); dfree(c->d_uddt); dfree(c->d_work_mm10); dfree(c->d_mm10_blks);
   dfree(c->d_rot_n); dfree(c->d_rot_n1); dfree(c->d_u_t); dfree(c->d_du_t);
+  dfree(c->d_counter);
+  for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+  c->graphs.clear();
   for (int ph = 0; ph < W3D_NUM_PHASES; ++ph) {
     for (int k = 0; k < 2; ++k) {
       if (c->ev_a[ph][k]) hipEventDestroy(c->ev_a[ph][k]);
@@ -591,6 +612,12 @@ int create_impl(w3d_ctx* c, const w3d_model* m) {
   HIPCHK(hipMemset(c->d_ek32, 0, std::max<int64_t>(ek, 1) * sizeof(float)));
   if ((rc = dalloc(&c->d_flags, 4))) return rc;
   HIPCHK(hipMemset(c->d_flags, 0, 4 * sizeof(int32_t)));
+  if ((rc = dalloc(&c->d_counter, 4))) return rc;
+  HIPCHK(hipMemset(c->d_counter, 0, 4 * sizeof(int32_t)));
+  HIPCHK(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount,
+                               c->device));
+  // persistent mm10: one wave per SIMD fits (registers), four per CU
+  c->mm10_grid = 4 * std::max(c->n_cu, 1);
   {
     std::vector<WorkItem> items;
     build_work(c, false, items, c->rec_groups, c->rec_start, c->rec_count);
@@ -624,11 +651,11 @@ int create_impl(w3d_ctx* c, const w3d_model* m) {
 }
 
 void prof_begin(w3d_ctx* c, int ph) {
-  if (!c->profiling || c->nseg[ph] >= 2) return;
+  if (!c->profiling || c->capturing || c->nseg[ph] >= 2) return;
   hipEventRecord(c->ev_a[ph][c->nseg[ph]], c->stream);
 }
 void prof_end(w3d_ctx* c, int ph) {
-  if (!c->profiling || c->nseg[ph] >= 2) return;
+  if (!c->profiling || c->capturing || c->nseg[ph] >= 2) return;
   hipEventRecord(c->ev_b[ph][c->nseg[ph]], c->stream);
   c->nseg[ph] += 1;
 }
@@ -668,11 +695,68 @@ int check_flags(w3d_ctx* c, int step) {
   return W3D_OK;
 }
 
-int recover_impl(w3d_ctx* c, int step, int iter, int extrapolated,
-                 const double* d_u, const double* d_du, double* d_ifv,
-                 double* energy, double* plastic_work, int first_blk,
-                 int nblk) {
-  HIPCHK(hipSetDevice(c->device));
+// Work lists of one recovery: every block, or the blocks
+// [first_blk, first_blk + nblk) (device copies owned by the caller).
+struct RecoverLists {
+  const WorkItem* d_w = nullptr;
+  const WorkItem* d_wm = nullptr;
+  int nwm = 0;
+  std::vector<int> sg, ss, sn;
+  WorkItem *d_sel = nullptr, *d_selm = nullptr;
+};
+
+int recover_lists(w3d_ctx* c, int first_blk, int nblk, RecoverLists& L) {
+  L.d_w = c->d_work_rec;
+  L.d_wm = c->d_work_mm10;
+  L.nwm = c->n_work_mm10;
+  L.sg = c->rec_groups;
+  L.ss = c->rec_start;
+  L.sn = c->rec_count;
+  if (first_blk == 1 && nblk == c->nelblk) return W3D_OK;
+  auto in = [&](const WorkItem& w) {
+    return w.blk >= first_blk - 1 && w.blk < first_blk - 1 + nblk;
+  };
+  std::vector<WorkItem> items;
+  std::vector<int> g, s, n;
+  build_work(c, false, items, g, s, n);
+  std::vector<WorkItem> sel;
+  L.sg.clear();
+  L.ss.clear();
+  L.sn.clear();
+  for (size_t k = 0; k < g.size(); ++k) {
+    int st = (int)sel.size();
+    for (int i = s[k]; i < s[k] + n[k]; ++i)
+      if (in(items[i])) sel.push_back(items[i]);
+    if ((int)sel.size() > st) {
+      L.sg.push_back(g[k]);
+      L.ss.push_back(st);
+      L.sn.push_back((int)sel.size() - st);
+    }
+  }
+  int rc;
+  if ((rc = upload(&L.d_sel, sel))) return rc;
+  L.d_w = L.d_sel;
+  if (c->has_mm10) {
+    std::vector<WorkItem> all(c->n_work_mm10), selm;
+    HIPCHK(hipMemcpy(all.data(), c->d_work_mm10, all.size() * sizeof(WorkItem),
+                     hipMemcpyDeviceToHost));
+    for (const WorkItem& w : all)
+      if (in(w)) selm.push_back(w);
+    if ((rc = upload(&L.d_selm, selm))) return rc;
+    L.d_wm = L.d_selm;
+    L.nwm = (int)selm.size();
+  }
+  return W3D_OK;
+}
+
+// Enqueue one recovery on the context's stream without host
+// synchronisation or memsets (so that it can be captured in a graph): the
+// flags cleared by a kernel, the recovery passes, mm10 with grad Fe, the
+// structure-force gather into d_ifv (if given) and the per-block energy
+// sums (if sums).
+int enqueue_recover(w3d_ctx* c, const RecoverLists& L, int step, int iter,
+                    int extrapolated, const double* d_u, const double* d_du,
+                    double* d_ifv, bool sums, int first_blk, int nblk) {
   if (c->has_geonl) {
     // the finite-strain update needs u; the tangent later uses X + u + du
     if (!d_u)
@@ -683,79 +767,34 @@ int recover_impl(w3d_ctx* c, int step, int iter, int extrapolated,
     HIPCHK(hipMemcpyAsync(c->d_du_t, d_du, nb, hipMemcpyDeviceToDevice,
                           c->stream));
   }
-  HIPCHK(hipMemsetAsync(c->d_flags, 0, 4 * sizeof(int32_t), c->stream));
+  int rc = launch_fill_i32(c->d_flags, 4, 0, c->stream);
+  if (rc) return fail(rc, "recover: flag reset launch failed");
+  c->last_step = step;
   const int nonlinear = (iter >= 1 || extrapolated) ? 1 : 0;
   const int want_energy = (iter > 0) ? 1 : 0;
   DevModel M = c->dev();
   // mm10's iteration-0 linear-elastic update (rstgp1.f:4453-4457)
   const int iter0_off = (iter == 0 && (!extrapolated || step == 1)) ? 1 : 0;
-  int rc;
-  const WorkItem* d_w = c->d_work_rec;
-  const WorkItem* d_wm = c->d_work_mm10;
-  int nwm = c->n_work_mm10;
-  std::vector<int> sg = c->rec_groups, ss = c->rec_start, sn = c->rec_count;
-  WorkItem *d_sel = nullptr, *d_selm = nullptr;
-  if (!(first_blk == 1 && nblk == c->nelblk)) {
-    // restrict the work lists to the requested blocks
-    auto in = [&](const WorkItem& w) {
-      return w.blk >= first_blk - 1 && w.blk < first_blk - 1 + nblk;
-    };
-    std::vector<WorkItem> items;
-    std::vector<int> g, s, n;
-    build_work(c, false, items, g, s, n);
-    std::vector<WorkItem> sel;
-    sg.clear();
-    ss.clear();
-    sn.clear();
-    for (size_t k = 0; k < g.size(); ++k) {
-      int st = (int)sel.size();
-      for (int i = s[k]; i < s[k] + n[k]; ++i)
-        if (in(items[i])) sel.push_back(items[i]);
-      if ((int)sel.size() > st) {
-        sg.push_back(g[k]);
-        ss.push_back(st);
-        sn.push_back((int)sel.size() - st);
-      }
-    }
-    if ((rc = upload(&d_sel, sel))) return rc;
-    d_w = d_sel;
-    if (c->has_mm10) {
-      std::vector<WorkItem> all(c->n_work_mm10), selm;
-      HIPCHK(hipMemcpy(all.data(), c->d_work_mm10,
-                       all.size() * sizeof(WorkItem), hipMemcpyDeviceToHost));
-      for (const WorkItem& w : all)
-        if (in(w)) selm.push_back(w);
-      if ((rc = upload(&d_selm, selm))) {
-        dfree(d_sel);
-        return rc;
-      }
-      d_wm = d_selm;
-      nwm = (int)selm.size();
-    }
-  }
   prof_begin(c, W3D_PHASE_RECOVER);
-  rc = launch_recover(M, c->blocks, d_w, sg, ss, sn, step, iter, nonlinear,
-                      d_u, d_du, want_energy, c->d_en_wg, 0, c->stream);
+  rc = launch_recover(M, c->blocks, L.d_w, L.sg, L.ss, L.sn, step, iter,
+                      nonlinear, d_u, d_du, want_energy, c->d_en_wg, 0,
+                      c->stream);
   prof_end(c, W3D_PHASE_RECOVER);
-  if (!rc && nwm > 0) {
+  if (!rc && L.nwm > 0) {
     prof_begin(c, W3D_PHASE_MM10);
-    rc = launch_mm10(M, d_wm, nwm, step, iter0_off, c->stream);
+    rc = launch_mm10(M, L.d_wm, L.nwm, c->d_counter, c->mm10_grid, step,
+                     iter0_off, c->stream);
     if (!rc)
       rc = launch_mm10_grads(M, c->d_mm10_blks, c->n_mm10_blks, first_blk - 1,
                              first_blk - 1 + nblk, step, c->stream);
     prof_end(c, W3D_PHASE_MM10);
     if (!rc) {
       prof_begin(c, W3D_PHASE_RECOVER);
-      rc = launch_recover(M, c->blocks, d_w, sg, ss, sn, step, iter,
+      rc = launch_recover(M, c->blocks, L.d_w, L.sg, L.ss, L.sn, step, iter,
                           nonlinear, d_u, d_du, want_energy, c->d_en_wg, 1,
                           c->stream);
       prof_end(c, W3D_PHASE_RECOVER);
     }
-  }
-  if (d_sel || d_selm) {
-    hipStreamSynchronize(c->stream);
-    dfree(d_sel);
-    dfree(d_selm);
   }
   if (rc) return fail(rc, "recover: kernel launch failed");
   if (d_ifv) {
@@ -765,13 +804,22 @@ int recover_impl(w3d_ctx* c, int step, int iter, int extrapolated,
     prof_end(c, W3D_PHASE_IFV);
     if (rc) return fail(rc, "recover: ifv gather launch failed");
   }
-  double en = 0.0, pw = 0.0;
-  if ((energy || plastic_work) && want_energy) {
+  if (sums && want_energy) {
     prof_begin(c, W3D_PHASE_ENERGY);
     rc = launch_energy_reduce(c->d_en_wg, c->d_blk_wfirst, c->d_blk_wcount,
                               c->nelblk, c->d_blk_sums, c->stream);
     prof_end(c, W3D_PHASE_ENERGY);
     if (rc) return fail(rc, "recover: energy reduction launch failed");
+  }
+  return W3D_OK;
+}
+
+// After the stream: energy and plastic work from the per-block sums (block
+// order), then the flags.
+int finish_recover(w3d_ctx* c, int step, int iter, double* energy,
+                   double* plastic_work) {
+  double en = 0.0, pw = 0.0;
+  if ((energy || plastic_work) && iter > 0) {
     std::vector<double> sums(2 * c->nelblk);
     HIPCHK(hipMemcpyAsync(sums.data(), c->d_blk_sums,
                           sums.size() * sizeof(double),
@@ -785,6 +833,25 @@ int recover_impl(w3d_ctx* c, int step, int iter, int extrapolated,
   if (energy) *energy = en;
   if (plastic_work) *plastic_work = pw;
   return check_flags(c, step);
+}
+
+int recover_impl(w3d_ctx* c, int step, int iter, int extrapolated,
+                 const double* d_u, const double* d_du, double* d_ifv,
+                 double* energy, double* plastic_work, int first_blk,
+                 int nblk) {
+  HIPCHK(hipSetDevice(c->device));
+  RecoverLists L;
+  int rc = recover_lists(c, first_blk, nblk, L);
+  if (!rc)
+    rc = enqueue_recover(c, L, step, iter, extrapolated, d_u, d_du, d_ifv,
+                         energy || plastic_work, first_blk, nblk);
+  if (L.d_sel || L.d_selm) {
+    hipStreamSynchronize(c->stream);
+    dfree(L.d_sel);
+    dfree(L.d_selm);
+  }
+  if (rc) return rc;
+  return finish_recover(c, step, iter, energy, plastic_work);
 }
 
 // Jacobi-PCG on an assembled matrix (diag, coef device arrays); b and x
@@ -1627,6 +1694,187 @@ int w3d_synchronize(w3d_ctx* c) {
   if (!c) return fail(W3D_ERR_ARG, "null context");
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipStreamSynchronize(c->stream));
+  return W3D_OK;
+}
+
+int w3d_iteration_dev(w3d_ctx* c, int32_t step, int32_t iter,
+                      int32_t extrapolated, const double* d_u,
+                      const double* d_du, double* d_ifv, int32_t precision,
+                      double* d_k_diag, double* d_k_coeffs, double* energy,
+                      double* plastic_work) {
+  if (!c || !d_du || !d_ifv)
+    return fail(W3D_ERR_ARG, "w3d_iteration_dev: null argument");
+  if (step < 1 || iter < 0)
+    return fail(W3D_ERR_ARG, "w3d_iteration_dev: step/iter");
+  if (precision != W3D_FP64 && precision != W3D_FP32)
+    return fail(W3D_ERR_ARG, "w3d_iteration_dev: precision");
+  if (!c->d_asm_hdr) return fail(W3D_ERR_ARG, "w3d_iteration_dev: no pattern");
+  if (!d_k_diag || (!d_k_coeffs && c->ncoeff))
+    return fail(W3D_ERR_ARG, "w3d_iteration_dev: null matrix output");
+  HIPCHK(hipSetDevice(c->device));
+  // the kernels see the step and the iteration only through these
+  // predicates: one graph per (step 1?, iteration 0?, extrapolated,
+  // precision, energy sums) and per set of caller buffers and dt
+  const int step_c = step == 1 ? 1 : 2, iter_c = iter == 0 ? 0 : 1;
+  const bool sums = energy || plastic_work;
+  const int key[5] = {step_c, iter_c, extrapolated ? 1 : 0, precision,
+                      sums ? 1 : 0};
+  const void* ptr[5] = {d_u, d_du, d_ifv, d_k_diag, d_k_coeffs};
+  w3d_ctx::GraphEntry* ge = nullptr;
+  for (auto& g : c->graphs)
+    if (!std::memcmp(g.key, key, sizeof key) &&
+        !std::memcmp(g.ptr, ptr, sizeof ptr) && g.dt == c->dt)
+      ge = &g;
+  if (!ge) {
+    if (c->graphs.size() >= 8) {
+      (void)hipGraphExecDestroy(c->graphs.front().exec);
+      c->graphs.erase(c->graphs.begin());
+    }
+    RecoverLists L;
+    int rc = recover_lists(c, 1, c->nelblk, L);  // the resident lists
+    if (rc) return rc;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    rc = enqueue_recover(c, L, step_c, iter_c, extrapolated ? 1 : 0, d_u, d_du,
+                         d_ifv, sums, 1, c->nelblk);
+    if (!rc)
+      rc = launch_tangent_mfma(c->dev(), c->d_work_tan, c->tan_groups,
+                               c->tan_start, c->tan_count, precision,
+                               c->stream);
+    if (!rc)
+      rc = launch_assemble(c->d_ek64, c->d_ek32, precision == W3D_FP32,
+                           c->d_asm_hdr, c->d_asm_src, c->d_asm_wbase,
+                           c->asm_nseg, c->neqns, d_k_diag, d_k_coeffs,
+                           c->stream);
+    c->capturing = false;
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (rc || e != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc ? rc : fail(W3D_ERR_DEVICE, "w3d_iteration_dev: capture");
+    }
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess)
+      return fail(W3D_ERR_DEVICE, std::string("hipGraphInstantiate: ") +
+                                      hipGetErrorString(e));
+    w3d_ctx::GraphEntry g;
+    std::memcpy(g.key, key, sizeof key);
+    std::memcpy(g.ptr, ptr, sizeof ptr);
+    g.dt = c->dt;
+    g.exec = exec;
+    c->graphs.push_back(g);
+    ge = &c->graphs.back();
+  }
+  HIPCHK(hipGraphLaunch(ge->exec, c->stream));
+  c->tangent_precision = precision;
+  c->assembled += 1;
+  return finish_recover(c, step, iter, energy, plastic_work);
+}
+
+int w3d_autotune(w3d_ctx* c, int32_t precision, int32_t* choices,
+                 int32_t capacity, int32_t* n_choices) {
+  if (!c) return fail(W3D_ERR_ARG, "w3d_autotune: null context");
+  if (precision != W3D_FP64 && precision != W3D_FP32)
+    return fail(W3D_ERR_ARG, "w3d_autotune: precision");
+  HIPCHK(hipSetDevice(c->device));
+  hipEvent_t ea, eb;
+  HIPCHK(hipEventCreate(&ea));
+  HIPCHK(hipEventCreate(&eb));
+  DevModel M = c->dev();
+  std::vector<int32_t> out;
+  int rc = W3D_OK;
+  // tangent: per kernel group the work-item size, in elements, with the
+  // shortest launch (the mean of three, after one untimed launch)
+  const std::vector<int> groups = c->tan_groups;
+  for (int key : groups) {
+    int best = 0;
+    float best_ms = 0.0f;
+    for (int chunk : {8, 16, 32, 64}) {
+      bool found = false;
+      for (auto& kc : c->tan_chunk)
+        if (kc.first == key) {
+          kc.second = chunk;
+          found = true;
+        }
+      if (!found) c->tan_chunk.push_back({key, chunk});
+      std::vector<WorkItem> items;
+      std::vector<int> g, st, cnt;
+      build_work(c, true, items, g, st, cnt);
+      size_t k = 0;
+      while (k < g.size() && g[k] != key) ++k;
+      if (k == g.size()) continue;
+      std::vector<WorkItem> mine(items.begin() + st[k],
+                                 items.begin() + st[k] + cnt[k]);
+      WorkItem* d = nullptr;
+      if ((rc = upload(&d, mine))) break;
+      const std::vector<int> gg{key}, ss{0}, nn{cnt[k]};
+      rc = launch_tangent_mfma(M, d, gg, ss, nn, precision, c->stream);
+      (void)hipEventRecord(ea, c->stream);
+      for (int r = 0; r < 3 && !rc; ++r)
+        rc = launch_tangent_mfma(M, d, gg, ss, nn, precision, c->stream);
+      (void)hipEventRecord(eb, c->stream);
+      (void)hipStreamSynchronize(c->stream);
+      dfree(d);
+      if (rc) break;
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, ea, eb);
+      if (!best || ms < best_ms) {
+        best = chunk;
+        best_ms = ms;
+      }
+    }
+    if (rc) break;
+    for (auto& kc : c->tan_chunk)
+      if (kc.first == key) kc.second = best;
+    out.push_back(key);
+    out.push_back(best);
+  }
+  // persistent mm10: workgroups per compute unit (after a recovery the
+  // increments are in place; the kernel rewrites the same outputs from the
+  // same inputs)
+  if (!rc && c->has_mm10 && c->n_work_mm10 > 0) {
+    const int iter0_off = 0;
+    int best = c->mm10_grid;
+    float best_ms = 0.0f;
+    for (int per_cu : {2, 4, 8}) {
+      const int grid = per_cu * std::max(c->n_cu, 1);
+      rc = launch_mm10(M, c->d_work_mm10, c->n_work_mm10, c->d_counter, grid,
+                       c->last_step, iter0_off, c->stream);
+      (void)hipEventRecord(ea, c->stream);
+      for (int r = 0; r < 2 && !rc; ++r)
+        rc = launch_mm10(M, c->d_work_mm10, c->n_work_mm10, c->d_counter,
+                         grid, c->last_step, iter0_off, c->stream);
+      (void)hipEventRecord(eb, c->stream);
+      (void)hipStreamSynchronize(c->stream);
+      if (rc) break;
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, ea, eb);
+      if (per_cu == 2 || ms < best_ms) {
+        best = grid;
+        best_ms = ms;
+      }
+    }
+    c->mm10_grid = best;
+    out.push_back(-1);
+    out.push_back(best);
+  }
+  (void)hipEventDestroy(ea);
+  (void)hipEventDestroy(eb);
+  if (rc) return fail(rc, "w3d_autotune: launch failed");
+  // the tangent work list with the chosen sizes; captured iterations are
+  // stale
+  std::vector<WorkItem> items;
+  build_work(c, true, items, c->tan_groups, c->tan_start, c->tan_count);
+  dfree(c->d_work_tan);
+  if ((rc = upload(&c->d_work_tan, items))) return rc;
+  for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+  c->graphs.clear();
+  if (choices)
+    for (int32_t k = 0; k < capacity && k < (int32_t)out.size(); ++k)
+      choices[k] = out[k];
+  if (n_choices) *n_choices = (int32_t)out.size();
   return W3D_OK;
 }
 

@@ -135,9 +135,14 @@ int launch_recover(const DevModel& M, const std::vector<BlockDesc>& hblocks,
                    const std::vector<int>& group_count, int step, int iter,
                    int nonlinear, const double* d_u, const double* d_du,
                    int want_energy, double* en_wg, int pass, hipStream_t s);
-// mm10 update, one thread per point over work items of <= 64 points.
+// mm10 update, one thread per point over work items of <= 64 points,
+// persistent: grid workgroups take items from the counter d_counter.
 int launch_mm10(const DevModel& M, const WorkItem* d_work, int nwork,
-                int step, int iter0_off, hipStream_t s);
+                int* d_counter, int grid, int step, int iter0_off,
+                hipStream_t s);
+// n int32 words of d set to v (a kernel, so that it may be captured in a
+// graph in place of a memset)
+int launch_fill_i32(int32_t* d, int n, int32_t v, hipStream_t s);
 // grad Fe words of the small-strain mm10 blocks listed in d_blks whose
 // index lies in [lo, hi) (rknstr_finish_cp), one workgroup per block.
 int launch_mm10_grads(const DevModel& M, const int32_t* d_blks, int nblks,

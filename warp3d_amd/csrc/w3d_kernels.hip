@@ -776,6 +776,18 @@ int launch_recover(const DevModel& M, const std::vector<BlockDesc>& hblocks,
   return hipGetLastError() == hipSuccess ? W3D_OK : W3D_ERR_DEVICE;
 }
 
+__global__ void k_fill_i32(int32_t* __restrict__ d, int n, int32_t v) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) d[k] = v;
+}
+
+int launch_fill_i32(int32_t* d, int n, int32_t v, hipStream_t s) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_fill_i32, dim3((n + 255) / 256), dim3(256), 0, s, d,
+                       n, v);
+  return hipGetLastError() == hipSuccess ? W3D_OK : W3D_ERR_DEVICE;
+}
+
 int launch_ifv_gather(const double* einfvec, const int64_t* inv_ptr,
                       const int64_t* inv_off, int nonode, double* ifv,
                       hipStream_t s) {

@@ -688,11 +688,9 @@ __device__ inline void euler_angles(const Props& p, const double* Rp,
 // ---------------------------------------------------------------------
 // One thread per Gauss point of an mm10 block (all its crystals).
 // ---------------------------------------------------------------------
-__global__ void __launch_bounds__(64)
-    k_mm10(DevModel M, const WorkItem* __restrict__ work, int step,
-           int iter0_off) {
-  __shared__ double s_geo[108 * kLanes];
-  const WorkItem wi = work[blockIdx.x];
+__device__ __forceinline__ void mm10_item(const DevModel& M,
+                                          const WorkItem wi, int step,
+                                          int iter0_off, double* s_geo) {
   if ((int)threadIdx.x >= wi.count) return;
   const BlockDesc bd = M.blocks[wi.blk];
   const MatDesc md = M.mats[bd.mat];
@@ -1182,11 +1180,34 @@ int launch_mm10_grads(const DevModel& M, const int32_t* d_blks, int nblks,
   return hipGetLastError() == hipSuccess ? W3D_OK : W3D_ERR_DEVICE;
 }
 
+// Persistent over the work list: each workgroup (one wave) takes the next
+// item of 64 points from a device counter until the list is exhausted, so
+// points whose local solves sub-step or iterate longer do not hold back a
+// fixed share of the grid.  The counter starts at zero (launch_mm10 clears
+// it with a fill kernel first).
+__global__ void __launch_bounds__(64)
+    k_mm10(DevModel M, const WorkItem* __restrict__ work, int nwork,
+           int* __restrict__ counter, int step, int iter0_off) {
+  __shared__ double s_geo[108 * kLanes];
+  __shared__ int s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int item = s_item;
+    __syncthreads();
+    if (item >= nwork) break;  // uniform across the workgroup
+    mm10_item(M, work[item], step, iter0_off, s_geo);
+  }
+}
+
 int launch_mm10(const DevModel& M, const WorkItem* d_work, int nwork,
-                int step, int iter0_off, hipStream_t s) {
-  if (nwork > 0)
-    hipLaunchKernelGGL(k_mm10, dim3(nwork), dim3(64), 0, s, M, d_work, step,
-                       iter0_off);
+                int* d_counter, int grid, int step, int iter0_off,
+                hipStream_t s) {
+  if (nwork <= 0) return W3D_OK;
+  int rc = launch_fill_i32(d_counter, 1, 0, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_mm10, dim3(grid < nwork ? grid : nwork), dim3(64), 0,
+                     s, M, d_work, nwork, d_counter, step, iter0_off);
   return hipGetLastError() == hipSuccess ? W3D_OK : W3D_ERR_DEVICE;
 }
 

@@ -63,6 +63,16 @@ def lib():
         L.w3d_reset_phase_times.restype = C.c_int32
         L.w3d_set_time_increment.argtypes = [C.c_void_p, C.c_double]
         L.w3d_set_time_increment.restype = C.c_int32
+        L.w3d_iteration_dev.argtypes = [C.c_void_p, C.c_int32, C.c_int32,
+                                        C.c_int32, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_int32, C.c_void_p,
+                                        C.c_void_p, C.POINTER(C.c_double),
+                                        C.POINTER(C.c_double)]
+        L.w3d_iteration_dev.restype = C.c_int32
+        L.w3d_autotune.argtypes = [C.c_void_p, C.c_int32,
+                                   C.POINTER(C.c_int32), C.c_int32,
+                                   C.POINTER(C.c_int32)]
+        L.w3d_autotune.restype = C.c_int32
         _LIB = L
     return _LIB
 
@@ -177,6 +187,29 @@ class BlockModel:
     def assemble_vss_dev(self, d_kdiag, d_kcoeffs):
         return _chk(lib().w3d_assemble_vss_dev(self.h, d_kdiag, d_kcoeffs),
                     "w3d_assemble_vss_dev")
+
+    def iteration_dev(self, step, iter_, d_u, d_du, d_ifv, precision,
+                      d_kdiag, d_kcoeffs, extrapolated=False):
+        """One graph-launched Newton iteration on device buffers
+        (recovery, tangent and assembly); returns (energy, plastic work,
+        status)."""
+        en = C.c_double()
+        pw = C.c_double()
+        rc = _chk(lib().w3d_iteration_dev(
+            self.h, step, iter_, int(extrapolated), d_u, d_du, d_ifv,
+            precision, d_kdiag, d_kcoeffs, C.byref(en), C.byref(pw)),
+            "w3d_iteration_dev")
+        return en.value, pw.value, rc
+
+    def autotune(self, precision=0):
+        """Runs the autotuner; returns {group key: elements per work item}
+        and, with mm10 blocks, {-1: mm10 workgroups}."""
+        buf = (C.c_int32 * 64)()
+        n = C.c_int32()
+        _chk(lib().w3d_autotune(self.h, precision, buf, 64, C.byref(n)),
+             "w3d_autotune")
+        v = list(buf)[:n.value]
+        return dict(zip(v[0::2], v[1::2]))
 
     # -- state -----------------------------------------------------------
     def commit_step(self):
