@@ -336,6 +336,75 @@ int w3d_get_phase_time(const w3d_ctx* ctx, int32_t phase, int64_t* launches,
                        double* total_ms);
 int w3d_reset_phase_times(w3d_ctx* ctx);
 
+/*
+ * Batched serving of independent problems (no reference counterpart; the
+ * reference solves one model per process).  A problem is a quasi-static
+ * solve on one context: nsteps load steps, each imposing the increments
+ * du0 gives on the constrained dofs of cstmap (nonzero = constrained), with
+ * Newton iterations of recovery, block tangent, assembly and a Jacobi-PCG
+ * solve of the free equations until the residual (-ifv on the free dofs)
+ * satisfies |R| <= tol * max(|F_c|, tol_abs), F_c the internal forces on
+ * the constrained dofs.  Step 1 starts from du0 (its free entries are the
+ * initial guess), every later step from the previous step's converged
+ * increment (extrapolation).  Mesh, state, displacements and solver
+ * vectors stay on the device.
+ *
+ * w3d_batch_advance runs one Newton iteration of every running problem of
+ * a batch: each kernel of the iteration is launched once for the whole
+ * batch where its work spans problems (the mm10 update, the residual, the
+ * Jacobi-PCG iterations, the displacement update) and per problem for the
+ * rest, on the batch's stream.  A problem's arithmetic does not depend on
+ * the other problems of the batch: it gives bitwise the same results alone
+ * and in any mix.  Problems may join or leave between calls.  The contexts
+ * of the problems must not be used by other calls during an advance.
+ */
+typedef struct w3d_problem w3d_problem;
+typedef struct w3d_batch w3d_batch;
+
+typedef struct w3d_newton_opts {
+  int32_t nsteps;     /* load steps                                     */
+  int32_t max_iters;  /* Newton iterations per step before failing      */
+  int32_t precision;  /* block tangent: W3D_FP64 or W3D_FP32            */
+  int32_t pcg_maxit;  /* Jacobi-PCG iterations per solve                */
+  double tol;         /* relative residual of convergence               */
+  double tol_abs;     /* floor of the reaction norm in the test         */
+  double pcg_rtol;    /* |r| <= pcg_rtol |b| ends a linear solve        */
+} w3d_newton_opts;
+
+enum {
+  W3D_PROBLEM_RUNNING = 0,
+  W3D_PROBLEM_STEP_DONE = 1, /* a step converged in the last advance    */
+  W3D_PROBLEM_DONE = 2,      /* every step converged                    */
+  W3D_PROBLEM_FAILED = 3     /* iteration limit or a material failure   */
+};
+
+/* Sets the context's pattern from cstmap and allocates the problem's
+ * device vectors; du0 has 3*nonode words. */
+int w3d_problem_create(w3d_ctx* ctx, const int32_t* cstmap, const double* du0,
+                       const w3d_newton_opts* opts, w3d_problem** out);
+int w3d_problem_destroy(w3d_problem* p);
+/* status (W3D_PROBLEM_*), current step (1-based) and Newton iteration */
+int w3d_problem_status(const w3d_problem* p, int32_t* status, int32_t* step,
+                       int32_t* iter);
+/* Record of converged step (1-based): Newton iterations, Jacobi-PCG
+ * iterations summed over them, the sum of the internal forces on the
+ * loaded dofs (constrained dofs with a nonzero increment), and the last
+ * residual and reaction norms. */
+int w3d_problem_step_record(const w3d_problem* p, int32_t step,
+                            int32_t* newton_iters, int32_t* pcg_iters,
+                            double* load, double* residual, double* reaction);
+/* displacements at the end of the last converged step (3*nonode) */
+int w3d_problem_get_u(const w3d_problem* p, double* u);
+/* device bytes the problem's context and vectors hold */
+int w3d_problem_device_bytes(const w3d_problem* p, int64_t* bytes);
+
+int w3d_batch_create(int32_t device, w3d_batch** out);
+int w3d_batch_destroy(w3d_batch* b);
+/* One Newton iteration of each of the n problems that are still running;
+ * status[k] receives problem k's status after it. */
+int w3d_batch_advance(w3d_batch* b, w3d_problem* const* problems, int32_t n,
+                      int32_t* status);
+
 #ifdef __cplusplus
 }
 #endif

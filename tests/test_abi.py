@@ -72,6 +72,10 @@ int main(void) {
   P(w3d_model, nmat); P(w3d_model, mats); P(w3d_model, nelblk);
   P(w3d_model, elblks); P(w3d_model, auto_size); P(w3d_model, eps_bbar);
   printf("w3d_model.size %zu\n", sizeof(w3d_model));
+  P(w3d_newton_opts, nsteps); P(w3d_newton_opts, precision);
+  P(w3d_newton_opts, pcg_maxit); P(w3d_newton_opts, tol);
+  P(w3d_newton_opts, tol_abs); P(w3d_newton_opts, pcg_rtol);
+  printf("w3d_newton_opts.size %zu\n", sizeof(w3d_newton_opts));
   return 0;
 }
 '''
@@ -87,7 +91,8 @@ int main(void) {
         key, val = line.split()
         struct, field = key.split(".")
         cls = {"w3d_material": _abi.W3DMaterial,
-               "w3d_model": _abi.W3DModel}[struct]
+               "w3d_model": _abi.W3DModel,
+               "w3d_newton_opts": _abi.W3DNewtonOpts}[struct]
         if field == "size":
             assert C.sizeof(cls) == int(val), key
         else:

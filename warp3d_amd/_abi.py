@@ -203,6 +203,19 @@ _D = C.POINTER(C.c_double)
 _IP = C.POINTER(C.c_int32)
 _LP = C.POINTER(C.c_int64)
 
+class W3DNewtonOpts(C.Structure):
+    """w3d_newton_opts: Newton and solver controls of a served problem."""
+    _fields_ = [("nsteps", C.c_int32), ("max_iters", C.c_int32),
+                ("precision", C.c_int32), ("pcg_maxit", C.c_int32),
+                ("tol", C.c_double), ("tol_abs", C.c_double),
+                ("pcg_rtol", C.c_double)]
+
+
+W3D_PROBLEM_RUNNING = 0
+W3D_PROBLEM_STEP_DONE = 1
+W3D_PROBLEM_DONE = 2
+W3D_PROBLEM_FAILED = 3
+
 PROTOS = {
     "get_blocking": (_I, [_P, _IP, _IP]),
     "block_info": (_I, [_P, _I, _IP]),
@@ -216,4 +229,19 @@ PROTOS = {
     "rollback": (_I, [_P]),
     "get_block_state": (_I, [_P, _I, _I, _D, C.c_int64]),
     "set_block_state": (_I, [_P, _I, _I, _D, C.c_int64]),
+}
+
+# the batched serving entry points (the library only; the oracle mirrors
+# the block interface above)
+SERVE_PROTOS = {
+    "problem_create": (_I, [_P, _IP, _D, C.POINTER(W3DNewtonOpts),
+                            C.POINTER(_P)]),
+    "problem_destroy": (_I, [_P]),
+    "problem_status": (_I, [_P, _IP, _IP, _IP]),
+    "problem_step_record": (_I, [_P, _I, _IP, _IP, _D, _D, _D]),
+    "problem_get_u": (_I, [_P, _D]),
+    "problem_device_bytes": (_I, [_P, _LP]),
+    "batch_create": (_I, [_I, C.POINTER(_P)]),
+    "batch_destroy": (_I, [_P]),
+    "batch_advance": (_I, [_P, C.POINTER(_P), _I, _IP]),
 }

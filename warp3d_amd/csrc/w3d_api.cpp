@@ -17,7 +17,7 @@
 #include <string>
 #include <vector>
 
-#include "w3d_internal.h"
+#include "w3d_ctx.h"
 
 namespace w3d {
 bool fill_gauss_table(int etype, int order, GaussTable* t);
@@ -28,40 +28,19 @@ int nnode_of(int etype);
 using namespace w3d;
 
 namespace {
-
 thread_local std::string g_err = "no error";
+}  // namespace
 
-int fail(int code, const std::string& msg) {
+thread_local int64_t w3d::t_dalloc_bytes = 0;
+
+int w3d::fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
 }
 
-#define HIPCHK(x)                                                        \
-  do {                                                                   \
-    hipError_t e_ = (x);                                                 \
-    if (e_ != hipSuccess)                                                \
-      return fail(W3D_ERR_DEVICE, std::string(#x) + ": " +               \
-                                      hipGetErrorString(e_));            \
-  } while (0)
+namespace {
 
-template <typename T>
-int dalloc(T** p, size_t n) {
-  *p = nullptr;
-  if (n == 0) n = 1;
-  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
-  if (e != hipSuccess)
-    return fail(W3D_ERR_NOMEM, std::string("hipMalloc: ") +
-                                   hipGetErrorString(e));
-  return W3D_OK;
-}
-
-template <typename T>
-void dfree(T*& p) {
-  if (p) (void)hipFree((void*)p);
-  p = nullptr;
-}
-
-// ranks of an element's nodes by ascending global number
+// H' = E Et / (E - Et) in single precision
 float hprime_real(float e, float et) {
   volatile float num = e * et;  // REAL arithmetic as elprp.f:872
   volatile float den = e - et;
@@ -70,132 +49,8 @@ float hprime_real(float e, float et) {
 
 }  // namespace
 
-struct w3d_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  int nonode = 0, noelem = 0, nelblk = 0;
-  std::vector<int32_t> etype, order, matnum, bbar, geonl;
-  bool has_geonl = false;
-  // finite strains: [R] at n and n+1 of every point (SoA like the
-  // stresses), and the displacements of the last recovery (the tangent's
-  // configuration)
-  double *d_rot_n = nullptr, *d_rot_n1 = nullptr;
-  double *d_u_t = nullptr, *d_du_t = nullptr;
-  std::vector<int32_t> elblks;
-  std::vector<BlockDesc> blocks;
-  std::vector<MatDesc> mats;
-  std::vector<GaussTable> tables;
-  std::vector<int32_t> conn;      // 0-based nodes, block order
-  std::vector<int32_t> elem_blk;  // block of each element (0-based)
-  // inverse incidences: per node, (element, local node) ascending element
-  std::vector<int64_t> inv_ptr;
-  std::vector<int32_t> inv_elem, inv_ln;
-  int64_t nq = 0, nhist = 0, nek = 0, nifv = 0;
-  // device memory
-  BlockDesc* d_blocks = nullptr;
-  MatDesc* d_mats = nullptr;
-  GaussTable* d_tables = nullptr;
-  double* d_coords = nullptr;
-  int32_t* d_conn = nullptr;
-  double *d_stress_n = nullptr, *d_stress_n1 = nullptr;
-  double *d_strain_n = nullptr, *d_strain_n1 = nullptr;
-  double *d_hist_n = nullptr, *d_hist_n1 = nullptr;
-  double* d_cep = nullptr;
-  double* d_ek64 = nullptr;
-  float* d_ek32 = nullptr;
-  double *d_einfvec = nullptr, *d_evol = nullptr, *d_en_wg = nullptr;
-  int32_t* d_flags = nullptr;
-  // mm10: crystal records, strain increments, per-point work items
-  CrystalDev* d_crys = nullptr;
-  double* d_uddt = nullptr;
-  WorkItem* d_work_mm10 = nullptr;
-  int n_work_mm10 = 0;
-  int32_t* d_mm10_blks = nullptr;  // indices of the mm10 blocks
-  int n_mm10_blks = 0;
-  int32_t* d_counter = nullptr;     // work counter of persistent kernels
-  int mm10_grid = 0;                // workgroups of the persistent mm10
-  int n_cu = 0;                     // compute units of the device
-  // tangent work-item size (elements) per kernel group, autotuned
-  std::vector<std::pair<int, int>> tan_chunk;
-  // captured Newton iterations (w3d_iteration_dev), newest last
-  struct GraphEntry {
-    int key[5];
-    const void* ptr[5];
-    double dt;
-    hipGraphExec_t exec;
-  };
-  std::vector<GraphEntry> graphs;
-  bool capturing = false;
-  int last_step = 1;                // step of the last recovery
-  bool has_mm10 = false;
-  double dt = 1.0;
-  WorkItem *d_work_rec = nullptr, *d_work_tan = nullptr;
-  std::vector<int> rec_groups, rec_start, rec_count;
-  std::vector<int> tan_groups, tan_start, tan_count;
-  int64_t *d_inv_ptr = nullptr, *d_inv_off = nullptr;
-  int32_t *d_blk_wfirst = nullptr, *d_blk_wcount = nullptr;
-  double* d_blk_sums = nullptr;
-  double *d_u = nullptr, *d_du = nullptr, *d_ifv = nullptr;
-  // sparsity + assembly map
-  int neqns = 0;
-  int64_t ncoeff = 0;
-  std::vector<int32_t> dof_eqn_map, eqn_node_map, k_ptrs, k_indexes;
-  std::vector<int64_t> row_start;
-  uint16_t* d_asm_hdr = nullptr;   // assembly segments (w3d_pattern)
-  uint32_t* d_asm_src = nullptr;
-  int64_t* d_asm_wbase = nullptr;
-  int64_t asm_nseg = 0;
-  double* d_kout = nullptr;  // neqns diagonal then ncoeff coefficients
-  int64_t assembled = 0;     // serial of the last assembly
-  // full symmetric CSR of the assembled matrix for the solver: row
-  // pointers, columns (0-based), the d_kout index of each entry, values
-  int64_t csr_nnz = 0, csr_serial = -1;
-  int64_t* d_csr_rp = nullptr;
-  int32_t* d_csr_col = nullptr;
-  uint32_t* d_csr_vidx = nullptr;
-  double* d_csr_val = nullptr;
-  double* d_pcg = nullptr;   // solver work vectors and scalars
-  int tangent_precision = -1;
-  double eps_bbar = 0.0;
-  // phase timing (w3d_set_profiling)
-  bool profiling = false;
-  // up to two timed segments per phase between collections
-  hipEvent_t ev_a[W3D_NUM_PHASES][2] = {}, ev_b[W3D_NUM_PHASES][2] = {};
-  int nseg[W3D_NUM_PHASES] = {};
-  int64_t prof_n[W3D_NUM_PHASES] = {};
-  double prof_ms[W3D_NUM_PHASES] = {};
 
-  DevModel dev() const {
-    DevModel M;
-    M.blocks = d_blocks;
-    M.mats = d_mats;
-    M.tables = d_tables;
-    M.coords = d_coords;
-    M.conn = d_conn;
-    M.stress_n = d_stress_n;
-    M.stress_n1 = d_stress_n1;
-    M.strain_n = d_strain_n;
-    M.strain_n1 = d_strain_n1;
-    M.hist_n = d_hist_n;
-    M.hist_n1 = d_hist_n1;
-    M.cep = d_cep;
-    M.ek64 = d_ek64;
-    M.ek32 = d_ek32;
-    M.einfvec = d_einfvec;
-    M.evol = d_evol;
-    M.flags = d_flags;
-    M.crys = d_crys;
-    M.uddt = d_uddt;
-    M.eps_bbar = eps_bbar;
-    M.dt = dt;
-    M.rot_n1 = d_rot_n1;
-    M.u_t = d_u_t;
-    M.du_t = d_du_t;
-    return M;
-  }
-};
-
-namespace {
+namespace w3d {
 
 bool compatible(const w3d_ctx* c, int a, int b) {
   return c->etype[a] == c->etype[b] && c->matnum[a] == c->matnum[b] &&
@@ -298,15 +153,6 @@ void free_ctx(w3d_ctx* c) {
   c->stream = nullptr;
 }
 
-template <typename T>
-int upload(T** d, const std::vector<T>& h) {
-  int rc = dalloc(d, h.size());
-  if (rc) return rc;
-  if (!h.empty())
-    HIPCHK(hipMemcpy(*d, h.data(), h.size() * sizeof(T),
-                     hipMemcpyHostToDevice));
-  return W3D_OK;
-}
 
 int create_impl(w3d_ctx* c, const w3d_model* m) {
   if (m->nonode <= 0 || m->noelem <= 0 || !m->c || !m->incid ||
@@ -695,16 +541,6 @@ int check_flags(w3d_ctx* c, int step) {
   return W3D_OK;
 }
 
-// Work lists of one recovery: every block, or the blocks
-// [first_blk, first_blk + nblk) (device copies owned by the caller).
-struct RecoverLists {
-  const WorkItem* d_w = nullptr;
-  const WorkItem* d_wm = nullptr;
-  int nwm = 0;
-  std::vector<int> sg, ss, sn;
-  WorkItem *d_sel = nullptr, *d_selm = nullptr;
-};
-
 int recover_lists(w3d_ctx* c, int first_blk, int nblk, RecoverLists& L) {
   L.d_w = c->d_work_rec;
   L.d_wm = c->d_work_mm10;
@@ -749,69 +585,102 @@ int recover_lists(w3d_ctx* c, int first_blk, int nblk, RecoverLists& L) {
   return W3D_OK;
 }
 
-// Enqueue one recovery on the context's stream without host
-// synchronisation or memsets (so that it can be captured in a graph): the
-// flags cleared by a kernel, the recovery passes, mm10 with grad Fe, the
-// structure-force gather into d_ifv (if given) and the per-block energy
-// sums (if sums).
-int enqueue_recover(w3d_ctx* c, const RecoverLists& L, int step, int iter,
-                    int extrapolated, const double* d_u, const double* d_du,
-                    double* d_ifv, bool sums, int first_blk, int nblk) {
-  if (c->has_geonl) {
-    // the finite-strain update needs u; the tangent later uses X + u + du
-    if (!d_u)
-      return fail(W3D_ERR_ARG, "recover: finite strains need the displacements u");
-    const size_t nb = 3 * (size_t)c->nonode * sizeof(double);
-    HIPCHK(hipMemcpyAsync(c->d_u_t, d_u, nb, hipMemcpyDeviceToDevice,
-                          c->stream));
-    HIPCHK(hipMemcpyAsync(c->d_du_t, d_du, nb, hipMemcpyDeviceToDevice,
-                          c->stream));
-  }
-  int rc = launch_fill_i32(c->d_flags, 4, 0, c->stream);
-  if (rc) return fail(rc, "recover: flag reset launch failed");
-  c->last_step = step;
+int enqueue_recover_stage(w3d_ctx* c, const RecoverLists& L, int stage,
+                          int step, int iter, int extrapolated,
+                          const double* d_u, const double* d_du, double* d_ifv,
+                          bool sums, int first_blk, int nblk) {
   const int nonlinear = (iter >= 1 || extrapolated) ? 1 : 0;
   const int want_energy = (iter > 0) ? 1 : 0;
   DevModel M = c->dev();
-  // mm10's iteration-0 linear-elastic update (rstgp1.f:4453-4457)
-  const int iter0_off = (iter == 0 && (!extrapolated || step == 1)) ? 1 : 0;
-  prof_begin(c, W3D_PHASE_RECOVER);
-  rc = launch_recover(M, c->blocks, L.d_w, L.sg, L.ss, L.sn, step, iter,
-                      nonlinear, d_u, d_du, want_energy, c->d_en_wg, 0,
-                      c->stream);
-  prof_end(c, W3D_PHASE_RECOVER);
-  if (!rc && L.nwm > 0) {
+  int rc = W3D_OK;
+  if (stage == 0) {
+    if (c->has_geonl) {
+      // the finite-strain update needs u; the tangent later uses X + u + du
+      if (!d_u)
+        return fail(W3D_ERR_ARG,
+                    "recover: finite strains need the displacements u");
+      const size_t nb = 3 * (size_t)c->nonode * sizeof(double);
+      HIPCHK(hipMemcpyAsync(c->d_u_t, d_u, nb, hipMemcpyDeviceToDevice,
+                            c->stream));
+      HIPCHK(hipMemcpyAsync(c->d_du_t, d_du, nb, hipMemcpyDeviceToDevice,
+                            c->stream));
+    }
+    rc = launch_fill_i32(c->d_flags, 4, 0, c->stream);
+    if (rc) return fail(rc, "recover: flag reset launch failed");
+    c->last_step = step;
+    prof_begin(c, W3D_PHASE_RECOVER);
+    rc = launch_recover(M, c->blocks, L.d_w, L.sg, L.ss, L.sn, step, iter,
+                        nonlinear, d_u, d_du, want_energy, c->d_en_wg, 0,
+                        c->stream);
+    prof_end(c, W3D_PHASE_RECOVER);
+  } else if (stage == 1) {
+    if (L.nwm <= 0) return W3D_OK;
     prof_begin(c, W3D_PHASE_MM10);
     rc = launch_mm10(M, L.d_wm, L.nwm, c->d_counter, c->mm10_grid, step,
-                     iter0_off, c->stream);
-    if (!rc)
-      rc = launch_mm10_grads(M, c->d_mm10_blks, c->n_mm10_blks, first_blk - 1,
-                             first_blk - 1 + nblk, step, c->stream);
+                     mm10_iter0_off(step, iter, extrapolated), c->stream);
     prof_end(c, W3D_PHASE_MM10);
-    if (!rc) {
+  } else if (stage == 2) {
+    if (L.nwm <= 0) return W3D_OK;
+    prof_begin(c, W3D_PHASE_MM10);
+    rc = launch_mm10_grads(M, c->d_mm10_blks, c->n_mm10_blks, first_blk - 1,
+                           first_blk - 1 + nblk, step, c->stream);
+    prof_end(c, W3D_PHASE_MM10);
+  } else {
+    if (L.nwm > 0) {
       prof_begin(c, W3D_PHASE_RECOVER);
       rc = launch_recover(M, c->blocks, L.d_w, L.sg, L.ss, L.sn, step, iter,
                           nonlinear, d_u, d_du, want_energy, c->d_en_wg, 1,
                           c->stream);
       prof_end(c, W3D_PHASE_RECOVER);
     }
+    if (!rc && d_ifv) {
+      prof_begin(c, W3D_PHASE_IFV);
+      rc = launch_ifv_gather(c->d_einfvec, c->d_inv_ptr, c->d_inv_off,
+                             c->nonode, d_ifv, c->stream);
+      prof_end(c, W3D_PHASE_IFV);
+      if (rc) return fail(rc, "recover: ifv gather launch failed");
+    }
+    if (!rc && sums && want_energy) {
+      prof_begin(c, W3D_PHASE_ENERGY);
+      rc = launch_energy_reduce(c->d_en_wg, c->d_blk_wfirst, c->d_blk_wcount,
+                                c->nelblk, c->d_blk_sums, c->stream);
+      prof_end(c, W3D_PHASE_ENERGY);
+      if (rc) return fail(rc, "recover: energy reduction launch failed");
+    }
   }
   if (rc) return fail(rc, "recover: kernel launch failed");
-  if (d_ifv) {
-    prof_begin(c, W3D_PHASE_IFV);
-    rc = launch_ifv_gather(c->d_einfvec, c->d_inv_ptr, c->d_inv_off,
-                           c->nonode, d_ifv, c->stream);
-    prof_end(c, W3D_PHASE_IFV);
-    if (rc) return fail(rc, "recover: ifv gather launch failed");
-  }
-  if (sums && want_energy) {
-    prof_begin(c, W3D_PHASE_ENERGY);
-    rc = launch_energy_reduce(c->d_en_wg, c->d_blk_wfirst, c->d_blk_wcount,
-                              c->nelblk, c->d_blk_sums, c->stream);
-    prof_end(c, W3D_PHASE_ENERGY);
-    if (rc) return fail(rc, "recover: energy reduction launch failed");
+  return W3D_OK;
+}
+
+int enqueue_recover(w3d_ctx* c, const RecoverLists& L, int step, int iter,
+                    int extrapolated, const double* d_u, const double* d_du,
+                    double* d_ifv, bool sums, int first_blk, int nblk) {
+  for (int stage = 0; stage < 4; ++stage) {
+    int rc = enqueue_recover_stage(c, L, stage, step, iter, extrapolated, d_u,
+                                   d_du, d_ifv, sums, first_blk, nblk);
+    if (rc) return rc;
   }
   return W3D_OK;
+}
+
+int enqueue_commit(w3d_ctx* c) {
+  HIPCHK(hipMemcpyAsync(c->d_stress_n, c->d_stress_n1, 9 * c->nq * 8,
+                        hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_strain_n, c->d_strain_n1, 6 * c->nq * 8,
+                        hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_hist_n, c->d_hist_n1, c->nhist * 8,
+                        hipMemcpyDeviceToDevice, c->stream));
+  if (c->has_geonl)  // adaptive_saves (mnralg.f:1305)
+    HIPCHK(hipMemcpyAsync(c->d_rot_n, c->d_rot_n1, 9 * c->nq * 8,
+                          hipMemcpyDeviceToDevice, c->stream));
+  return W3D_OK;
+}
+
+int enqueue_csr_values(w3d_ctx* c, const double* d_diag,
+                       const double* d_coef) {
+  int rc = solver_csr_values(d_diag, d_coef, c->neqns, c->d_csr_vidx,
+                             c->csr_nnz, c->d_csr_val, c->stream);
+  return rc ? fail(rc, "csr values: launch failed") : W3D_OK;
 }
 
 // After the stream: energy and plastic work from the per-block sums (block
@@ -936,7 +805,7 @@ struct View {
   int64_t n = 0;
 };
 
-}  // namespace
+}  // namespace w3d
 
 extern "C" {
 
@@ -956,7 +825,9 @@ int w3d_create(w3d_ctx** ctx, const w3d_model* model, int device) {
     return fail(W3D_ERR_ARG, "w3d_create: bad device ordinal");
   w3d_ctx* c = new w3d_ctx();
   c->device = device;
+  const int64_t bytes0 = t_dalloc_bytes;
   int rc = create_impl(c, model);
+  c->dev_bytes = t_dalloc_bytes - bytes0;
   if (rc) {
     std::string keep = g_err;
     free_ctx(c);
@@ -1098,6 +969,7 @@ int w3d_tangent(w3d_ctx* c, int32_t step, int32_t iter, int32_t precision) {
 int w3d_pattern(w3d_ctx* c, const int32_t* cstmap, int32_t* neqns_out,
                 int64_t* ncoeff_out) {
   if (!c || !cstmap) return fail(W3D_ERR_ARG, "w3d_pattern: null argument");
+  const int64_t bytes0 = t_dalloc_bytes;
   const int nd = 3 * c->nonode;
   // dof_map (assemble_code.f:561)
   c->dof_eqn_map.assign(nd, 0);
@@ -1397,6 +1269,7 @@ int w3d_pattern(w3d_ctx* c, const int32_t* cstmap, int32_t* neqns_out,
     c->csr_serial = -1;
     c->assembled = 0;
   }
+  c->dev_bytes += t_dalloc_bytes - bytes0;
   if (neqns_out) *neqns_out = neq;
   if (ncoeff_out) *ncoeff_out = c->ncoeff;
   return W3D_OK;
@@ -1465,15 +1338,8 @@ int w3d_assemble_vss(w3d_ctx* c, double* k_diag, double* k_coeffs) {
 int w3d_commit_step(w3d_ctx* c) {
   if (!c) return fail(W3D_ERR_ARG, "null context");
   HIPCHK(hipSetDevice(c->device));
-  HIPCHK(hipMemcpyAsync(c->d_stress_n, c->d_stress_n1, 9 * c->nq * 8,
-                        hipMemcpyDeviceToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(c->d_strain_n, c->d_strain_n1, 6 * c->nq * 8,
-                        hipMemcpyDeviceToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(c->d_hist_n, c->d_hist_n1, c->nhist * 8,
-                        hipMemcpyDeviceToDevice, c->stream));
-  if (c->has_geonl)  // adaptive_saves (mnralg.f:1305)
-    HIPCHK(hipMemcpyAsync(c->d_rot_n, c->d_rot_n1, 9 * c->nq * 8,
-                          hipMemcpyDeviceToDevice, c->stream));
+  int rc = enqueue_commit(c);
+  if (rc) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
   return W3D_OK;
 }

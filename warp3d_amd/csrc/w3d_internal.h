@@ -193,6 +193,82 @@ int solver_update_xr(int n, const double* p, const double* q,
 int solver_residual(int n, const double* b, double* r, hipStream_t s);
 int solver_shift(double* sc, hipStream_t s);
 
+// ---------------------------------------------------------------------
+// Batches of independent problems (w3d_serve.cpp): one launch covers the
+// work of every problem in the batch.
+// ---------------------------------------------------------------------
+// One problem's share of a batched mm10 update: its model, its work list
+// and the predicates of its step and iteration.  Global item g of the
+// launch belongs to the last slot with item0 <= g.
+struct Mm10Slot {
+  DevModel M;
+  const WorkItem* work;
+  int32_t item0;
+  int32_t nwork;
+  int32_t step;
+  int32_t iter0_off;
+};
+int launch_mm10_batch(const Mm10Slot* d_slots, int nslot, int nwork_total,
+                      int* d_counter, int grid, hipStream_t s);
+
+// Rows of one tile of a batched vector kernel (one workgroup each).
+constexpr int kTileRows = 128;
+
+// One system of a batched Jacobi-PCG: the full CSR of its assembled
+// matrix, its diagonal, right-hand side and vectors, and its tiles.
+// Per system the batch keeps 7 partial-sum rows of ntile words (b.b, then
+// p.q, r.r and r.z by iteration parity), the state (done: 0 running,
+// 1 converged, 2 iteration limit; iterations) and the scalars (|b|, r.r).
+struct PcgSys {
+  const int64_t* rp;
+  const int32_t* col;
+  const double* val;
+  const double* diag;
+  const double* b;
+  double* x;
+  double* r;
+  double* p;
+  double* q;
+  double rtol;            // |r| <= rtol |b| ends the solve
+  int32_t maxit;          // or this many iterations
+  int32_t n;
+  int32_t tile0;
+  int32_t ntile;
+};
+// x = 0, r = b, p = D^-1 b and the partial sums; then |b| per system
+int launch_pcg_batch_init(const PcgSys* d_sys, int nsys, int ntile_total,
+                          double* part, int32_t* state, double* sc,
+                          hipStream_t s);
+// iteration it (0-based) of every running system: q = A p; x, r; p
+int launch_pcg_batch_iter(const PcgSys* d_sys, int nsys, int ntile_total,
+                          int it, double* part, int32_t* state, double* sc,
+                          hipStream_t s);
+
+// One problem's Newton bookkeeping for the batched residual and update:
+// rhs = -ifv on the equations, |rhs|^2, the reaction |ifv|^2 on the
+// constrained dofs, and the sum of ifv over the loaded dofs; du += x on
+// the equations of the solved systems.
+struct ResSys {
+  const double* ifv;
+  const int32_t* eqdof;   // dof of each equation
+  const int32_t* cdof;    // constrained dofs
+  const int32_t* ldof;    // loaded dofs (a subset of cdof)
+  const int32_t* flags;   // the context's recovery flags
+  double* rhs;
+  double* du;
+  const double* x;
+  int32_t neq, ncdof, nldof;
+  int32_t tile0, ntile;
+  int32_t solve;          // du += x for this system (launch_du_batch)
+};
+// out[8 s + k]: k = 0 |rhs|^2, 1 reaction^2, 2 load, 3.. 6 the flags
+int launch_residual_batch(const ResSys* d_sys, int nsys, int ntile_total,
+                          double* part, double* out, hipStream_t s);
+int launch_du_batch(const ResSys* d_sys, int nsys, int ntile_total,
+                    hipStream_t s);
+// u += du (n words)
+int launch_axpy1(int64_t n, double* u, const double* du, hipStream_t s);
+
 // Host set-up of one crystal record from the material and its angles
 // (w3d_mm10_setup.cpp); false for an unsupported slip / elastic type.
 bool mm10_crystal_setup(const w3d_material& m, const double* angles,

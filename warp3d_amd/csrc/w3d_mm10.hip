@@ -1200,6 +1200,44 @@ __global__ void __launch_bounds__(64)
   }
 }
 
+// The batched form: the items of every slot in one list, slot s owning
+// global items [item0_s, item0_s + nwork_s); the workgroup finds its slot
+// by bisection over the slots' item0 (ascending).
+__global__ void __launch_bounds__(64)
+    k_mm10_batch(const Mm10Slot* __restrict__ slots, int nslot,
+                 int nwork_total, int* __restrict__ counter) {
+  __shared__ double s_geo[108 * kLanes];
+  __shared__ int s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int item = __builtin_amdgcn_readfirstlane(s_item);
+    __syncthreads();
+    if (item >= nwork_total) break;  // uniform across the workgroup
+    int lo = 0, hi = nslot - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (slots[mid].item0 <= item)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const Mm10Slot& sl = slots[lo];
+    mm10_item(sl.M, sl.work[item - sl.item0], sl.step, sl.iter0_off, s_geo);
+  }
+}
+
+int launch_mm10_batch(const Mm10Slot* d_slots, int nslot, int nwork_total,
+                      int* d_counter, int grid, hipStream_t s) {
+  if (nwork_total <= 0 || nslot <= 0) return W3D_OK;
+  int rc = launch_fill_i32(d_counter, 1, 0, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_mm10_batch,
+                     dim3(grid < nwork_total ? grid : nwork_total), dim3(64),
+                     0, s, d_slots, nslot, nwork_total, d_counter);
+  return hipGetLastError() == hipSuccess ? W3D_OK : W3D_ERR_DEVICE;
+}
+
 int launch_mm10(const DevModel& M, const WorkItem* d_work, int nwork,
                 int* d_counter, int grid, int step, int iter0_off,
                 hipStream_t s) {

@@ -118,6 +118,321 @@ __global__ void k_residual(int n, const double* __restrict__ b,
 // sc[0] = sc[1] (keep the previous r.z)
 __global__ void k_shift(double* sc) { sc[0] = sc[1]; }
 
+// ---------------------------------------------------------------------
+// Batched Jacobi-PCG over independent systems.  Tile t of a launch is
+// tile t - tile0 of the system that owns it; each system's reductions run
+// over its own tiles in a fixed order, so a system's iterates do not
+// depend on the other systems of the batch.
+// ---------------------------------------------------------------------
+constexpr int kPartBB = 0, kPartPQ = 1, kPartRR = 3, kPartRZ = 5;
+constexpr int kSpmvLanes = 16;
+
+// the system owning global tile t (tile0 ascending, system 0 at tile 0)
+template <typename S>
+__device__ __forceinline__ int owner(const S* __restrict__ sys, int nsys,
+                                     int t) {
+  int lo = 0, hi = nsys - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (sys[mid].tile0 <= t)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return __builtin_amdgcn_readfirstlane(lo);
+}
+
+// sum of v over the workgroup (256 threads), the same order every time;
+// every thread gets the result
+__device__ __forceinline__ double block_sum(double v, double* s_red) {
+  s_red[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = kRedBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      s_red[threadIdx.x] = s_red[threadIdx.x] + s_red[threadIdx.x + w];
+    __syncthreads();
+  }
+  const double r = s_red[0];
+  __syncthreads();
+  return r;
+}
+
+// sum of n partials (strided per thread, then the tree)
+__device__ __forceinline__ double sum_partials(const double* __restrict__ v,
+                                               int n, double* s_red) {
+  double t = 0.0;
+  for (int i = threadIdx.x; i < n; i += kRedBlock) t = t + v[i];
+  return block_sum(t, s_red);
+}
+
+__global__ void __launch_bounds__(kRedBlock)
+    k_pcg_init(const PcgSys* __restrict__ sys, int nsys,
+               double* __restrict__ part) {
+  __shared__ double s_red[kRedBlock];
+  const int si = owner(sys, nsys, blockIdx.x);
+  const PcgSys& S = sys[si];
+  const int t = blockIdx.x - S.tile0;
+  const int row = t * kTileRows + threadIdx.x;
+  double bb = 0.0, rz = 0.0;
+  if (threadIdx.x < kTileRows && row < S.n) {
+    const double bi = S.b[row];
+    const double zi = bi / S.diag[row];
+    S.x[row] = 0.0;
+    S.r[row] = bi;
+    S.p[row] = zi;
+    bb = bi * bi;
+    rz = bi * zi;
+  }
+  bb = block_sum(bb, s_red);
+  rz = block_sum(rz, s_red);
+  if (threadIdx.x == 0) {
+    double* P = part + 7 * (int64_t)S.tile0;
+    P[kPartBB * S.ntile + t] = bb;
+    P[(kPartRZ + 1) * S.ntile + t] = rz;  // the r.z of iteration -1
+  }
+}
+
+// one workgroup per system: |b|, and a zero right-hand side is solved
+__global__ void __launch_bounds__(kRedBlock)
+    k_pcg_init2(const PcgSys* __restrict__ sys, const double* __restrict__ part,
+                int32_t* __restrict__ state, double* __restrict__ sc) {
+  __shared__ double s_red[kRedBlock];
+  const PcgSys& S = sys[blockIdx.x];
+  const double* P = part + 7 * (int64_t)S.tile0;
+  const double bb = sum_partials(P + kPartBB * S.ntile, S.ntile, s_red);
+  if (threadIdx.x == 0) {
+    sc[2 * blockIdx.x] = sqrt(bb);
+    sc[2 * blockIdx.x + 1] = bb;
+    state[2 * blockIdx.x] = bb == 0.0 ? 1 : 0;
+    state[2 * blockIdx.x + 1] = 0;
+  }
+}
+
+// q = A p on the tile's rows (16 lanes per row) and the partial p.q
+__global__ void __launch_bounds__(kRedBlock)
+    k_pcg_spmv(const PcgSys* __restrict__ sys, int nsys, int it,
+               double* __restrict__ part, const int32_t* __restrict__ state) {
+  __shared__ double s_red[kRedBlock];
+  const int si = owner(sys, nsys, blockIdx.x);
+  if (state[2 * si]) return;
+  const PcgSys& S = sys[si];
+  const int t = blockIdx.x - S.tile0;
+  const int lane = threadIdx.x % kSpmvLanes;
+  constexpr int kRowsPerPass = kRedBlock / kSpmvLanes;
+  double pq = 0.0;
+  for (int k = threadIdx.x / kSpmvLanes; k < kTileRows; k += kRowsPerPass) {
+    const int row = t * kTileRows + k;
+    double v = 0.0;
+    if (row < S.n) {
+      for (int64_t j = S.rp[row] + lane; j < S.rp[row + 1]; j += kSpmvLanes)
+        v = v + S.val[j] * S.p[S.col[j]];
+    }
+#pragma unroll
+    for (int off = kSpmvLanes / 2; off >= 1; off >>= 1)
+      v = v + __shfl_xor(v, off, kSpmvLanes);
+    if (row < S.n && lane == 0) {
+      S.q[row] = v;
+      pq = pq + S.p[row] * v;
+    }
+  }
+  pq = block_sum(pq, s_red);
+  if (threadIdx.x == 0)
+    part[7 * (int64_t)S.tile0 + (kPartPQ + (it & 1)) * S.ntile + t] = pq;
+}
+
+// alpha = r.z / p.q; x += alpha p, r -= alpha q; partial r.r and r.z
+__global__ void __launch_bounds__(kRedBlock)
+    k_pcg_xr(const PcgSys* __restrict__ sys, int nsys, int it,
+             double* __restrict__ part, const int32_t* __restrict__ state) {
+  __shared__ double s_red[kRedBlock];
+  const int si = owner(sys, nsys, blockIdx.x);
+  if (state[2 * si]) return;
+  const PcgSys& S = sys[si];
+  const int t = blockIdx.x - S.tile0;
+  double* P = part + 7 * (int64_t)S.tile0;
+  const double pq =
+      sum_partials(P + (kPartPQ + (it & 1)) * S.ntile, S.ntile, s_red);
+  const double rz =
+      sum_partials(P + (kPartRZ + ((it - 1) & 1)) * S.ntile, S.ntile, s_red);
+  const double alpha = rz / pq;
+  const int row = t * kTileRows + threadIdx.x;
+  double rr = 0.0, rzn = 0.0;
+  if (threadIdx.x < kTileRows && row < S.n) {
+    S.x[row] = S.x[row] + alpha * S.p[row];
+    const double ri = S.r[row] - alpha * S.q[row];
+    S.r[row] = ri;
+    rr = ri * ri;
+    rzn = ri * (ri / S.diag[row]);
+  }
+  rr = block_sum(rr, s_red);
+  rzn = block_sum(rzn, s_red);
+  if (threadIdx.x == 0) {
+    P[(kPartRR + (it & 1)) * S.ntile + t] = rr;
+    P[(kPartRZ + (it & 1)) * S.ntile + t] = rzn;
+  }
+}
+
+// convergence |r| <= rtol |b| (or the iteration limit) ends the system;
+// otherwise p = D^-1 r + (r.z / r.z old) p.  Every workgroup of a system
+// reaches the same decision from the same partials; the first tile's
+// writes the state, which a sibling in this launch may or may not see:
+// either way it leaves p alone or updates a p that is no longer read.
+__global__ void __launch_bounds__(kRedBlock)
+    k_pcg_p(const PcgSys* __restrict__ sys, int nsys, int it,
+            const double* __restrict__ part,
+            int32_t* __restrict__ state, double* __restrict__ sc) {
+  __shared__ double s_red[kRedBlock];
+  const int si = owner(sys, nsys, blockIdx.x);
+  if (state[2 * si]) return;
+  const PcgSys& S = sys[si];
+  const int t = blockIdx.x - S.tile0;
+  const double* P = part + 7 * (int64_t)S.tile0;
+  const double rr =
+      sum_partials(P + (kPartRR + (it & 1)) * S.ntile, S.ntile, s_red);
+  const bool conv = sqrt(rr) <= S.rtol * sc[2 * si];
+  if (conv || it + 1 >= S.maxit) {
+    if (t == 0 && threadIdx.x == 0) {
+      state[2 * si] = conv ? 1 : 2;
+      state[2 * si + 1] = it + 1;
+      sc[2 * si + 1] = rr;
+    }
+    return;
+  }
+  const double rzn =
+      sum_partials(P + (kPartRZ + (it & 1)) * S.ntile, S.ntile, s_red);
+  const double rzo =
+      sum_partials(P + (kPartRZ + ((it - 1) & 1)) * S.ntile, S.ntile, s_red);
+  const double beta = rzn / rzo;
+  const int row = t * kTileRows + threadIdx.x;
+  if (threadIdx.x < kTileRows && row < S.n)
+    S.p[row] = S.r[row] / S.diag[row] + beta * S.p[row];
+}
+
+int launch_pcg_batch_init(const PcgSys* d_sys, int nsys, int ntile_total,
+                          double* part, int32_t* state, double* sc,
+                          hipStream_t s) {
+  if (nsys <= 0) return W3D_OK;
+  if (ntile_total > 0)
+    hipLaunchKernelGGL(k_pcg_init, dim3(ntile_total), dim3(kRedBlock), 0, s,
+                       d_sys, nsys, part);
+  hipLaunchKernelGGL(k_pcg_init2, dim3(nsys), dim3(kRedBlock), 0, s, d_sys,
+                     part, state, sc);
+  return hipGetLastError() == hipSuccess ? W3D_OK : W3D_ERR_DEVICE;
+}
+
+int launch_pcg_batch_iter(const PcgSys* d_sys, int nsys, int ntile_total,
+                          int it, double* part, int32_t* state, double* sc,
+                          hipStream_t s) {
+  if (nsys <= 0 || ntile_total <= 0) return W3D_OK;
+  hipLaunchKernelGGL(k_pcg_spmv, dim3(ntile_total), dim3(kRedBlock), 0, s,
+                     d_sys, nsys, it, part, state);
+  hipLaunchKernelGGL(k_pcg_xr, dim3(ntile_total), dim3(kRedBlock), 0, s,
+                     d_sys, nsys, it, part, state);
+  hipLaunchKernelGGL(k_pcg_p, dim3(ntile_total), dim3(kRedBlock), 0, s, d_sys,
+                     nsys, it, part, state, sc);
+  return hipGetLastError() == hipSuccess ? W3D_OK : W3D_ERR_DEVICE;
+}
+
+// ---------------------------------------------------------------------
+// Batched Newton bookkeeping (one tile = kTileRows equations and
+// kTileRows constrained dofs of one problem)
+// ---------------------------------------------------------------------
+__global__ void __launch_bounds__(kRedBlock)
+    k_residual_batch(const ResSys* __restrict__ sys, int nsys,
+                     double* __restrict__ part) {
+  __shared__ double s_red[kRedBlock];
+  const int si = owner(sys, nsys, blockIdx.x);
+  const ResSys& S = sys[si];
+  const int t = blockIdx.x - S.tile0;
+  const int k = t * kTileRows + threadIdx.x;
+  double res = 0.0, rea = 0.0, load = 0.0;
+  if (threadIdx.x < kTileRows) {
+    if (k < S.neq) {
+      const double v = -S.ifv[S.eqdof[k]];
+      S.rhs[k] = v;
+      res = v * v;
+    }
+    if (k < S.ncdof) {
+      const double v = S.ifv[S.cdof[k]];
+      rea = v * v;
+    }
+    if (k < S.nldof) load = S.ifv[S.ldof[k]];
+  }
+  res = block_sum(res, s_red);
+  rea = block_sum(rea, s_red);
+  load = block_sum(load, s_red);
+  if (threadIdx.x == 0) {
+    double* P = part + 3 * (int64_t)S.tile0;
+    P[t] = res;
+    P[S.ntile + t] = rea;
+    P[2 * S.ntile + t] = load;
+  }
+}
+
+__global__ void __launch_bounds__(kRedBlock)
+    k_residual_sums(const ResSys* __restrict__ sys,
+                    const double* __restrict__ part, double* __restrict__ out) {
+  __shared__ double s_red[kRedBlock];
+  const ResSys& S = sys[blockIdx.x];
+  const double* P = part + 3 * (int64_t)S.tile0;
+  const double res = sum_partials(P, S.ntile, s_red);
+  const double rea = sum_partials(P + S.ntile, S.ntile, s_red);
+  const double load = sum_partials(P + 2 * S.ntile, S.ntile, s_red);
+  if (threadIdx.x == 0) {
+    double* o = out + 8 * (int64_t)blockIdx.x;
+    o[0] = res;
+    o[1] = rea;
+    o[2] = load;
+    for (int f = 0; f < 4; ++f) o[3 + f] = (double)S.flags[f];
+    o[7] = 0.0;
+  }
+}
+
+__global__ void __launch_bounds__(kRedBlock)
+    k_du_batch(const ResSys* __restrict__ sys, int nsys) {
+  const int si = owner(sys, nsys, blockIdx.x);
+  const ResSys& S = sys[si];
+  if (!S.solve) return;
+  const int t = blockIdx.x - S.tile0;
+  const int k = t * kTileRows + threadIdx.x;
+  if (threadIdx.x < kTileRows && k < S.neq) {
+    const int d = S.eqdof[k];
+    S.du[d] = S.du[d] + S.x[k];
+  }
+}
+
+__global__ void k_axpy1(int64_t n, double* __restrict__ u,
+                        const double* __restrict__ du) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) u[i] = u[i] + du[i];
+}
+
+int launch_residual_batch(const ResSys* d_sys, int nsys, int ntile_total,
+                          double* part, double* out, hipStream_t s) {
+  if (nsys <= 0) return W3D_OK;
+  if (ntile_total > 0)
+    hipLaunchKernelGGL(k_residual_batch, dim3(ntile_total), dim3(kRedBlock),
+                       0, s, d_sys, nsys, part);
+  hipLaunchKernelGGL(k_residual_sums, dim3(nsys), dim3(kRedBlock), 0, s,
+                     d_sys, part, out);
+  return hipGetLastError() == hipSuccess ? W3D_OK : W3D_ERR_DEVICE;
+}
+
+int launch_du_batch(const ResSys* d_sys, int nsys, int ntile_total,
+                    hipStream_t s) {
+  if (nsys <= 0 || ntile_total <= 0) return W3D_OK;
+  hipLaunchKernelGGL(k_du_batch, dim3(ntile_total), dim3(kRedBlock), 0, s,
+                     d_sys, nsys);
+  return hipGetLastError() == hipSuccess ? W3D_OK : W3D_ERR_DEVICE;
+}
+
+int launch_axpy1(int64_t n, double* u, const double* du, hipStream_t s) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_axpy1, dim3((unsigned)((n + 255) / 256)), dim3(256),
+                       0, s, n, u, du);
+  return hipGetLastError() == hipSuccess ? W3D_OK : W3D_ERR_DEVICE;
+}
+
 namespace {
 inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 }

@@ -11,7 +11,8 @@ import os
 
 import numpy as np
 
-from ._abi import (PROTOS, W3D_CUT_STEP, W3DModel, dptr, iptr)
+from ._abi import (PROTOS, SERVE_PROTOS, W3D_CUT_STEP, W3DModel, dptr,
+                   iptr)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libw3dblk.so")
@@ -30,7 +31,7 @@ def lib():
             raise W3DError(f"{LIB_PATH} is not built: run "
                            "python -m warp3d_amd.build")
         L = C.CDLL(LIB_PATH)
-        for name, (res, args) in PROTOS.items():
+        for name, (res, args) in {**PROTOS, **SERVE_PROTOS}.items():
             f = getattr(L, "w3d_" + name)
             f.restype = res
             f.argtypes = args
@@ -73,6 +74,16 @@ def lib():
                                    C.POINTER(C.c_int32), C.c_int32,
                                    C.POINTER(C.c_int32)]
         L.w3d_autotune.restype = C.c_int32
+        L.w3d_solve_pcg.argtypes = [C.c_void_p, C.POINTER(C.c_double),
+                                    C.POINTER(C.c_double), C.c_double,
+                                    C.c_int32, C.POINTER(C.c_int32),
+                                    C.POINTER(C.c_double)]
+        L.w3d_solve_pcg.restype = C.c_int32
+        L.w3d_solve_pcg_dev.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_void_p, C.c_double,
+                                        C.c_int32, C.POINTER(C.c_int32),
+                                        C.POINTER(C.c_double)]
+        L.w3d_solve_pcg_dev.restype = C.c_int32
         _LIB = L
     return _LIB
 
@@ -210,6 +221,29 @@ class BlockModel:
              "w3d_autotune")
         v = list(buf)[:n.value]
         return dict(zip(v[0::2], v[1::2]))
+
+    # -- linear solve (Jacobi-PCG on the last assembly) ------------------
+    def solve_pcg(self, rhs, x0=None, rtol=1e-10, maxit=10000):
+        """Host vectors; returns (x, iterations, relative residual)."""
+        x = np.zeros(self.neqns) if x0 is None else np.array(x0, float)
+        it = C.c_int32()
+        rr = C.c_double()
+        _chk(lib().w3d_solve_pcg(self.h, dptr(np.ascontiguousarray(rhs,
+                                                                   float)),
+                                 dptr(x), rtol, maxit, C.byref(it),
+                                 C.byref(rr)), "w3d_solve_pcg")
+        return x, it.value, rr.value
+
+    def solve_pcg_dev(self, d_kdiag, d_kcoeffs, d_rhs, d_x, rtol=1e-10,
+                      maxit=10000):
+        """Device vectors (d_x: initial guess in, solution out); returns
+        (iterations, relative residual)."""
+        it = C.c_int32()
+        rr = C.c_double()
+        _chk(lib().w3d_solve_pcg_dev(self.h, d_kdiag, d_kcoeffs, d_rhs, d_x,
+                                     rtol, maxit, C.byref(it), C.byref(rr)),
+             "w3d_solve_pcg_dev")
+        return it.value, rr.value
 
     # -- state -----------------------------------------------------------
     def commit_step(self):
