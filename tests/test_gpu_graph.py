@@ -110,12 +110,13 @@ def test_autotuned_lists_give_the_same_results(W):
     choices = b.dev.autotune(_abi.W3D_FP32)
     assert all(v in (8, 16, 32, 64) for k, v in choices.items() if k >= 0)
     assert -1 in choices and choices[-1] > 0
-    for r, tune in ((a, False), (b, True)):
-        r.eager(1, 2, _abi.W3D_FP32)
+    ea = a.eager(1, 2, _abi.W3D_FP32)
+    assert b.eager(1, 2, _abi.W3D_FP32) == ea
     for x, y in zip(a.outputs(), b.outputs()):
         assert np.array_equal(x, y)
-    # and the graph path with the tuned lists
+    # and the graph path with the tuned lists (the same status: this
+    # increment may ask for a step cut)
     eb = b.graph(1, 2, _abi.W3D_FP32)
     for x, y in zip(a.outputs(), b.outputs()):
         assert np.array_equal(x, y)
-    assert eb[2] == 0
+    assert eb == ea
