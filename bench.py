@@ -31,7 +31,11 @@ FP32 tangent (the library's mixed-precision path), reported beside the plain
 FP64 path measured in the same run.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
-                    [--workload hex8|hex20|mixed]
+                    [--workload hex8|hex20|mixed|rve]
+
+The rve workload serves a stream of RVE requests instead: --steps is the
+number of requests timed, --warmup the requests served before; see
+run_rve.
 """
 import argparse
 import json
@@ -79,6 +83,11 @@ WORKLOADS = {
              "spans 32-256, mm10 fcc Voce crystal plasticity with seeded "
              "SO(3)-uniform orientations, one Newton iteration of the "
              "element-block path per step",
+    "rve": "Poisson stream of crystal-plasticity RVE requests, each an n^3 "
+           "hex8 unit cube (n uniform in [8, 24]) of 64 seeded Voronoi "
+           "grains with SO(3)-uniform fcc orientations, mm10 Voce, FP32 "
+           "tangent, symmetry planes and a strain-controlled top face, 10 "
+           "steps to 2 %; served by continuous batching",
 }
 
 
@@ -303,6 +312,93 @@ def cpu_baseline(arr, du, ngp, budget_s=10.0):
                       f"iterations in {t1 - t0:.1f} s"}
 
 
+def run_rve(args):
+    """RVE serving: --warmup requests served first (untimed), then a
+    Poisson stream of --steps requests through the continuous-batching
+    server; beside it --plain-sample requests of the same stream solved one
+    at a time through the plain path (FP64 tangent, host-driven Newton and
+    Jacobi-PCG).  value = RVE solves/s of the server over the stream."""
+    from warp3d_amd import _abi
+    from warp3d_amd import serve as S
+    opts = S.newton_opts(precision=_abi.W3D_FP32)
+    eng = S.DeviceEngine(opts)
+    try:
+        if args.warmup:
+            warm = S.poisson_arrivals(args.warmup, rate=0.0, nmin=8,
+                                      nmax=12, seed=99)
+            S.Server(eng, max_slots=args.slots).run(warm)
+            log(f"rve warmup: {args.warmup} requests")
+        arr = S.poisson_arrivals(args.steps, rate=args.rate, nmin=8,
+                                 nmax=args.nmax, seed=1)
+        srv = S.Server(eng, max_slots=args.slots, hbm_budget=64 << 30)
+        done, st = srv.run(arr)
+    finally:
+        eng.close()
+    log("server: " + json.dumps(st))
+    newton = [r["newton"] for j in done if j.result
+              for r in j.result["records"]]
+    pcg = [r["pcg"] for j in done if j.result for r in j.result["records"]]
+    # the plain path on a sample of the same stream: the requests at evenly
+    # spaced size quantiles, so that its mean time per request estimates
+    # the stream's
+    by_size = sorted(range(len(arr)), key=lambda k: arr[k][1].n)
+    k = args.plain_sample
+    pick = [by_size[int((i + 0.5) * len(arr) / k)] for i in range(k)] \
+        if k and arr else []
+    plain_t, plain_steps = [], []
+    popts = S.newton_opts(precision=_abi.W3D_FP64)
+    for i in pick:
+        spec = arr[i][1]
+        t0 = time.perf_counter()
+        _, _, times = S.solve_plain(S.make_rve(spec), popts)
+        plain_t.append(time.perf_counter() - t0)
+        plain_steps += times
+        log(f"plain n={spec.n}: {plain_t[-1]:.2f} s")
+    mean_elems_stream = np.mean([s.noelem for _, s in arr])
+    plain_rate = len(plain_t) / sum(plain_t) if plain_t else None
+    out = {
+        "metric": "RVE solves/s at p99 load-step latency",
+        "value": round(st["solves_per_s"], 3),
+        "unit": "RVE solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * st["wall_s"] / max(args.steps, 1), 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (recovery, mm10, residual, Jacobi-PCG) + f32 (tangent)",
+        "data": "synthetic (seeded Voronoi grains, orientations, sizes and "
+                "arrival times)",
+        "config": {"workload": WORKLOADS["rve"], "requests": args.steps,
+                   "rate_per_s": args.rate, "slots": args.slots,
+                   "n_range": [8, args.nmax],
+                   "mean_elements": round(float(mean_elems_stream), 1)},
+        "latency": {"step_p50_ms": st["step_p50_ms"],
+                    "step_p99_ms": st["step_p99_ms"],
+                    "request_p50_s": st["request_p50_s"],
+                    "request_p99_s": st["request_p99_s"]},
+        "server": {k: st[k] for k in ("solved", "failed", "wall_s",
+                                      "iterations", "peak_slots",
+                                      "peak_bytes", "steps")},
+        "newton_iters_per_step": round(float(np.mean(newton)), 3)
+        if newton else None,
+        "pcg_iters_per_step": round(float(np.mean(pcg)), 1) if pcg else None,
+        "plain": {"sample_n": [arr[i][1].n for i in pick],
+                  "solves_per_s": plain_rate,
+                  "step_p50_ms": 1e3 * float(np.percentile(plain_steps, 50))
+                  if plain_steps else None,
+                  "step_p99_ms": 1e3 * float(np.percentile(plain_steps, 99))
+                  if plain_steps else None},
+        "speedup_vs_plain": round(st["solves_per_s"] / plain_rate, 2)
+        if plain_rate else None,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -314,6 +410,14 @@ def main():
                     help="first run 3 device-to-device copies of MIB MiB "
                          "(calibrates HBM byte counters against a copy of "
                          "known size)")
+    ap.add_argument("--rate", type=float, default=100.0,
+                    help="rve: request arrivals per second")
+    ap.add_argument("--slots", type=int, default=16,
+                    help="rve: most requests on the device at once")
+    ap.add_argument("--nmax", type=int, default=24,
+                    help="rve: largest n of an n^3 request")
+    ap.add_argument("--plain-sample", type=int, default=3,
+                    help="rve: requests timed through the plain path")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not "
@@ -324,6 +428,8 @@ def main():
     if hip.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         return 1
+    if args.workload == "rve":
+        return run_rve(args)
     if args.copy_probe:
         nb = args.copy_probe << 20
         a = hip.DeviceArray(nb, np.uint8)
