@@ -346,8 +346,13 @@ int w3d_reset_phase_times(w3d_ctx* ctx);
  * satisfies |R| <= tol * max(|F_c|, tol_abs), F_c the internal forces on
  * the constrained dofs.  Step 1 starts from du0 (its free entries are the
  * initial guess), every later step from the previous step's converged
- * increment (extrapolation).  Mesh, state, displacements and solver
- * vectors stay on the device.
+ * increment (extrapolation).  With adaptive on, an increment whose
+ * recovery asks for a cut (or that reaches max_iters) is restored and
+ * split into four sub-increments, recursively down to 1/16 of the step,
+ * with the stack, the first-sub-increment restart and the extrapolation
+ * scales of the reference's adaptive solution (mnralg.f, adapt_check.f),
+ * from step 1 on.  Mesh, state, displacements and solver vectors stay on
+ * the device.
  *
  * w3d_batch_advance runs one Newton iteration of every running problem of
  * a batch: each kernel of the iteration is launched once for the whole
@@ -363,9 +368,11 @@ typedef struct w3d_batch w3d_batch;
 
 typedef struct w3d_newton_opts {
   int32_t nsteps;     /* load steps                                     */
-  int32_t max_iters;  /* Newton iterations per step before failing      */
+  int32_t max_iters;  /* Newton iterations per (sub-)increment          */
   int32_t precision;  /* block tangent: W3D_FP64 or W3D_FP32            */
   int32_t pcg_maxit;  /* Jacobi-PCG iterations per solve                */
+  int32_t adaptive;   /* 1: split failing increments (adapt_check.f)    */
+  int32_t reserved;   /* 0                                              */
   double tol;         /* relative residual of convergence               */
   double tol_abs;     /* floor of the reaction norm in the test         */
   double pcg_rtol;    /* |r| <= pcg_rtol |b| ends a linear solve        */
@@ -386,13 +393,14 @@ int w3d_problem_destroy(w3d_problem* p);
 /* status (W3D_PROBLEM_*), current step (1-based) and Newton iteration */
 int w3d_problem_status(const w3d_problem* p, int32_t* status, int32_t* step,
                        int32_t* iter);
-/* Record of converged step (1-based): Newton iterations, Jacobi-PCG
- * iterations summed over them, the sum of the internal forces on the
- * loaded dofs (constrained dofs with a nonzero increment), and the last
- * residual and reaction norms. */
+/* Record of converged step (1-based): Newton iterations (all attempts),
+ * Jacobi-PCG iterations summed over them, adaptive cuts, the sum of the
+ * internal forces on the loaded dofs (constrained dofs with a nonzero
+ * increment), and the last residual and reaction norms. */
 int w3d_problem_step_record(const w3d_problem* p, int32_t step,
                             int32_t* newton_iters, int32_t* pcg_iters,
-                            double* load, double* residual, double* reaction);
+                            int32_t* cuts, double* load, double* residual,
+                            double* reaction);
 /* displacements at the end of the last converged step (3*nonode) */
 int w3d_problem_get_u(const w3d_problem* p, double* u);
 /* device bytes the problem's context and vectors hold */

@@ -73,7 +73,8 @@ int main(void) {
   P(w3d_model, elblks); P(w3d_model, auto_size); P(w3d_model, eps_bbar);
   printf("w3d_model.size %zu\n", sizeof(w3d_model));
   P(w3d_newton_opts, nsteps); P(w3d_newton_opts, precision);
-  P(w3d_newton_opts, pcg_maxit); P(w3d_newton_opts, tol);
+  P(w3d_newton_opts, pcg_maxit); P(w3d_newton_opts, adaptive);
+  P(w3d_newton_opts, tol);
   P(w3d_newton_opts, tol_abs); P(w3d_newton_opts, pcg_rtol);
   printf("w3d_newton_opts.size %zu\n", sizeof(w3d_newton_opts));
   return 0;

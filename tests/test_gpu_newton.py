@@ -46,11 +46,14 @@ def c1():
 TIGHT = 1e-5  # cvtest test-2 tolerance (%) of the FP32 comparison
 
 
-def run_drive(tmp_path, arr, cst, steps, precision, res_tol=0.01):
+def run_drive(tmp_path, arr, cst, steps, precision, res_tol=0.01, **kw):
+    """The deck and the driver; without keywords the Newton options of the
+    oracle reference's default path (no extrapolation)."""
+    kw.setdefault("extrapolate", False)
     deck = os.path.join(tmp_path, f"c1_{precision}.deck")
     res = os.path.join(tmp_path, f"c1_{precision}.res")
     D.write_deck(deck, arr, cst, np.array(steps), precision=precision,
-                 pcg_rtol=1e-13, res_tol=res_tol)
+                 pcg_rtol=1e-13, res_tol=res_tol, **kw)
     out, steps_out = D.run(deck, res)
     print(out, flush=True)
     return steps_out
@@ -60,7 +63,7 @@ def test_fortran_driver_fp64_matches_oracle(c1, tmp_path):
     m, arr, cst, steps, ref = c1
     got = run_drive(str(tmp_path), arr, cst, steps, _abi.W3D_FP64)
     assert len(got) == len(ref)
-    for s, (g, (conv, it, ratios, u)) in enumerate(zip(got, ref), start=1):
+    for s, (g, (conv, it, ratios, u, _)) in enumerate(zip(got, ref), start=1):
         assert g["converged"] and g["status"] == 0, s
         assert g["iters"] == it, (s, g["iters"], it)
         err = np.abs(g["u"] - u).max() / np.abs(u).max()
@@ -75,8 +78,72 @@ def test_fortran_driver_fp32_tangent(c1, tmp_path):
     got = run_drive(str(tmp_path), arr, cst, steps, _abi.W3D_FP32,
                     res_tol=TIGHT)
     assert len(got) == len(ref)
-    for s, (g, (conv, it, ratios, u)) in enumerate(zip(got, ref), start=1):
+    for s, (g, (conv, it, ratios, u, _)) in enumerate(zip(got, ref), start=1):
         assert g["converged"], s
         assert g["iters"] <= it + 1, (s, g["iters"], it)
         err = np.abs(g["u"] - u).max() / np.abs(u).max()
         assert err <= 1e-7, (s, err)
+
+
+def compare(got, ref, tol=1e-8):
+    assert len(got) == len(ref)
+    for s, (g, (conv, it, ratios, u, info)) in enumerate(zip(got, ref),
+                                                         start=1):
+        assert g["converged"] == conv, s
+        assert g["iters"] == it, (s, g["iters"], it)
+        assert g["cuts"] == info["cuts"], (s, g["cuts"], info["cuts"])
+        err = np.abs(g["u"] - u).max() / np.abs(u).max()
+        assert err <= tol, (s, err)
+
+
+def test_fortran_driver_extrapolation_and_line_search(c1, tmp_path):
+    """mnralg's default extrapolation and the mnralg_ls line search."""
+    m, arr, cst, steps, _ = c1
+    opts = NR.NewtonOptions(extrapolate=True, line_search=True)
+    ref = NR.run_history(arr, cst, steps, opts=opts)
+    got = run_drive(str(tmp_path), arr, cst, steps, _abi.W3D_FP64,
+                    extrapolate=True, line_search=True)
+    compare(got, ref)
+
+
+def test_fortran_driver_adaptive_cuts(tmp_path):
+    """Adaptive sub-increments (adapt_check.f), nested down to 1/16, on an
+    mm10 cube whose 0.4 % steps the crystal update and the iteration limit
+    cannot take whole."""
+    m = M.box_hex8(3, jitter=0.05, seed=5)
+    cp = _abi.mm10_material(tau_y=50.0, tau_v=100.0, theta_0=250.0,
+                            e=66.7e3, nu=0.42, mu=75e3, elastic="cubic")
+    arr = m.model(cp, auto_size=16,
+                  angles=M.random_orientations(m.noelem, seed=6))
+    cst, top, _ = M.uniaxial_bc(m, 0.001)
+    d1 = np.zeros(3 * m.nonode)
+    d1[top] = 0.001
+    d2 = np.zeros(3 * m.nonode)
+    d2[top] = 0.004
+    steps = [d1, d2, d2]
+    opts = NR.NewtonOptions(extrapolate=True, adaptive=True, mxiter=10)
+    ref = NR.run_history(arr, cst, steps, opts=opts, mfs=[1.0, 4.0, 4.0])
+    assert all(h[0] for h in ref) and sum(h[4]["cuts"] for h in ref) >= 4
+    got = run_drive(str(tmp_path), arr, cst, steps, _abi.W3D_FP64,
+                    extrapolate=True, adaptive=True, mxiter=10,
+                    mfs=[1.0, 4.0, 4.0])
+    compare(got, ref)
+
+
+def test_fortran_driver_load_control(tmp_path):
+    """External nodal loads (load control, pbar of uppbar.f) with
+    extrapolation, against the oracle."""
+    m = M.box_hex8(3, jitter=0.05, seed=9)
+    mat = _abi.mm01_material(200e3, 0.3, 400.0, hprime=2000.0)
+    arr = m.model(mat, auto_size=16)
+    cst, top, _ = M.uniaxial_bc(m, 0.0)
+    cst[top] = 0
+    dl = np.zeros(3 * m.nonode)
+    dl[top] = 120.0 / top.sum()
+    steps = [np.zeros(3 * m.nonode)] * 6
+    opts = NR.NewtonOptions(extrapolate=True)
+    ref = NR.run_history(arr, cst, steps, opts=opts, dloads=[dl] * 6)
+    assert all(h[0] for h in ref)
+    got = run_drive(str(tmp_path), arr, cst, steps, _abi.W3D_FP64,
+                    extrapolate=True, dloads=[dl] * 6)
+    compare(got, ref)

@@ -31,7 +31,7 @@ def test_newton_uniaxial_bar_matches_closed_form():
     hp = float(O.hprime(np.float32(200e3), mat.tan_e))
     om = O.OracleModel(arr)
     om.pattern(cst)
-    for s, (conv, it, ratios, u) in enumerate(hist, start=1):
+    for s, (conv, it, ratios, u, _) in enumerate(hist, start=1):
         assert conv
         # reaction at the loaded end: internal forces of the top dofs
         ifv, _, _, _ = om.recover(1, 1, np.zeros_like(u), u)  # one-shot
@@ -40,3 +40,135 @@ def test_newton_uniaxial_bar_matches_closed_form():
                                                           ref)
         # Newton iterations: 1 while elastic, 2 once plastic
         assert it == (1 if e * s * de <= 400.0 else 2)
+
+
+def test_line_search_decisions():
+    """mnralg_ls: accept the first length whose |s/s0| < 0.5; back to the
+    full step when r stops falling by 0.9 or the length drops below the
+    minimum."""
+    opts = NR.NewtonOptions(line_search=True)
+
+    def run(rfun):
+        seen = []
+
+        def ev(a):
+            seen.append(round(a, 12))
+            return rfun(a)
+        return NR.line_search(ev, opts), seen
+
+    # accepted at once
+    a, seen = run(lambda a: 0.1)
+    assert a == 1.0 and seen == [1.0]
+    # backtracks by 0.7 until r < 0.5
+    a, seen = run(lambda a: {1.0: 0.9, 0.7: 0.6}.get(round(a, 12), 0.3))
+    assert seen == [1.0, 0.7, 0.49] and abs(a - 0.49) < 1e-12
+    # r not falling by 10 %: full step again
+    a, seen = run(lambda a: {1.0: 0.9, 0.7: 0.85}.get(round(a, 12), 0.0))
+    assert seen == [1.0, 0.7, 1.0] and a == 1.0
+    # falling by more than 10 % each time but never below 0.5 (r may
+    # exceed 1): down past the minimum length, then the full step
+    vals = iter(100.0 * 0.85 ** np.arange(40))
+    a, seen = run(lambda a: next(vals))
+    assert a == 1.0 and seen[-1] == 1.0
+    assert len(seen) == 14 and abs(seen[-2] - 0.7 ** 12) < 1e-12
+    assert 0.7 ** 13 < 0.01 <= 0.7 ** 12
+
+
+def test_adaptive_stack_sequences():
+    """adapt_check: one cut gives four quarter increments, the first
+    without extrapolation scale, the others at scale 1, and the step ends
+    after the fourth; a cut inside the third gives sixteenths, then the
+    fourth quarter with scale 4; a sixteenth cannot be cut."""
+    ad = NR.Adaptive()
+    assert ad.cut()
+    seq = [(ad.stack[-1][0], ad.load_fact, ad.disp_fact, ad.scaling)]
+    while ad.advance():
+        seq.append((ad.stack[-1][0], ad.load_fact, ad.disp_fact, ad.scaling))
+    assert seq == [(0.0, 0.25, 0.25, 0.25), (0.25, 0.5, 0.25, 1.0),
+                   (0.5, 0.75, 0.25, 1.0), (0.75, 1.0, 0.25, 1.0)]
+    # nested
+    ad = NR.Adaptive()
+    ad.cut()
+    ad.advance()
+    ad.advance()                    # now on [0.5, 0.75]
+    assert ad.stack[-1][:2] == [0.5, 0.75]
+    assert ad.cut()
+    seq = [(ad.stack[-1][0], ad.load_fact, ad.disp_fact)]
+    scales = [ad.scaling]
+    while ad.advance():
+        seq.append((ad.stack[-1][0], ad.load_fact, ad.disp_fact))
+        scales.append(ad.scaling)
+    assert [x[0] for x in seq] == [0.5, 0.5625, 0.625, 0.6875, 0.75]
+    assert [x[2] for x in seq] == [0.0625] * 4 + [0.25]
+    assert scales == [0.25, 1.0, 1.0, 1.0, 4.0]
+    # the smallest increment cannot be cut
+    ad = NR.Adaptive()
+    ad.cut()
+    assert ad.cut()
+    assert ad.disp_fact == 0.0625 and not ad.cut()
+
+
+def test_extrapolation_keeps_the_closed_form_and_saves_iterations():
+    m = M.box_hex8(4, jitter=0.1, seed=11)
+    mat = _abi.mm01_material(200e3, 0.3, 400.0, hprime=2000.0)
+    arr = m.model(mat, auto_size=16)
+    de = 0.0008
+    cst, top, val = M.uniaxial_bc(m, de)
+    du = np.zeros(3 * m.nonode)
+    du[top] = val
+    off = NR.run_history(arr, cst, [du] * 6)
+    on = NR.run_history(arr, cst, [du] * 6,
+                        opts=NR.NewtonOptions(extrapolate=True))
+    e = float(np.float32(200e3))
+    hp = float(O.hprime(np.float32(200e3), mat.tan_e))
+    om = O.OracleModel(arr)
+    om.pattern(cst)
+    for s, ((c0, i0, _, u0, _), (c1, i1, _, u1, _)) in enumerate(
+            zip(off, on), start=1):
+        assert c0 and c1
+        ifv, _, _, _ = om.recover(1, 1, np.zeros_like(u1), u1)
+        ref = uniaxial_stress(s * de, e, 400.0, hp)
+        assert abs(ifv[top].sum() - ref) <= 1e-9 * ref
+        assert i1 <= i0
+    # the elastic second step is exact from the extrapolated increment
+    assert on[1][1] == 1 and on[1][2][0] < 1e-10
+
+
+def test_load_control_matches_the_closed_form():
+    """Pressure-like nodal loads on the top face (load control): the bar
+    stays homogeneous, so u_z(top) = eps(sigma) and the Newton loop
+    converges on the bilinear curve in both regimes."""
+    m = M.box_hex8(3)
+    mat = _abi.mm01_material(200e3, 0.3, 400.0, hprime=2000.0)
+    arr = m.model(mat, auto_size=16)
+    cst, top, _ = M.uniaxial_bc(m, 0.0)
+    cst[top] = 0                   # the top face is free, loaded instead
+    x = m.coords
+    # consistent nodal forces of a unit traction on the top face of a
+    # uniform grid of n x n squares: corner 1/4, edge 1/2, interior 1 of
+    # one square's quarter share
+    n = 3
+    h = 1.0 / n
+    topn = np.flatnonzero(np.isclose(x[:, 2], 1.0))
+    w = np.zeros(m.nonode)
+    for k in topn:
+        cx = 1 if np.isclose(x[k, 0], 0) or np.isclose(x[k, 0], 1) else 2
+        cy = 1 if np.isclose(x[k, 1], 0) or np.isclose(x[k, 1], 1) else 2
+        w[k] = cx * cy * h * h / 4.0
+    dsig = 100.0
+    dl = np.zeros(3 * m.nonode)
+    dl[2::3] = dsig * w
+    assert abs(dl.sum() - dsig) < 1e-12
+    hist = NR.run_history(arr, cst, [np.zeros(3 * m.nonode)] * 6,
+                          opts=NR.NewtonOptions(extrapolate=True,
+                                                res_tol=1e-6),
+                          dloads=[dl] * 6)
+    e = float(np.float32(200e3))
+    hp = float(O.hprime(np.float32(200e3), mat.tan_e))
+    for s, (conv, it, _, u, _) in enumerate(hist, start=1):
+        assert conv, s
+        sig = s * dsig
+        eps = sig / e if sig <= 400.0 else \
+            400.0 / e + (sig - 400.0) * (1.0 / hp + 1.0 / e)
+        uz = u[2::3][topn]
+        assert np.allclose(uz, eps, rtol=1e-7), (s, uz.mean(), eps)

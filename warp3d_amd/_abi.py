@@ -207,6 +207,7 @@ class W3DNewtonOpts(C.Structure):
     """w3d_newton_opts: Newton and solver controls of a served problem."""
     _fields_ = [("nsteps", C.c_int32), ("max_iters", C.c_int32),
                 ("precision", C.c_int32), ("pcg_maxit", C.c_int32),
+                ("adaptive", C.c_int32), ("reserved", C.c_int32),
                 ("tol", C.c_double), ("tol_abs", C.c_double),
                 ("pcg_rtol", C.c_double)]
 
@@ -238,7 +239,7 @@ SERVE_PROTOS = {
                             C.POINTER(_P)]),
     "problem_destroy": (_I, [_P]),
     "problem_status": (_I, [_P, _IP, _IP, _IP]),
-    "problem_step_record": (_I, [_P, _I, _IP, _IP, _D, _D, _D]),
+    "problem_step_record": (_I, [_P, _I, _IP, _IP, _IP, _D, _D, _D]),
     "problem_get_u": (_I, [_P, _D]),
     "problem_device_bytes": (_I, [_P, _LP]),
     "batch_create": (_I, [_I, C.POINTER(_P)]),

@@ -676,6 +676,19 @@ int enqueue_commit(w3d_ctx* c) {
   return W3D_OK;
 }
 
+int enqueue_rollback(w3d_ctx* c) {
+  HIPCHK(hipMemcpyAsync(c->d_stress_n1, c->d_stress_n, 9 * c->nq * 8,
+                        hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_strain_n1, c->d_strain_n, 6 * c->nq * 8,
+                        hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->d_hist_n1, c->d_hist_n, c->nhist * 8,
+                        hipMemcpyDeviceToDevice, c->stream));
+  if (c->has_geonl)  // adaptive restart (mnralg.f:1393)
+    HIPCHK(hipMemcpyAsync(c->d_rot_n1, c->d_rot_n, 9 * c->nq * 8,
+                          hipMemcpyDeviceToDevice, c->stream));
+  return W3D_OK;
+}
+
 int enqueue_csr_values(w3d_ctx* c, const double* d_diag,
                        const double* d_coef) {
   int rc = solver_csr_values(d_diag, d_coef, c->neqns, c->d_csr_vidx,
@@ -1347,15 +1360,8 @@ int w3d_commit_step(w3d_ctx* c) {
 int w3d_rollback(w3d_ctx* c) {
   if (!c) return fail(W3D_ERR_ARG, "null context");
   HIPCHK(hipSetDevice(c->device));
-  HIPCHK(hipMemcpyAsync(c->d_stress_n1, c->d_stress_n, 9 * c->nq * 8,
-                        hipMemcpyDeviceToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(c->d_strain_n1, c->d_strain_n, 6 * c->nq * 8,
-                        hipMemcpyDeviceToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(c->d_hist_n1, c->d_hist_n, c->nhist * 8,
-                        hipMemcpyDeviceToDevice, c->stream));
-  if (c->has_geonl)  // adaptive restart (mnralg.f:1393)
-    HIPCHK(hipMemcpyAsync(c->d_rot_n1, c->d_rot_n, 9 * c->nq * 8,
-                          hipMemcpyDeviceToDevice, c->stream));
+  int rc = enqueue_rollback(c);
+  if (rc) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
   return W3D_OK;
 }

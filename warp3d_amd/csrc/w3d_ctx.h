@@ -226,8 +226,10 @@ int enqueue_recover_stage(w3d_ctx* c, const RecoverLists& L, int stage,
 int enqueue_recover(w3d_ctx* c, const RecoverLists& L, int step, int iter,
                     int extrapolated, const double* d_u, const double* d_du,
                     double* d_ifv, bool sums, int first_blk, int nblk);
-// The [n+1] -> [n] state copies of w3d_commit_step, enqueued only.
+// The [n+1] -> [n] state copies of w3d_commit_step, and the [n] ->
+// [n+1] copies of w3d_rollback, enqueued only.
 int enqueue_commit(w3d_ctx* c);
+int enqueue_rollback(w3d_ctx* c);
 // CSR values of the solver from an assembly (diag, coef) on the stream.
 int enqueue_csr_values(w3d_ctx* c, const double* d_diag, const double* d_coef);
 

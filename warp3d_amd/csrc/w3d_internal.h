@@ -268,6 +268,12 @@ int launch_du_batch(const ResSys* d_sys, int nsys, int ntile_total,
                     hipStream_t s);
 // u += du (n words)
 int launch_axpy1(int64_t n, double* u, const double* du, hipStream_t s);
+// the start of a load increment: du *= sf on the equations' dofs, and
+// du = disp * cimp on the constrained dofs (cimp: the full step's imposed
+// increments, in cdof order)
+int launch_start_increment(const int32_t* eqdof, int neq, const int32_t* cdof,
+                           const double* cimp, int ncdof, double sf,
+                           double disp, double* du, hipStream_t s);
 
 // Host set-up of one crystal record from the material and its angles
 // (w3d_mm10_setup.cpp); false for an unsupported slip / elastic type.

@@ -29,6 +29,66 @@
 
 using namespace w3d;
 
+// adapt_check.f's stack of sub-increments: rows (start, end, increment,
+// extrapolation scale), the current one at row level - 1
+struct AdaptStack {
+  static constexpr int kDiv = 4, kRows = 64;
+  double st[kRows][4] = {};
+  int level = 1;
+  double scaling = 1.0, disp = 1.0;
+  // process type 3: a new step
+  void init() {
+    st[0][0] = 0.0;
+    st[0][1] = 1.0;
+    st[0][2] = 1.0;
+    st[0][3] = 1.0;
+    level = 1;
+    scaling = 1.0;
+    disp = 1.0;
+  }
+  // process type 1: split the current increment (false: already 1/16)
+  bool cut() {
+    if (disp <= 1.0 / (kDiv * kDiv) || level + kDiv > kRows) return false;
+    const int lv = level - 1, top = lv + kDiv;
+    const double a = st[lv][0], b = st[lv][1], inc = (b - a) / kDiv;
+    double start = a;
+    for (int i = top; i >= lv + 1; --i) {
+      st[i][0] = start;
+      st[i][1] = start + inc;
+      st[i][2] = inc;
+      start = start + inc;
+    }
+    st[top][3] = st[lv][3] / kDiv;
+    st[lv][3] = st[lv][3] / st[top][3];
+    for (int i = lv + 1; i < top; ++i) st[i][3] = 1.0;
+    level = top + 1;
+    scaling = st[top][3];
+    disp = st[top][2];
+    return true;
+  }
+  // process type 2: after a converged sub-increment (true: another
+  // follows)
+  bool next() {
+    if (level <= 1) return false;
+    if (st[level - 1][1] == st[level - 2][1]) {
+      level -= 2;
+      if (level == 0) {
+        scaling = st[0][3];
+        level = 1;
+        return false;
+      }
+      st[level - 1][3] = kDiv * st[level - 1][3];
+      scaling = st[level - 1][3];
+    } else {
+      level -= 1;
+    }
+    if (level <= 1) return false;
+    scaling = st[level - 1][3];
+    disp = st[level - 1][2];
+    return true;
+  }
+};
+
 struct w3d_problem {
   w3d_ctx* c = nullptr;
   w3d_newton_opts o{};
@@ -38,15 +98,18 @@ struct w3d_problem {
   double* d_k = nullptr;  // assembled diagonal, then coefficients
   double *d_rhs = nullptr, *d_x = nullptr, *d_r = nullptr, *d_p = nullptr,
          *d_q = nullptr;
+  double* d_cimp = nullptr;  // a full step's imposed increments (cdof order)
   int32_t *d_eqdof = nullptr, *d_cdof = nullptr, *d_ldof = nullptr;
   int64_t bytes = 0;
   int status = W3D_PROBLEM_RUNNING;
   int step = 1, iter = 0;
-  bool pending_commit = false;
-  int pcg_iters_step = 0;
+  // what the next advance does before its recovery
+  enum Pending { kNone, kCommitStep, kCommitSub } pending = kNone;
+  AdaptStack ad;
+  int iters_step = 0, pcg_iters_step = 0, cuts_step = 0;
   double res = 0.0, react = 0.0;
   struct Record {
-    int newton = 0, pcg = 0;
+    int newton = 0, pcg = 0, cuts = 0;
     double load = 0.0, res = 0.0, react = 0.0;
   };
   std::vector<Record> records;
@@ -75,7 +138,16 @@ namespace {
 void free_problem(w3d_problem* p) {
   dfree(p->d_u); dfree(p->d_du); dfree(p->d_ifv); dfree(p->d_k);
   dfree(p->d_rhs); dfree(p->d_x); dfree(p->d_r); dfree(p->d_p); dfree(p->d_q);
+  dfree(p->d_cimp);
   dfree(p->d_eqdof); dfree(p->d_cdof); dfree(p->d_ldof);
+}
+
+// du for the start of an increment: the free entries scaled by sf (0: no
+// extrapolation), the constrained ones the imposed values times disp
+int start_increment(w3d_problem* p, double sf, double disp, hipStream_t s) {
+  int rc = launch_start_increment(p->d_eqdof, p->neq, p->d_cdof, p->d_cimp,
+                                  p->ncdof, sf, disp, p->d_du, s);
+  return rc ? fail(rc, "w3d_batch_advance: increment launch") : W3D_OK;
 }
 
 template <typename T>
@@ -154,7 +226,8 @@ int w3d_problem_create(w3d_ctx* c, const int32_t* cstmap, const double* du0,
   *out = nullptr;
   if (o->nsteps < 1 || o->max_iters < 1 || o->pcg_maxit < 1 ||
       !(o->tol > 0.0) || !(o->tol_abs >= 0.0) || !(o->pcg_rtol > 0.0) ||
-      (o->precision != W3D_FP64 && o->precision != W3D_FP32))
+      (o->precision != W3D_FP64 && o->precision != W3D_FP32) ||
+      (o->adaptive != 0 && o->adaptive != 1))
     return fail(W3D_ERR_ARG, "w3d_problem_create: bad options");
   HIPCHK(hipSetDevice(c->device));
   int32_t neq = 0;
@@ -167,14 +240,17 @@ int w3d_problem_create(w3d_ctx* c, const int32_t* cstmap, const double* du0,
   p->neq = neq;
   p->nd = 3 * (int64_t)c->nonode;
   std::vector<int32_t> eqdof(std::max(neq, 1)), cdof, ldof;
+  std::vector<double> cimp;
   for (int64_t d = 0; d < p->nd; ++d) {
     if (cstmap[d] == 0) {
       eqdof[c->dof_eqn_map[d] - 1] = (int32_t)d;
     } else {
       cdof.push_back((int32_t)d);
+      cimp.push_back(du0[d]);
       if (du0[d] != 0.0) ldof.push_back((int32_t)d);
     }
   }
+  p->ad.init();
   p->ncdof = (int)cdof.size();
   p->nldof = (int)ldof.size();
   const int64_t b0 = t_dalloc_bytes;
@@ -184,7 +260,8 @@ int w3d_problem_create(w3d_ctx* c, const int32_t* cstmap, const double* du0,
       (rc = dalloc(&p->d_k, neq + (size_t)nco)) ||
       (rc = dalloc(&p->d_rhs, neq)) || (rc = dalloc(&p->d_x, neq)) ||
       (rc = dalloc(&p->d_r, neq)) || (rc = dalloc(&p->d_p, neq)) ||
-      (rc = dalloc(&p->d_q, neq)) || (rc = upload(&p->d_eqdof, eqdof)) ||
+      (rc = dalloc(&p->d_q, neq)) || (rc = upload(&p->d_cimp, cimp)) ||
+      (rc = upload(&p->d_eqdof, eqdof)) ||
       (rc = upload(&p->d_cdof, cdof)) || (rc = upload(&p->d_ldof, ldof))) {
     free_problem(p);
     delete p;
@@ -220,13 +297,15 @@ int w3d_problem_status(const w3d_problem* p, int32_t* status, int32_t* step,
 
 int w3d_problem_step_record(const w3d_problem* p, int32_t step,
                             int32_t* newton_iters, int32_t* pcg_iters,
-                            double* load, double* residual, double* reaction) {
+                            int32_t* cuts, double* load, double* residual,
+                            double* reaction) {
   if (!p) return fail(W3D_ERR_ARG, "w3d_problem_step_record: null problem");
   if (step < 1 || step > (int32_t)p->records.size())
     return fail(W3D_ERR_ARG, "w3d_problem_step_record: step not converged");
   const w3d_problem::Record& r = p->records[step - 1];
   if (newton_iters) *newton_iters = r.newton;
   if (pcg_iters) *pcg_iters = r.pcg;
+  if (cuts) *cuts = r.cuts;
   if (load) *load = r.load;
   if (residual) *residual = r.res;
   if (reaction) *reaction = r.react;
@@ -237,7 +316,7 @@ int w3d_problem_get_u(const w3d_problem* p, double* u) {
   if (!p || !u) return fail(W3D_ERR_ARG, "w3d_problem_get_u: null argument");
   HIPCHK(hipSetDevice(p->c->device));
   HIPCHK(hipMemcpy(u, p->d_u, p->nd * sizeof(double), hipMemcpyDeviceToHost));
-  if (p->pending_commit) {  // the converged step's increment is not in u yet
+  if (p->pending != w3d_problem::kNone) {  // the increment is not in u yet
     std::vector<double> du(p->nd);
     HIPCHK(hipMemcpy(du.data(), p->d_du, p->nd * sizeof(double),
                      hipMemcpyDeviceToHost));
@@ -313,19 +392,29 @@ int w3d_batch_advance(w3d_batch* b, w3d_problem* const* ps, int32_t n,
   StreamSwap swap;
   for (w3d_problem* p : run) swap.add(p->c, s);
   int rc = W3D_OK;
-  // 1. step starts
+  // 1. commits and the starts of steps and sub-increments
   for (w3d_problem* p : run) {
-    if (p->pending_commit) {
+    if (p->pending != w3d_problem::kNone) {
       if ((rc = enqueue_commit(p->c)) ||
           (rc = launch_axpy1(p->nd, p->d_u, p->d_du, s)))
         return fail(rc ? rc : W3D_ERR_DEVICE, "w3d_batch_advance: commit");
-      p->pending_commit = false;
-      p->step += 1;
+      if (p->pending == w3d_problem::kCommitStep) {
+        // a new step extrapolates the last converged increment unscaled
+        // (the adaptive stack restarts at 1)
+        p->step += 1;
+        p->ad.init();
+        p->iters_step = p->pcg_iters_step = p->cuts_step = 0;
+        rc = start_increment(p, 1.0, 1.0, s);
+      } else {
+        rc = start_increment(p, p->ad.scaling, p->ad.disp, s);
+      }
+      if (rc) return rc;
+      p->pending = w3d_problem::kNone;
       p->iter = 0;
-      p->pcg_iters_step = 0;
     }
     p->status = W3D_PROBLEM_RUNNING;
     p->iter += 1;
+    p->iters_step += 1;
   }
   // residual tiles (also the update's)
   int ntile_res = 0;
@@ -405,15 +494,22 @@ int w3d_batch_advance(w3d_batch* b, w3d_problem* const* ps, int32_t n,
     const double* o = b->h_out + 8 * k;
     p->res = std::sqrt(o[0]);
     p->react = std::sqrt(o[1]);
-    // flags: [0] negative plastic multiplier, [1] cut step, [3] det F <= 0
-    if (o[3] != 0.0 || o[4] != 0.0 || o[6] != 0.0) {
+    // flags: [0] negative plastic multiplier (fatal), [1] a material asks
+    // for a cut, [3] det F <= 0
+    if (o[3] != 0.0) {
       p->status = W3D_PROBLEM_FAILED;
       continue;
     }
-    if (p->res <= p->o.tol * std::max(p->react, p->o.tol_abs)) {
+    const bool cut = o[4] != 0.0 || o[6] != 0.0;
+    if (!cut && p->res <= p->o.tol * std::max(p->react, p->o.tol_abs)) {
+      if (p->o.adaptive && p->ad.next()) {
+        p->pending = w3d_problem::kCommitSub;  // the next sub-increment
+        continue;
+      }
       w3d_problem::Record r;
-      r.newton = p->iter;
+      r.newton = p->iters_step;
       r.pcg = p->pcg_iters_step;
+      r.cuts = p->cuts_step;
       r.load = o[2];
       r.res = p->res;
       r.react = p->react;
@@ -424,13 +520,22 @@ int w3d_batch_advance(w3d_batch* b, w3d_problem* const* ps, int32_t n,
           return fail(rc ? rc : W3D_ERR_DEVICE, "w3d_batch_advance: commit");
         p->status = W3D_PROBLEM_DONE;
       } else {
-        p->pending_commit = true;
+        p->pending = w3d_problem::kCommitStep;
         p->status = W3D_PROBLEM_STEP_DONE;
       }
       continue;
     }
-    if (p->iter >= p->o.max_iters) {
-      p->status = W3D_PROBLEM_FAILED;
+    if (cut || p->iter >= p->o.max_iters) {
+      // restore n and restart on the first sub-increment of a split,
+      // without extrapolation
+      if (p->o.adaptive && p->ad.cut()) {
+        if ((rc = enqueue_rollback(p->c))) return rc;
+        if ((rc = start_increment(p, 0.0, p->ad.disp, s))) return rc;
+        p->cuts_step += 1;
+        p->iter = 0;
+      } else {
+        p->status = W3D_PROBLEM_FAILED;
+      }
       continue;
     }
     solve.push_back(k);

@@ -407,6 +407,29 @@ __global__ void k_axpy1(int64_t n, double* __restrict__ u,
   if (i < n) u[i] = u[i] + du[i];
 }
 
+__global__ void k_start_increment(const int32_t* __restrict__ eqdof, int neq,
+                                  const int32_t* __restrict__ cdof,
+                                  const double* __restrict__ cimp, int ncdof,
+                                  double sf, double disp,
+                                  double* __restrict__ du) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < neq) {
+    const int d = eqdof[k];
+    du[d] = sf * du[d];
+  }
+  if (k < ncdof) du[cdof[k]] = disp * cimp[k];
+}
+
+int launch_start_increment(const int32_t* eqdof, int neq, const int32_t* cdof,
+                           const double* cimp, int ncdof, double sf,
+                           double disp, double* du, hipStream_t s) {
+  const int n = neq > ncdof ? neq : ncdof;
+  if (n > 0)
+    hipLaunchKernelGGL(k_start_increment, dim3((n + 255) / 256), dim3(256), 0,
+                       s, eqdof, neq, cdof, cimp, ncdof, sf, disp, du);
+  return hipGetLastError() == hipSuccess ? W3D_OK : W3D_ERR_DEVICE;
+}
+
 int launch_residual_batch(const ResSys* d_sys, int nsys, int ntile_total,
                           double* part, double* out, hipStream_t s) {
   if (nsys <= 0) return W3D_OK;

@@ -2,10 +2,12 @@
 (warp3d_amd/bin/w3d_drive, built from warp3d_amd/fortran/w3d_drive.f90).
 
 A deck holds a model (the w3d_model arrays and materials), the absolute
-constraints and the imposed displacement increments of every load step; the
-driver runs each step with the Fortran Newton loop over libw3dblk on the GPU
-and writes, per step, its status, iteration count, PCG iterations, test-2
-residual ratios, energy, plastic work and the displacements after the step.
+constraints, the Newton options and, per load step, the load multiplier,
+the imposed displacement increments and the external nodal load increments;
+the driver runs each step with the Fortran Newton loop over libw3dblk on the
+GPU and writes, per step, its status, iteration counts, adaptive cuts, PCG
+iterations, test-2 residual ratios, energy, plastic work and the
+displacements after the step.
 """
 import os
 import subprocess
@@ -18,10 +20,16 @@ DRIVE = os.path.join(HERE, "bin", "w3d_drive")
 
 def write_deck(path, arrays, cstmap, du_steps, precision=0, res_tol=0.01,
                mniter=1, mxiter=15, pcg_rtol=1e-12, pcg_maxit=50000,
-               dt=1.0):
+               dt=1.0, extrapolate=True, line_search=False, adaptive=False,
+               divergence_check=True, ls_rho=0.7, ls_slack=0.5, ls_min=0.01,
+               ls_max=1.0, dloads=None, mfs=None, mf_changes=None):
     """arrays: warp3d_amd._abi.ModelArrays (single-type connectivity);
     du_steps: (nsteps, 3*nonode) imposed increments (zero off the
-    constrained dofs)."""
+    constrained dofs); dloads: external load increments per step (default
+    none); mfs: load multipliers of the steps (default 1: proportional
+    steps); mf_changes: steps whose load pattern changes.  The Newton
+    options default to the reference's (mnralg: extrapolation on, line
+    search and adaptive off, divergence check on)."""
     nnpe = int(np.diff(arrays.incmap).max()) if arrays.noelem > 1 else \
         arrays.incid.size
     if not np.array_equal(arrays.incmap,
@@ -29,11 +37,20 @@ def write_deck(path, arrays, cstmap, du_steps, precision=0, res_tol=0.01,
         raise ValueError("write_deck: one element type per deck")
     du_steps = np.ascontiguousarray(du_steps, dtype=np.float64)
     nsteps = du_steps.shape[0]
+    nd = 3 * arrays.nonode
+    dloads = (np.zeros((nsteps, nd)) if dloads is None else
+              np.ascontiguousarray(dloads, dtype=np.float64))
+    mfs = np.ones(nsteps) if mfs is None else np.asarray(mfs, float)
+    chg = (np.zeros(nsteps) if mf_changes is None else
+           np.asarray(mf_changes, float))
     with open(path, "wb") as f:
         np.array([arrays.nonode, arrays.noelem, nnpe,
                   len(arrays.materials), nsteps, precision, mniter, mxiter,
-                  arrays.auto_size, pcg_maxit], np.int32).tofile(f)
-        np.array([res_tol, pcg_rtol, dt], np.float64).tofile(f)
+                  arrays.auto_size, pcg_maxit, int(extrapolate),
+                  int(line_search), int(adaptive), int(divergence_check)],
+                 np.int32).tofile(f)
+        np.array([res_tol, pcg_rtol, dt, ls_rho, ls_slack, ls_min, ls_max],
+                 np.float64).tofile(f)
         arrays.coords.astype(np.float64).tofile(f)
         for a in (arrays.incid, arrays.etype, arrays.int_order, arrays.matnum,
                   arrays.bbar):
@@ -53,27 +70,32 @@ def write_deck(path, arrays, cstmap, du_steps, precision=0, res_tol=0.01,
         if has:
             arrays.angles.astype(np.float64).tofile(f)
         np.asarray(cstmap, np.int32).tofile(f)
-        du_steps.tofile(f)
+        for k in range(nsteps):
+            np.array([mfs[k], chg[k]], np.float64).tofile(f)
+            du_steps[k].tofile(f)
+            dloads[k].tofile(f)
 
 
 def read_result(path):
-    """List of per-step dicts (status, converged, iters, pcg_iters, ratio,
-    energy, plastic_work, u)."""
+    """List of per-step dicts (status, converged, iters (all attempts),
+    last_iters, cuts, pcg_iters and ratio of the last sub-increment's
+    iterations, energy, plastic_work, u)."""
     steps = []
     with open(path, "rb") as f:
         nsteps, nodof = np.fromfile(f, np.int32, 2)
         for _ in range(nsteps):
-            head = np.fromfile(f, np.int32, 3)
-            if head.size < 3:
+            head = np.fromfile(f, np.int32, 5)
+            if head.size < 5:
                 break
-            status, conv, iters = (int(x) for x in head)
-            pcg = np.fromfile(f, np.int32, iters)
-            ratio = np.fromfile(f, np.float64, iters)
+            status, conv, iters, last, cuts = (int(x) for x in head)
+            pcg = np.fromfile(f, np.int32, last)
+            ratio = np.fromfile(f, np.float64, last)
             en, pw = np.fromfile(f, np.float64, 2)
             u = np.fromfile(f, np.float64, nodof)
             steps.append(dict(status=status, converged=bool(conv),
-                              iters=iters, pcg_iters=pcg, ratio=ratio,
-                              energy=en, plastic_work=pw, u=u))
+                              iters=iters, last_iters=last, cuts=cuts,
+                              pcg_iters=pcg, ratio=ratio, energy=en,
+                              plastic_work=pw, u=u))
     return steps
 
 

@@ -1,22 +1,25 @@
-! w3d_drive.f90 -- runs a displacement-controlled load history on the GPU
-! with the Fortran Newton driver (module w3d_newton) over libw3dblk.
+! w3d_drive.f90 -- runs a static load history on the GPU with the Fortran
+! Newton driver (module w3d_newton) over libw3dblk.
 !
 !   w3d_drive <deck file> <result file> [device]
 !
 ! Deck (stream, native byte order): int32 nonode, noelem, nnpe, nmat,
-! nsteps, precision, mniter, mxiter, auto_size, pcg_maxit; double res_tol,
-! pcg_rtol, dt; double coords(3*nonode); int32 incid(nnpe*noelem),
-! etype(noelem), int_order(noelem), matnum(noelem), bbar(noelem); per
-! material: int32 model, linear_elastic, slip_type, elastic_type, miter,
-! alter_mode, angle_type, ncrystals; real32 e, nu, beta, tan_e, yld_pt;
-! double cp_e, cp_nu, cp_mu, harden_n, theta_0, tau_y, tau_v, voce_m, iD_v,
-! eps_dot_0_y, atol, atol1, rtol, rtol1; int32 has_angles, angle_stride and
-! (if has_angles) double angles(3*angle_stride*noelem); int32
-! cstmap(3*nonode); then per step double du_imposed(3*nonode).
+! nsteps, precision, mniter, mxiter, auto_size, pcg_maxit, extrapolate,
+! line_search, adaptive, divergence_check; double res_tol, pcg_rtol, dt,
+! ls_rho, ls_slack, ls_min, ls_max; double coords(3*nonode); int32
+! incid(nnpe*noelem), etype(noelem), int_order(noelem), matnum(noelem),
+! bbar(noelem); per material: int32 model, linear_elastic, slip_type,
+! elastic_type, miter, alter_mode, angle_type, ncrystals; real32 e, nu,
+! beta, tan_e, yld_pt; double cp_e, cp_nu, cp_mu, harden_n, theta_0, tau_y,
+! tau_v, voce_m, iD_v, eps_dot_0_y, atol, atol1, rtol, rtol1; int32
+! has_angles, angle_stride and (if has_angles) double
+! angles(3*angle_stride*noelem); int32 cstmap(3*nonode); then per step
+! double mf (the step's load multiplier), mf_change (1: the load pattern
+! changed), du_imposed(3*nonode), dload(3*nonode).
 !
 ! Result (stream): int32 nsteps, nodof; per step int32 status, converged,
-! iters, pcg_iters(iters); double ratio(iters), energy, plastic_work,
-! u(nodof) after the step.
+! iters (all attempts), last_iters, cuts, pcg_iters(last_iters); double
+! ratio(last_iters), energy, plastic_work, u(nodof) after the step.
 program w3d_drive
   use, intrinsic :: iso_c_binding
   use w3d
@@ -28,8 +31,10 @@ program w3d_drive
   integer(c_int32_t) :: has_angles, neqns, rc, ival(8)
   integer(c_int64_t) :: ncoeff
   real(c_float) :: fval(5)
-  real(c_double) :: dval(14), dt
-  real(c_double), allocatable, target :: coords(:), angles(:), u(:), dui(:)
+  real(c_double) :: dval(14), dt, ls(4), mfv(2), mf_prev, sload
+  integer(c_int32_t) :: flags(4)
+  real(c_double), allocatable, target :: coords(:), angles(:), u(:), dui(:), &
+       du(:), load_n(:), dload(:)
   integer(c_int32_t), allocatable, target :: incid(:), incmap(:), etype(:), &
        order(:), matnum(:), bbar(:), cstmap(:), dmap(:)
   type(w3d_material), allocatable, target :: mats(:)
@@ -49,12 +54,21 @@ program w3d_drive
        status='old', iostat=ios)
   if (ios /= 0) stop 'w3d_drive: cannot open the deck'
   read(fu) nonode, noelem, nnpe, nmat, nsteps, opts%precision, &
-       opts%mniter, opts%mxiter, model%auto_size, opts%pcg_maxit
-  read(fu) opts%res_tol, opts%pcg_rtol, dt
+       opts%mniter, opts%mxiter, model%auto_size, opts%pcg_maxit, flags
+  opts%extrapolate = flags(1) /= 0
+  opts%line_search = flags(2) /= 0
+  opts%adaptive = flags(3) /= 0
+  opts%divergence_check = flags(4) /= 0
+  read(fu) opts%res_tol, opts%pcg_rtol, dt, ls
+  opts%ls_rho = ls(1)
+  opts%ls_slack = ls(2)
+  opts%ls_min = ls(3)
+  opts%ls_max = ls(4)
   nodof = 3 * nonode
   allocate(coords(nodof), incid(nnpe * noelem), incmap(noelem), &
            etype(noelem), order(noelem), matnum(noelem), bbar(noelem), &
-           mats(nmat), cstmap(nodof), dmap(nodof), u(nodof), dui(nodof))
+           mats(nmat), cstmap(nodof), dmap(nodof), u(nodof), dui(nodof), &
+           du(nodof), load_n(nodof), dload(nodof))
   read(fu) coords, incid, etype, order, matnum, bbar
   do k = 1, nmat
     read(fu) ival, fval, dval
@@ -119,18 +133,29 @@ program w3d_drive
        status='replace')
   write(fo) nsteps, nodof
   u = 0.0d0
+  du = 0.0d0
+  load_n = 0.0d0
+  mf_prev = 1.0d0
   do step = 1, nsteps
-    read(fu) dui
-    call newton_step(ctx, step, nodof, neqns, cstmap, dmap, u, dui, opts, rep)
+    read(fu) mfv, dui, dload
+    ! scaling of the extrapolated increment: mf / mf at the last step
+    ! (mnralg.f:144-146)
+    if (abs(mf_prev) <= 1.0d-20) mf_prev = 1.0d0
+    sload = mfv(1) / mf_prev
+    if (step == 1) sload = 1.0d0
+    mf_prev = mfv(1)
+    call newton_step(ctx, step, nodof, neqns, cstmap, dmap, u, du, load_n, &
+                     dui, dload, sload, mfv(2) /= 0.0d0, opts, rep)
     write(fo) int(rep%status, c_int32_t), merge(1_c_int32_t, 0_c_int32_t, &
-         rep%converged), int(rep%iters, c_int32_t)
-    write(fo) (int(rep%pcg_iters(k), c_int32_t), k = 1, rep%iters)
-    write(fo) (rep%ratio(k), k = 1, rep%iters)
+         rep%converged), int(rep%iters, c_int32_t), &
+         int(rep%last_iters, c_int32_t), int(rep%cuts, c_int32_t)
+    write(fo) (int(rep%pcg_iters(k), c_int32_t), k = 1, rep%last_iters)
+    write(fo) (rep%ratio(k), k = 1, rep%last_iters)
     write(fo) rep%energy, rep%plastic_work
     write(fo) u
-    write(*, '(a,i4,a,i3,a,l2,a,es10.3)') ' step', step, '  iterations', &
-         rep%iters, '  converged', rep%converged, '  ratio %', &
-         rep%ratio(max(rep%iters, 1))
+    write(*, '(a,i4,a,i3,a,i2,a,l2,a,es10.3)') ' step', step, &
+         '  iterations', rep%iters, '  cuts', rep%cuts, '  converged', &
+         rep%converged, '  ratio %', rep%ratio(max(rep%last_iters, 1))
     if (.not. rep%converged) then
       if (rep%status /= W3D_OK) print *, 'status ', rep%status, ': ', &
            w3d_error_message()

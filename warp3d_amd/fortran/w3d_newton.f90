@@ -1,23 +1,33 @@
 ! w3d_newton.f90 -- the Newton driver of a load step, in Fortran, calling the
 ! GPU element-block library through ISO_C_BINDING (module w3d).
 !
-! Restates the static, displacement-controlled path of the reference's
-! Newton loop mnralg.f:177-600:
-!   iteration 0: strain-stress recovery with the imposed displacement
-!     increments (drive_eps_sig_internal_forces(step, 0), mnralg.f:358),
-!     element stiffnesses (stifup, :367) and the effective load of the step
-!     res = pbar - ifv on the unconstrained dofs (upres_iter_0, upres.f:14);
-!   iterations 1, 2, ...: assemble and solve K idu = res (eqn_solve,
-!     :503), du = du + idu, recovery at du (mnralg_ls without line search,
-!     :1102), residual res = pbar - ifv with the load norm of upres
-!     (upres.f:148-234: reactions |ifv| on constrained dofs, |load| on the
-!     others), convergence test 2 of cvtest (cvtest.f:163-186: ||res|| /
-!     ||load|| * 100 <= tol), new element stiffnesses for the next iteration
-!     (stifup, :439); at convergence (and iter >= mniter) the n+1 state is
-!     committed (update, :600).
-! External nodal loads are zero (pbar = 0); the constrained dofs carry the
-! imposed increments.  The equations are solved with the library's device
-! Jacobi-PCG.
+! Restates the static path of the reference's Newton loop (mnralg.f):
+!   start of a step or adaptive sub-increment (mnralg.f:211-300): the last
+!     converged increment extrapolated by the load-ratio and adaptive scale
+!     factors (clamped: < 0 -> 0, > 5 -> 1), not at step 1, after a change
+!     of load pattern (the increment is then kept unscaled) or on the first
+!     sub-increment after a cut; imposed increments scaled by the adaptive
+!     displacement factor;
+!   iteration 0: strain-stress recovery with that increment
+!     (drive_eps_sig_internal_forces(step, 0), mnralg.f:358), element
+!     stiffnesses (stifup, :367) and the effective load res = pbar - ifv on
+!     the unconstrained dofs, pbar = load at n + adapt_load_fact * dload
+!     (uppbar.f, upres_iter_0, upres.f:14);
+!   iterations 1, 2, ...: assemble and solve K idu = res (eqn_solve, :503),
+!     the update of mnralg_ls (:758-914: with the line search on and
+!     s0 = -res . idu < 0, step lengths 1, rho, rho^2, ... until
+!     |s / s0| < slack, back to 1 when that ratio falls by less than 10 %
+!     or the length drops below its minimum), residual and load norm
+!     (upres.f:148-234), convergence test 2 of cvtest (cvtest.f:163-186)
+!     and its divergence check (three non-decreasing residual norms,
+!     :94-107), new element stiffnesses for the next iteration (stifup,
+!     :439); at convergence (and iter >= mniter) the n+1 state is committed
+!     (update, :600);
+!   failure (iteration limit, divergence, a recovery asking for a step
+!     cut): with adaptive on and step > 1 the n state is restored and the
+!     increment split by the adaptive stack of adapt_check.f (4 divisions,
+!     smallest factor 1/16); otherwise the step fails.
+! The equations are solved with the library's device Jacobi-PCG.
 module w3d_newton
   use, intrinsic :: iso_c_binding
   use w3d
@@ -30,10 +40,19 @@ module w3d_newton
     integer(c_int32_t) :: precision = W3D_FP64
     real(c_double) :: pcg_rtol = 1.0d-12
     integer(c_int32_t) :: pcg_maxit = 50000
+    logical :: extrapolate = .true.
+    logical :: line_search = .false.
+    real(c_double) :: ls_rho = 0.7d0, ls_slack = 0.5d0
+    real(c_double) :: ls_min = 0.01d0, ls_max = 1.0d0
+    logical :: adaptive = .false.
+    logical :: divergence_check = .true.
   end type newton_options
 
   type :: newton_report
-    integer :: iters = 0                    ! Newton iterations of the step
+    integer :: iters = 0                    ! Newton iterations, all attempts
+    integer :: last_iters = 0               ! of the last sub-increment
+    integer :: cuts = 0                     ! adaptive cuts
+    integer :: subincrements = 0
     logical :: converged = .false.
     integer :: status = W3D_OK              ! library status of a failure
     real(c_double) :: ratio(32) = 0         ! test-2 ratio after each iter
@@ -41,95 +60,296 @@ module w3d_newton
     real(c_double) :: energy = 0, plastic_work = 0
   end type newton_report
 
+  ! adapt_check.f's stack: rows (start, end, increment, extrapolation
+  ! scale), the current sub-increment at row level
+  integer, parameter :: adapt_divisions = 4, adapt_rows = 64
+  type :: adaptive_state
+    real(c_double) :: stack(4, adapt_rows) = 0
+    integer :: level = 1
+    real(c_double) :: scaling = 1, disp_fact = 1, load_fact = 1
+    real(c_double) :: min_fact = 1.0d0 / adapt_divisions**2
+  end type adaptive_state
+
 contains
 
-  ! One load step.  u (3*nonode) is the displacement at n and is advanced to
-  ! n+1 on convergence; du_imposed holds the step's increments on the
-  ! constrained dofs (cstmap /= 0) and zeros elsewhere; dof_eqn_map is the
+  ! process type 3: a new step
+  subroutine adapt_init(ad)
+    type(adaptive_state), intent(out) :: ad
+    ad%stack = 0
+    ad%stack(:, 1) = [0.0d0, 1.0d0, 1.0d0, 1.0d0]
+    ad%level = 1
+    ad%scaling = 1
+    ad%disp_fact = 1
+    ad%load_fact = 1
+  end subroutine adapt_init
+
+  ! process type 1: split the current increment; .false. when it is
+  ! already the smallest allowed (or the stack is full)
+  logical function adapt_cut(ad) result(ok)
+    type(adaptive_state), intent(inout) :: ad
+    real(c_double) :: a, b, inc, start
+    integer :: i, lv, top
+    ok = .false.
+    if (ad%disp_fact <= ad%min_fact) return
+    lv = ad%level
+    top = lv + adapt_divisions
+    if (top > adapt_rows) return
+    a = ad%stack(1, lv)
+    b = ad%stack(2, lv)
+    inc = (b - a) / dble(adapt_divisions)
+    ad%stack(3, lv + 1:top) = inc
+    start = a
+    do i = top, lv + 1, -1
+      ad%stack(1, i) = start
+      ad%stack(2, i) = start + inc
+      start = start + inc
+    end do
+    ad%stack(4, top) = ad%stack(4, lv) / adapt_divisions
+    ad%stack(4, lv) = ad%stack(4, lv) / ad%stack(4, top)
+    ad%stack(4, lv + 1:top - 1) = 1.0d0
+    ad%level = top
+    ad%scaling = ad%stack(4, top)
+    ad%disp_fact = ad%stack(3, top)
+    ad%load_fact = ad%stack(2, top)
+    ok = .true.
+  end function adapt_cut
+
+  ! process type 2: after a converged sub-increment; .true. if another
+  ! one follows
+  logical function adapt_next(ad) result(more)
+    type(adaptive_state), intent(inout) :: ad
+    more = .false.
+    if (ad%level <= 1) return
+    if (ad%stack(2, ad%level) == ad%stack(2, ad%level - 1)) then
+      ad%level = ad%level - 2
+      if (ad%level == 0) then
+        ad%scaling = ad%stack(4, 1)
+        ad%level = 1
+        return
+      end if
+      ad%stack(4, ad%level) = adapt_divisions * ad%stack(4, ad%level)
+      ad%scaling = ad%stack(4, ad%level)
+    else
+      ad%level = ad%level - 1
+    end if
+    if (ad%level <= 1) return
+    ad%scaling = ad%stack(4, ad%level)
+    ad%disp_fact = ad%stack(3, ad%level)
+    ad%load_fact = ad%stack(2, ad%level)
+    more = .true.
+  end function adapt_next
+
+  ! One load step with its adaptive sub-increments.  u (3*nonode) is the
+  ! displacement at n and is advanced on convergence; du holds the last
+  ! converged increment (the extrapolation source) and is replaced by this
+  ! step's; load_n is the external load at n (advanced by dload on
+  ! convergence); du_imposed holds the step's increments on the
+  ! constrained dofs (cstmap /= 0); scaling_load = mf / mf_prev and
+  ! mf_change the load-pattern change of the step; dof_eqn_map is the
   ! library's equation number of each dof (0 for constrained ones).
   subroutine newton_step(ctx, step, nodof, neqns, cstmap, dof_eqn_map, &
-                         u, du_imposed, opts, rep)
+                         u, du, load_n, du_imposed, dload, scaling_load, &
+                         mf_change, opts, rep)
     type(c_ptr), intent(in) :: ctx
     integer(c_int32_t), intent(in) :: step, nodof, neqns
     integer(c_int32_t), intent(in) :: cstmap(nodof), dof_eqn_map(nodof)
-    real(c_double), intent(inout) :: u(nodof)
-    real(c_double), intent(in) :: du_imposed(nodof)
+    real(c_double), intent(inout) :: u(nodof), du(nodof), load_n(nodof)
+    real(c_double), intent(in) :: du_imposed(nodof), dload(nodof)
+    real(c_double), intent(in) :: scaling_load
+    logical, intent(in) :: mf_change
     type(newton_options), intent(in) :: opts
     type(newton_report), intent(out) :: rep
-    real(c_double), allocatable :: du(:), ifv(:), res(:), rhs(:), x(:)
-    real(c_double) :: en, pw, mgload, norm_res, relres
-    integer(c_int32_t) :: iter, pits, rc
+    real(c_double), allocatable :: ifv(:), res(:), rhs(:), x(:), idu(:), &
+         du0(:), pbar(:)
+    real(c_double) :: en, pw, mgload, norm_res, relres, sf, sload
+    real(c_double) :: norms(64)
+    integer(c_int32_t) :: iter, pits, rc, extrap
     integer :: j
-    logical :: cnverg
+    logical :: cnverg, diverging, failed, adaptive, first_subinc
+    type(adaptive_state) :: ad
 
-    allocate(du(nodof), ifv(nodof), res(nodof), rhs(neqns), x(neqns))
-    du = du_imposed
-    ! ---- iteration 0: imposed increments, stiffness, effective load
-    rc = w3d_recover(ctx, step, 0, 0, u, du, ifv, en, pw)
-    if (rc /= W3D_OK) then
-      rep%status = rc
-      return
-    end if
-    rc = w3d_tangent(ctx, step, 0, opts%precision)
-    if (rc /= W3D_OK) then
-      rep%status = rc
-      return
-    end if
-    call residual(nodof, cstmap, ifv, res, mgload, norm_res)
-    iter = 1
+    allocate(ifv(nodof), res(nodof), rhs(neqns), x(neqns), idu(nodof), &
+             du0(nodof), pbar(nodof))
+    call adapt_init(ad)
+    adaptive = opts%adaptive .and. step > 1
+    first_subinc = .false.
+    sload = scaling_load
+    rc = W3D_OK
     do
-      if (iter > 1) then
-        rc = w3d_tangent(ctx, step, iter, opts%precision)
-        if (rc /= W3D_OK) exit
+      ! ---- start of the step or sub-increment
+      sf = sload * ad%scaling
+      extrap = 0
+      if (first_subinc .or. .not. (opts%extrapolate .and. step > 1)) then
+        du = 0.0d0
+      else if (.not. mf_change) then
+        if (sf < 0.0d0) sf = 0.0d0
+        if (sf > 5.0d0) sf = 1.0d0
+        extrap = 1
+        du = sf * du
       end if
-      rc = w3d_assemble_vss(ctx, c_null_ptr, c_null_ptr)
-      if (rc /= W3D_OK) exit
-      do j = 1, nodof
-        if (dof_eqn_map(j) > 0) rhs(dof_eqn_map(j)) = res(j)
-      end do
-      x = 0.0d0
-      rc = w3d_solve_pcg(ctx, rhs, x, opts%pcg_rtol, opts%pcg_maxit, pits, &
-                         relres)
-      if (rc /= W3D_OK) exit
-      do j = 1, nodof
-        if (dof_eqn_map(j) > 0) du(j) = du(j) + x(dof_eqn_map(j))
-      end do
-      rc = w3d_recover(ctx, step, iter, 0, u, du, ifv, en, pw)
-      if (rc /= W3D_OK) exit
-      call residual(nodof, cstmap, ifv, res, mgload, norm_res)
-      ! cvtest test 2
-      if (mgload == 0.0d0) then
-        rep%ratio(min(iter, 32)) = 0.0d0
-        cnverg = .true.
-      else
-        rep%ratio(min(iter, 32)) = (norm_res / mgload) * 100.0d0
-        cnverg = rep%ratio(min(iter, 32)) <= opts%res_tol
-        if (mgload <= 1.0d-5 .and. norm_res <= 1.0d-8) cnverg = .true.
-      end if
-      rep%pcg_iters(min(iter, 32)) = pits
-      rep%iters = iter
-      if (cnverg .and. iter >= opts%mniter) then
-        rep%converged = .true.
+      where (cstmap /= 0) du = du_imposed * ad%disp_fact
+      pbar = load_n + ad%load_fact * dload
+      failed = .false.
+      iter = 0
+      rc = w3d_recover(ctx, step, 0, extrap, u, du, ifv, en, pw)
+      if (rc == W3D_CUT_STEP) then
+        failed = .true.
+      else if (rc /= W3D_OK) then
         exit
       end if
-      iter = iter + 1
-      if (iter > opts%mxiter) exit
-    end do
-    rep%status = rc
-    if (rep%converged) then
+      if (.not. failed) then
+        rc = w3d_tangent(ctx, step, 0, opts%precision)
+        if (rc /= W3D_OK) exit
+        call residual(nodof, cstmap, ifv, pbar, res, mgload, norm_res)
+        iter = 1
+        do
+          if (iter > 1) then
+            rc = w3d_tangent(ctx, step, iter, opts%precision)
+            if (rc /= W3D_OK) exit
+          end if
+          rc = w3d_assemble_vss(ctx, c_null_ptr, c_null_ptr)
+          if (rc /= W3D_OK) exit
+          do j = 1, nodof
+            if (dof_eqn_map(j) > 0) rhs(dof_eqn_map(j)) = res(j)
+          end do
+          x = 0.0d0
+          rc = w3d_solve_pcg(ctx, rhs, x, opts%pcg_rtol, opts%pcg_maxit, &
+                             pits, relres)
+          if (rc /= W3D_OK) exit
+          idu = 0.0d0
+          do j = 1, nodof
+            if (dof_eqn_map(j) > 0) idu(j) = x(dof_eqn_map(j))
+          end do
+          du0 = du
+          call line_search_update(rc)
+          if (rc == W3D_CUT_STEP) then
+            failed = .true.
+            rc = W3D_OK
+            exit
+          end if
+          if (rc /= W3D_OK) exit
+          call residual(nodof, cstmap, ifv, pbar, res, mgload, norm_res)
+          ! cvtest test 2 and the divergence check
+          if (mgload == 0.0d0) then
+            rep%ratio(min(iter, 32)) = 0.0d0
+            cnverg = .true.
+          else
+            rep%ratio(min(iter, 32)) = (norm_res / mgload) * 100.0d0
+            cnverg = rep%ratio(min(iter, 32)) <= opts%res_tol
+            if (mgload <= 1.0d-5 .and. norm_res <= 1.0d-8) cnverg = .true.
+          end if
+          norms(min(iter, 64)) = norm_res
+          diverging = .false.
+          if (iter >= 3 .and. iter <= 64) diverging = &
+               norms(iter) >= norms(iter - 1) .and. &
+               norms(iter - 1) >= norms(iter - 2)
+          rep%pcg_iters(min(iter, 32)) = pits
+          if (cnverg .and. iter >= opts%mniter) exit
+          iter = iter + 1
+          if (iter > opts%mxiter .or. (opts%divergence_check .and. &
+              diverging)) then
+            iter = iter - 1
+            failed = .true.
+            exit
+          end if
+        end do
+        if (rc /= W3D_OK) exit
+      end if
+      rep%iters = rep%iters + iter
+      rep%last_iters = iter
+      if (failed) then
+        if (adaptive) then
+          if (adapt_cut(ad)) then
+            rc = w3d_rollback(ctx)
+            if (rc /= W3D_OK) exit
+            rep%cuts = rep%cuts + 1
+            first_subinc = .true.
+            cycle
+          end if
+        end if
+        rc = W3D_CUT_STEP
+        exit
+      end if
       rc = w3d_commit_step(ctx)
-      rep%status = rc
+      if (rc /= W3D_OK) exit
       u = u + du
+      rep%subincrements = rep%subincrements + 1
       rep%energy = en
       rep%plastic_work = pw
-    end if
+      if (adaptive) then
+        if (adapt_next(ad)) then
+          first_subinc = .false.
+          sload = 1.0d0
+          cycle
+        end if
+      end if
+      rep%converged = .true.
+      load_n = load_n + dload
+      exit
+    end do
+    rep%status = rc
+    if (rep%converged) rep%status = W3D_OK
+
+  contains
+
+    ! mnralg_ls: du = du0 + alpha idu and the recovery at it; rc as
+    ! w3d_recover's (W3D_CUT_STEP: the material asks for a cut)
+    subroutine line_search_update(rc)
+      integer(c_int32_t), intent(out) :: rc
+      real(c_double) :: s0, s, r, rprev, alpha
+      integer :: ell
+      s0 = 0.0d0
+      do j = 1, nodof
+        if (cstmap(j) == 0) s0 = s0 - res(j) * idu(j)
+      end do
+      if (s0 >= 0.0d0 .or. .not. opts%line_search) then
+        call evaluate(1.0d0, rc)
+        return
+      end if
+      alpha = opts%ls_max
+      ell = 1
+      rprev = 0.0d0
+      do
+        call evaluate(alpha, rc)
+        if (rc /= W3D_OK) return
+        call residual(nodof, cstmap, ifv, pbar, res, mgload, norm_res)
+        s = 0.0d0
+        do j = 1, nodof
+          if (cstmap(j) == 0) s = s - res(j) * idu(j)
+        end do
+        r = abs(s / s0)
+        if (r < opts%ls_slack) return
+        if (ell >= 2) then
+          if (r > 0.9d0 * rprev) then
+            call evaluate(1.0d0, rc)
+            return
+          end if
+        end if
+        alpha = opts%ls_rho * alpha
+        if (alpha < opts%ls_min) then
+          call evaluate(1.0d0, rc)
+          return
+        end if
+        rprev = r
+        ell = ell + 1
+      end do
+    end subroutine line_search_update
+
+    subroutine evaluate(alpha, rc)
+      real(c_double), intent(in) :: alpha
+      integer(c_int32_t), intent(out) :: rc
+      du = du0 + alpha * idu
+      rc = w3d_recover(ctx, step, iter, extrap, u, du, ifv, en, pw)
+    end subroutine evaluate
+
   end subroutine newton_step
 
-  ! res = pbar - ifv (pbar = 0) with zeros on constrained dofs, and the
-  ! load norm of upres: the reactions |ifv| on the constrained dofs
-  subroutine residual(nodof, cstmap, ifv, res, mgload, norm_res)
+  ! upres: res = pbar - ifv on the free dofs (0 on constrained ones), the
+  ! load norm over the nonzero terms |pbar| (free dofs) and |ifv|
+  ! (constrained dofs: the reactions), and the residual norm
+  subroutine residual(nodof, cstmap, ifv, pbar, res, mgload, norm_res)
     integer(c_int32_t), intent(in) :: nodof, cstmap(nodof)
-    real(c_double), intent(in) :: ifv(nodof)
+    real(c_double), intent(in) :: ifv(nodof), pbar(nodof)
     real(c_double), intent(out) :: res(nodof), mgload, norm_res
     real(c_double) :: term
     integer :: i
@@ -137,8 +357,8 @@ contains
     norm_res = 0.0d0
     do i = 1, nodof
       if (cstmap(i) == 0) then
-        res(i) = -ifv(i)
-        term = 0.0d0
+        res(i) = pbar(i) - ifv(i)
+        term = abs(pbar(i))
       else
         term = abs(ifv(i))
         res(i) = 0.0d0

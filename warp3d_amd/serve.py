@@ -103,10 +103,16 @@ def make_rve(spec, auto_size=128):
     return Rve(spec, m, arr, cst, du0, top)
 
 
-def newton_opts(nsteps=10, precision=_abi.W3D_FP32, tol=1e-5, tol_abs=1e-8,
-                max_iters=12, pcg_rtol=1e-6, pcg_maxit=5000):
+def newton_opts(nsteps=10, precision=_abi.W3D_FP32, tol=1e-4, tol_abs=1e-8,
+                max_iters=20, pcg_rtol=1e-6, pcg_maxit=5000, adaptive=True):
+    """The serving defaults: the reference's default convergence test
+    (0.01 % of the reaction norm), adaptive sub-increments on (the crystal
+    update asks for them at 0.2 % strain per step), a symmetric tangent,
+    so that convergence is linear and needs more iterations than the
+    reference's default limit of 10."""
     return _abi.W3DNewtonOpts(nsteps=nsteps, max_iters=max_iters,
                               precision=precision, pcg_maxit=pcg_maxit,
+                              adaptive=int(bool(adaptive)), reserved=0,
                               tol=tol, tol_abs=tol_abs, pcg_rtol=pcg_rtol)
 
 
@@ -137,19 +143,20 @@ class Problem:
         return st.value, step.value, it.value
 
     def records(self):
-        """Per converged step: Newton iterations, PCG iterations, load,
-        residual and reaction norms."""
+        """Per converged step: Newton iterations (all attempts), PCG
+        iterations, adaptive cuts, load, residual and reaction norms."""
         out = []
         for s in range(1, self.nsteps + 1):
-            ni, pi = C.c_int32(), C.c_int32()
+            ni, pi, cu = C.c_int32(), C.c_int32(), C.c_int32()
             ld, rs, rc = C.c_double(), C.c_double(), C.c_double()
             r = self._lib().w3d_problem_step_record(
-                self.h, s, C.byref(ni), C.byref(pi), C.byref(ld), C.byref(rs),
-                C.byref(rc))
+                self.h, s, C.byref(ni), C.byref(pi), C.byref(cu), C.byref(ld),
+                C.byref(rs), C.byref(rc))
             if r != 0:
                 break
-            out.append(dict(newton=ni.value, pcg=pi.value, load=ld.value,
-                            residual=rs.value, reaction=rc.value))
+            out.append(dict(newton=ni.value, pcg=pi.value, cuts=cu.value,
+                            load=ld.value, residual=rs.value,
+                            reaction=rc.value))
         return out
 
     def u(self):
@@ -401,21 +408,22 @@ def poisson_arrivals(nreq, rate, nmin=8, nmax=24, seed=0, **spec_kw):
 def solve_plain(rve, opts, device=0):
     """One RVE through the per-call entry points: w3d_recover (host
     vectors), w3d_tangent, w3d_assemble_vss_dev, w3d_solve_pcg_dev, with
-    the residual, the convergence test and the update on the host; the
-    same test and tolerances as w3d_batch_advance.  Returns the per-step
-    records, the final displacements and the per-step wall times."""
+    the residual, the convergence test, the adaptive sub-increments and the
+    update on the host; the same algorithm and tolerances as
+    w3d_batch_advance.  Returns the per-step records, the final
+    displacements and the per-step wall times."""
     from . import hip
     from .model import BlockModel
+    from .steps import Adaptive
     dev = BlockModel(rve.arrays, device)
     try:
         neq, nco = dev.pattern(rve.cstmap)
         d, _, _, _ = dev.get_pattern()
         free = np.flatnonzero(rve.cstmap == 0)
-        eq_of = d[free] - 1          # equation of each free dof
-        order = np.argsort(eq_of)
-        eqdof = free[order]          # dof of each equation
+        eqdof = free[np.argsort(d[free] - 1)]   # dof of each equation
         cdof = np.flatnonzero(rve.cstmap != 0)
         ldof = np.flatnonzero((rve.cstmap != 0) & (rve.du0 != 0.0))
+        imp = rve.du0[cdof]
         kd = hip.DeviceArray(neq)
         kc = hip.DeviceArray(max(nco, 1))
         rhs = hip.DeviceArray(neq)
@@ -423,24 +431,44 @@ def solve_plain(rve, opts, device=0):
         u = np.zeros(3 * rve.mesh.nonode)
         du = rve.du0.copy()
         records, times = [], []
+
+        def start(sf, disp):
+            du[eqdof] = sf * du[eqdof]
+            du[cdof] = disp * imp
+
         for step in range(1, opts.nsteps + 1):
             t0 = time.perf_counter()
-            pcg = 0
+            ad = Adaptive()
+            if step > 1:
+                start(1.0, 1.0)
+            iters = pcg = cuts = 0
             it = 0
             while True:
                 it += 1
+                iters += 1
                 ifv, _, _, rc = dev.recover(step, it, u, du, extrapolated=True)
-                if rc != 0:
-                    raise RuntimeError(f"step {step}: recovery asks for a cut")
+                cut = rc != 0
                 r = -ifv[eqdof]
                 res = float(np.sqrt(np.dot(r, r)))
                 react = float(np.sqrt(np.dot(ifv[cdof], ifv[cdof])))
-                if res <= opts.tol * max(react, opts.tol_abs):
-                    records.append(dict(newton=it, pcg=pcg,
+                if not cut and res <= opts.tol * max(react, opts.tol_abs):
+                    dev.commit_step()
+                    u = u + du
+                    if opts.adaptive and ad.advance():
+                        start(ad.scaling, ad.disp_fact)
+                        it = 0
+                        continue
+                    records.append(dict(newton=iters, pcg=pcg, cuts=cuts,
                                         load=float(ifv[ldof].sum()),
                                         residual=res, reaction=react))
                     break
-                if it >= opts.max_iters:
+                if cut or it >= opts.max_iters:
+                    if opts.adaptive and ad.cut():
+                        dev.rollback()
+                        start(0.0, ad.disp_fact)
+                        cuts += 1
+                        it = 0
+                        continue
                     raise RuntimeError(f"step {step}: no convergence")
                 dev.tangent(step, it, opts.precision)
                 dev.assemble_vss_dev(kd.ptr, kc.ptr)
@@ -450,8 +478,6 @@ def solve_plain(rve, opts, device=0):
                                             opts.pcg_rtol, opts.pcg_maxit)
                 pcg += n_it
                 du[eqdof] += x.download()
-            dev.commit_step()
-            u = u + du
             times.append(time.perf_counter() - t0)
         for a in (kd, kc, rhs, x):
             a.free()
