@@ -331,9 +331,14 @@ def run_rve(args):
         arr = S.poisson_arrivals(args.steps, rate=args.rate, nmin=8,
                                  nmax=args.nmax, seed=1)
         srv = S.Server(eng, max_slots=args.slots, hbm_budget=64 << 30)
+        ph0, na0, np0 = eng.batch.phase_times()
         done, st = srv.run(arr)
+        ph1, na1, np1 = eng.batch.phase_times()
     finally:
         eng.close()
+    phases = {k: round(ph1[k] - ph0[k], 1) for k in ph1}
+    log("server phases (device ms over the stream): " + json.dumps(phases)
+        + f", advances {na1 - na0}, batched PCG iterations {np1 - np0}")
     log("server: " + json.dumps(st))
     newton = [r["newton"] for j in done if j.result
               for r in j.result["records"]]
@@ -347,15 +352,22 @@ def run_rve(args):
         if k and arr else []
     plain_t, plain_steps = [], []
     popts = S.newton_opts(precision=_abi.W3D_FP64)
+    plain_failed = 0
     for i in pick:
         spec = arr[i][1]
         t0 = time.perf_counter()
-        _, _, times = S.solve_plain(S.make_rve(spec), popts)
+        try:
+            _, _, times = S.solve_plain(S.make_rve(spec), popts)
+        except RuntimeError as e:      # counted, and its time kept
+            plain_failed += 1
+            times = []
+            log(f"plain n={spec.n}: {e}")
         plain_t.append(time.perf_counter() - t0)
         plain_steps += times
         log(f"plain n={spec.n}: {plain_t[-1]:.2f} s")
     mean_elems_stream = np.mean([s.noelem for _, s in arr])
-    plain_rate = len(plain_t) / sum(plain_t) if plain_t else None
+    plain_rate = (len(plain_t) - plain_failed) / sum(plain_t) \
+        if plain_t else None
     out = {
         "metric": "RVE solves/s at p99 load-step latency",
         "value": round(st["solves_per_s"], 3),
@@ -381,10 +393,13 @@ def run_rve(args):
         "server": {k: st[k] for k in ("solved", "failed", "wall_s",
                                       "iterations", "peak_slots",
                                       "peak_bytes", "steps")},
+        "server_phase_ms": phases,
+        "batched_pcg_iterations": np1 - np0,
         "newton_iters_per_step": round(float(np.mean(newton)), 3)
         if newton else None,
         "pcg_iters_per_step": round(float(np.mean(pcg)), 1) if pcg else None,
         "plain": {"sample_n": [arr[i][1].n for i in pick],
+                  "failed": plain_failed,
                   "solves_per_s": plain_rate,
                   "step_p50_ms": 1e3 * float(np.percentile(plain_steps, 50))
                   if plain_steps else None,
@@ -392,6 +407,9 @@ def run_rve(args):
                   if plain_steps else None},
         "speedup_vs_plain": round(st["solves_per_s"] / plain_rate, 2)
         if plain_rate else None,
+        "newton_opts": {"tol": opts.tol, "max_iters": opts.max_iters,
+                        "pcg_rtol": opts.pcg_rtol,
+                        "adaptive": bool(opts.adaptive)},
         "roofline": None,
         "cpu_baseline": None,
     }

@@ -412,6 +412,13 @@ int w3d_batch_destroy(w3d_batch* b);
  * status[k] receives problem k's status after it. */
 int w3d_batch_advance(w3d_batch* b, w3d_problem* const* problems, int32_t n,
                       int32_t* status);
+/* Device milliseconds of the batch's advances so far, per phase: ms[0]
+ * recovery (all passes), ms[1] the batched mm10 inside it, ms[2] residual,
+ * ms[3] tangents and assemblies, ms[4] the batched Jacobi-PCG (with its
+ * host checks), ms[5] the update; the number of advances and of batched
+ * PCG iterations. */
+int w3d_batch_phase_times(const w3d_batch* b, double* ms, int64_t* advances,
+                          int64_t* pcg_iterations);
 
 #ifdef __cplusplus
 }

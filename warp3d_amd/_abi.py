@@ -245,4 +245,5 @@ SERVE_PROTOS = {
     "batch_create": (_I, [_I, C.POINTER(_P)]),
     "batch_destroy": (_I, [_P]),
     "batch_advance": (_I, [_P, C.POINTER(_P), _I, _IP]),
+    "batch_phase_times": (_I, [_P, _D, _LP, _LP]),
 }

@@ -43,16 +43,25 @@ struct Props {
 // crystal_state of one crystal (the Voce subset).  The slip tensors
 // (mm10_setup's ms, qs; qc equals qs bitwise for R = I) depend only on the
 // plastic rotation at n, so every sub-step shares one copy.
-// The tensors live in LDS, one column per lane (word w of lane t at
-// g[w * kLanes + t]), so that the local problem's registers stay free.
+// The rotated ms live in LDS, one column per lane (word w of lane t at
+// g[w * kLanes + t]), so that the local problem's registers stay free; qs
+// is formed where it is used from the crystal's reference vectors and the
+// 3x3 rotation rw (mm10_rt2rvw), which keeps the LDS of a workgroup at
+// 72 words a lane (four one-wave workgroups per compute unit).
 constexpr int kLanes = 64;
+constexpr int kGeoWords = 72;
 struct CState {
   double D[6];
   double tt_rate, tinc, dg;
-  const double* g;  // 72 ms words then 36 qs words, lane-strided
+  const double* g;           // 72 ms words, lane-strided
+  const double (*q0)[3];     // reference qs of the 12 systems
+  double rw[9];              // column-major
   __device__ double ms(int i, int k) const { return g[(6 * i + k) * kLanes]; }
-  __device__ double qs(int i, int k) const {
-    return g[(72 + 3 * i + k) * kLanes];
+  // rw q0[i] (mv3)
+  __device__ void qs3(int i, double* q) const {
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      q[k] = rw[k] * q0[i][0] + rw[k + 3] * q0[i][1] + rw[k + 6] * q0[i][2];
   }
 };
 
@@ -242,30 +251,29 @@ __device__ inline void setup(const Props& p, CState& s) {
 
 // mm10_setup: the slip tensors rotated by the plastic rotation at n
 __device__ inline void slip_geometry(const Props& p, const double* Rpn,
-                                     double* gl) {
-  double tn[9], RE[36], RW[9];
+                                     double* gl, CState& s) {
+  double tn[9], RE[36];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j) tn[i + 3 * j] = Rpn[j + 3 * i];
   rt2rve(tn, RE);
-  rt2rvw(tn, RW);
+  rt2rvw(tn, s.rw);
+  s.q0 = p.cr->qs;
 #pragma unroll
   for (int i = 0; i < NS; ++i) {
     const double* m0 = p.cr->ms[i];
-    double m[6], q[3];
+    double m[6];
 #pragma unroll
     for (int r = 0; r < 6; ++r) m[r] = RE[r] * m0[0];
 #pragma unroll
     for (int j = 1; j < 6; ++j)
 #pragma unroll
       for (int r = 0; r < 6; ++r) m[r] = m[r] + RE[r + 6 * j] * m0[j];
-    mv3(q, RW, p.cr->qs[i]);
 #pragma unroll
     for (int r = 0; r < 6; ++r) gl[(6 * i + r) * kLanes] = m[r];
-#pragma unroll
-    for (int r = 0; r < 3; ++r) gl[(72 + 3 * i + r) * kLanes] = q[r];
   }
+  s.g = gl;
 }
 
 __device__ inline double rs_of(const CState& s, const double* x, int i) {
@@ -298,20 +306,22 @@ __device__ inline void form_w(const Props& p, const CState& s,
   w[0] = w[1] = w[2] = 0.0;
 #pragma unroll
   for (int i = 0; i < NS; ++i) {
+    double q[3];
+    s.qs3(i, q);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) w[k] = w[k] + gam[i] * s.qs(i, k);
+    for (int k = 0; k < 3; ++k) w[k] = w[k] + gam[i] * q[k];
     double c = rs[i] * s.tinc * p.iD_v;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) w[k] = w[k] + c * s.qs(i, k);
+    for (int k = 0; k < 3; ++k) w[k] = w[k] + c * q[k];
   }
 }
 
-// formR1 at x with tau tt
-__device__ inline void formR1(const Props& p, const CState& s,
-                              const double* sn, const double* x, double tt,
-                              double* R1) {
-  double rs[NS], gam[NS], dbar[6], wp[3], sw[6], w1[6];
-  slips(p, s, x, tt, rs, gam);
+// formR1 at x with tau tt, from the slips (rs, gam) at (x, tt)
+__device__ inline void formR1_slips(const Props& p, const CState& s,
+                                    const double* sn, const double* x,
+                                    const double* rs, const double* gam,
+                                    double* R1) {
+  double dbar[6], wp[3], sw[6], w1[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k) dbar[k] = 0.0;
 #pragma unroll
@@ -336,6 +346,15 @@ __device__ inline void formR1(const Props& p, const CState& s,
   for (int k = 0; k < 6; ++k) R1[k] = x[k] - sn[k] - w2[k] + 2.0 * sw[k];
 }
 
+// formR1 at x with tau tt
+__device__ inline void formR1(const Props& p, const CState& s,
+                              const double* sn, const double* x, double tt,
+                              double* R1) {
+  double rs[NS], gam[NS];
+  slips(p, s, x, tt, rs, gam);
+  formR1_slips(p, s, sn, x, rs, gam, R1);
+}
+
 // h_voche (tau_l = 0)
 __device__ inline double h_voche(const Props& p, const double* gam, double tt,
                                  double tau_n) {
@@ -348,31 +367,35 @@ __device__ inline double h_voche(const Props& p, const double* gam, double tt,
 }
 
 // formR (7 residuals); records the hardening rate
+// (the reference evaluates the slips twice, in formR1 and here, from the
+// same arguments: once here, bitwise the same values)
 __device__ inline void formR(const Props& p, CState& s, const double* sn,
                              double tau_n, const double* x, double* R) {
-  formR1(p, s, sn, x, x[6], R);
   double rs[NS], gam[NS];
   slips(p, s, x, x[6], rs, gam);
+  formR1_slips(p, s, sn, x, rs, gam, R);
   double h = h_voche(p, gam, x[6], tau_n);
   R[6] = x[6] - h;
   s.tt_rate = (h - tau_n) / s.tinc;
 }
 
-// formJ11 (6x6, column-major)
-__device__ inline void formJ11(const Props& p, const CState& s,
-                               const double* x, double tt, double* J11) {
-  double rs[NS], gam[NS];
-  slips(p, s, x, tt, rs, gam);
+// formJ11 (6x6, column-major) from the slips (rs, gam) at (x, tt)
+__device__ inline void formJ11_slips(const Props& p, const CState& s,
+                                     const double* x, double tt,
+                                     const double* rs, const double* gam,
+                                     double* J11) {
   const double* C = p.cr->C;
+  const double ptt = pow(tt, p.rate_n);
 #pragma unroll
   for (int k = 0; k < 36; ++k) J11[k] = 0.0;
 #pragma unroll
   for (int i = 0; i < NS; ++i) {
     double d = pow(fabs(rs[i]), p.rate_n - 1.0);
-    d = s.dg * p.rate_n / pow(tt, p.rate_n) * d;
+    d = s.dg * p.rate_n / ptt * d;
     double dgdt = d + s.tinc * p.iD_v;
     double st[6], wv[6];
-    double qv[3] = {s.qs(i, 0), s.qs(i, 1), s.qs(i, 2)};
+    double qv[3];
+    s.qs3(i, qv);
     symsw(x, qv, st);
 #pragma unroll
     for (int r = 0; r < 6; ++r) wv[r] = 2.0 * st[r];
@@ -415,20 +438,28 @@ __device__ inline void formJ11(const Props& p, const CState& s,
   for (int i = 0; i < 6; ++i) J11[i + 6 * i] = J11[i + 6 * i] + 1.0;
 }
 
+__device__ inline void formJ11(const Props& p, const CState& s,
+                               const double* x, double tt, double* J11) {
+  double rs[NS], gam[NS];
+  slips(p, s, x, tt, rs, gam);
+  formJ11_slips(p, s, x, tt, rs, gam, J11);
+}
+
 // formJ: the 7x7 Jacobian (column-major, leading dimension 7)
 __device__ inline void formJ(const Props& p, const CState& s, const double* x,
                              double* J) {
   const double tt = x[6];
-  double J11[36];
-  formJ11(p, s, x, tt, J11);
   double rs[NS], gam[NS];
-  slips(p, s, x, tt, rs, gam);
+  slips(p, s, x, tt, rs, gam);  // once for J11 and the rest (bitwise)
+  double J11[36];
+  formJ11_slips(p, s, x, tt, rs, gam, J11);
   const double* C = p.cr->C;
   double J12[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
   for (int i = 0; i < NS; ++i) {
     double st[6], tm[6];
-    double qv[3] = {s.qs(i, 0), s.qs(i, 1), s.qs(i, 2)};
+    double qv[3];
+    s.qs3(i, qv);
     symsw(x, qv, st);
     double dgdh = -p.rate_n / tt * gam[i];
 #pragma unroll
@@ -764,9 +795,8 @@ __device__ __forceinline__ void mm10_item(const DevModel& M,
 
     // np1 (mm10_setup_np1 + mm10_setup)
     double* gl = s_geo + threadIdx.x;
-    slip_geometry(p, Rpn, gl);
     CState np1;
-    np1.g = gl;
+    slip_geometry(p, Rpn, gl, np1);
 #pragma unroll
     for (int k = 0; k < 6; ++k) np1.D[k] = uddt[k];
     np1.tinc = dt;
@@ -966,10 +996,11 @@ __device__ __forceinline__ void mm10_item(const DevModel& M,
 #pragma unroll
         for (int k = 0; k < 6; ++k) ep[k] = ep[k] / np1.tinc;
         n_eff = 0.0;
+        const double ptt = pow(tt, p.rate_n);
 #pragma unroll
         for (int i = 0; i < NS; ++i) {
           double d = pow(fabs(rs[i]), p.rate_n - 1.0);
-          d = np1.dg * p.rate_n / pow(tt, p.rate_n) * d;
+          d = np1.dg * p.rate_n / ptt * d;
           double dgdt = d + np1.tinc * p.iD_v;
           double a = np1.ms(i, 0) * ep[0] + np1.ms(i, 1) * ep[1] +
                      np1.ms(i, 2) * ep[2];
@@ -1188,7 +1219,7 @@ int launch_mm10_grads(const DevModel& M, const int32_t* d_blks, int nblks,
 __global__ void __launch_bounds__(64)
     k_mm10(DevModel M, const WorkItem* __restrict__ work, int nwork,
            int* __restrict__ counter, int step, int iter0_off) {
-  __shared__ double s_geo[108 * kLanes];
+  __shared__ double s_geo[kGeoWords * kLanes];
   __shared__ int s_item;
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
@@ -1206,7 +1237,7 @@ __global__ void __launch_bounds__(64)
 __global__ void __launch_bounds__(64)
     k_mm10_batch(const Mm10Slot* __restrict__ slots, int nslot,
                  int nwork_total, int* __restrict__ counter) {
-  __shared__ double s_geo[108 * kLanes];
+  __shared__ double s_geo[kGeoWords * kLanes];
   __shared__ int s_item;
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);

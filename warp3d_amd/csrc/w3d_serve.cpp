@@ -131,6 +131,20 @@ struct w3d_batch {
   int32_t *d_state = nullptr, *h_state = nullptr;
   double *d_sc = nullptr, *d_out = nullptr, *h_out = nullptr;
   int* d_counter = nullptr;
+  // device time per phase of the advances (w3d_batch_phase_times):
+  // 0 recovery (all passes), 1 of which the batched mm10, 2 residual,
+  // 3 tangents and assemblies, 4 the batched solve, 5 the update
+  static constexpr int kPhases = 6;
+  hipEvent_t ev[kPhases][2] = {};
+  bool ev_used[kPhases] = {};
+  double phase_ms[kPhases] = {};
+  int64_t advances = 0, pcg_iterations = 0;
+  void mark(int ph, int k) {
+    if (ev[ph][k]) {
+      (void)hipEventRecord(ev[ph][k], stream);
+      if (k == 1) ev_used[ph] = true;
+    }
+  }
 };
 
 namespace {
@@ -349,6 +363,9 @@ int w3d_batch_create(int32_t device, w3d_batch** out) {
   }
   b->n_cu = prop.multiProcessorCount;
   b->mm10_grid = 4 * std::max(b->n_cu, 1);
+  for (int ph = 0; ph < w3d_batch::kPhases; ++ph)
+    for (int k = 0; k < 2; ++k)
+      if (hipEventCreate(&b->ev[ph][k]) != hipSuccess) b->ev[ph][k] = nullptr;
   int rc = dalloc(&b->d_counter, 1);
   if (rc) {
     (void)hipStreamDestroy(b->stream);
@@ -366,8 +383,21 @@ int w3d_batch_destroy(w3d_batch* b) {
   free_batch_tables(b);
   dfree(b->d_part);
   dfree(b->d_counter);
+  for (int ph = 0; ph < w3d_batch::kPhases; ++ph)
+    for (int k = 0; k < 2; ++k)
+      if (b->ev[ph][k]) (void)hipEventDestroy(b->ev[ph][k]);
   (void)hipStreamDestroy(b->stream);
   delete b;
+  return W3D_OK;
+}
+
+int w3d_batch_phase_times(const w3d_batch* b, double* ms, int64_t* advances,
+                          int64_t* pcg_iterations) {
+  if (!b) return fail(W3D_ERR_ARG, "w3d_batch_phase_times: null batch");
+  if (ms)
+    for (int ph = 0; ph < w3d_batch::kPhases; ++ph) ms[ph] = b->phase_ms[ph];
+  if (advances) *advances = b->advances;
+  if (pcg_iterations) *pcg_iterations = b->pcg_iterations;
   return W3D_OK;
 }
 
@@ -424,7 +454,9 @@ int w3d_batch_advance(w3d_batch* b, w3d_problem* const* ps, int32_t n,
     ntile_res += tiles(std::max({run[k]->neq, run[k]->ncdof, run[k]->nldof}));
   }
   if ((rc = reserve(b, nr, 3 * (int64_t)ntile_res))) return rc;
+  for (int ph = 0; ph < w3d_batch::kPhases; ++ph) b->ev_used[ph] = false;
   // 2. recovery: per-problem passes around one mm10 launch
+  b->mark(0, 0);
   std::vector<RecoverLists> lists(nr);
   for (int k = 0; k < nr && !rc; ++k)
     rc = recover_lists(run[k]->c, 1, run[k]->c->nelblk, lists[k]);
@@ -449,8 +481,10 @@ int w3d_batch_advance(w3d_batch* b, w3d_problem* const* ps, int32_t n,
   if (nslot > 0) {
     HIPCHK(hipMemcpyAsync(b->d_slots, b->h_slots, nslot * sizeof(Mm10Slot),
                           hipMemcpyHostToDevice, s));
+    b->mark(1, 0);
     rc = launch_mm10_batch(b->d_slots, nslot, nitems, b->d_counter,
                            b->mm10_grid, s);
+    b->mark(1, 1);
     if (rc) return fail(rc, "w3d_batch_advance: mm10 launch");
   }
   for (int stage = 2; stage <= 3 && !rc; ++stage)
@@ -459,6 +493,7 @@ int w3d_batch_advance(w3d_batch* b, w3d_problem* const* ps, int32_t n,
                                  run[k]->iter, 1, run[k]->d_u, run[k]->d_du,
                                  run[k]->d_ifv, false, 1, run[k]->c->nelblk);
   if (rc) return rc;
+  b->mark(0, 1);
   // 3. residuals and the decisions
   for (int k = 0; k < nr; ++k) {
     w3d_problem* p = run[k];
@@ -481,9 +516,11 @@ int w3d_batch_advance(w3d_batch* b, w3d_problem* const* ps, int32_t n,
   if (nr > 0) {
     HIPCHK(hipMemcpyAsync(b->d_res, b->h_res, nr * sizeof(ResSys),
                           hipMemcpyHostToDevice, s));
+    b->mark(2, 0);
     if ((rc = launch_residual_batch(b->d_res, nr, ntile_res, b->d_part,
                                     b->d_out, s)))
       return fail(rc, "w3d_batch_advance: residual launch");
+    b->mark(2, 1);
     HIPCHK(hipMemcpyAsync(b->h_out, b->d_out, 8 * nr * sizeof(double),
                           hipMemcpyDeviceToHost, s));
   }
@@ -544,6 +581,7 @@ int w3d_batch_advance(w3d_batch* b, w3d_problem* const* ps, int32_t n,
   const int ns = (int)solve.size();
   if (ns > 0) {
     int ntile = 0, maxit = 0;
+    b->mark(3, 0);
     for (int i = 0; i < ns; ++i) {
       w3d_problem* p = run[solve[i]];
       w3d_ctx* c = p->c;
@@ -575,12 +613,14 @@ int w3d_batch_advance(w3d_batch* b, w3d_problem* const* ps, int32_t n,
       ntile += S.ntile;
       maxit = std::max(maxit, S.maxit);
     }
+    b->mark(3, 1);
     // the solve's partial sums follow the residual's in d_part
     if ((rc = reserve(b, nr, 3 * (int64_t)ntile_res + 7 * (int64_t)ntile)))
       return rc;
     double* part = b->d_part + 3 * (int64_t)ntile_res;
     HIPCHK(hipMemcpyAsync(b->d_pcg, b->h_pcg, ns * sizeof(PcgSys),
                           hipMemcpyHostToDevice, s));
+    b->mark(4, 0);
     if ((rc = launch_pcg_batch_init(b->d_pcg, ns, ntile, part, b->d_state,
                                     b->d_sc, s)))
       return fail(rc, "w3d_batch_advance: solver launch");
@@ -600,6 +640,8 @@ int w3d_batch_advance(w3d_batch* b, w3d_problem* const* ps, int32_t n,
       all_done = true;
       for (int i = 0; i < ns; ++i) all_done = all_done && b->h_state[2 * i];
     }
+    b->mark(4, 1);
+    b->pcg_iterations += it;
     for (int i = 0; i < ns; ++i) {
       run[solve[i]]->pcg_iters_step += b->h_state[2 * i + 1];
       b->h_res[solve[i]].solve = 1;
@@ -607,10 +649,19 @@ int w3d_batch_advance(w3d_batch* b, w3d_problem* const* ps, int32_t n,
     // 5. du += x
     HIPCHK(hipMemcpyAsync(b->d_res, b->h_res, nr * sizeof(ResSys),
                           hipMemcpyHostToDevice, s));
+    b->mark(5, 0);
     if ((rc = launch_du_batch(b->d_res, nr, ntile_res, s)))
       return fail(rc, "w3d_batch_advance: update launch");
+    b->mark(5, 1);
   }
   HIPCHK(hipStreamSynchronize(s));
+  b->advances += 1;
+  for (int ph = 0; ph < w3d_batch::kPhases; ++ph) {
+    float t = 0.0f;
+    if (b->ev_used[ph] &&
+        hipEventElapsedTime(&t, b->ev[ph][0], b->ev[ph][1]) == hipSuccess)
+      b->phase_ms[ph] += t;
+  }
   for (int k = 0; k < nr; ++k) {
     if (lists[k].d_sel || lists[k].d_selm) {
       dfree(lists[k].d_sel);

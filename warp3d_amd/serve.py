@@ -104,12 +104,12 @@ def make_rve(spec, auto_size=128):
 
 
 def newton_opts(nsteps=10, precision=_abi.W3D_FP32, tol=1e-4, tol_abs=1e-8,
-                max_iters=20, pcg_rtol=1e-6, pcg_maxit=5000, adaptive=True):
+                max_iters=30, pcg_rtol=1e-6, pcg_maxit=5000, adaptive=True):
     """The serving defaults: the reference's default convergence test
     (0.01 % of the reaction norm), adaptive sub-increments on (the crystal
     update asks for them at 0.2 % strain per step), a symmetric tangent,
     so that convergence is linear and needs more iterations than the
-    reference's default limit of 10."""
+    reference's default limit of 10 (up to 30 per increment)."""
     return _abi.W3DNewtonOpts(nsteps=nsteps, max_iters=max_iters,
                               precision=precision, pcg_maxit=pcg_maxit,
                               adaptive=int(bool(adaptive)), reserved=0,
@@ -201,6 +201,19 @@ class Batch:
             self.h, C.cast(arr, C.POINTER(C.c_void_p)), n, _abi.iptr(st)),
             "w3d_batch_advance")
         return st[:n].tolist()
+
+    PHASES = ("recover", "mm10", "residual", "tangent_assembly", "pcg",
+              "update")
+
+    def phase_times(self):
+        """Device ms per phase over all advances, the advances and the
+        batched PCG iterations."""
+        ms = np.zeros(6)
+        na, npcg = C.c_int64(), C.c_int64()
+        self._chk(self._lib().w3d_batch_phase_times(
+            self.h, _abi.dptr(ms), C.byref(na), C.byref(npcg)),
+            "w3d_batch_phase_times")
+        return dict(zip(self.PHASES, ms.tolist())), na.value, npcg.value
 
     def close(self):
         if getattr(self, "h", None):
