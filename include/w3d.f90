@@ -57,6 +57,23 @@ module w3d
     real(c_double) :: eps_bbar = 0.0d0
   end type w3d_model
 
+  ! w3d_newton_opts: controls of a served problem (w3d_problem_create)
+  type, bind(C) :: w3d_newton_opts
+    integer(c_int32_t) :: nsteps = 1
+    integer(c_int32_t) :: max_iters = 30
+    integer(c_int32_t) :: precision = 1
+    integer(c_int32_t) :: pcg_maxit = 5000
+    integer(c_int32_t) :: adaptive = 1
+    integer(c_int32_t) :: reserved = 0
+    real(c_double) :: tol = 1.0d-4
+    real(c_double) :: tol_abs = 1.0d-8
+    real(c_double) :: pcg_rtol = 1.0d-6
+  end type w3d_newton_opts
+
+  integer(c_int32_t), parameter :: W3D_PROBLEM_RUNNING = 0, &
+       W3D_PROBLEM_STEP_DONE = 1, W3D_PROBLEM_DONE = 2, &
+       W3D_PROBLEM_FAILED = 3
+
   interface
     function w3d_last_error() bind(C, name="w3d_last_error") result(p)
       import :: c_ptr
@@ -157,6 +174,68 @@ module w3d
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: precision, capacity
       integer(c_int32_t), intent(out) :: choices(*), n_choices
+      integer(c_int) :: rc
+    end function
+    function w3d_problem_create(ctx, cstmap, du0, opts, prob) &
+         bind(C, name="w3d_problem_create") result(rc)
+      import :: c_ptr, c_int, c_int32_t, c_double, w3d_newton_opts
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(in) :: cstmap(*)
+      real(c_double), intent(in) :: du0(*)
+      type(w3d_newton_opts), intent(in) :: opts
+      type(c_ptr), intent(out) :: prob
+      integer(c_int) :: rc
+    end function
+    function w3d_problem_destroy(prob) bind(C, name="w3d_problem_destroy") &
+         result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: prob
+      integer(c_int) :: rc
+    end function
+    function w3d_problem_status(prob, status, step, iter) &
+         bind(C, name="w3d_problem_status") result(rc)
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: prob
+      integer(c_int32_t), intent(out) :: status, step, iter
+      integer(c_int) :: rc
+    end function
+    function w3d_problem_step_record(prob, step, newton_iters, pcg_iters, &
+         cuts, load, residual, reaction) &
+         bind(C, name="w3d_problem_step_record") result(rc)
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: prob
+      integer(c_int32_t), value :: step
+      integer(c_int32_t), intent(out) :: newton_iters, pcg_iters, cuts
+      real(c_double), intent(out) :: load, residual, reaction
+      integer(c_int) :: rc
+    end function
+    function w3d_problem_get_u(prob, u) bind(C, name="w3d_problem_get_u") &
+         result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: prob
+      real(c_double), intent(out) :: u(*)
+      integer(c_int) :: rc
+    end function
+    function w3d_batch_create(device, batch) bind(C, name="w3d_batch_create") &
+         result(rc)
+      import :: c_ptr, c_int, c_int32_t
+      integer(c_int32_t), value :: device
+      type(c_ptr), intent(out) :: batch
+      integer(c_int) :: rc
+    end function
+    function w3d_batch_destroy(batch) bind(C, name="w3d_batch_destroy") &
+         result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: batch
+      integer(c_int) :: rc
+    end function
+    function w3d_batch_advance(batch, probs, n, status) &
+         bind(C, name="w3d_batch_advance") result(rc)
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: batch
+      type(c_ptr), intent(in) :: probs(*)
+      integer(c_int32_t), value :: n
+      integer(c_int32_t), intent(out) :: status(*)
       integer(c_int) :: rc
     end function
     function w3d_commit_step(ctx) bind(C, name="w3d_commit_step") result(rc)
