@@ -102,7 +102,10 @@ struct w3d_problem {
   int32_t *d_eqdof = nullptr, *d_cdof = nullptr, *d_ldof = nullptr;
   int64_t bytes = 0;
   int status = W3D_PROBLEM_RUNNING;
+  // the next recovery's iteration (0: the start of an increment, as the
+  // reference's iteration 0) and whether its increment is extrapolated
   int step = 1, iter = 0;
+  bool extrap = true;
   // what the next advance does before its recovery
   enum Pending { kNone, kCommitStep, kCommitSub } pending = kNone;
   AdaptStack ad;
@@ -440,11 +443,9 @@ int w3d_batch_advance(w3d_batch* b, w3d_problem* const* ps, int32_t n,
       }
       if (rc) return rc;
       p->pending = w3d_problem::kNone;
-      p->iter = 0;
     }
     p->status = W3D_PROBLEM_RUNNING;
-    p->iter += 1;
-    p->iters_step += 1;
+    if (p->iter >= 1) p->iters_step += 1;
   }
   // residual tiles (also the update's)
   int ntile_res = 0;
@@ -462,8 +463,9 @@ int w3d_batch_advance(w3d_batch* b, w3d_problem* const* ps, int32_t n,
     rc = recover_lists(run[k]->c, 1, run[k]->c->nelblk, lists[k]);
   for (int k = 0; k < nr && !rc; ++k)
     rc = enqueue_recover_stage(run[k]->c, lists[k], 0, run[k]->step,
-                               run[k]->iter, 1, run[k]->d_u, run[k]->d_du,
-                               run[k]->d_ifv, false, 1, run[k]->c->nelblk);
+                               run[k]->iter, run[k]->extrap, run[k]->d_u,
+                               run[k]->d_du, run[k]->d_ifv, false, 1,
+                               run[k]->c->nelblk);
   if (rc) return rc;
   int nslot = 0, nitems = 0;
   for (int k = 0; k < nr; ++k) {
@@ -475,7 +477,8 @@ int w3d_batch_advance(w3d_batch* b, w3d_problem* const* ps, int32_t n,
     sl.item0 = nitems;
     sl.nwork = L.nwm;
     sl.step = run[k]->step;
-    sl.iter0_off = mm10_iter0_off(run[k]->step, run[k]->iter, 1);
+    sl.iter0_off = mm10_iter0_off(run[k]->step, run[k]->iter,
+                                  run[k]->extrap ? 1 : 0);
     nitems += L.nwm;
   }
   if (nslot > 0) {
@@ -490,8 +493,9 @@ int w3d_batch_advance(w3d_batch* b, w3d_problem* const* ps, int32_t n,
   for (int stage = 2; stage <= 3 && !rc; ++stage)
     for (int k = 0; k < nr && !rc; ++k)
       rc = enqueue_recover_stage(run[k]->c, lists[k], stage, run[k]->step,
-                                 run[k]->iter, 1, run[k]->d_u, run[k]->d_du,
-                                 run[k]->d_ifv, false, 1, run[k]->c->nelblk);
+                                 run[k]->iter, run[k]->extrap, run[k]->d_u,
+                                 run[k]->d_du, run[k]->d_ifv, false, 1,
+                                 run[k]->c->nelblk);
   if (rc) return rc;
   b->mark(0, 1);
   // 3. residuals and the decisions
@@ -538,7 +542,12 @@ int w3d_batch_advance(w3d_batch* b, w3d_problem* const* ps, int32_t n,
       continue;
     }
     const bool cut = o[4] != 0.0 || o[6] != 0.0;
-    if (!cut && p->res <= p->o.tol * std::max(p->react, p->o.tol_abs)) {
+    // iteration 0 is never accepted: it only forms the increment's first
+    // residual and stiffness (mnralg.f's iteration 0)
+    if (!cut && p->iter >= 1 &&
+        p->res <= p->o.tol * std::max(p->react, p->o.tol_abs)) {
+      p->iter = 0;
+      p->extrap = true;
       if (p->o.adaptive && p->ad.next()) {
         p->pending = w3d_problem::kCommitSub;  // the next sub-increment
         continue;
@@ -570,11 +579,13 @@ int w3d_batch_advance(w3d_batch* b, w3d_problem* const* ps, int32_t n,
         if ((rc = start_increment(p, 0.0, p->ad.disp, s))) return rc;
         p->cuts_step += 1;
         p->iter = 0;
+        p->extrap = false;
       } else {
         p->status = W3D_PROBLEM_FAILED;
       }
       continue;
     }
+    p->iter += 1;  // the next recovery follows this iteration's solve
     solve.push_back(k);
   }
   // 4. tangents, assemblies and the batched solve

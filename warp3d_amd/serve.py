@@ -455,21 +455,25 @@ def solve_plain(rve, opts, device=0):
             if step > 1:
                 start(1.0, 1.0)
             iters = pcg = cuts = 0
-            it = 0
+            it, extrap = 0, True
             while True:
-                it += 1
-                iters += 1
-                ifv, _, _, rc = dev.recover(step, it, u, du, extrapolated=True)
+                # iteration 0 starts an increment: its recovery forms the
+                # first residual and stiffness and is never accepted
+                if it >= 1:
+                    iters += 1
+                ifv, _, _, rc = dev.recover(step, it, u, du,
+                                            extrapolated=extrap)
                 cut = rc != 0
                 r = -ifv[eqdof]
                 res = float(np.sqrt(np.dot(r, r)))
                 react = float(np.sqrt(np.dot(ifv[cdof], ifv[cdof])))
-                if not cut and res <= opts.tol * max(react, opts.tol_abs):
+                if not cut and it >= 1 and \
+                        res <= opts.tol * max(react, opts.tol_abs):
                     dev.commit_step()
                     u = u + du
+                    it, extrap = 0, True
                     if opts.adaptive and ad.advance():
                         start(ad.scaling, ad.disp_fact)
-                        it = 0
                         continue
                     records.append(dict(newton=iters, pcg=pcg, cuts=cuts,
                                         load=float(ifv[ldof].sum()),
@@ -480,7 +484,7 @@ def solve_plain(rve, opts, device=0):
                         dev.rollback()
                         start(0.0, ad.disp_fact)
                         cuts += 1
-                        it = 0
+                        it, extrap = 0, False
                         continue
                     raise RuntimeError(f"step {step}: no convergence")
                 dev.tangent(step, it, opts.precision)
@@ -491,6 +495,7 @@ def solve_plain(rve, opts, device=0):
                                             opts.pcg_rtol, opts.pcg_maxit)
                 pcg += n_it
                 du[eqdof] += x.download()
+                it += 1
             times.append(time.perf_counter() - t0)
         for a in (kd, kc, rhs, x):
             a.free()
