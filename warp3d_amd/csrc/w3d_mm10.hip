@@ -504,9 +504,12 @@ __device__ inline void formJ(const Props& p, const CState& s, const double* x,
   J[48] = etau;
 }
 
-__device__ inline bool any_nan(const double* x, int n) {
+// (unrolled, so that the arrays it is given stay in registers)
+template <int N>
+__device__ inline bool any_nan(const double* x) {
   bool b = false;
-  for (int i = 0; i < n; ++i) b |= isnan(x[i]);
+#pragma unroll
+  for (int i = 0; i < N; ++i) b |= isnan(x[i]);
   return b;
 }
 
@@ -516,11 +519,12 @@ __device__ inline double dot6(const double* a, const double* b) {
 }
 
 // mm10_solve: predictor (6x6 Newton at fixed extrapolated tau) then the
-// 7x7 corrector; J receives the Jacobian of the last corrector iteration
+// 7x7 corrector; Jg (49 words at stride jstride) receives the Jacobian of
+// the last corrector iteration
 __device__ void solve(const Props& p, CState& s, const double* sn,
                       double tau_n, double tt_rate_n, const double* Dn,
                       double* stress, double* tt, bool& fail, double dtinc,
-                      double* J) {
+                      double* Jg, int64_t jstride) {
   const double c = 1.0e-4, red = 0.5;
   const int mls = 10, mmin = 1;
   double x[7], inR1;
@@ -572,22 +576,23 @@ __device__ void solve(const Props& p, CState& s, const double* sn,
     bool ok = true;
     int iter = 0;
     while (nR1 > p.atol1 && nR1 / inR1 > p.rtol1) {
-      double J11[36], mJ[36], dx1[6], wv[6];
+      double J11[36], dx1[6], wv[6];
       formJ11(p, s, x1, x2, J11);
-#pragma unroll
-      for (int k = 0; k < 6; ++k) dx1[k] = R1[k];
-#pragma unroll
-      for (int k = 0; k < 36; ++k) mJ[k] = -J11[k];
-      lu_solve<6, 1>(mJ, dx1);
-      double alpha = 1.0;
-      double dotR1 = R1[0] * R1[0] + R1[1] * R1[1] + R1[2] * R1[2] +
-                     R1[3] * R1[3] + R1[4] * R1[4] + R1[5] * R1[5];
-      double ls1 = 0.5 * dotR1;
+      // J11^T R1 first: the factorisation below overwrites J11
 #pragma unroll
       for (int j = 0; j < 6; ++j)
         wv[j] = J11[0 + 6 * j] * R1[0] + J11[1 + 6 * j] * R1[1] +
                 J11[2 + 6 * j] * R1[2] + J11[3 + 6 * j] * R1[3] +
                 J11[4 + 6 * j] * R1[4] + J11[5 + 6 * j] * R1[5];
+      // J11 dx1 = -R1 in place: bitwise the solution of (-J11) dx1 = R1
+      // (the same pivots and multipliers; every other value negated)
+#pragma unroll
+      for (int k = 0; k < 6; ++k) dx1[k] = -R1[k];
+      lu_solve<6, 1>(J11, dx1);
+      double alpha = 1.0;
+      double dotR1 = R1[0] * R1[0] + R1[1] * R1[1] + R1[2] * R1[2] +
+                     R1[3] * R1[3] + R1[4] * R1[4] + R1[5] * R1[5];
+      double ls1 = 0.5 * dotR1;
       double ls2 = c * dot6(dx1, wv);
       int ls = 0;
       for (;;) {
@@ -608,7 +613,7 @@ __device__ void solve(const Props& p, CState& s, const double* sn,
         ls = ls + 1;
       }
       iter = iter + 1;
-      if (iter > p.miter || any_nan(x1, 6)) {
+      if (iter > p.miter || any_nan<6>(x1)) {
         fail = true;
         ok = false;
         break;
@@ -630,14 +635,12 @@ __device__ void solve(const Props& p, CState& s, const double* sn,
     double nR = sqrt(dotR), inR = nR;
     if (inR == 0.0) inR = inR1;
     while ((nR > p.atol && nR / inR > p.rtol) || iter < mmin) {
-      double mJ[49], dx[7], wv[7];
+      double J[49], dx[7], wv[7];
       formJ(p, s, x, J);
+      // the tangent takes the last iteration's Jacobian: parked in
+      // global memory instead of held in registers across the loop
 #pragma unroll
-      for (int k = 0; k < 7; ++k) dx[k] = R[k];
-#pragma unroll
-      for (int k = 0; k < 49; ++k) mJ[k] = -J[k];
-      lu_solve<7, 1>(mJ, dx);
-      double alpha = 1.0;
+      for (int k = 0; k < 49; ++k) Jg[k * jstride] = J[k];
       dotR = 0.0;
 #pragma unroll
       for (int k = 0; k < 7; ++k) dotR = dotR + R[k] * R[k];
@@ -649,6 +652,11 @@ __device__ void solve(const Props& p, CState& s, const double* sn,
         for (int l = 0; l < 7; ++l) t = t + J[l + 7 * j] * R[l];
         wv[j] = t;
       }
+      // J dx = -R in place: bitwise the solution of (-J) dx = R
+#pragma unroll
+      for (int k = 0; k < 7; ++k) dx[k] = -R[k];
+      lu_solve<7, 1>(J, dx);
+      double alpha = 1.0;
       double dd = 0.0;
 #pragma unroll
       for (int k = 0; k < 7; ++k) dd = dd + dx[k] * wv[k];
@@ -673,7 +681,7 @@ __device__ void solve(const Props& p, CState& s, const double* sn,
         ls = ls + 1;
       }
       iter = iter + 1;
-      if (iter > p.miter || any_nan(x, 7)) {
+      if (iter > p.miter || any_nan<7>(x)) {
         fail = true;
         break;
       }
@@ -809,16 +817,29 @@ __device__ __forceinline__ void mm10_item(const DevModel& M,
     double temp1 = dot6(np1.D, np1.D);
     double temp2 = dot6(stress, stress);
     const bool no_load = temp2 == 0.0 && temp1 == 0.0;
-    double J[49];
-#pragma unroll
-    for (int k = 0; k < 49; ++k) J[k] = 0.0;
+    // the last corrector Jacobian lives in the crystal's n+1 history words
+    // 0-48 until the tangent has read it (mm10_store_cryhist then writes
+    // those words)
+    double* Jg = hn1 + cb;
     bool fail = false;
     double curr_tt_rate = 0.0;
     double tangent[36];
+    // the crystal's tangent into the point's average (history words 0-35;
+    // stored as soon as it is formed, so that it does not stay live)
+    auto store_tangent = [&]() {
+      if (c == 0) {
+#pragma unroll
+        for (int k = 0; k < 36; ++k) hn1[k * SG] = tangent[k];
+      } else {
+#pragma unroll
+        for (int k = 0; k < 36; ++k) hn1[k * SG] = hn1[k * SG] + tangent[k];
+      }
+    };
     if (iter0_off || no_load) {
       const double* C = p.cr->C;
 #pragma unroll
       for (int k = 0; k < 36; ++k) tangent[k] = C[k];
+      store_tangent();
       if (!no_load) {
         double R1[6];
         formR1(p, np1, sn, stress, tt, R1);
@@ -837,7 +858,7 @@ __device__ __forceinline__ void mm10_item(const DevModel& M,
         setup(p, curr);
         tt = tau_n;
         solve(p, curr, sn, tau_n, tt_rate_n, Dn, stress, &tt, fail,
-              np1.tinc * sstep, J);
+              np1.tinc * sstep, Jg, SG);
         if (fail) {
 #pragma unroll
           for (int k = 0; k < 6; ++k) stress[k] = ostress[k];
@@ -853,7 +874,7 @@ __device__ __forceinline__ void mm10_item(const DevModel& M,
           frac = frac + sstep;
         }
       }
-      if (fail || isnan(tt) || any_nan(stress, 6)) {
+      if (fail || isnan(tt) || any_nan<6>(stress)) {
         cut = true;
         break;
       }
@@ -873,7 +894,9 @@ __device__ __forceinline__ void mm10_item(const DevModel& M,
     for (int k = 0; k < 15; ++k) u[k] = 0.0;
     if (!(iter0_off || no_load)) {
       // ---- mm10_tangent: T = (J11 - J12 J21 / J22)^-1 C, symmetrised
-      double JJ[36], beta[12];
+      double J[49], JJ[36], beta[12];
+#pragma unroll
+      for (int k = 0; k < 49; ++k) J[k] = Jg[k * SG];
 #pragma unroll
       for (int cc = 0; cc < 6; ++cc)
 #pragma unroll
@@ -905,6 +928,7 @@ __device__ __forceinline__ void mm10_item(const DevModel& M,
           w[i + 6 * j] = (tangent[i + 6 * j] + tangent[j + 6 * i]) * 0.5;
 #pragma unroll
       for (int k = 0; k < 36; ++k) tangent[k] = w[k];
+      store_tangent();
       // ---- slips at the converged state (np1: full-step dg)
       double rs[NS], gam[NS];
       slips(p, np1, stress, tt, rs, gam);
@@ -1055,13 +1079,6 @@ __device__ __forceinline__ void mm10_item(const DevModel& M,
     // accumulate for the averages
 #pragma unroll
     for (int k = 0; k < 6; ++k) sig_avg[k] = sig_avg[k] + stress[k];
-    if (c == 0) {
-#pragma unroll
-      for (int k = 0; k < 36; ++k) hn1[k * SG] = tangent[k];
-    } else {
-#pragma unroll
-      for (int k = 0; k < 36; ++k) hn1[k * SG] = hn1[k * SG] + tangent[k];
-    }
 #pragma unroll
     for (int k = 0; k < NS; ++k) slip_avg[k] = slip_avg[k] + slip[k];
     t_work = t_work + work_inc;
