@@ -85,7 +85,8 @@ WORKLOADS = {
              "element-block path per step",
     "rve": "Poisson stream of crystal-plasticity RVE requests, each an n^3 "
            "hex8 unit cube (n uniform in [8, 24]) of 64 seeded Voronoi "
-           "grains with SO(3)-uniform fcc orientations, mm10 Voce, FP32 "
+           "grains with SO(3)-uniform fcc orientations, mm10 Voce (the "
+           "test82 Voce constants), FP32 "
            "tangent, symmetry planes and a strain-controlled top face, 10 "
            "steps to 2 %; served by continuous batching",
 }

@@ -105,7 +105,7 @@ def _solve(om, cst, free, eq, res):
 
 def newton_step(om, step, cst, dmap, u, du_imposed, res_tol=0.01, mniter=1,
                 mxiter=15, opts=None, du_last=None, load_n=None, dload=None,
-                scaling_load=1.0, mf_change=False):
+                scaling_load=1.0, mf_change=False, dt=1.0):
     """One load step (with its adaptive sub-increments).  Returns
     (converged, iters, ratios, u_new, du_last_new, info): iters summed over
     sub-increments, ratios of the last, du_last_new the last converged
@@ -142,6 +142,8 @@ def newton_step(om, step, cst, dmap, u, du_imposed, res_tol=0.01, mniter=1,
         # starts without extrapolation (mnralg.f:267-271)
         du[cst != 0] = du_imposed[cst != 0] * ad.disp_fact
         pbar = load_n + ad.load_fact * dload
+        if adaptive:  # a sub-increment's share of the time (mnralg.f:306)
+            om.set_time_increment(dt * ad.disp_fact)
         ifv, _, _, rc = om.recover(step, 0, u, du, extrapolated)
         failed = rc != 0
         ratios, norms = [], []
@@ -200,6 +202,8 @@ def newton_step(om, step, cst, dmap, u, du_imposed, res_tol=0.01, mniter=1,
                 info["cuts"] += 1
                 first_subinc = True
                 continue
+            if adaptive:
+                om.set_time_increment(dt)
             return False, total_iters, ratios, u, du, info
         om.commit_step()
         u = u + du
@@ -208,6 +212,8 @@ def newton_step(om, step, cst, dmap, u, du_imposed, res_tol=0.01, mniter=1,
             first_subinc = False
             scaling_load = 1.0
             continue
+        if adaptive:
+            om.set_time_increment(dt)
         return True, total_iters, ratios, u, du, info
 
 

@@ -7,9 +7,9 @@ Bars:
   in a mix of other problems that join and leave at other iterations;
 * with the FP64 tangent and tight solver tolerances the batched path and
   the plain path (per-call entry points, host Newton, host-driven PCG)
-  take the same Newton iterations and adaptive cuts per step (0.2 % strain
-  steps, which the crystal update cannot take whole) and give loads within
-  1e-9 relative and displacements within 1e-8 relative;
+  take the same Newton iterations and adaptive cuts per step (0.4 % strain
+  steps, which the crystal update cannot always take whole) and give loads
+  within 1e-9 relative and displacements within 1e-8 relative;
 * the FP32 tangent pays at most one extra Newton iteration per step, its
   loads within 1e-6 relative and displacements within 1e-6 relative of
   the FP64 solution (both converged to |R| <= 1e-7 |F_c|);
@@ -42,7 +42,7 @@ def run_alone(spec, opts):
 
 def test_alone_and_in_the_mix_are_identical():
     opts = S.newton_opts(nsteps=4)
-    kw = dict(nsteps=4, strain=0.006)
+    kw = dict(nsteps=4, strain=0.012)
     specs = {"a": S.RveSpec(6, seed=11, **kw),
              "b": S.RveSpec(5, seed=12, **kw),
              "c": S.RveSpec(7, seed=13, **kw)}
@@ -81,7 +81,7 @@ def test_alone_and_in_the_mix_are_identical():
 
 
 def test_batched_fp64_matches_the_plain_path_and_fp32_tangent():
-    spec = S.RveSpec(6, seed=21, nsteps=3, strain=0.006)
+    spec = S.RveSpec(6, seed=21, nsteps=3, strain=0.012)
     tight = dict(nsteps=3, tol=1e-7, pcg_rtol=1e-11, pcg_maxit=20000,
                  max_iters=40)
     rve = S.make_rve(spec)
@@ -115,7 +115,7 @@ def test_batched_fp64_matches_the_plain_path_and_fp32_tangent():
 def test_server_gives_each_request_its_alone_result():
     opts = S.newton_opts(nsteps=3)
     arr = S.poisson_arrivals(6, rate=400.0, nmin=4, nmax=7, seed=3,
-                             nsteps=3, strain=0.003)
+                             nsteps=3, strain=0.006)
     eng = S.DeviceEngine(opts)
     try:
         srv = S.Server(eng, max_slots=3, hbm_budget=8 << 30, prep_workers=2,

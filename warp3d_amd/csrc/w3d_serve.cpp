@@ -106,6 +106,7 @@ struct w3d_problem {
   // reference's iteration 0) and whether its increment is extrapolated
   int step = 1, iter = 0;
   bool extrap = true;
+  double dt0 = 1.0;  // the context's time increment of a whole step
   // what the next advance does before its recovery
   enum Pending { kNone, kCommitStep, kCommitSub } pending = kNone;
   AdaptStack ad;
@@ -160,8 +161,11 @@ void free_problem(w3d_problem* p) {
 }
 
 // du for the start of an increment: the free entries scaled by sf (0: no
-// extrapolation), the constrained ones the imposed values times disp
+// extrapolation), the constrained ones the imposed values times disp; the
+// time increment scaled by disp as well (mnralg.f:306-309: a sub-increment
+// takes its share of the step's time, so rates stay those of the step)
 int start_increment(w3d_problem* p, double sf, double disp, hipStream_t s) {
+  p->c->dt = p->dt0 * disp;
   int rc = launch_start_increment(p->d_eqdof, p->neq, p->d_cdof, p->d_cimp,
                                   p->ncdof, sf, disp, p->d_du, s);
   return rc ? fail(rc, "w3d_batch_advance: increment launch") : W3D_OK;
@@ -254,6 +258,7 @@ int w3d_problem_create(w3d_ctx* c, const int32_t* cstmap, const double* du0,
   w3d_problem* p = new w3d_problem();
   p->c = c;
   p->o = *o;
+  p->dt0 = c->dt;
   p->neq = neq;
   p->nd = 3 * (int64_t)c->nonode;
   std::vector<int32_t> eqdof(std::max(neq, 1)), cdof, ldof;

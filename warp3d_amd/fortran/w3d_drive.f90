@@ -122,6 +122,7 @@ program w3d_drive
     stop 1
   end if
   rc = w3d_set_time_increment(ctx, dt)
+  opts%dt = dt
   rc = w3d_pattern(ctx, cstmap, neqns, ncoeff)
   if (rc /= W3D_OK) then
     print *, 'w3d_pattern: ', w3d_error_message()

@@ -46,6 +46,7 @@ module w3d_newton
     real(c_double) :: ls_min = 0.01d0, ls_max = 1.0d0
     logical :: adaptive = .false.
     logical :: divergence_check = .true.
+    real(c_double) :: dt = 1.0d0            ! time increment of a step
   end type newton_options
 
   type :: newton_report
@@ -189,6 +190,8 @@ contains
       end if
       where (cstmap /= 0) du = du_imposed * ad%disp_fact
       pbar = load_n + ad%load_fact * dload
+      ! a sub-increment's share of the step's time (mnralg.f:306-309)
+      if (adaptive) rc = w3d_set_time_increment(ctx, opts%dt * ad%disp_fact)
       failed = .false.
       iter = 0
       rc = w3d_recover(ctx, step, 0, extrap, u, du, ifv, en, pw)
@@ -287,6 +290,7 @@ contains
       load_n = load_n + dload
       exit
     end do
+    if (adaptive) j = w3d_set_time_increment(ctx, opts%dt)
     rep%status = rc
     if (rep%converged) rep%status = W3D_OK
 
