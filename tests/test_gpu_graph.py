@@ -2,7 +2,9 @@
 autotuner against the separate calls.
 
 Bar: bit-identical to the eager sequence w3d_recover_dev + w3d_tangent +
-w3d_assemble_vss_dev on the same inputs, for every replay of every
+w3d_assemble_vss_dev on the same inputs (NaN where NaN: the n+1 history
+of a point whose crystal update asks for a step cut holds the diverged
+local iterates, which the caller discards), for every replay of every
 captured graph (steps 1 and 2, iterations 0-3, with eager work and commits
 between the replays), for mm01 small strain, mm01 finite strain and mm10;
 and the autotuned work lists give the same tangents and matrices.
@@ -92,7 +94,7 @@ def test_graph_iteration_is_the_eager_sequence(W, name, prec):
             eb = b.graph(step, it, prec)
             assert ea == eb, (ea, eb)
             for x, y in zip(a.outputs(), b.outputs()):
-                assert np.array_equal(x, y)
+                assert np.array_equal(x, y, equal_nan=True)
         for r in (a, b):
             r.dev.commit_step()
         u = u + duf
@@ -113,10 +115,10 @@ def test_autotuned_lists_give_the_same_results(W):
     ea = a.eager(1, 2, _abi.W3D_FP32)
     assert b.eager(1, 2, _abi.W3D_FP32) == ea
     for x, y in zip(a.outputs(), b.outputs()):
-        assert np.array_equal(x, y)
+        assert np.array_equal(x, y, equal_nan=True)
     # and the graph path with the tuned lists (the same status: this
     # increment may ask for a step cut)
     eb = b.graph(1, 2, _abi.W3D_FP32)
     for x, y in zip(a.outputs(), b.outputs()):
-        assert np.array_equal(x, y)
+        assert np.array_equal(x, y, equal_nan=True)
     assert eb == ea
