@@ -744,11 +744,8 @@ __device__ __forceinline__ void mm10_item(const DevModel& M,
   double* sn1 = M.stress_n1 + 9 * bd.q_off + q;
   double* cp = M.cep + 21 * bd.q_off + q;
   const double* ud = M.uddt + 6 * bd.q_off + q;
-  double uddt[6], urcs_n[9];
-#pragma unroll
-  for (int k = 0; k < 6; ++k) uddt[k] = ud[k * SG];
-#pragma unroll
-  for (int k = 0; k < 9; ++k) urcs_n[k] = sn9[k * SG];
+  // (inputs and sums are read from and kept in memory where they are
+  // needed, not held in registers across the local solves)
   const double dt = M.dt;
 
   Props p;
@@ -767,13 +764,10 @@ __device__ __forceinline__ void mm10_item(const DevModel& M,
   p.alter_mode = md.cp.alter_mode;
   p.angle_type = md.cp.angle_type;
 
-  // crystal sums of the tangent and the slips accumulate in their output
-  // words (history 1-36 and the cep slots), read back only by this thread
-  double sig_avg[6] = {0, 0, 0, 0, 0, 0};
+  // crystal sums of the tangent, the stresses and the slips accumulate in
+  // their output words (history 1-36, stress n+1 1-6, history 76-87), read
+  // back only by this thread
   double t_work = 0.0, p_work = 0.0, p_strain = 0.0;
-  double slip_avg[NS];
-#pragma unroll
-  for (int k = 0; k < NS; ++k) slip_avg[k] = 0.0;
   bool cut = false;
 
   for (int c = 0; c < ncry; ++c) {
@@ -784,7 +778,7 @@ __device__ __forceinline__ void mm10_item(const DevModel& M,
     double sn[6], Rpn[9], Dn[6], tau_n, tt_rate_n;
     if (step == 1) {
 #pragma unroll
-      for (int k = 0; k < 6; ++k) sn[k] = urcs_n[k];
+      for (int k = 0; k < 6; ++k) sn[k] = sn9[k * SG];
 #pragma unroll
       for (int k = 0; k < 9; ++k) Rpn[k] = (k % 4 == 0) ? 1.0 : 0.0;
 #pragma unroll
@@ -806,7 +800,7 @@ __device__ __forceinline__ void mm10_item(const DevModel& M,
     CState np1;
     slip_geometry(p, Rpn, gl, np1);
 #pragma unroll
-    for (int k = 0; k < 6; ++k) np1.D[k] = uddt[k];
+    for (int k = 0; k < 6; ++k) np1.D[k] = ud[k * SG];
     np1.tinc = dt;
     np1.tt_rate = 0.0;
     setup(p, np1);
@@ -932,8 +926,14 @@ __device__ __forceinline__ void mm10_item(const DevModel& M,
       // ---- slips at the converged state (np1: full-step dg)
       double rs[NS], gam[NS];
       slips(p, np1, stress, tt, rs, gam);
-      // ---- mm10_update_rotation: Rp = expm(W(wbarp)) Rp_n
+      // ---- mm10_update_rotation: Rp = expm(W(wbarp)) Rp_n (Rp_n read
+      // again here)
       {
+        double Rpe[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+          Rpe[k] = step == 1 ? ((k % 4 == 0) ? 1.0 : 0.0)
+                             : hn[cb + (9 + k) * SG];
         double wb[3];
         form_w(p, np1, rs, gam, wb);
         double W[9] = {0.0, -wb[2], -wb[1], wb[2], 0.0, -wb[0],
@@ -968,8 +968,8 @@ __device__ __forceinline__ void mm10_item(const DevModel& M,
         for (int j = 0; j < 3; ++j)
 #pragma unroll
           for (int i = 0; i < 3; ++i)
-            Rp1[i + 3 * j] = A[i] * Rpn[0 + 3 * j] + A[i + 3] * Rpn[1 + 3 * j] +
-                             A[i + 6] * Rpn[2 + 3 * j];
+            Rp1[i + 3 * j] = A[i] * Rpe[0 + 3 * j] + A[i + 3] * Rpe[1 + 3 * j] +
+                             A[i + 6] * Rpe[2 + 3 * j];
       }
       // ---- mm10_output
 #pragma unroll
@@ -1078,9 +1078,11 @@ __device__ __forceinline__ void mm10_item(const DevModel& M,
     }
     // accumulate for the averages
 #pragma unroll
-    for (int k = 0; k < 6; ++k) sig_avg[k] = sig_avg[k] + stress[k];
+    for (int k = 0; k < 6; ++k)
+      sn1[k * SG] = c == 0 ? stress[k] : sn1[k * SG] + stress[k];
 #pragma unroll
-    for (int k = 0; k < NS; ++k) slip_avg[k] = slip_avg[k] + slip[k];
+    for (int k = 0; k < NS; ++k)
+      hn1[(75 + k) * SG] = c == 0 ? slip[k] : hn1[(75 + k) * SG] + slip[k];
     t_work = t_work + work_inc;
     p_work = p_work + p_work_inc;
     p_strain = p_strain + p_strain_inc;
@@ -1112,22 +1114,18 @@ __device__ __forceinline__ void mm10_item(const DevModel& M,
     return;
   }
   const double rn = (double)ncry;
-#pragma unroll
-  for (int k = 0; k < 6; ++k) sig_avg[k] = sig_avg[k] / rn;
   double tang_avg[36];
 #pragma unroll
   for (int k = 0; k < 36; ++k) tang_avg[k] = hn1[k * SG] / rn;
-#pragma unroll
-  for (int k = 0; k < NS; ++k) slip_avg[k] = slip_avg[k] / rn;
   t_work = t_work / rn;
   p_work = p_work / rn;
   p_strain = p_strain / rn;
   // mm10_a_store_crystal
 #pragma unroll
-  for (int k = 0; k < 6; ++k) sn1[k * SG] = sig_avg[k];
-  sn1[6 * SG] = urcs_n[6] + t_work;
-  sn1[7 * SG] = urcs_n[7] + p_work;
-  sn1[8 * SG] = urcs_n[8] + p_strain;
+  for (int k = 0; k < 6; ++k) sn1[k * SG] = sn1[k * SG] / rn;
+  sn1[6 * SG] = sn9[6 * SG] + t_work;
+  sn1[7 * SG] = sn9[7 * SG] + p_work;
+  sn1[8 * SG] = sn9[8 * SG] + p_strain;
 #pragma unroll
   for (int k = 0; k < 36; ++k) hn1[k * SG] = tang_avg[k];
 #pragma unroll
@@ -1143,7 +1141,7 @@ __device__ __forceinline__ void mm10_item(const DevModel& M,
 #pragma unroll
   for (int k = 0; k < NS; ++k) {
     double base = step == 1 ? 0.0 : hn[(75 + k) * SG];
-    hn1[(75 + k) * SG] = base + slip_avg[k];
+    hn1[(75 + k) * SG] = base + hn1[(75 + k) * SG] / rn;
   }
   // [D] for the tangent kernel: the lower triangle of history words 1-36
 #pragma unroll
