@@ -218,12 +218,15 @@ def newton_step(om, step, cst, dmap, u, du_imposed, res_tol=0.01, mniter=1,
 
 
 def run_history(arrays, cst, du_steps, opts=None, dloads=None, mfs=None,
-                **kw):
+                dt=1.0, om=None, **kw):
     """All steps on the oracle; returns a list of (converged, iters,
     ratios, u, info) per step.  dloads: external load increments per step;
     mfs: the steps' load multipliers (extrapolation scales by mf/mf_prev;
-    None = proportional steps)."""
-    om = O.OracleModel(arrays)
+    None = proportional steps); dt: the time increment of a step; om: an
+    oracle model to run on (its state is the result), else a new one."""
+    if om is None:
+        om = O.OracleModel(arrays)
+    om.set_time_increment(dt)
     om.pattern(cst)
     dmap, _, _, _ = om.get_pattern()
     nd = 3 * arrays.nonode
@@ -239,7 +242,7 @@ def run_history(arrays, cst, du_steps, opts=None, dloads=None, mfs=None,
             sl = mfs[s - 1] / (prev if abs(prev) > 1e-20 else 1.0)
         conv, it, ratios, u, du_last, info = newton_step(
             om, s, cst, dmap, u, dui, opts=opts, du_last=du_last,
-            load_n=load, dload=dl, scaling_load=sl, **kw)
+            load_n=load, dload=dl, scaling_load=sl, dt=dt, **kw)
         load = load + dl
         out.append((conv, it, ratios, u.copy(), info))
         if not conv:
