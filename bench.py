@@ -57,9 +57,9 @@ def kernel_name(phase, workload, fp32=True):
     tmpl = {"hex8": "8, 8, true", "hex20": "20, 27, false",
             "mixed": "8, 8, true"}[workload]
     prec = "float" if fp32 else "double"
-    return {"recover": f"k_recover<{tmpl}, 0>",
+    return {"recover": f"k_recover<{tmpl}, 0, false>",
             "ifv_gather": "k_ifv_gather",
-            "tangent": f"k_tangent_mfma<{tmpl}, {prec}>",
+            "tangent": f"k_tangent_mfma<{tmpl}, {prec}, false>",
             "assemble": f"k_assemble<{prec}>",
             "energy_reduce": "k_energy_reduce",
             "mm10": "k_mm10"}[phase]

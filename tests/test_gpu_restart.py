@@ -27,8 +27,9 @@ def test_device_state_round_trip_and_into_the_oracle(tmp_path):
     a.pattern(cst)
     u = np.zeros(3 * m.nonode)
     for s in (1, 2):
-        du = smooth_field(m, 0.003, seed=s)
-        a.recover(s, 1, u, du)
+        du = smooth_field(m, 0.0015, seed=s)
+        rc = a.recover(s, 1, u, du)[3]
+        assert rc == 0, f"step {s}: the crystal update asks for a cut"
         a.commit_step()
         u = u + du
     path = str(tmp_path / "dev.db")
@@ -42,9 +43,11 @@ def test_device_state_round_trip_and_into_the_oracle(tmp_path):
     for blk in range(1, a.nelblk + 1):
         for w in (R.HIST_N, R.STRAIN_N, R.STRESS_N):
             x = a.get_block_state(blk, w)
-            assert np.array_equal(x, b.get_block_state(blk, w))
+            y = b.get_block_state(blk, w)
+            bad = np.flatnonzero(x != y)
+            assert bad.size == 0, (blk, w, bad[:8], np.isnan(x).sum())
             assert np.array_equal(x, o.get_block_state(blk, w))
-    du = smooth_field(m, 0.003, seed=3)
+    du = smooth_field(m, 0.0015, seed=3)
     fa = a.recover(3, 1, u, du)[0]
     fb = b.recover(3, 1, u, du)[0]
     fo = o.recover(3, 1, u, du)[0]
