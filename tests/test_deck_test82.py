@@ -18,11 +18,12 @@ from warp3d_amd import deck_input as DI
 HERE = os.path.join(os.path.dirname(__file__), "golden", "test82_voche")
 
 
-def run_case(om=None):
+def run_case(make=None):
+    """The deck's 8 steps on a model made by make(arrays) (the oracle's by
+    default)."""
     dk = DI.read(os.path.join(HERE, "warp3d_voche.inp"))
     pb = DI.build(dk)
-    if om is None:
-        om = O.OracleModel(pb.arrays)
+    om = (make or O.OracleModel)(pb.arrays)
     opts = NR.NewtonOptions(**pb.newton)
     hist = NR.run_history(pb.arrays, pb.cstmap,
                           [np.zeros(3 * dk.nonode)] * pb.nsteps, opts=opts,

@@ -42,11 +42,14 @@ def test_device_state_round_trip_and_into_the_oracle(tmp_path):
     R.load_element_state(o, path)
     for blk in range(1, a.nelblk + 1):
         for w in (R.HIST_N, R.STRAIN_N, R.STRESS_N):
-            x = a.get_block_state(blk, w)
-            y = b.get_block_state(blk, w)
+            # compared as bit patterns: words that hold NaN (the parked
+            # corrector Jacobian of a crystal whose last local solve was
+            # abandoned) must read back as the same NaN
+            x = a.get_block_state(blk, w).view(np.uint64)
+            y = b.get_block_state(blk, w).view(np.uint64)
             bad = np.flatnonzero(x != y)
-            assert bad.size == 0, (blk, w, bad[:8], np.isnan(x).sum())
-            assert np.array_equal(x, o.get_block_state(blk, w))
+            assert bad.size == 0, (blk, w, bad[:8], x.size)
+            assert np.array_equal(x, o.get_block_state(blk, w).view(np.uint64))
     du = smooth_field(m, 0.0015, seed=3)
     fa = a.recover(3, 1, u, du)[0]
     fb = b.recover(3, 1, u, du)[0]
