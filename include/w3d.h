@@ -244,6 +244,33 @@ int w3d_get_pattern(const w3d_ctx* ctx, int32_t* dof_eqn_map,
 int w3d_assemble_vss(w3d_ctx* ctx, double* k_diag, double* k_coeffs);
 int w3d_assemble_vss_dev(w3d_ctx* ctx, double* d_k_diag, double* d_k_coeffs);
 
+/* Asymmetric assembly, the solution parameter "assembly asymmetric on"
+ * (main_data asymmetric_assembly):
+ *  - mm10 keeps the tangent it forms unsymmetrised (mm10_a.f:1165);
+ *  - w3d_tangent (FP64 only, whole model) forms the full Ke of every
+ *    element, bdbt_asym (gptns1.f:1513): word col*T + row of an element
+ *    holds B(row)^T D B(col), D the point's history words 1-36
+ *    (drive_10_cnst, gptns1.f:1371) or mm01's cep (extract_D_symmetric);
+ *  - w3d_assemble_csr sums them into the values of the full CSR pattern
+ *    of w3d_get_csr_pattern (per row: the diagonal and every coupled
+ *    equation, ascending 0-based columns), row srow gathering the words
+ *    (erow-1)*T + ecol of each element on its node in inverse-incidence
+ *    order, assem_by_row_asymmetric / assem_a_row_asym
+ *    (assemble_code.f:1050) -- the CSR that convert_vss_csr hands the
+ *    asymmetric solver (drive_assemble_solve.f:757);
+ *  - the symmetric paths (w3d_assemble_vss*, w3d_iteration_dev, the
+ *    batched server) refuse the context while it is on; finite-strain
+ *    blocks are refused (W3D_ERR_UNSUPPORTED).
+ * Switching clears the last tangent.  w3d_get_block_ek_full copies block
+ * blk's full Ke (span * T * T words, element after element). */
+int w3d_set_asymmetric(w3d_ctx* ctx, int32_t on);
+int w3d_get_csr_pattern(const w3d_ctx* ctx, int64_t* nnz, int64_t* rp,
+                        int32_t* col);
+int w3d_assemble_csr(w3d_ctx* ctx, double* val);
+int w3d_assemble_csr_dev(w3d_ctx* ctx, double* d_val);
+int w3d_get_block_ek_full(const w3d_ctx* ctx, int32_t blk, double* out,
+                          int64_t n_out);
+
 /*
  * Jacobi-preconditioned conjugate gradients, FP64, on the device, for the
  * matrix of the last w3d_assemble_vss call (its outputs may be NULL to

@@ -32,6 +32,12 @@ def lib():
         L.orc_last_error.restype = C.c_char_p
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_tangent.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.orc_set_asymmetric.argtypes = [C.c_void_p, C.c_int]
+        L.orc_assemble_csr.argtypes = [C.c_void_p, C.POINTER(C.c_int64),
+                                       C.POINTER(C.c_int32),
+                                       C.POINTER(C.c_double)]
+        L.orc_get_ek_full.argtypes = [C.c_void_p, C.c_int,
+                                      C.POINTER(C.c_double), C.c_int64]
         L.orc_ngp.restype = C.c_int
         L.orc_ngp.argtypes = [C.c_int, C.c_int]
         L.orc_getgpts.argtypes = [C.c_int, C.c_int, C.c_int] + \
@@ -56,7 +62,7 @@ def lib():
         L.orc_mm10_gp.restype = C.c_int
         L.orc_mm10_gp.argtypes = [C.POINTER(W3DMaterial), C.POINTER(Crystal),
                                   C.c_int, C.c_int, C.c_double, _D, _D, _D,
-                                  _D, _D]
+                                  _D, _D, C.c_int]
         L.orc_mm10_residual.argtypes = [C.POINTER(W3DMaterial),
                                         C.POINTER(Crystal), _D, _D,
                                         C.c_double, _D, _D, _D]
@@ -180,7 +186,8 @@ def mm10_crystal(mat, angles):
     return cr
 
 
-def mm10_gp(mat, crystals, step, iter0_off, dt, uddt, urcs_n, hist_n):
+def mm10_gp(mat, crystals, step, iter0_off, dt, uddt, urcs_n, hist_n,
+            asym=False):
     """One mm10 point: returns (rc, urcs_n1, hist_n1, hist_n as used)."""
     ncry = len(crystals)
     arr = (Crystal * ncry)(*crystals)
@@ -191,7 +198,7 @@ def mm10_gp(mat, crystals, step, iter0_off, dt, uddt, urcs_n, hist_n):
                            step, int(iter0_off), dt,
                            dptr(np.ascontiguousarray(uddt, dtype=float)),
                            dptr(np.ascontiguousarray(urcs_n, dtype=float)),
-                           dptr(u1), dptr(hn), dptr(hn1))
+                           dptr(u1), dptr(hn), dptr(hn1), int(asym))
     return rc, u1, hn1, hn
 
 
@@ -401,6 +408,52 @@ class OracleModel:
         kc = np.zeros(max(self.ncoeff, 1))
         _chk(lib().orc_assemble_vss(self.h, dptr(kd), dptr(kc)), "assemble")
         return kd, kc[:self.ncoeff]
+
+    asym = False
+
+    def set_asymmetric(self, on):
+        _chk(lib().orc_set_asymmetric(self.h, int(bool(on))), "asymmetric")
+        self.asym = bool(on)
+
+    def csr_pattern(self):
+        """The full CSR pattern of the equations (rows: the diagonal and
+        every column coupled to it, ascending, 0-based) from the upper
+        triangle of the sparsity pattern."""
+        _, _, kp, ki = self.get_pattern()
+        n = self.neqns
+        rows = np.repeat(np.arange(n), kp)
+        cols = ki.astype(np.int64) - 1
+        r = np.concatenate([np.arange(n), rows, cols])
+        c = np.concatenate([np.arange(n), cols, rows])
+        order = np.lexsort((c, r))
+        r, c = r[order], c[order]
+        rp = np.zeros(n + 1, np.int64)
+        np.cumsum(np.bincount(r, minlength=n), out=rp[1:])
+        return rp, c.astype(np.int32)
+
+    def assemble_csr(self, rp=None, col=None):
+        """(rp, col, values) of the full CSR pattern (this model's own when
+        rp is None) from the full element matrices of the last tangent
+        (asymmetric assembly)."""
+        if rp is None:
+            rp, col = self.csr_pattern()
+        rp = np.ascontiguousarray(rp, np.int64)
+        col = np.ascontiguousarray(col, np.int32)
+        val = np.zeros(max(int(rp[-1]), 1))
+        _chk(lib().orc_assemble_csr(
+            self.h, rp.ctypes.data_as(C.POINTER(C.c_int64)),
+            col.ctypes.data_as(C.POINTER(C.c_int32)), dptr(val)),
+            "assemble_csr")
+        return rp, col, val[:int(rp[-1])]
+
+    def get_ek_full(self, blk):
+        """(span, T, T): element e's word [e, c, r] = B(r)^T D B(c)."""
+        info = self.block_info(blk)
+        span, T = info["span"], info["totdof"]
+        out = np.zeros(span * T * T)
+        _chk(lib().orc_get_ek_full(self.h, blk, dptr(out), out.size),
+             "get_ek_full")
+        return out.reshape(span, T, T)
 
     def commit_step(self):
         _chk(lib().orc_commit_step(self.h), "commit")

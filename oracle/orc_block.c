@@ -60,6 +60,7 @@ typedef struct {
   orc_crystal* crys;          /* mm10: (ncry, span) crystal constants    */
   int geonl;                  /* finite strains and rotations            */
   double *rot_n, *rot_n1;     /* geonl: (9, ngp, span) [R] column-major  */
+  double* ekf;                /* asymmetric: (T*T, span) full Ke         */
 } orc_block;
 
 struct orc_model {
@@ -72,6 +73,7 @@ struct orc_model {
   double eps_bbar;
   int32_t* elblks; /* (4, nelblk) */
   double dt;       /* time increment of the load step */
+  int asym;        /* asymmetric assembly                */
   orc_block* blk;
   int32_t* elem_blk; /* (noelem) 0-based block of each element */
   /* inverse incidences: per node, elements in ascending order */
@@ -340,6 +342,7 @@ void orc_destroy(orc_model* om) {
       free(B->urcs_n); free(B->urcs_n1); free(B->eps_n); free(B->eps_n1);
       free(B->hist_n); free(B->hist_n1); free(B->cep); free(B->ek);
       free(B->einfvec); free(B->evol); free(B->edest); free(B->crys);
+      free(B->ekf);
       free(B->rot_n); free(B->rot_n1);
     }
   free(om->blk); free(om->elem_blk); free(om->c); free(om->incid);
@@ -539,7 +542,8 @@ static int recover_block(orc_model* om, int b, int step, int iter,
         int iter0_off = iter == 0 && (!extrapolated || step == 1);
         double* h1 = B->hist_n1 + q * H;
         if (orc_mm10_gp(mat, B->crys + (int64_t)i * B->ncry, step, iter0_off,
-                        om->dt, deps, cgn, cgn1, hloc + q * H, h1))
+                        om->dt, deps, cgn, cgn1, hloc + q * H, h1,
+                        om->asym))
           cut = 1;
         /* [D] as stored for the tangent: history words 1-36 (6x6,
            column-major, drive_10_cnst) */
@@ -765,6 +769,8 @@ static void tangent_block(orc_model* om, int b) {
     elem_kinematics(B, &t, ce, dj, gama, vol);
     double* ek = B->ek + (int64_t)i * U; /* transposed layout (utsz,span) */
     memset(ek, 0, sizeof(double) * U);
+    double* ef = om->asym ? B->ekf + (int64_t)i * T * T : NULL;
+    if (ef) memset(ef, 0, sizeof(double) * T * T);
     for (int g = 0; g < G; ++g) {
       orc_bmatrix(N, gama[g], t.nxi[g], t.neta[g], t.nzeta[g],
                   B->bbar ? vol : NULL, om->eps_bbar, bmat);
@@ -809,6 +815,18 @@ static void tangent_block(orc_model* om, int b) {
                            dm[3 * 6 + kk] * bmat[3 * T + j] +
                            dm[4 * 6 + kk] * bmat[4 * T + j] +
                            dm[5 * 6 + kk] * bmat[5 * T + j];
+      if (ef) { /* bdbt_asym: word j = col T + row */
+        int j = 0;
+        for (int col = 0; col < T; ++col)
+          for (int row = 0; row < T; ++row, ++j)
+            ef[j] = ef[j] + bmat[0 * T + col] * bd[0 * T + row] +
+                    bmat[1 * T + col] * bd[1 * T + row] +
+                    bmat[2 * T + col] * bd[2 * T + row] +
+                    bmat[3 * T + col] * bd[3 * T + row] +
+                    bmat[4 * T + col] * bd[4 * T + row] +
+                    bmat[5 * T + col] * bd[5 * T + row];
+        continue;
+      }
       for (int j = 0; j < U; ++j) {
         int row = icp_r[j] - 1, col = icp_c[j] - 1;
         ek[j] = ek[j] + bmat[0 * T + col] * bd[0 * T + row] +
@@ -832,6 +850,15 @@ int orc_tangent_blocks(orc_model* om, int first, int n, int step, int iter) {
   (void)iter;
   if (first < 1 || n < 0 || first - 1 + n > om->nelblk)
     return fail(W3D_ERR_ARG, "orc_tangent_blocks: block range");
+  for (int b = first - 1; b < first - 1 + n && om->asym; ++b) {
+    orc_block* B = &om->blk[b];
+    if (B->geonl)
+      return fail(W3D_ERR_UNSUPPORTED,
+                  "asymmetric assembly of finite-strain blocks");
+    if (!B->ekf)
+      B->ekf = (double*)malloc(sizeof(double) * B->totdof * B->totdof *
+                               (size_t)B->span);
+  }
 #pragma omp parallel for schedule(dynamic, 1) \
     num_threads(g_threads > 0 ? g_threads : omp_get_max_threads())
   for (int b = first - 1; b < first - 1 + n; ++b) tangent_block(om, b);
@@ -979,6 +1006,69 @@ int orc_assemble_vss(orc_model* om, double* k_diag, double* k_coeffs) {
     }
     free(coeff_row);
   }
+  return 0;
+}
+
+int orc_set_asymmetric(orc_model* om, int on) {
+  om->asym = on != 0;
+  return 0;
+}
+
+/* assem_by_row_asymmetric / assem_a_row_asym (assemble_code.f:1050): the
+   row of equation srow gathers, over the elements on its node in
+   inverse-incidence order, the Ke words (erow - 1) T + ecol of the
+   element row erow that maps to srow */
+int orc_assemble_csr(orc_model* om, const int64_t* rp, const int32_t* col,
+                     double* val) {
+  if (!om->dof_eqn_map) return fail(W3D_ERR_ARG, "no pattern");
+  if (!om->asym) return fail(W3D_ERR_ARG, "asymmetric assembly is off");
+  for (int b = 0; b < om->nelblk; ++b)
+    if (!om->blk[b].ekf) return fail(W3D_ERR_ARG, "no full Ke: run tangent");
+  int neq = om->neqns, bad = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(| : bad) \
+    num_threads(g_threads > 0 ? g_threads : omp_get_max_threads())
+  for (int srow = 1; srow <= neq; ++srow) {
+    const int64_t r0 = rp[srow - 1], r1 = rp[srow];
+    for (int64_t k = r0; k < r1; ++k) val[k] = 0.0;
+    int snode = om->eqn_node_map[srow - 1];
+    for (int p = om->inv_ptr[snode - 1]; p < om->inv_ptr[snode]; ++p) {
+      int e = om->inv_list[p] - 1;
+      const orc_block* B = &om->blk[om->elem_blk[e]];
+      int i = e - (B->felem - 1);
+      const int32_t* ed = B->edest + (int64_t)i * B->totdof;
+      int T = B->totdof;
+      const double* emat = B->ekf + (int64_t)i * T * T;
+      int lsc[MXDOF];
+      for (int k = 0; k < T; ++k) lsc[k] = om->dof_eqn_map[ed[k] - 1];
+      for (int erow = 0; erow < T; ++erow) {
+        if (lsc[erow] != srow) continue;
+        for (int ecol = 0; ecol < T; ++ecol) {
+          int scol = lsc[ecol];
+          if (scol == 0) continue;
+          int64_t lo = r0, hi = r1 - 1;
+          while (lo < hi) {
+            int64_t mid = (lo + hi) / 2;
+            if (col[mid] < scol - 1) lo = mid + 1; else hi = mid;
+          }
+          if (lo >= r1 || col[lo] != scol - 1) { bad = 1; continue; }
+          val[lo] = val[lo] + emat[erow * T + ecol];
+        }
+        break; /* no repeated incidences in these meshes */
+      }
+    }
+  }
+  return bad ? fail(W3D_ERR_ARG, "orc_assemble_csr: pattern lacks a column")
+             : 0;
+}
+
+int orc_get_ek_full(const orc_model* om, int blk, double* out,
+                    int64_t n_out) {
+  if (blk < 1 || blk > om->nelblk) return fail(W3D_ERR_ARG, "block");
+  const orc_block* B = &om->blk[blk - 1];
+  int64_t n = (int64_t)B->totdof * B->totdof * B->span;
+  if (!B->ekf) return fail(W3D_ERR_ARG, "no full Ke: run tangent");
+  if (n_out < n) return fail(W3D_ERR_ARG, "output too small");
+  memcpy(out, B->ekf, sizeof(double) * n);
   return 0;
 }
 

@@ -1044,7 +1044,7 @@ int orc_mm10_hist_size(int ncry) { return NCOMMON + NCRYS * ncry; }
 int orc_mm10_gp(const w3d_material* mat, const orc_crystal* crys, int step,
                 int iter0_off, double dt, const double* uddt,
                 const double* urcs_n, double* urcs_n1, double* hn,
-                double* hn1) {
+                double* hn1, int asym) {
   const int ncry = mat->ncrystals > 0 ? mat->ncrystals : 1;
   static const double I3[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
   if (step == 1) { /* mm10_init_general_hist, _uout_hist, _slip_hist */
@@ -1108,7 +1108,8 @@ int orc_mm10_gp(const w3d_material* mat, const orc_crystal* crys, int step,
     if (fail) return 1;
     if (!(iter0_off || no_load)) {
       tangent(&p, &np1, J);
-      make_symm(np1.tangent);
+      /* mm10_a.f:1165: kept as formed for asymmetric assembly */
+      if (!asym) make_symm(np1.tangent);
       update_rotation(&p, &np1, &n);
       output(&p, &np1, pst);
     }

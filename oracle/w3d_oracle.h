@@ -68,11 +68,12 @@ int orc_mm10_hist_size(int ncrystals);
 /* mm10 for one Gauss point (all its crystals).  hn is the point's gathered
    n-history (re-initialised in place at step 1), hn1 receives the n+1
    history, urcs 9 words.  Returns 0, or 1 when the local solve fails (the
-   reference's material_cut_step). */
+   reference's material_cut_step).  asym: the tangent is not symmetrised
+   (asymmetric assembly, mm10_a.f:1165). */
 int orc_mm10_gp(const w3d_material* mat, const orc_crystal* crys, int step,
                 int iter0_off, double dt, const double* uddt,
                 const double* urcs_n, double* urcs_n1, double* hn,
-                double* hn1);
+                double* hn1, int asym);
 void orc_mm10_residual(const w3d_material* mat, const orc_crystal* cr,
                        const double* hist_crystal, const double* uddt,
                        double dt, const double* x, double* R, double* J);
@@ -127,6 +128,18 @@ int orc_get_pattern(const orc_model* om, int32_t* dof_eqn_map,
                     int32_t* eqn_node_map, int32_t* k_ptrs,
                     int32_t* k_indexes);
 int orc_assemble_vss(orc_model* om, double* k_diag, double* k_coeffs);
+/* Asymmetric assembly (the solution parameter asymmetric_assembly): mm10
+   keeps its tangent unsymmetrised, the tangent forms the full Ke of every
+   element (bdbt_asym, gptns1.f:1513) and orc_assemble_csr sums it into the
+   values of a full CSR pattern given by the caller (rp: neqns+1 offsets,
+   col: 0-based equations, ascending in each row) in the reference's order
+   (assem_a_row_asym, assemble_code.f:1050). */
+int orc_set_asymmetric(orc_model* om, int on);
+int orc_assemble_csr(orc_model* om, const int64_t* rp, const int32_t* col,
+                     double* val);
+/* The full Ke of the elements of block blk (span x T*T, each in bdbt_asym's
+   order: word col*T + row holds B(row)^T D B(col)). */
+int orc_get_ek_full(const orc_model* om, int blk, double* out, int64_t n_out);
 int orc_commit_step(orc_model* om);
 int orc_rollback(orc_model* om);
 int orc_get_block_state(const orc_model* om, int blk, int which, double* out,

@@ -93,8 +93,12 @@ def line_search(evaluate, opts):
 
 
 def _solve(om, cst, free, eq, res):
-    kd, kc = om.assemble_vss()
-    K = vss_matrix(om, kd, kc)
+    if om.asym:  # asymmetric assembly: the full CSR rows as assembled
+        rp, col, val = om.assemble_csr()
+        K = sp.csr_matrix((val, col, rp), shape=(om.neqns, om.neqns)).tocsc()
+    else:
+        kd, kc = om.assemble_vss()
+        K = vss_matrix(om, kd, kc)
     rhs = np.zeros(om.neqns)
     rhs[eq] = res[free]
     x = spla.spsolve(K, rhs)

@@ -1,0 +1,80 @@
+"""Asymmetric assembly on the device against the oracle (whose full Ke and
+CSR gather are pinned in test_asym.py): mm10's unsymmetrised tangent in
+the history, the full element matrices and the CSR values within the FP64
+bar (1e-10 relative), the CSR pattern identical."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests._util import RTOL_FP64, assert_close, smooth_field
+from warp3d_amd import _abi
+from warp3d_amd import mesh as M
+from warp3d_amd.serve import RVE_MATERIAL
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(kind):
+    if kind.startswith("mm10"):
+        m = M.box_hex8(3, jitter=0.05, seed=5)
+        cp = _abi.mm10_material(**RVE_MATERIAL)
+        arr = m.model(cp, auto_size=8, bbar=1 if kind == "mm10-bbar" else 0,
+                      angles=M.random_orientations(m.noelem, seed=6))
+        return m, arr, 0.004
+    mat = _abi.mm01_material(200e3, 0.3, 400.0, hprime=2000.0)
+    m = {"hex20": lambda: M.box_hex20(2, jitter=0.05, seed=3),
+         "tet10": lambda: M.box_tet10(2, jitter=0.05, seed=3)}[kind]()
+    return m, m.model(mat, auto_size=8), 0.003
+
+
+def _drive(model, m, amp):
+    model.set_asymmetric(True)
+    cst = np.zeros(3 * m.nonode, np.int32)
+    cst[:3] = 1
+    cst[5] = 1
+    model.pattern(cst)
+    u = np.zeros(3 * m.nonode)
+    for s in (1, 2):
+        du = smooth_field(m, amp, seed=s)
+        assert model.recover(s, 1, u, du)[3] == 0
+        model.commit_step()
+        u = u + du
+    du = smooth_field(m, amp, seed=3)
+    ifv = model.recover(3, 1, u, du)[0]
+    model.tangent(3, 1)
+    return ifv
+
+
+@pytest.mark.parametrize("kind", ["mm10-bbar", "mm10", "hex20", "tet10"])
+def test_full_matrices_and_csr_match_the_oracle(kind):
+    from warp3d_amd.model import BlockModel
+    m, arr, amp = _case(kind)
+    d, o = BlockModel(arr), O.OracleModel(arr)
+    fd, fo = _drive(d, m, amp), _drive(o, m, amp)
+    assert_close(fd, fo, what="forces")
+    for b in range(1, o.nelblk + 1):
+        assert_close(d.get_block_state(b, _abi.HIST_N1),
+                     o.get_block_state(b, _abi.HIST_N1), what=f"hist {b}")
+        assert_close(d.get_ek_full(b), o.get_ek_full(b), what=f"Ke {b}")
+    rp, col, val = d.assemble_csr()
+    rpo, colo = o.csr_pattern()
+    assert np.array_equal(rp, rpo) and np.array_equal(col, colo)
+    _, _, valo = o.assemble_csr(rp, col)
+    assert_close(val, valo, what="CSR values")
+    if kind.startswith("mm10"):  # the matrix is genuinely unsymmetric
+        n = d.neqns
+        rows = np.repeat(np.arange(n), np.diff(rp))
+        import scipy.sparse as sp
+        K = sp.csr_matrix((val, col, rp), shape=(n, n))
+        assert abs(K - K.T).max() > 1e-8 * abs(K).max()
+
+
+def test_symmetric_paths_refuse_an_asymmetric_context():
+    from warp3d_amd.model import BlockModel, W3DError
+    m, arr, amp = _case("hex20")
+    d = BlockModel(arr)
+    _drive(d, m, amp)
+    with pytest.raises(W3DError):
+        d.assemble_vss()
+    with pytest.raises(W3DError):
+        d.tangent(3, 1, precision=_abi.W3D_FP32)

@@ -232,6 +232,16 @@ PROTOS = {
     "set_block_state": (_I, [_P, _I, _I, _D, C.c_int64]),
 }
 
+# asymmetric assembly (the library; the oracle's counterparts take the CSR
+# pattern from the caller)
+ASYM_PROTOS = {
+    "set_asymmetric": (_I, [_P, _I]),
+    "get_csr_pattern": (_I, [_P, _LP, _LP, _IP]),
+    "assemble_csr": (_I, [_P, _D]),
+    "assemble_csr_dev": (_I, [_P, _P]),
+    "get_block_ek_full": (_I, [_P, _I, _D, C.c_int64]),
+}
+
 # the batched serving entry points (the library only; the oracle mirrors
 # the block interface above)
 SERVE_PROTOS = {

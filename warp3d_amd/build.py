@@ -17,7 +17,7 @@ ARCH = "gfx950"
 
 SOURCES = ["w3d_api.cpp", "w3d_elements.cpp", "w3d_mm10_setup.cpp",
            "w3d_kernels.hip", "w3d_tangent.hip", "w3d_mm10.hip",
-           "w3d_solver.hip", "w3d_serve.cpp"]
+           "w3d_solver.hip", "w3d_serve.cpp", "w3d_asym.hip"]
 
 
 def hipcc():

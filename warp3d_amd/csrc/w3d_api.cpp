@@ -137,6 +137,10 @@ void free_ctx(w3d_ctx* c) {
   dfree(c->d_asm_hdr); dfree(c->d_asm_src); dfree(c->d_asm_wbase); dfree(c->d_kout);
   dfree(c->d_csr_rp); dfree(c->d_csr_col); dfree(c->d_csr_vidx);
   dfree(c->d_csr_val); dfree(c->d_pcg);
+  dfree(c->d_ekf); dfree(c->d_ekf_off);
+  dfree(c->d_fa_eqmap); dfree(c->d_fa_eqdof); dfree(c->d_fa_elem);
+  dfree(c->d_fa_ln); dfree(c->d_fa_eblk); dfree(c->d_fa_bad);
+  dfree(c->d_fa_invptr);
   dfree(c->d_crys); dfree(c->d_uddt); dfree(c->d_work_mm10); dfree(c->d_mm10_blks);
   dfree(c->d_rot_n); dfree(c->d_rot_n1); dfree(c->d_u_t); dfree(c->d_du_t);
   dfree(c->d_counter);
@@ -927,6 +931,34 @@ int w3d_recover_blocks(w3d_ctx* c, int32_t first_blk, int32_t nblk,
                       c->d_du, nullptr, nullptr, nullptr, first_blk, nblk);
 }
 
+// bdbt_asym for every element (asymmetric assembly), FP64
+static int tangent_full(w3d_ctx* c) {
+  int rc;
+  if (!c->d_ekf) {
+    std::vector<int64_t> off(c->nelblk + 1, 0);
+    for (int b = 0; b < c->nelblk; ++b) {
+      const BlockDesc& B = c->blocks[b];
+      off[b + 1] = off[b] + (int64_t)B.span * B.totdof * B.totdof;
+    }
+    c->n_ekf = off[c->nelblk];
+    const int64_t bytes0 = t_dalloc_bytes;
+    if ((rc = upload(&c->d_ekf_off, off))) return rc;
+    if ((rc = dalloc(&c->d_ekf, c->n_ekf))) return rc;
+    c->dev_bytes += t_dalloc_bytes - bytes0;
+  }
+  prof_begin(c, W3D_PHASE_TANGENT);
+  rc = launch_tangent_full(c->dev(), c->d_work_tan, c->tan_groups,
+                           c->tan_start, c->tan_count, c->d_ekf_off, c->d_ekf,
+                           c->stream);
+  prof_end(c, W3D_PHASE_TANGENT);
+  if (rc) return fail(rc, "w3d_tangent: full Ke launch failed");
+  c->tangent_precision = W3D_FP64;
+  c->asym_tangent = true;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  prof_collect(c);
+  return W3D_OK;
+}
+
 int w3d_tangent_blocks(w3d_ctx* c, int32_t first_blk, int32_t nblk,
                        int32_t step, int32_t iter, int32_t precision) {
   (void)step;
@@ -937,6 +969,15 @@ int w3d_tangent_blocks(w3d_ctx* c, int32_t first_blk, int32_t nblk,
   if (first_blk < 1 || nblk < 1 || first_blk - 1 + nblk > c->nelblk)
     return fail(W3D_ERR_ARG, "w3d_tangent_blocks: block range");
   HIPCHK(hipSetDevice(c->device));
+  if (c->asym) {
+    if (precision != W3D_FP64)
+      return fail(W3D_ERR_UNSUPPORTED,
+                  "w3d_tangent: asymmetric assembly takes the FP64 tangent");
+    if (first_blk != 1 || nblk != c->nelblk)
+      return fail(W3D_ERR_UNSUPPORTED,
+                  "w3d_tangent_blocks: asymmetric assembly forms every block");
+    return tangent_full(c);
+  }
   DevModel M = c->dev();
   int rc;
   if (first_blk == 1 && nblk == c->nelblk) {
@@ -1271,6 +1312,8 @@ int w3d_pattern(w3d_ctx* c, const int32_t* cstmap, int32_t* neqns_out,
     dfree(c->d_csr_vidx);
     dfree(c->d_csr_val);
     dfree(c->d_pcg);
+    dfree(c->d_fa_eqmap);
+    dfree(c->d_fa_eqdof);
     if ((rc = upload(&c->d_csr_rp, rp))) return rc;
     if ((rc = upload(&c->d_csr_col, col))) return rc;
     if ((rc = upload(&c->d_csr_vidx, vidx))) return rc;
@@ -1303,8 +1346,111 @@ int w3d_get_pattern(const w3d_ctx* c, int32_t* dof_eqn_map,
   return W3D_OK;
 }
 
+int w3d_set_asymmetric(w3d_ctx* c, int32_t on) {
+  if (!c) return fail(W3D_ERR_ARG, "w3d_set_asymmetric: null context");
+  if (on)
+    for (const BlockDesc& B : c->blocks)
+      if (B.geonl)
+        return fail(W3D_ERR_UNSUPPORTED,
+                    "w3d_set_asymmetric: finite-strain blocks");
+  c->asym = on != 0;
+  c->asym_tangent = false;
+  c->tangent_precision = -1;
+  return W3D_OK;
+}
+
+int w3d_get_csr_pattern(const w3d_ctx* c, int64_t* nnz, int64_t* rp,
+                        int32_t* col) {
+  if (!c || !c->d_csr_rp)
+    return fail(W3D_ERR_ARG, "w3d_get_csr_pattern: no pattern");
+  HIPCHK(hipSetDevice(c->device));
+  if (nnz) *nnz = c->csr_nnz;
+  if (rp)
+    HIPCHK(hipMemcpy(rp, c->d_csr_rp, 8 * (size_t)(c->neqns + 1),
+                     hipMemcpyDeviceToHost));
+  if (col && c->csr_nnz)
+    HIPCHK(hipMemcpy(col, c->d_csr_col, 4 * (size_t)c->csr_nnz,
+                     hipMemcpyDeviceToHost));
+  return W3D_OK;
+}
+
+int w3d_assemble_csr_dev(w3d_ctx* c, double* d_val) {
+  if (!c || !c->d_csr_rp)
+    return fail(W3D_ERR_ARG, "w3d_assemble_csr: no pattern");
+  if (!c->asym || !c->asym_tangent)
+    return fail(W3D_ERR_ARG,
+                "w3d_assemble_csr: no full Ke (asymmetric assembly off, or "
+                "no tangent since it was switched on)");
+  if (!d_val && c->csr_nnz) return fail(W3D_ERR_ARG, "w3d_assemble_csr: null");
+  HIPCHK(hipSetDevice(c->device));
+  int rc;
+  if (!c->d_fa_eqmap) {
+    const int64_t bytes0 = t_dalloc_bytes;
+    std::vector<int32_t> eqdof(c->neqns > 0 ? c->neqns : 1, 0);
+    for (int d = 0; d < 3 * c->nonode; ++d)
+      if (c->dof_eqn_map[d]) eqdof[c->dof_eqn_map[d] - 1] = d;
+    dfree(c->d_fa_elem); dfree(c->d_fa_ln); dfree(c->d_fa_eblk);
+    dfree(c->d_fa_bad); dfree(c->d_fa_invptr);
+    if ((rc = upload(&c->d_fa_eqmap, c->dof_eqn_map))) return rc;
+    if ((rc = upload(&c->d_fa_eqdof, eqdof))) return rc;
+    if ((rc = upload(&c->d_fa_invptr, c->inv_ptr))) return rc;
+    if ((rc = upload(&c->d_fa_elem, c->inv_elem))) return rc;
+    if ((rc = upload(&c->d_fa_ln, c->inv_ln))) return rc;
+    if ((rc = upload(&c->d_fa_eblk, c->elem_blk))) return rc;
+    if ((rc = dalloc(&c->d_fa_bad, 1))) return rc;
+    c->dev_bytes += t_dalloc_bytes - bytes0;
+  }
+  HIPCHK(hipMemsetAsync(c->d_fa_bad, 0, sizeof(int32_t), c->stream));
+  prof_begin(c, W3D_PHASE_ASSEMBLE);
+  rc = launch_assemble_full(c->d_ekf, c->d_ekf_off, c->d_blocks, c->d_conn,
+                            c->d_fa_eqmap, c->d_fa_eqdof, c->d_fa_invptr,
+                            c->d_fa_elem, c->d_fa_ln, c->d_fa_eblk,
+                            c->d_csr_rp, c->d_csr_col, d_val, c->neqns,
+                            c->d_fa_bad, c->stream);
+  prof_end(c, W3D_PHASE_ASSEMBLE);
+  if (rc) return fail(rc, "w3d_assemble_csr: kernel launch failed");
+  int32_t bad = 0;
+  HIPCHK(hipMemcpyAsync(&bad, c->d_fa_bad, sizeof bad, hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  prof_collect(c);
+  if (bad) return fail(W3D_ERR_ARG, "w3d_assemble_csr: pattern lacks a column");
+  return W3D_OK;
+}
+
+int w3d_assemble_csr(w3d_ctx* c, double* val) {
+  int rc = w3d_assemble_csr_dev(c, c ? c->d_csr_val : nullptr);
+  if (rc) return rc;
+  if (val && c->csr_nnz)
+    HIPCHK(hipMemcpy(val, c->d_csr_val, 8 * (size_t)c->csr_nnz,
+                     hipMemcpyDeviceToHost));
+  c->csr_serial = -1;  // the solver's CSR values are no longer the VSS's
+  return W3D_OK;
+}
+
+int w3d_get_block_ek_full(const w3d_ctx* c, int32_t blk, double* out,
+                          int64_t n_out) {
+  if (!c || blk < 1 || blk > c->nelblk)
+    return fail(W3D_ERR_ARG, "w3d_get_block_ek_full: block");
+  if (!c->asym_tangent)
+    return fail(W3D_ERR_ARG, "w3d_get_block_ek_full: no full Ke");
+  const BlockDesc& B = c->blocks[blk - 1];
+  const int64_t n = (int64_t)B.span * B.totdof * B.totdof;
+  if (!out || n_out < n)
+    return fail(W3D_ERR_ARG, "w3d_get_block_ek_full: output too small");
+  int64_t off = 0;
+  for (int b = 0; b < blk - 1; ++b)
+    off += (int64_t)c->blocks[b].span * c->blocks[b].totdof *
+           c->blocks[b].totdof;
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipMemcpy(out, c->d_ekf + off, 8 * (size_t)n, hipMemcpyDeviceToHost));
+  return W3D_OK;
+}
+
 int w3d_assemble_vss_dev(w3d_ctx* c, double* d_k_diag, double* d_k_coeffs) {
   if (!c || !c->d_asm_hdr) return fail(W3D_ERR_ARG, "w3d_assemble: no pattern");
+  if (c->asym)
+    return fail(W3D_ERR_ARG, "w3d_assemble_vss: asymmetric assembly is on");
   if (c->tangent_precision < 0)
     return fail(W3D_ERR_ARG, "w3d_assemble: no tangent computed");
   if (!d_k_diag || (!d_k_coeffs && c->ncoeff))
@@ -1324,6 +1470,8 @@ int w3d_assemble_vss_dev(w3d_ctx* c, double* d_k_diag, double* d_k_coeffs) {
 
 int w3d_assemble_vss(w3d_ctx* c, double* k_diag, double* k_coeffs) {
   if (!c || !c->d_asm_hdr) return fail(W3D_ERR_ARG, "w3d_assemble: no pattern");
+  if (c->asym)
+    return fail(W3D_ERR_ARG, "w3d_assemble_vss: asymmetric assembly is on");
   if (c->tangent_precision < 0)
     return fail(W3D_ERR_ARG, "w3d_assemble: no tangent computed");
   HIPCHK(hipSetDevice(c->device));
@@ -1576,6 +1724,9 @@ int w3d_iteration_dev(w3d_ctx* c, int32_t step, int32_t iter,
                       double* plastic_work) {
   if (!c || !d_du || !d_ifv)
     return fail(W3D_ERR_ARG, "w3d_iteration_dev: null argument");
+  if (c->asym)
+    return fail(W3D_ERR_UNSUPPORTED,
+                "w3d_iteration_dev: asymmetric assembly (symmetric VSS only)");
   if (step < 1 || iter < 0)
     return fail(W3D_ERR_ARG, "w3d_iteration_dev: step/iter");
   if (precision != W3D_FP64 && precision != W3D_FP32)

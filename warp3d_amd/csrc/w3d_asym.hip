@@ -1,0 +1,289 @@
+// w3d_asym.hip -- asymmetric assembly: the full element matrices and their
+// gather into full CSR rows, FP64.
+//
+// Reference: gptns1.f:1513 bdbt_asym (BD(j,k) = sum_m D(m,k) B(m,j), then
+// ek_full(j) += sum_r B(r,col) BD(row,r) for j = (col-1) T + row, every
+// row and column), gptns1.f:1371 drive_10_cnst ([D] = the point's history
+// words 1-36, column-major, times w |J|), gptns1.f:1174 extract_D_symmetric
+// (mm01), gpmas1.f:399 adjust_cnst (1/6 for tets), assemble_code.f:1050
+// assem_by_row_asymmetric / assem_a_row_asym (row srow gathers the words
+// (erow-1) T + ecol of each element on its node, elements in
+// inverse-incidence order, the column found by binary search).
+//
+// One 256-lane workgroup per element: the point loop stages B and [D] B of
+// one point in LDS and every lane accumulates its share of the T*T words
+// in registers.  The assembly runs one lane per equation row.
+#include <hip/hip_runtime.h>
+
+#include "w3d_internal.h"
+
+namespace w3d {
+
+template <int NN, int G, bool BBAR>
+__global__ void __launch_bounds__(256)
+    k_tangent_full(DevModel M, const WorkItem* __restrict__ work,
+                   const int64_t* __restrict__ ekf_off,
+                   double* __restrict__ ekf) {
+  constexpr int T = 3 * NN;
+  constexpr int NE = (T * T + 255) / 256;
+  __shared__ double s_ce[T];
+  __shared__ double s_g[G][9];
+  __shared__ double s_dj[G];
+  __shared__ double s_vol[24];
+  __shared__ double s_b[6 * T];
+  __shared__ double s_bd[6 * T];
+  __shared__ double s_d[36];
+  const WorkItem wi = work[blockIdx.x];
+  const BlockDesc bd = M.blocks[wi.blk];
+  const GaussTable* gt = M.tables + bd.table;
+  const int tid = threadIdx.x;
+  const int64_t SG = (int64_t)bd.span * G;
+  const double scalar = bd.etype == W3D_TET10 ? 1.0 / 6.0 : 1.0;
+  const double one = 1.0, two = 2.0, third = 1.0 / 3.0;
+  const double alpha = (two + M.eps_bbar) * third;
+  const double beta = (one - M.eps_bbar) * third;
+  for (int e = 0; e < wi.count; ++e) {
+    const int i = wi.first + e;
+    __syncthreads();
+    if (tid < NN) {
+      const int node = M.conn[bd.conn_off + (int64_t)i * NN + tid];
+      s_ce[tid] = M.coords[3 * node];
+      s_ce[NN + tid] = M.coords[3 * node + 1];
+      s_ce[2 * NN + tid] = M.coords[3 * node + 2];
+    }
+    __syncthreads();
+    if (tid < G) {  // jacob1
+      const double* nxi = gt->dn[0][tid];
+      const double* neta = gt->dn[1][tid];
+      const double* nzeta = gt->dn[2][tid];
+      double jac[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) jac[k] = 0.0;
+      for (int j = 0; j < NN; ++j) {
+        const double x = s_ce[j], y = s_ce[NN + j], z = s_ce[2 * NN + j];
+        jac[0] = jac[0] + nxi[j] * x;
+        jac[1] = jac[1] + nxi[j] * y;
+        jac[2] = jac[2] + nxi[j] * z;
+        jac[3] = jac[3] + neta[j] * x;
+        jac[4] = jac[4] + neta[j] * y;
+        jac[5] = jac[5] + neta[j] * z;
+        jac[6] = jac[6] + nzeta[j] * x;
+        jac[7] = jac[7] + nzeta[j] * y;
+        jac[8] = jac[8] + nzeta[j] * z;
+      }
+      const double j1 = jac[4] * jac[8] - jac[5] * jac[7];
+      const double j2 = jac[3] * jac[8] - jac[5] * jac[6];
+      const double j3 = jac[3] * jac[7] - jac[4] * jac[6];
+      const double d = jac[0] * j1 - jac[1] * j2 + jac[2] * j3;
+      s_dj[tid] = d;
+      double* g = s_g[tid];
+      g[0] = j1 / d;
+      g[3] = -j2 / d;
+      g[6] = j3 / d;
+      g[1] = (jac[7] * jac[2] - jac[1] * jac[8]) / d;
+      g[4] = (jac[0] * jac[8] - jac[6] * jac[2]) / d;
+      g[7] = (jac[1] * jac[6] - jac[0] * jac[7]) / d;
+      g[2] = (jac[1] * jac[5] - jac[2] * jac[4]) / d;
+      g[5] = (jac[2] * jac[3] - jac[0] * jac[5]) / d;
+      g[8] = (jac[0] * jac[4] - jac[1] * jac[3]) / d;
+    }
+    __syncthreads();
+    if (BBAR && tid < 24) {  // vol_terms, vol_avg
+      const int k = tid % 3, j = tid / 3;
+      double v = 0.0, volume = 0.0;
+      for (int g = 0; g < G; ++g) {
+        const double* a = s_g[g];
+        v = v + (a[3 * k] * gt->dn[0][g][j] + a[3 * k + 1] * gt->dn[1][g][j] +
+                 a[3 * k + 2] * gt->dn[2][g][j]) *
+                    s_dj[g];
+        volume = volume + s_dj[g];
+      }
+      s_vol[3 * j + k] = v * (1.0 / volume);
+    }
+    double acc[NE];
+#pragma unroll
+    for (int n = 0; n < NE; ++n) acc[n] = 0.0;
+    for (int g = 0; g < G; ++g) {
+      __syncthreads();
+      for (int k = tid; k < 6 * T; k += 256) s_b[k] = 0.0;
+      __syncthreads();
+      if (tid < NN) {  // the B columns of node j (blcmp1)
+        const int j = tid;
+        const double* a = s_g[g];
+        const double nx = gt->dn[0][g][j], ny = gt->dn[1][g][j],
+                     nz = gt->dn[2][g][j];
+        const double bx = a[0] * nx + a[1] * ny + a[2] * nz;
+        const double by = a[3] * nx + a[4] * ny + a[5] * nz;
+        const double bz = a[6] * nx + a[7] * ny + a[8] * nz;
+        s_b[0 * T + j] = bx;
+        s_b[3 * T + j] = by;
+        s_b[5 * T + j] = bz;
+        s_b[1 * T + NN + j] = by;
+        s_b[3 * T + NN + j] = bx;
+        s_b[4 * T + NN + j] = bz;
+        s_b[2 * T + 2 * NN + j] = bz;
+        s_b[4 * T + 2 * NN + j] = by;
+        s_b[5 * T + 2 * NN + j] = bx;
+      }
+      if (tid >= 64 && tid < 100) {  // [D] of the point times w |J|
+        const int t = tid - 64, m = t % 6, kk = t / 6;
+        const double f = gt->w[g] * s_dj[g];
+        const int64_t q = (int64_t)i * G + g;
+        double v;
+        if (bd.model == W3D_MM10) {
+          v = M.hist_n1[bd.hist_off + (int64_t)(m + 6 * kk) * SG + q] * f;
+        } else {
+          const int r = m > kk ? m : kk, c = m > kk ? kk : m;
+          v = f * M.cep[21 * bd.q_off + (int64_t)(r * (r + 1) / 2 + c) * SG +
+                        q];
+        }
+        if (scalar != 1.0) v = scalar * v;
+        s_d[6 * m + kk] = v;
+      }
+      __syncthreads();
+      if (BBAR && tid < 8) {  // bmod, node j
+        const int j = tid;
+        double* r1 = s_b;
+        double* r2 = s_b + T;
+        double* r3 = s_b + 2 * T;
+        const double d = r1[j];
+        r1[j] = alpha * d + beta * s_vol[3 * j + 0];
+        r2[j] = beta * (s_vol[3 * j + 0] - d);
+        r1[8 + j] = beta * (s_vol[3 * j + 1] - r2[8 + j]);
+        r1[16 + j] = beta * (s_vol[3 * j + 2] - r3[16 + j]);
+        r2[8 + j] = alpha * r2[8 + j] + beta * s_vol[3 * j + 1];
+        r2[16 + j] = r1[16 + j];
+        r3[j] = r2[j];
+        r3[8 + j] = r1[8 + j];
+        r3[16 + j] = alpha * r3[16 + j] + beta * s_vol[3 * j + 2];
+      }
+      __syncthreads();
+      for (int k = tid; k < 6 * T; k += 256) {
+        const int kk = k / T, j = k % T;
+        s_bd[k] = s_d[0 * 6 + kk] * s_b[0 * T + j] +
+                  s_d[1 * 6 + kk] * s_b[1 * T + j] +
+                  s_d[2 * 6 + kk] * s_b[2 * T + j] +
+                  s_d[3 * 6 + kk] * s_b[3 * T + j] +
+                  s_d[4 * 6 + kk] * s_b[4 * T + j] +
+                  s_d[5 * 6 + kk] * s_b[5 * T + j];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int n = 0; n < NE; ++n) {
+        const int w = tid + 256 * n;
+        if (w < T * T) {
+          const int col = w / T, row = w % T;
+          acc[n] = acc[n] + s_b[0 * T + col] * s_bd[0 * T + row] +
+                   s_b[1 * T + col] * s_bd[1 * T + row] +
+                   s_b[2 * T + col] * s_bd[2 * T + row] +
+                   s_b[3 * T + col] * s_bd[3 * T + row] +
+                   s_b[4 * T + col] * s_bd[4 * T + row] +
+                   s_b[5 * T + col] * s_bd[5 * T + row];
+        }
+      }
+    }
+    double* out = ekf + ekf_off[wi.blk] + (int64_t)i * T * T;
+#pragma unroll
+    for (int n = 0; n < NE; ++n) {
+      const int w = tid + 256 * n;
+      if (w < T * T) out[w] = acc[n];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    k_assemble_full(const double* __restrict__ ekf,
+                    const int64_t* __restrict__ ekf_off,
+                    const BlockDesc* __restrict__ blocks,
+                    const int32_t* __restrict__ conn,
+                    const int32_t* __restrict__ eqmap,
+                    const int32_t* __restrict__ eq_dof,
+                    const int64_t* __restrict__ inv_ptr,
+                    const int32_t* __restrict__ inv_elem,
+                    const int32_t* __restrict__ inv_ln,
+                    const int32_t* __restrict__ elem_blk,
+                    const int64_t* __restrict__ rp,
+                    const int32_t* __restrict__ col, double* __restrict__ val,
+                    int neq, int32_t* __restrict__ bad) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= neq) return;
+  const int64_t r0 = rp[r], r1 = rp[r + 1];
+  for (int64_t k = r0; k < r1; ++k) val[k] = 0.0;
+  const int dof = eq_dof[r], node = dof / 3, comp = dof % 3;
+  for (int64_t p = inv_ptr[node]; p < inv_ptr[node + 1]; ++p) {
+    const int e = inv_elem[p], b = elem_blk[e];
+    const BlockDesc B = blocks[b];
+    const int NN = B.nnode, T = 3 * NN, i = e - (B.felem - 1);
+    const int erow = comp * NN + inv_ln[p];
+    const double* em = ekf + ekf_off[b] + (int64_t)i * T * T + erow * T;
+    const int32_t* cn = conn + B.conn_off + (int64_t)i * NN;
+    for (int ecol = 0; ecol < T; ++ecol) {
+      const int scol = eqmap[3 * cn[ecol % NN] + ecol / NN];
+      if (scol == 0) continue;
+      int64_t lo = r0, hi = r1 - 1;
+      while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (col[mid] < scol - 1)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      if (lo >= r1 || col[lo] != scol - 1) {
+        bad[0] = 1;
+        continue;
+      }
+      val[lo] = val[lo] + em[ecol];
+    }
+  }
+}
+
+#define W3D_FULL_GROUPS(X) \
+  X(8, 8, true)            \
+  X(8, 8, false)           \
+  X(20, 27, false)         \
+  X(20, 8, false)          \
+  X(20, 14, false)         \
+  X(20, 9, false)          \
+  X(10, 4, false)          \
+  X(10, 1, false)          \
+  X(10, 5, false)
+
+int launch_tangent_full(const DevModel& M, const WorkItem* d_work,
+                        const std::vector<int>& groups,
+                        const std::vector<int>& gstart,
+                        const std::vector<int>& gcount,
+                        const int64_t* d_ekf_off, double* d_ekf,
+                        hipStream_t s) {
+  for (size_t k = 0; k < groups.size(); ++k) {
+    if (gcount[k] == 0) continue;
+    if (groups[k] & 1) return W3D_ERR_UNSUPPORTED;  // finite strains
+    const WorkItem* w = d_work + gstart[k];
+    bool done = false;
+#define X(NN, G, BB)                                                      \
+  if (!done && (groups[k] >> 2) == (group_key(NN, G, BB, 0, 0) >> 2)) {  \
+    hipLaunchKernelGGL((k_tangent_full<NN, G, BB>), dim3(gcount[k]),      \
+                       dim3(256), 0, s, M, w, d_ekf_off, d_ekf);          \
+    done = true;                                                          \
+  }
+    W3D_FULL_GROUPS(X)
+#undef X
+    if (!done) return W3D_ERR_UNSUPPORTED;
+  }
+  return hipGetLastError() == hipSuccess ? W3D_OK : W3D_ERR_DEVICE;
+}
+
+int launch_assemble_full(const double* ekf, const int64_t* ekf_off,
+                         const BlockDesc* blocks, const int32_t* conn,
+                         const int32_t* eqmap, const int32_t* eq_dof,
+                         const int64_t* inv_ptr, const int32_t* inv_elem,
+                         const int32_t* inv_ln, const int32_t* elem_blk,
+                         const int64_t* rp, const int32_t* col, double* val,
+                         int neq, int32_t* bad, hipStream_t s) {
+  if (neq == 0) return W3D_OK;
+  hipLaunchKernelGGL(k_assemble_full, dim3((neq + 255) / 256), dim3(256), 0,
+                     s, ekf, ekf_off, blocks, conn, eqmap, eq_dof, inv_ptr,
+                     inv_elem, inv_ln, elem_blk, rp, col, val, neq, bad);
+  return hipGetLastError() == hipSuccess ? W3D_OK : W3D_ERR_DEVICE;
+}
+
+}  // namespace w3d

@@ -156,6 +156,16 @@ struct w3d_ctx {
   uint32_t* d_csr_vidx = nullptr;
   double* d_csr_val = nullptr;
   double* d_pcg = nullptr;   // solver work vectors and scalars
+  // asymmetric assembly (w3d_set_asymmetric): the full Ke of every element
+  // (block b from d_ekf_off[b]), whether the last tangent formed it, and
+  // the gather tables of the CSR assembly (rebuilt after w3d_pattern)
+  bool asym = false, asym_tangent = false;
+  double* d_ekf = nullptr;
+  int64_t* d_ekf_off = nullptr;
+  int64_t n_ekf = 0;
+  int32_t *d_fa_eqmap = nullptr, *d_fa_eqdof = nullptr, *d_fa_elem = nullptr;
+  int32_t *d_fa_ln = nullptr, *d_fa_eblk = nullptr, *d_fa_bad = nullptr;
+  int64_t* d_fa_invptr = nullptr;
   int tangent_precision = -1;
   double eps_bbar = 0.0;
   // phase timing (w3d_set_profiling)
@@ -192,6 +202,7 @@ struct w3d_ctx {
     M.rot_n1 = d_rot_n1;
     M.u_t = d_u_t;
     M.du_t = d_du_t;
+    M.asym = asym ? 1 : 0;
     return M;
   }
 };

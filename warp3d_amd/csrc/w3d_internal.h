@@ -123,6 +123,7 @@ struct DevModel {
   double* rot_n1;
   const double* u_t;
   const double* du_t;
+  int32_t asym;           // asymmetric assembly: mm10 tangent unsymmetrised
 };
 
 // Kernel launchers (w3d_kernels.hip, w3d_tangent.hip).
@@ -161,6 +162,22 @@ int launch_assemble(const double* ek64, const float* ek32, int fp32,
                     const uint16_t* hdr, const uint32_t* src,
                     const int64_t* wbase, int64_t nseg, int64_t neq,
                     double* diag, double* coef, hipStream_t s);
+// Asymmetric assembly (w3d_asym.hip): full FP64 Ke of every element of
+// the work list into ekf (block b's elements from ekf_off[b], T*T words
+// each), and its gather into the values of the full CSR rows.
+int launch_tangent_full(const DevModel& M, const WorkItem* d_work,
+                        const std::vector<int>& groups,
+                        const std::vector<int>& gstart,
+                        const std::vector<int>& gcount,
+                        const int64_t* d_ekf_off, double* d_ekf,
+                        hipStream_t s);
+int launch_assemble_full(const double* ekf, const int64_t* ekf_off,
+                         const BlockDesc* blocks, const int32_t* conn,
+                         const int32_t* eqmap, const int32_t* eq_dof,
+                         const int64_t* inv_ptr, const int32_t* inv_elem,
+                         const int32_t* inv_ln, const int32_t* elem_blk,
+                         const int64_t* rp, const int32_t* col, double* val,
+                         int neq, int32_t* bad, hipStream_t s);
 int launch_tangent_mfma(const DevModel& M, const WorkItem* d_work,
                         const std::vector<int>& groups,
                         const std::vector<int>& group_start,

@@ -888,6 +888,7 @@ __device__ __forceinline__ void mm10_item(const DevModel& M,
     for (int k = 0; k < 15; ++k) u[k] = 0.0;
     if (!(iter0_off || no_load)) {
       // ---- mm10_tangent: T = (J11 - J12 J21 / J22)^-1 C, symmetrised
+      // unless the assembly is asymmetric
       double J[49], JJ[36], beta[12];
 #pragma unroll
       for (int k = 0; k < 49; ++k) J[k] = Jg[k * SG];
@@ -914,14 +915,16 @@ __device__ __forceinline__ void mm10_item(const DevModel& M,
 #pragma unroll
       for (int k = 0; k < 36; ++k) tangent[k] = C[k] - 0.0 - 0.0;
       lu_solve<6, 6>(JJ, tangent);
-      double w[36];
+      if (!M.asym) {  // kept as formed for asymmetric assembly (:1165)
+        double w[36];
 #pragma unroll
-      for (int j = 0; j < 6; ++j)
+        for (int j = 0; j < 6; ++j)
 #pragma unroll
-        for (int i = 0; i < 6; ++i)
-          w[i + 6 * j] = (tangent[i + 6 * j] + tangent[j + 6 * i]) * 0.5;
+          for (int i = 0; i < 6; ++i)
+            w[i + 6 * j] = (tangent[i + 6 * j] + tangent[j + 6 * i]) * 0.5;
 #pragma unroll
-      for (int k = 0; k < 36; ++k) tangent[k] = w[k];
+        for (int k = 0; k < 36; ++k) tangent[k] = w[k];
+      }
       store_tangent();
       // ---- slips at the converged state (np1: full-step dg)
       double rs[NS], gam[NS];

@@ -245,6 +245,10 @@ int w3d_problem_create(w3d_ctx* c, const int32_t* cstmap, const double* du0,
   if (!c || !cstmap || !du0 || !o || !out)
     return fail(W3D_ERR_ARG, "w3d_problem_create: null argument");
   *out = nullptr;
+  if (c->asym)
+    return fail(W3D_ERR_UNSUPPORTED,
+                "w3d_problem_create: asymmetric assembly (Jacobi-PCG needs "
+                "the symmetric matrix)");
   if (o->nsteps < 1 || o->max_iters < 1 || o->pcg_maxit < 1 ||
       !(o->tol > 0.0) || !(o->tol_abs >= 0.0) || !(o->pcg_rtol > 0.0) ||
       (o->precision != W3D_FP64 && o->precision != W3D_FP32) ||

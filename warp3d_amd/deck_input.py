@@ -303,6 +303,7 @@ class Problem:
     nsteps: int
     dt: float
     newton: dict
+    asymmetric: bool = False    # assembly asymmetric on
 
 
 def build(dk):
@@ -317,6 +318,7 @@ def build(dk):
     newton = dict(res_tol=0.01, mxiter=5, mniter=1, extrapolate=True,
                   line_search=False, adaptive=False, divergence_check=True)
     dt = 1.0
+    asym = False
     for k, t in nl.items():
         if k.startswith("time step"):
             dt = float(t[-1])
@@ -334,6 +336,8 @@ def build(dk):
             newton["extrapolate"] = t[-1] == "on"
         elif k.startswith("line search"):
             newton["line_search"] = t[-1] == "on"
+        elif k.startswith("assembly asymmetric"):
+            asym = t[-1] == "on"
         elif k.startswith("bbar stabilization factor"):
             eps_bbar = float(t[-1])
     arr = _abi.ModelArrays(
@@ -358,7 +362,7 @@ def build(dk):
         d[cst != 0] = 0.0
         dloads.append(d)
         mfs.append(mf)
-    return Problem(arr, cst, dloads, mfs, nsteps, dt, newton)
+    return Problem(arr, cst, dloads, mfs, nsteps, dt, newton, asym)
 
 
 def center_stress(model, elem, auto_size_blocks):

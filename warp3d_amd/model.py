@@ -11,7 +11,7 @@ import os
 
 import numpy as np
 
-from ._abi import (PROTOS, SERVE_PROTOS, W3D_CUT_STEP, W3DModel, dptr,
+from ._abi import (ASYM_PROTOS, PROTOS, SERVE_PROTOS, W3D_CUT_STEP, W3DModel, dptr,
                    iptr)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -31,7 +31,8 @@ def lib():
             raise W3DError(f"{LIB_PATH} is not built: run "
                            "python -m warp3d_amd.build")
         L = C.CDLL(LIB_PATH)
-        for name, (res, args) in {**PROTOS, **SERVE_PROTOS}.items():
+        for name, (res, args) in {**PROTOS, **SERVE_PROTOS,
+                                  **ASYM_PROTOS}.items():
             f = getattr(L, "w3d_" + name)
             f.restype = res
             f.argtypes = args
@@ -194,6 +195,41 @@ class BlockModel:
         _chk(lib().w3d_assemble_vss(self.h, dptr(kd), dptr(kc)),
              "w3d_assemble_vss")
         return kd, kc[:self.ncoeff]
+
+    asym = False
+
+    def set_asymmetric(self, on):
+        """Asymmetric assembly (w3d_set_asymmetric)."""
+        _chk(lib().w3d_set_asymmetric(self.h, int(bool(on))),
+             "w3d_set_asymmetric")
+        self.asym = bool(on)
+
+    def csr_pattern(self):
+        nnz = C.c_int64()
+        _chk(lib().w3d_get_csr_pattern(self.h, C.byref(nnz), None, None),
+             "w3d_get_csr_pattern")
+        rp = np.zeros(self.neqns + 1, np.int64)
+        col = np.zeros(max(nnz.value, 1), np.int32)
+        _chk(lib().w3d_get_csr_pattern(
+            self.h, None, rp.ctypes.data_as(C.POINTER(C.c_int64)),
+            col.ctypes.data_as(C.POINTER(C.c_int32))), "w3d_get_csr_pattern")
+        return rp, col[:nnz.value]
+
+    def assemble_csr(self):
+        """(rp, col, values) of the full CSR rows (asymmetric assembly)."""
+        rp, col = self.csr_pattern()
+        val = np.zeros(max(col.size, 1))
+        _chk(lib().w3d_assemble_csr(self.h, dptr(val)), "w3d_assemble_csr")
+        return rp, col, val[:col.size]
+
+    def get_ek_full(self, blk):
+        """(span, T, T): element e's word [e, c, r] = B(r)^T D B(c)."""
+        info = self.block_info(blk)
+        span, T = info["span"], info["totdof"]
+        out = np.zeros(span * T * T)
+        _chk(lib().w3d_get_block_ek_full(self.h, blk, dptr(out), out.size),
+             "w3d_get_block_ek_full")
+        return out.reshape(span, T, T)
 
     def assemble_vss_dev(self, d_kdiag, d_kcoeffs):
         return _chk(lib().w3d_assemble_vss_dev(self.h, d_kdiag, d_kcoeffs),
