@@ -222,6 +222,10 @@ class BlockModel:
         _chk(lib().w3d_assemble_csr(self.h, dptr(val)), "w3d_assemble_csr")
         return rp, col, val[:col.size]
 
+    def assemble_csr_dev(self, d_val):
+        return _chk(lib().w3d_assemble_csr_dev(self.h, d_val),
+                    "w3d_assemble_csr_dev")
+
     def get_ek_full(self, blk):
         """(span, T, T): element e's word [e, c, r] = B(r)^T D B(c)."""
         info = self.block_info(blk)
