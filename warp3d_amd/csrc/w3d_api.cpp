@@ -1285,6 +1285,9 @@ int w3d_pattern(w3d_ctx* c, const int32_t* cstmap, int32_t* neqns_out,
     for (int r = 0; r < neq; ++r)
       rp[r + 1] = rp[r] + low[r + 1] + 1 + c->k_ptrs[r];
     const int64_t nnz = rp[neq];
+    c->csr_maxrow = 0;
+    for (int r = 0; r < neq; ++r)
+      c->csr_maxrow = std::max<int>(c->csr_maxrow, (int)(rp[r + 1] - rp[r]));
     std::vector<int32_t> col(nnz);
     std::vector<uint32_t> vidx(nnz);
     std::vector<int64_t> fill(neq);
@@ -1382,6 +1385,10 @@ int w3d_assemble_csr_dev(w3d_ctx* c, double* d_val) {
                 "w3d_assemble_csr: no full Ke (asymmetric assembly off, or "
                 "no tangent since it was switched on)");
   if (!d_val && c->csr_nnz) return fail(W3D_ERR_ARG, "w3d_assemble_csr: null");
+  if (c->csr_maxrow > kAsmMaxRow)
+    return fail(W3D_ERR_UNSUPPORTED,
+                "w3d_assemble_csr: a row couples more than " +
+                    std::to_string(kAsmMaxRow) + " equations");
   HIPCHK(hipSetDevice(c->device));
   int rc;
   if (!c->d_fa_eqmap) {

@@ -12,7 +12,7 @@
 //
 // One 256-lane workgroup per element: the point loop stages B and [D] B of
 // one point in LDS and every lane accumulates its share of the T*T words
-// in registers.  The assembly runs one lane per equation row.
+// in registers.  The assembly runs one wave per equation row.
 #include <hip/hip_runtime.h>
 
 #include "w3d_internal.h"
@@ -191,7 +191,15 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-__global__ void __launch_bounds__(256)
+// One wave per equation row: the row's values are summed in LDS, element
+// after element in inverse-incidence order, lane ecol adding the element's
+// word (erow-1) T + ecol (distinct columns within an element, so no two
+// lanes meet); the Ke words of an element row and the row's values are
+// read and written contiguously.
+constexpr int kAsmWaves = 4;
+__device__ inline void wave_fence() { __builtin_amdgcn_wave_barrier(); }
+
+__global__ void __launch_bounds__(64 * kAsmWaves)
     k_assemble_full(const double* __restrict__ ekf,
                     const int64_t* __restrict__ ekf_off,
                     const BlockDesc* __restrict__ blocks,
@@ -205,36 +213,44 @@ __global__ void __launch_bounds__(256)
                     const int64_t* __restrict__ rp,
                     const int32_t* __restrict__ col, double* __restrict__ val,
                     int neq, int32_t* __restrict__ bad) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= neq) return;
+  __shared__ double s_row[kAsmWaves][kAsmMaxRow];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int r = blockIdx.x * kAsmWaves + wv;
+  if (r >= neq) return;  // uniform across the wave
   const int64_t r0 = rp[r], r1 = rp[r + 1];
-  for (int64_t k = r0; k < r1; ++k) val[k] = 0.0;
+  const int len = (int)(r1 - r0);  // <= kAsmMaxRow (checked on the host)
+  double* row = s_row[wv];
+  for (int k = lane; k < len; k += 64) row[k] = 0.0;
+  wave_fence();
   const int dof = eq_dof[r], node = dof / 3, comp = dof % 3;
   for (int64_t p = inv_ptr[node]; p < inv_ptr[node + 1]; ++p) {
     const int e = inv_elem[p], b = elem_blk[e];
-    const BlockDesc B = blocks[b];
-    const int NN = B.nnode, T = 3 * NN, i = e - (B.felem - 1);
+    const int NN = blocks[b].nnode, T = 3 * NN;
+    const int i = e - (blocks[b].felem - 1);
     const int erow = comp * NN + inv_ln[p];
-    const double* em = ekf + ekf_off[b] + (int64_t)i * T * T + erow * T;
-    const int32_t* cn = conn + B.conn_off + (int64_t)i * NN;
-    for (int ecol = 0; ecol < T; ++ecol) {
-      const int scol = eqmap[3 * cn[ecol % NN] + ecol / NN];
-      if (scol == 0) continue;
-      int64_t lo = r0, hi = r1 - 1;
-      while (lo < hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (col[mid] < scol - 1)
-          lo = mid + 1;
+    if (lane < T) {
+      const int ecol = lane;
+      const int nd = conn[blocks[b].conn_off + (int64_t)i * NN + ecol % NN];
+      const int scol = eqmap[3 * nd + ecol / NN];
+      if (scol != 0) {
+        int lo = 0, hi = len - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (col[r0 + mid] < scol - 1)
+            lo = mid + 1;
+          else
+            hi = mid;
+        }
+        if (col[r0 + lo] != scol - 1)
+          bad[0] = 1;
         else
-          hi = mid;
+          row[lo] = row[lo] +
+                    ekf[ekf_off[b] + (int64_t)i * T * T + erow * T + ecol];
       }
-      if (lo >= r1 || col[lo] != scol - 1) {
-        bad[0] = 1;
-        continue;
-      }
-      val[lo] = val[lo] + em[ecol];
     }
+    wave_fence();
   }
+  for (int k = lane; k < len; k += 64) val[r0 + k] = row[k];
 }
 
 #define W3D_FULL_GROUPS(X) \
@@ -280,7 +296,9 @@ int launch_assemble_full(const double* ekf, const int64_t* ekf_off,
                          const int64_t* rp, const int32_t* col, double* val,
                          int neq, int32_t* bad, hipStream_t s) {
   if (neq == 0) return W3D_OK;
-  hipLaunchKernelGGL(k_assemble_full, dim3((neq + 255) / 256), dim3(256), 0,
+  hipLaunchKernelGGL(k_assemble_full,
+                     dim3((neq + kAsmWaves - 1) / kAsmWaves),
+                     dim3(64 * kAsmWaves), 0,
                      s, ekf, ekf_off, blocks, conn, eqmap, eq_dof, inv_ptr,
                      inv_elem, inv_ln, elem_blk, rp, col, val, neq, bad);
   return hipGetLastError() == hipSuccess ? W3D_OK : W3D_ERR_DEVICE;

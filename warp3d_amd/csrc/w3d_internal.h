@@ -171,6 +171,7 @@ int launch_tangent_full(const DevModel& M, const WorkItem* d_work,
                         const std::vector<int>& gcount,
                         const int64_t* d_ekf_off, double* d_ekf,
                         hipStream_t s);
+constexpr int kAsmMaxRow = 768;  // longest CSR row the gather takes
 int launch_assemble_full(const double* ekf, const int64_t* ekf_off,
                          const BlockDesc* blocks, const int32_t* conn,
                          const int32_t* eqmap, const int32_t* eq_dof,
