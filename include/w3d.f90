@@ -147,6 +147,40 @@ module w3d
       type(c_ptr), value :: k_diag, k_coeffs
       integer(c_int) :: rc
     end function
+    ! asymmetric assembly (assembly asymmetric on)
+    function w3d_set_asymmetric(ctx, on) &
+         bind(C, name="w3d_set_asymmetric") result(rc)
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: on
+      integer(c_int) :: rc
+    end function
+    function w3d_get_csr_pattern(ctx, nnz, rp, col) &
+         bind(C, name="w3d_get_csr_pattern") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, nnz, rp, col
+      integer(c_int) :: rc
+    end function
+    function w3d_assemble_csr(ctx, val) &
+         bind(C, name="w3d_assemble_csr") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, val
+      integer(c_int) :: rc
+    end function
+    function w3d_assemble_csr_dev(ctx, d_val) &
+         bind(C, name="w3d_assemble_csr_dev") result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, d_val
+      integer(c_int) :: rc
+    end function
+    function w3d_get_block_ek_full(ctx, blk, out, n_out) &
+         bind(C, name="w3d_get_block_ek_full") result(rc)
+      import :: c_ptr, c_int, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx, out
+      integer(c_int32_t), value :: blk
+      integer(c_int64_t), value :: n_out
+      integer(c_int) :: rc
+    end function
     function w3d_solve_pcg(ctx, rhs, x, rtol, maxit, iters, relres) &
          bind(C, name="w3d_solve_pcg") result(rc)
       import :: c_ptr, c_int, c_int32_t, c_double
