@@ -193,6 +193,18 @@ module w3d
       real(c_double), intent(out) :: relres
       integer(c_int) :: rc
     end function
+    function w3d_solve_bicgstab(ctx, rhs, x, rtol, maxit, iters, relres) &
+         bind(C, name="w3d_solve_bicgstab") result(rc)
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: rhs(*)
+      real(c_double), intent(inout) :: x(*)
+      real(c_double), value :: rtol
+      integer(c_int32_t), value :: maxit
+      integer(c_int32_t), intent(out) :: iters
+      real(c_double), intent(out) :: relres
+      integer(c_int) :: rc
+    end function
     function w3d_iteration_dev(ctx, step, iter, extrapolated, d_u, d_du, &
          d_ifv, precision, d_k_diag, d_k_coeffs, energy, plastic_work) &
          bind(C, name="w3d_iteration_dev") result(rc)

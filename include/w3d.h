@@ -270,6 +270,15 @@ int w3d_assemble_csr(w3d_ctx* ctx, double* val);
 int w3d_assemble_csr_dev(w3d_ctx* ctx, double* d_val);
 int w3d_get_block_ek_full(const w3d_ctx* ctx, int32_t blk, double* out,
                           int64_t n_out);
+/* The nonsymmetric solve of the asymmetric CSR of the last
+ * w3d_assemble_csr, in place of drive_pardiso_asym (drive_pardiso_asym.f,
+ * Pardiso mtype 1, iparm(12) = 0: A x = b with A the rows as assembled):
+ * Jacobi-preconditioned BiCGStab from x = 0, FP64, to |r| <= rtol |b|.
+ * iters/relres as w3d_solve_pcg; W3D_ERR_NUMERIC after maxit iterations or
+ * on a breakdown, x holding the last iterate. */
+int w3d_solve_bicgstab(w3d_ctx* ctx, const double* rhs, double* x,
+                       double rtol, int32_t maxit, int32_t* iters,
+                       double* relres);
 
 /*
  * Jacobi-preconditioned conjugate gradients, FP64, on the device, for the

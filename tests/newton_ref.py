@@ -93,15 +93,20 @@ def line_search(evaluate, opts):
 
 
 def _solve(om, cst, free, eq, res):
+    rhs = np.zeros(om.neqns)
+    rhs[eq] = res[free]
     if om.asym:  # asymmetric assembly: the full CSR rows as assembled
         rp, col, val = om.assemble_csr()
-        K = sp.csr_matrix((val, col, rp), shape=(om.neqns, om.neqns)).tocsc()
+        if getattr(om, "device_solve", False):  # the library's BiCGStab
+            x = om.solve_bicgstab(rhs, rtol=1e-13)[0]
+        else:
+            K = sp.csr_matrix((val, col, rp),
+                              shape=(om.neqns, om.neqns)).tocsc()
+            x = spla.spsolve(K, rhs)
     else:
         kd, kc = om.assemble_vss()
         K = vss_matrix(om, kd, kc)
-    rhs = np.zeros(om.neqns)
-    rhs[eq] = res[free]
-    x = spla.spsolve(K, rhs)
+        x = spla.spsolve(K, rhs)
     idu = np.zeros_like(res)
     idu[free] = x[eq]
     return idu

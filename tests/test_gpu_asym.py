@@ -78,3 +78,24 @@ def test_symmetric_paths_refuse_an_asymmetric_context():
         d.assemble_vss()
     with pytest.raises(W3DError):
         d.tangent(3, 1, precision=_abi.W3D_FP32)
+
+
+def test_bicgstab_solves_the_asymmetric_csr():
+    """The device BiCGStab against a sparse LU of the same CSR."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    from warp3d_amd.model import BlockModel
+    m, arr, amp = _case("mm10-bbar")
+    d = BlockModel(arr)
+    _drive(d, m, amp)
+    rp, col, val = d.assemble_csr()
+    n = d.neqns
+    K = sp.csr_matrix((val, col, rp), shape=(n, n))
+    b = np.random.default_rng(7).standard_normal(n)
+    x, it, rr = d.solve_bicgstab(b, rtol=1e-12, maxit=5000)
+    xr = spla.spsolve(K.tocsc(), b)
+    assert rr <= 1e-12 and it > 0
+    assert np.linalg.norm(K @ x - b) <= 1e-11 * np.linalg.norm(b)
+    assert np.abs(x - xr).max() <= 1e-8 * np.abs(xr).max()
+    # bitwise repeatable
+    assert np.array_equal(d.solve_bicgstab(b, rtol=1e-12, maxit=5000)[0], x)

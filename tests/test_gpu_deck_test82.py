@@ -15,14 +15,22 @@ from warp3d_amd import deck_input as DI
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("asym", [True, False])
-def test_voce_case_on_the_device(asym):
+@pytest.mark.parametrize("asym,device_solve",
+                         [(True, False), (True, True), (False, False)])
+def test_voce_case_on_the_device(asym, device_solve):
     """asym: the deck's asymmetric assembly (full Ke, CSR rows), else the
-    symmetrised tangent."""
+    symmetrised tangent; device_solve: the library's BiCGStab (to a
+    relative residual of 1e-13) in place of the sparse LU."""
     from warp3d_amd.model import BlockModel
+
+    def make(arrays):
+        b = BlockModel(arrays)
+        b.device_solve = device_solve
+        return b
+
     want = json.load(open(os.path.join(HERE, "expected.json")))
     _, _, ho, om, blocks = run_case(asym=asym)
-    _, _, hd, dm, blocks_d = run_case(BlockModel, asym=asym)
+    _, _, hd, dm, blocks_d = run_case(make, asym=asym)
     assert blocks_d == blocks
     assert len(hd) == len(ho) == want["step"] and all(h[0] for h in hd)
     assert [h[1] for h in hd] == [h[1] for h in ho]

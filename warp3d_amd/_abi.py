@@ -240,6 +240,8 @@ ASYM_PROTOS = {
     "assemble_csr": (_I, [_P, _D]),
     "assemble_csr_dev": (_I, [_P, _P]),
     "get_block_ek_full": (_I, [_P, _I, _D, C.c_int64]),
+    "solve_bicgstab": (_I, [_P, _D, _D, C.c_double, _I, _IP,
+                            C.POINTER(C.c_double)]),
 }
 
 # the batched serving entry points (the library only; the oracle mirrors

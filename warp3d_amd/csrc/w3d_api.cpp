@@ -954,6 +954,7 @@ static int tangent_full(w3d_ctx* c) {
   if (rc) return fail(rc, "w3d_tangent: full Ke launch failed");
   c->tangent_precision = W3D_FP64;
   c->asym_tangent = true;
+  c->csr_asym_ready = false;
   HIPCHK(hipStreamSynchronize(c->stream));
   prof_collect(c);
   return W3D_OK;
@@ -1358,6 +1359,7 @@ int w3d_set_asymmetric(w3d_ctx* c, int32_t on) {
                     "w3d_set_asymmetric: finite-strain blocks");
   c->asym = on != 0;
   c->asym_tangent = false;
+  c->csr_asym_ready = false;
   c->tangent_precision = -1;
   return W3D_OK;
 }
@@ -1432,6 +1434,45 @@ int w3d_assemble_csr(w3d_ctx* c, double* val) {
     HIPCHK(hipMemcpy(val, c->d_csr_val, 8 * (size_t)c->csr_nnz,
                      hipMemcpyDeviceToHost));
   c->csr_serial = -1;  // the solver's CSR values are no longer the VSS's
+  c->csr_asym_ready = true;
+  return W3D_OK;
+}
+
+int w3d_solve_bicgstab(w3d_ctx* c, const double* rhs, double* x, double rtol,
+                       int32_t maxit, int32_t* iters, double* relres) {
+  if (!c || !c->d_csr_rp)
+    return fail(W3D_ERR_ARG, "w3d_solve_bicgstab: no pattern");
+  if (!c->asym || !c->csr_asym_ready)
+    return fail(W3D_ERR_ARG, "w3d_solve_bicgstab: no w3d_assemble_csr since "
+                             "the last tangent");
+  if (!rhs || !x) return fail(W3D_ERR_ARG, "w3d_solve_bicgstab: null argument");
+  if (!(rtol > 0.0) || maxit < 1)
+    return fail(W3D_ERR_ARG, "w3d_solve_bicgstab: rtol > 0 and maxit >= 1");
+  HIPCHK(hipSetDevice(c->device));
+  const int n = c->neqns;
+  double* d = nullptr;
+  int rc;
+  if ((rc = dalloc(&d, 12 * (int64_t)n + 8 + solver_npart(n)))) return rc;
+  double* d_b = d;
+  double* d_x = d + n;
+  HIPCHK(hipMemcpyAsync(d_b, rhs, 8 * (size_t)n, hipMemcpyHostToDevice,
+                        c->stream));
+  int it = 0;
+  double rr = 0.0;
+  rc = bicgstab(n, c->d_csr_rp, c->d_csr_col, c->d_csr_val, d_b, d_x,
+                d + 2 * (int64_t)n, rtol, maxit, &it, &rr, c->stream);
+  if (hipMemcpyAsync(x, d_x, 8 * (size_t)n, hipMemcpyDeviceToHost,
+                     c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    rc = W3D_ERR_DEVICE;
+  dfree(d);
+  if (iters) *iters = it;
+  if (relres) *relres = rr;
+  if (rc == W3D_ERR_NUMERIC)
+    return fail(rc, "w3d_solve_bicgstab: no convergence in " +
+                        std::to_string(it) + " iterations (relres " +
+                        std::to_string(rr) + ")");
+  if (rc) return fail(rc, "w3d_solve_bicgstab: device failure");
   return W3D_OK;
 }
 

@@ -15,6 +15,8 @@
 // in registers.  The assembly runs one wave per equation row.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
 #include "w3d_internal.h"
 
 namespace w3d {
@@ -252,6 +254,145 @@ __global__ void __launch_bounds__(64 * kAsmWaves)
   }
   for (int k = lane; k < len; k += 64) val[r0 + k] = row[k];
 }
+
+// ---------------------------------------------------------------------
+// Jacobi-preconditioned BiCGStab (van der Vorst 1992, right
+// preconditioning) on the full CSR rows, FP64.  The scalars live in
+// device memory (sc: rho, rho_prev, alpha, omega, (rh, v), (t, s), (t, t),
+// (r, r)) so an iteration is a fixed sequence of launches; dot products
+// use the solver's fixed reduction tree, so a solve is bitwise repeatable.
+// ---------------------------------------------------------------------
+__global__ void k_csr_diag_inv(const int64_t* __restrict__ rp,
+                               const int32_t* __restrict__ col,
+                               const double* __restrict__ val, int n,
+                               double* __restrict__ dinv) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double d = 0.0;
+  for (int64_t k = rp[r]; k < rp[r + 1]; ++k)
+    if (col[k] == r) d = val[k];
+  dinv[r] = d != 0.0 ? 1.0 / d : 1.0;
+}
+
+// p = r + beta (p - omega v), y = p / diag
+__global__ void k_bicg_p(int n, const double* __restrict__ sc, int first,
+                         const double* __restrict__ r,
+                         const double* __restrict__ v,
+                         const double* __restrict__ dinv,
+                         double* __restrict__ p, double* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double pi;
+  if (first) {
+    pi = r[i];
+  } else {
+    const double beta = (sc[0] / sc[1]) * (sc[2] / sc[3]);
+    pi = r[i] + beta * (p[i] - sc[3] * v[i]);
+  }
+  p[i] = pi;
+  y[i] = pi * dinv[i];
+}
+
+// alpha = rho / (rh, v); s = r - alpha v; x += alpha y; z = s / diag
+__global__ void k_bicg_s(int n, const double* __restrict__ sc,
+                         const double* __restrict__ r,
+                         const double* __restrict__ v,
+                         const double* __restrict__ y,
+                         const double* __restrict__ dinv,
+                         double* __restrict__ s, double* __restrict__ x,
+                         double* __restrict__ z) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double alpha = sc[0] / sc[4];
+  const double si = r[i] - alpha * v[i];
+  s[i] = si;
+  x[i] = x[i] + alpha * y[i];
+  z[i] = si * dinv[i];
+}
+
+// omega = (t, s) / (t, t); x += omega z; r = s - omega t
+__global__ void k_bicg_x(int n, const double* __restrict__ sc,
+                         const double* __restrict__ s,
+                         const double* __restrict__ t,
+                         const double* __restrict__ z,
+                         double* __restrict__ x, double* __restrict__ r) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double omega = sc[5] / sc[6];
+  x[i] = x[i] + omega * z[i];
+  r[i] = s[i] - omega * t[i];
+}
+
+// the next iteration's rho_prev, alpha, omega
+__global__ void k_bicg_shift(double* __restrict__ sc) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double alpha = sc[0] / sc[4];
+  const double omega = sc[5] / sc[6];
+  sc[1] = sc[0];
+  sc[2] = alpha;
+  sc[3] = omega;
+}
+
+#define ACHK(x)                                  \
+  do {                                           \
+    if ((x) != hipSuccess) return W3D_ERR_DEVICE; \
+  } while (0)
+
+int bicgstab(int n, const int64_t* rp, const int32_t* col, const double* val,
+             const double* b, double* x, double* work, double rtol, int maxit,
+             int* iters, double* relres, hipStream_t st) {
+  double* r = work;
+  double* rh = r + n;
+  double* p = rh + n;
+  double* v = p + n;
+  double* s = v + n;
+  double* t = s + n;
+  double* y = t + n;
+  double* z = y + n;
+  double* dinv = z + n;
+  double* sc = dinv + n;
+  double* part = sc + 8;
+  const dim3 gb((n + 255) / 256), tb(256);
+  hipLaunchKernelGGL(k_csr_diag_inv, gb, tb, 0, st, rp, col, val, n, dinv);
+  ACHK(hipMemcpyAsync(r, b, 8 * (size_t)n, hipMemcpyDeviceToDevice, st));
+  ACHK(hipMemcpyAsync(rh, b, 8 * (size_t)n, hipMemcpyDeviceToDevice, st));
+  ACHK(hipMemsetAsync(x, 0, 8 * (size_t)n, st));
+  ACHK(hipMemsetAsync(v, 0, 8 * (size_t)n, st));
+  int rc;
+  if ((rc = solver_dot(n, b, b, part, sc + 7, st))) return rc;
+  double bb = 0.0;
+  ACHK(hipMemcpyAsync(&bb, sc + 7, 8, hipMemcpyDeviceToHost, st));
+  ACHK(hipStreamSynchronize(st));
+  const double bnorm = std::sqrt(bb);
+  *iters = 0;
+  *relres = 0.0;
+  if (bnorm == 0.0) return W3D_OK;
+  for (int it = 1; it <= maxit; ++it) {
+    if ((rc = solver_dot(n, rh, r, part, sc + 0, st))) return rc;
+    hipLaunchKernelGGL(k_bicg_p, gb, tb, 0, st, n, sc, it == 1 ? 1 : 0, r, v,
+                       dinv, p, y);
+    if ((rc = solver_spmv(n, rp, col, val, y, v, st))) return rc;
+    if ((rc = solver_dot(n, rh, v, part, sc + 4, st))) return rc;
+    hipLaunchKernelGGL(k_bicg_s, gb, tb, 0, st, n, sc, r, v, y, dinv, s, x,
+                       z);
+    if ((rc = solver_spmv(n, rp, col, val, z, t, st))) return rc;
+    if ((rc = solver_dot(n, t, s, part, sc + 5, st))) return rc;
+    if ((rc = solver_dot(n, t, t, part, sc + 6, st))) return rc;
+    hipLaunchKernelGGL(k_bicg_x, gb, tb, 0, st, n, sc, s, t, z, x, r);
+    hipLaunchKernelGGL(k_bicg_shift, dim3(1), dim3(64), 0, st, sc);
+    if ((rc = solver_dot(n, r, r, part, sc + 7, st))) return rc;
+    double rr = 0.0;
+    ACHK(hipMemcpyAsync(&rr, sc + 7, 8, hipMemcpyDeviceToHost, st));
+    ACHK(hipStreamSynchronize(st));
+    *iters = it;
+    *relres = std::sqrt(rr) / bnorm;
+    if (!std::isfinite(rr)) return W3D_ERR_NUMERIC;  // breakdown
+    if (*relres <= rtol) return W3D_OK;
+  }
+  return W3D_ERR_NUMERIC;
+}
+
+#undef ACHK
 
 #define W3D_FULL_GROUPS(X) \
   X(8, 8, true)            \

@@ -167,6 +167,7 @@ struct w3d_ctx {
   int32_t *d_fa_ln = nullptr, *d_fa_eblk = nullptr, *d_fa_bad = nullptr;
   int64_t* d_fa_invptr = nullptr;
   int csr_maxrow = 0;  // longest row of the full CSR pattern
+  bool csr_asym_ready = false;  // d_csr_val holds the asymmetric CSR
   int tangent_precision = -1;
   double eps_bbar = 0.0;
   // phase timing (w3d_set_profiling)

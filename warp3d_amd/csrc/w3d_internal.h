@@ -172,6 +172,12 @@ int launch_tangent_full(const DevModel& M, const WorkItem* d_work,
                         const int64_t* d_ekf_off, double* d_ekf,
                         hipStream_t s);
 constexpr int kAsmMaxRow = 768;  // longest CSR row the gather takes
+// Jacobi-BiCGStab on full CSR rows from x = 0; work holds 10 n + 8 +
+// solver_npart(n) doubles.  W3D_ERR_NUMERIC after maxit iterations or on a
+// breakdown (x the last iterate).
+int bicgstab(int n, const int64_t* rp, const int32_t* col, const double* val,
+             const double* b, double* x, double* work, double rtol, int maxit,
+             int* iters, double* relres, hipStream_t st);
 int launch_assemble_full(const double* ekf, const int64_t* ekf_off,
                          const BlockDesc* blocks, const int32_t* conn,
                          const int32_t* eqmap, const int32_t* eq_dof,

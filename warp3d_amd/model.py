@@ -222,6 +222,17 @@ class BlockModel:
         _chk(lib().w3d_assemble_csr(self.h, dptr(val)), "w3d_assemble_csr")
         return rp, col, val[:col.size]
 
+    def solve_bicgstab(self, rhs, rtol=1e-12, maxit=20000):
+        """x of the asymmetric CSR of the last assemble_csr (BiCGStab);
+        returns (x, iterations, relative residual)."""
+        x = np.zeros(self.neqns)
+        it = C.c_int32()
+        rr = C.c_double()
+        _chk(lib().w3d_solve_bicgstab(
+            self.h, dptr(np.ascontiguousarray(rhs, float)), dptr(x), rtol,
+            maxit, C.byref(it), C.byref(rr)), "w3d_solve_bicgstab")
+        return x, it.value, rr.value
+
     def assemble_csr_dev(self, d_val):
         return _chk(lib().w3d_assemble_csr_dev(self.h, d_val),
                     "w3d_assemble_csr_dev")
