@@ -27,11 +27,12 @@ def _case(kind):
     return m, m.model(mat, auto_size=8), 0.003
 
 
-def _drive(model, m, amp):
+def _drive(model, m, amp, cst=None):
     model.set_asymmetric(True)
-    cst = np.zeros(3 * m.nonode, np.int32)
-    cst[:3] = 1
-    cst[5] = 1
+    if cst is None:
+        cst = np.zeros(3 * m.nonode, np.int32)
+        cst[:3] = 1
+        cst[5] = 1
     model.pattern(cst)
     u = np.zeros(3 * m.nonode)
     for s in (1, 2):
@@ -87,7 +88,11 @@ def test_bicgstab_solves_the_asymmetric_csr():
     from warp3d_amd.model import BlockModel
     m, arr, amp = _case("mm10-bbar")
     d = BlockModel(arr)
-    _drive(d, m, amp)
+    # the bottom face held (a nonsingular system)
+    cst = np.zeros(3 * m.nonode, np.int32)
+    for k in range(3):
+        cst[k::3][m.coords[:, 2] == 0.0] = 1
+    _drive(d, m, amp, cst)
     rp, col, val = d.assemble_csr()
     n = d.neqns
     K = sp.csr_matrix((val, col, rp), shape=(n, n))

@@ -318,7 +318,8 @@ __global__ void k_bicg_x(int n, const double* __restrict__ sc,
                          double* __restrict__ x, double* __restrict__ r) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double omega = sc[5] / sc[6];
+  // (t, t) = 0: s is already the residual of x (converged half step)
+  const double omega = sc[6] != 0.0 ? sc[5] / sc[6] : 0.0;
   x[i] = x[i] + omega * z[i];
   r[i] = s[i] - omega * t[i];
 }
@@ -327,7 +328,7 @@ __global__ void k_bicg_x(int n, const double* __restrict__ sc,
 __global__ void k_bicg_shift(double* __restrict__ sc) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const double alpha = sc[0] / sc[4];
-  const double omega = sc[5] / sc[6];
+  const double omega = sc[6] != 0.0 ? sc[5] / sc[6] : 0.0;
   sc[1] = sc[0];
   sc[2] = alpha;
   sc[3] = omega;
