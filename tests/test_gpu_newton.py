@@ -147,3 +147,27 @@ def test_fortran_driver_load_control(tmp_path):
     got = run_drive(str(tmp_path), arr, cst, steps, _abi.W3D_FP64,
                     extrapolate=True, dloads=[dl] * 6)
     compare(got, ref)
+
+
+def test_fortran_driver_asymmetric_mm10(tmp_path):
+    """Asymmetric assembly in the Fortran driver (full Ke, CSR rows,
+    BiCGStab) against the oracle's loop with the same assembly (sparse LU):
+    the same Newton iterations, displacements within 1e-8."""
+    from oracle import oracle as O
+    from warp3d_amd.serve import RVE_MATERIAL
+    m = M.box_hex8(3, jitter=0.05, seed=5)
+    cp = _abi.mm10_material(**RVE_MATERIAL)
+    arr = m.model(cp, auto_size=16,
+                  angles=M.random_orientations(m.noelem, seed=6))
+    cst, top, _ = M.uniaxial_bc(m, 0.001)
+    d1 = np.zeros(3 * m.nonode)
+    d1[top] = 0.002
+    steps = [d1, d1, d1]
+    om = O.OracleModel(arr)
+    om.set_asymmetric(True)
+    opts = NR.NewtonOptions(extrapolate=True)
+    ref = NR.run_history(arr, cst, steps, opts=opts, om=om)
+    assert all(h[0] for h in ref)
+    got = run_drive(str(tmp_path), arr, cst, steps, _abi.W3D_FP64,
+                    extrapolate=True, asymmetric=True)
+    compare(got, ref)

@@ -22,7 +22,8 @@ def write_deck(path, arrays, cstmap, du_steps, precision=0, res_tol=0.01,
                mniter=1, mxiter=15, pcg_rtol=1e-12, pcg_maxit=50000,
                dt=1.0, extrapolate=True, line_search=False, adaptive=False,
                divergence_check=True, ls_rho=0.7, ls_slack=0.5, ls_min=0.01,
-               ls_max=1.0, dloads=None, mfs=None, mf_changes=None):
+               ls_max=1.0, dloads=None, mfs=None, mf_changes=None,
+               asymmetric=False):
     """arrays: warp3d_amd._abi.ModelArrays (single-type connectivity);
     du_steps: (nsteps, 3*nonode) imposed increments (zero off the
     constrained dofs); dloads: external load increments per step (default
@@ -47,7 +48,8 @@ def write_deck(path, arrays, cstmap, du_steps, precision=0, res_tol=0.01,
         np.array([arrays.nonode, arrays.noelem, nnpe,
                   len(arrays.materials), nsteps, precision, mniter, mxiter,
                   arrays.auto_size, pcg_maxit, int(extrapolate),
-                  int(line_search), int(adaptive), int(divergence_check)],
+                  int(line_search), int(adaptive), int(divergence_check),
+                  int(asymmetric)],
                  np.int32).tofile(f)
         np.array([res_tol, pcg_rtol, dt, ls_rho, ls_slack, ls_min, ls_max],
                  np.float64).tofile(f)

@@ -5,7 +5,8 @@
 !
 ! Deck (stream, native byte order): int32 nonode, noelem, nnpe, nmat,
 ! nsteps, precision, mniter, mxiter, auto_size, pcg_maxit, extrapolate,
-! line_search, adaptive, divergence_check; double res_tol, pcg_rtol, dt,
+! line_search, adaptive, divergence_check, asymmetric; double res_tol,
+! pcg_rtol (of the PCG, or of the BiCGStab when asymmetric), dt,
 ! ls_rho, ls_slack, ls_min, ls_max; double coords(3*nonode); int32
 ! incid(nnpe*noelem), etype(noelem), int_order(noelem), matnum(noelem),
 ! bbar(noelem); per material: int32 model, linear_elastic, slip_type,
@@ -32,7 +33,7 @@ program w3d_drive
   integer(c_int64_t) :: ncoeff
   real(c_float) :: fval(5)
   real(c_double) :: dval(14), dt, ls(4), mfv(2), mf_prev, sload
-  integer(c_int32_t) :: flags(4)
+  integer(c_int32_t) :: flags(5)
   real(c_double), allocatable, target :: coords(:), angles(:), u(:), dui(:), &
        du(:), load_n(:), dload(:)
   integer(c_int32_t), allocatable, target :: incid(:), incmap(:), etype(:), &
@@ -59,6 +60,7 @@ program w3d_drive
   opts%line_search = flags(2) /= 0
   opts%adaptive = flags(3) /= 0
   opts%divergence_check = flags(4) /= 0
+  opts%asymmetric = flags(5) /= 0
   read(fu) opts%res_tol, opts%pcg_rtol, dt, ls
   opts%ls_rho = ls(1)
   opts%ls_slack = ls(2)
@@ -120,6 +122,13 @@ program w3d_drive
   if (rc /= W3D_OK) then
     print *, 'w3d_create: ', w3d_error_message()
     stop 1
+  end if
+  if (opts%asymmetric) then
+    rc = w3d_set_asymmetric(ctx, 1)
+    if (rc /= W3D_OK) then
+      print *, 'w3d_set_asymmetric: ', w3d_error_message()
+      stop 1
+    end if
   end if
   rc = w3d_set_time_increment(ctx, dt)
   opts%dt = dt

@@ -46,6 +46,9 @@ module w3d_newton
     real(c_double) :: ls_min = 0.01d0, ls_max = 1.0d0
     logical :: adaptive = .false.
     logical :: divergence_check = .true.
+    ! asymmetric assembly (the context switched by w3d_set_asymmetric): full
+    ! CSR rows and BiCGStab in place of the VSS and Jacobi-PCG
+    logical :: asymmetric = .false.
     real(c_double) :: dt = 1.0d0            ! time increment of a step
   end type newton_options
 
@@ -210,14 +213,23 @@ contains
             rc = w3d_tangent(ctx, step, iter, opts%precision)
             if (rc /= W3D_OK) exit
           end if
-          rc = w3d_assemble_vss(ctx, c_null_ptr, c_null_ptr)
+          if (opts%asymmetric) then
+            rc = w3d_assemble_csr(ctx, c_null_ptr)
+          else
+            rc = w3d_assemble_vss(ctx, c_null_ptr, c_null_ptr)
+          end if
           if (rc /= W3D_OK) exit
           do j = 1, nodof
             if (dof_eqn_map(j) > 0) rhs(dof_eqn_map(j)) = res(j)
           end do
           x = 0.0d0
-          rc = w3d_solve_pcg(ctx, rhs, x, opts%pcg_rtol, opts%pcg_maxit, &
-                             pits, relres)
+          if (opts%asymmetric) then
+            rc = w3d_solve_bicgstab(ctx, rhs, x, opts%pcg_rtol, &
+                                    opts%pcg_maxit, pits, relres)
+          else
+            rc = w3d_solve_pcg(ctx, rhs, x, opts%pcg_rtol, opts%pcg_maxit, &
+                               pits, relres)
+          end if
           if (rc /= W3D_OK) exit
           idu = 0.0d0
           do j = 1, nodof
