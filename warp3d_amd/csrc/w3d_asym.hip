@@ -11,7 +11,8 @@
 // inverse-incidence order, the column found by binary search).
 //
 // One 256-lane workgroup per element: the point loop stages B and [D] B of
-// one point in LDS and every lane accumulates its share of the T*T words
+// one point in LDS and every lane accumulates a 4x4 (or 2x2) block of the
+// T*T words
 // in registers.  The assembly runs one wave per equation row.
 #include <hip/hip_runtime.h>
 
@@ -27,7 +28,11 @@ __global__ void __launch_bounds__(256)
                    const int64_t* __restrict__ ekf_off,
                    double* __restrict__ ekf) {
   constexpr int T = 3 * NN;
-  constexpr int NE = (T * T + 255) / 256;
+  // each lane owns an RB x RB block of (column, row) words: RB values of
+  // B and RB of [D] B per strain component feed RB^2 products
+  constexpr int RB = (T % 4 == 0 && (T / 4) * (T / 4) <= 256 && T >= 48) ? 4 : 2;
+  constexpr int NB = T / RB;
+  static_assert(T % RB == 0 && NB * NB <= 256, "register blocking");
   __shared__ double s_ce[T];
   __shared__ double s_g[G][9];
   __shared__ double s_dj[G];
@@ -102,9 +107,13 @@ __global__ void __launch_bounds__(256)
       }
       s_vol[3 * j + k] = v * (1.0 / volume);
     }
-    double acc[NE];
+    const bool own = tid < NB * NB;
+    const int col0 = RB * (tid / NB), row0 = RB * (tid % NB);
+    double acc[RB][RB];
 #pragma unroll
-    for (int n = 0; n < NE; ++n) acc[n] = 0.0;
+    for (int a = 0; a < RB; ++a)
+#pragma unroll
+      for (int c = 0; c < RB; ++c) acc[a][c] = 0.0;
     for (int g = 0; g < G; ++g) {
       __syncthreads();
       for (int k = tid; k < 6 * T; k += 256) s_b[k] = 0.0;
@@ -170,25 +179,29 @@ __global__ void __launch_bounds__(256)
                   s_d[5 * 6 + kk] * s_b[5 * T + j];
       }
       __syncthreads();
+      if (own) {
+        // word (col, row) += sum_r B(r, col) BD(r, row), r in order
 #pragma unroll
-      for (int n = 0; n < NE; ++n) {
-        const int w = tid + 256 * n;
-        if (w < T * T) {
-          const int col = w / T, row = w % T;
-          acc[n] = acc[n] + s_b[0 * T + col] * s_bd[0 * T + row] +
-                   s_b[1 * T + col] * s_bd[1 * T + row] +
-                   s_b[2 * T + col] * s_bd[2 * T + row] +
-                   s_b[3 * T + col] * s_bd[3 * T + row] +
-                   s_b[4 * T + col] * s_bd[4 * T + row] +
-                   s_b[5 * T + col] * s_bd[5 * T + row];
+        for (int r = 0; r < 6; ++r) {
+          double bc[RB], br[RB];
+#pragma unroll
+          for (int a = 0; a < RB; ++a) {
+            bc[a] = s_b[r * T + col0 + a];
+            br[a] = s_bd[r * T + row0 + a];
+          }
+#pragma unroll
+          for (int a = 0; a < RB; ++a)
+#pragma unroll
+            for (int c = 0; c < RB; ++c) acc[a][c] = acc[a][c] + bc[a] * br[c];
         }
       }
     }
     double* out = ekf + ekf_off[wi.blk] + (int64_t)i * T * T;
+    if (own) {
 #pragma unroll
-    for (int n = 0; n < NE; ++n) {
-      const int w = tid + 256 * n;
-      if (w < T * T) out[w] = acc[n];
+      for (int a = 0; a < RB; ++a)
+#pragma unroll
+        for (int c = 0; c < RB; ++c) out[(col0 + a) * T + row0 + c] = acc[a][c];
     }
   }
 }
