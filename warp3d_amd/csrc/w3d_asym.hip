@@ -212,6 +212,7 @@ __global__ void __launch_bounds__(256)
 // lanes meet); the Ke words of an element row and the row's values are
 // read and written contiguously.
 constexpr int kAsmWaves = 4;
+constexpr int kAsmChunk = 8;  // elements whose loads are in flight together
 __device__ inline void wave_fence() { __builtin_amdgcn_wave_barrier(); }
 
 __global__ void __launch_bounds__(64 * kAsmWaves)
@@ -238,32 +239,48 @@ __global__ void __launch_bounds__(64 * kAsmWaves)
   for (int k = lane; k < len; k += 64) row[k] = 0.0;
   wave_fence();
   const int dof = eq_dof[r], node = dof / 3, comp = dof % 3;
-  for (int64_t p = inv_ptr[node]; p < inv_ptr[node + 1]; ++p) {
-    const int e = inv_elem[p], b = elem_blk[e];
-    const int NN = blocks[b].nnode, T = 3 * NN;
-    const int i = e - (blocks[b].felem - 1);
-    const int erow = comp * NN + inv_ln[p];
-    if (lane < T) {
+  const int64_t pb = inv_ptr[node], pe = inv_ptr[node + 1];
+  // elements in chunks of kAsmChunk: the chunk's index searches and Ke
+  // loads are independent and overlap; the sums then go into LDS in
+  // element order
+  for (int64_t p0 = pb; p0 < pe; p0 += kAsmChunk) {
+    int at[kAsmChunk];
+    double v[kAsmChunk];
+#pragma unroll
+    for (int j = 0; j < kAsmChunk; ++j) {
+      at[j] = -1;
+      v[j] = 0.0;
+      const int64_t p = p0 + j;
+      if (p >= pe) continue;
+      const int e = inv_elem[p], b = elem_blk[e];
+      const int NN = blocks[b].nnode, T = 3 * NN;
+      if (lane >= T) continue;
+      const int i = e - (blocks[b].felem - 1);
+      const int erow = comp * NN + inv_ln[p];
       const int ecol = lane;
       const int nd = conn[blocks[b].conn_off + (int64_t)i * NN + ecol % NN];
       const int scol = eqmap[3 * nd + ecol / NN];
-      if (scol != 0) {
-        int lo = 0, hi = len - 1;
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (col[r0 + mid] < scol - 1)
-            lo = mid + 1;
-          else
-            hi = mid;
-        }
-        if (col[r0 + lo] != scol - 1)
-          bad[0] = 1;
+      if (scol == 0) continue;
+      int lo = 0, hi = len - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (col[r0 + mid] < scol - 1)
+          lo = mid + 1;
         else
-          row[lo] = row[lo] +
-                    ekf[ekf_off[b] + (int64_t)i * T * T + erow * T + ecol];
+          hi = mid;
       }
+      if (col[r0 + lo] != scol - 1) {
+        bad[0] = 1;
+        continue;
+      }
+      at[j] = lo;
+      v[j] = ekf[ekf_off[b] + (int64_t)i * T * T + erow * T + ecol];
     }
-    wave_fence();
+#pragma unroll
+    for (int j = 0; j < kAsmChunk; ++j) {
+      if (at[j] >= 0) row[at[j]] = row[at[j]] + v[j];
+      wave_fence();
+    }
   }
   for (int k = lane; k < len; k += 64) val[r0 + k] = row[k];
 }
