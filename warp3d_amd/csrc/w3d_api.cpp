@@ -138,9 +138,8 @@ void free_ctx(w3d_ctx* c) {
   dfree(c->d_csr_rp); dfree(c->d_csr_col); dfree(c->d_csr_vidx);
   dfree(c->d_csr_val); dfree(c->d_pcg);
   dfree(c->d_ekf); dfree(c->d_ekf_off);
-  dfree(c->d_fa_eqmap); dfree(c->d_fa_eqdof); dfree(c->d_fa_elem);
-  dfree(c->d_fa_ln); dfree(c->d_fa_eblk); dfree(c->d_fa_bad);
-  dfree(c->d_fa_invptr);
+  dfree(c->d_fa_eqdof); dfree(c->d_fa_ln); dfree(c->d_fa_invptr);
+  dfree(c->d_fa_ebase); dfree(c->d_fa_atoff); dfree(c->d_fa_at);
   dfree(c->d_crys); dfree(c->d_uddt); dfree(c->d_work_mm10); dfree(c->d_mm10_blks);
   dfree(c->d_rot_n); dfree(c->d_rot_n1); dfree(c->d_u_t); dfree(c->d_du_t);
   dfree(c->d_counter);
@@ -1316,7 +1315,6 @@ int w3d_pattern(w3d_ctx* c, const int32_t* cstmap, int32_t* neqns_out,
     dfree(c->d_csr_vidx);
     dfree(c->d_csr_val);
     dfree(c->d_pcg);
-    dfree(c->d_fa_eqmap);
     dfree(c->d_fa_eqdof);
     if ((rc = upload(&c->d_csr_rp, rp))) return rc;
     if ((rc = upload(&c->d_csr_col, col))) return rc;
@@ -1393,37 +1391,84 @@ int w3d_assemble_csr_dev(w3d_ctx* c, double* d_val) {
                     std::to_string(kAsmMaxRow) + " equations");
   HIPCHK(hipSetDevice(c->device));
   int rc;
-  if (!c->d_fa_eqmap) {
+  if (!c->d_fa_eqdof) {
+    // the gather map (once per pattern): per (node, element) incidence p,
+    // the element's Ke base and, per element column, its position in the
+    // node's CSR rows (the three rows of a node share their columns)
     const int64_t bytes0 = t_dalloc_bytes;
-    std::vector<int32_t> eqdof(c->neqns > 0 ? c->neqns : 1, 0);
+    const int neq = c->neqns;
+    std::vector<int32_t> eqdof(neq > 0 ? neq : 1, 0);
     for (int d = 0; d < 3 * c->nonode; ++d)
       if (c->dof_eqn_map[d]) eqdof[c->dof_eqn_map[d] - 1] = d;
-    dfree(c->d_fa_elem); dfree(c->d_fa_ln); dfree(c->d_fa_eblk);
-    dfree(c->d_fa_bad); dfree(c->d_fa_invptr);
-    if ((rc = upload(&c->d_fa_eqmap, c->dof_eqn_map))) return rc;
+    std::vector<int64_t> rp(neq + 1);
+    std::vector<int32_t> col(c->csr_nnz > 0 ? c->csr_nnz : 1);
+    HIPCHK(hipMemcpy(rp.data(), c->d_csr_rp, 8 * (size_t)(neq + 1),
+                     hipMemcpyDeviceToHost));
+    if (c->csr_nnz)
+      HIPCHK(hipMemcpy(col.data(), c->d_csr_col, 4 * (size_t)c->csr_nnz,
+                       hipMemcpyDeviceToHost));
+    std::vector<int64_t> boff(c->nelblk + 1, 0);
+    for (int b = 0; b < c->nelblk; ++b)
+      boff[b + 1] = boff[b] + (int64_t)c->blocks[b].span *
+                                  c->blocks[b].totdof * c->blocks[b].totdof;
+    const int64_t ninv = c->inv_ptr[c->nonode];
+    std::vector<int64_t> ebase(ninv > 0 ? ninv : 1), atoff(ninv + 1, 0);
+    for (int64_t p = 0; p < ninv; ++p) {
+      const BlockDesc& B = c->blocks[c->elem_blk[c->inv_elem[p]]];
+      atoff[p + 1] = atoff[p] + B.totdof;
+    }
+    std::vector<uint16_t> at(atoff[ninv] > 0 ? atoff[ninv] : 1, kAsmNone);
+    for (int n = 0; n < c->nonode; ++n) {
+      int r = -1;  // a free row of the node
+      for (int k = 0; k < 3 && r < 0; ++k)
+        if (c->dof_eqn_map[3 * n + k]) r = c->dof_eqn_map[3 * n + k] - 1;
+      for (int k = 0; k < 3 && r >= 0; ++k) {  // the rows share columns
+        const int q = c->dof_eqn_map[3 * n + k] - 1;
+        if (q < 0) continue;
+        if (rp[q + 1] - rp[q] != rp[r + 1] - rp[r] ||
+            !std::equal(col.begin() + rp[q], col.begin() + rp[q + 1],
+                        col.begin() + rp[r]))
+          return fail(W3D_ERR_ARG,
+                      "w3d_assemble_csr: rows of a node differ in columns");
+      }
+      for (int64_t p = c->inv_ptr[n]; p < c->inv_ptr[n + 1]; ++p) {
+        const int e = c->inv_elem[p], b = c->elem_blk[e];
+        const BlockDesc& B = c->blocks[b];
+        const int NN = B.nnode, T = B.totdof, i = e - (B.felem - 1);
+        ebase[p] = boff[b] + (int64_t)i * T * T;
+        if (r < 0) continue;
+        const int32_t* cb = col.data() + rp[r];
+        const int32_t* ce = col.data() + rp[r + 1];
+        for (int ecol = 0; ecol < T; ++ecol) {
+          const int nd = c->conn[B.conn_off + (int64_t)i * NN + ecol % NN];
+          const int scol = c->dof_eqn_map[3 * nd + ecol / NN];
+          if (scol == 0) continue;
+          const int32_t* f = std::lower_bound(cb, ce, scol - 1);
+          if (f == ce || *f != scol - 1)
+            return fail(W3D_ERR_ARG, "w3d_assemble_csr: pattern lacks a column");
+          at[atoff[p] + ecol] = (uint16_t)(f - cb);
+        }
+      }
+    }
+    dfree(c->d_fa_invptr); dfree(c->d_fa_ln); dfree(c->d_fa_ebase);
+    dfree(c->d_fa_atoff); dfree(c->d_fa_at);
     if ((rc = upload(&c->d_fa_eqdof, eqdof))) return rc;
     if ((rc = upload(&c->d_fa_invptr, c->inv_ptr))) return rc;
-    if ((rc = upload(&c->d_fa_elem, c->inv_elem))) return rc;
     if ((rc = upload(&c->d_fa_ln, c->inv_ln))) return rc;
-    if ((rc = upload(&c->d_fa_eblk, c->elem_blk))) return rc;
-    if ((rc = dalloc(&c->d_fa_bad, 1))) return rc;
+    if ((rc = upload(&c->d_fa_ebase, ebase))) return rc;
+    if ((rc = upload(&c->d_fa_atoff, atoff))) return rc;
+    if ((rc = upload(&c->d_fa_at, at))) return rc;
     c->dev_bytes += t_dalloc_bytes - bytes0;
   }
-  HIPCHK(hipMemsetAsync(c->d_fa_bad, 0, sizeof(int32_t), c->stream));
   prof_begin(c, W3D_PHASE_ASSEMBLE);
-  rc = launch_assemble_full(c->d_ekf, c->d_ekf_off, c->d_blocks, c->d_conn,
-                            c->d_fa_eqmap, c->d_fa_eqdof, c->d_fa_invptr,
-                            c->d_fa_elem, c->d_fa_ln, c->d_fa_eblk,
-                            c->d_csr_rp, c->d_csr_col, d_val, c->neqns,
-                            c->d_fa_bad, c->stream);
+  rc = launch_assemble_full(c->d_ekf, c->d_fa_ebase, c->d_fa_atoff,
+                            c->d_fa_at, c->d_fa_eqdof, c->d_fa_invptr,
+                            c->d_fa_ln, c->d_csr_rp, d_val, c->neqns,
+                            c->stream);
   prof_end(c, W3D_PHASE_ASSEMBLE);
   if (rc) return fail(rc, "w3d_assemble_csr: kernel launch failed");
-  int32_t bad = 0;
-  HIPCHK(hipMemcpyAsync(&bad, c->d_fa_bad, sizeof bad, hipMemcpyDeviceToHost,
-                        c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   prof_collect(c);
-  if (bad) return fail(W3D_ERR_ARG, "w3d_assemble_csr: pattern lacks a column");
   return W3D_OK;
 }
 

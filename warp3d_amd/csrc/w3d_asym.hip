@@ -13,7 +13,10 @@
 // One 256-lane workgroup per element: the point loop stages B and [D] B of
 // one point in LDS and every lane accumulates a 4x4 (or 2x2) block of the
 // T*T words
-// in registers.  The assembly runs one wave per equation row.
+// in registers.  The assembly runs one wave per equation row over a map
+// built once per pattern: for each (node, element) incidence, the Ke base of
+// the element and, per element column, its position in the CSR rows of the
+// node (the same for the node's three rows), or none if constrained.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -217,18 +220,14 @@ __device__ inline void wave_fence() { __builtin_amdgcn_wave_barrier(); }
 
 __global__ void __launch_bounds__(64 * kAsmWaves)
     k_assemble_full(const double* __restrict__ ekf,
-                    const int64_t* __restrict__ ekf_off,
-                    const BlockDesc* __restrict__ blocks,
-                    const int32_t* __restrict__ conn,
-                    const int32_t* __restrict__ eqmap,
+                    const int64_t* __restrict__ ebase,
+                    const int64_t* __restrict__ atoff,
+                    const uint16_t* __restrict__ at,
                     const int32_t* __restrict__ eq_dof,
                     const int64_t* __restrict__ inv_ptr,
-                    const int32_t* __restrict__ inv_elem,
                     const int32_t* __restrict__ inv_ln,
-                    const int32_t* __restrict__ elem_blk,
-                    const int64_t* __restrict__ rp,
-                    const int32_t* __restrict__ col, double* __restrict__ val,
-                    int neq, int32_t* __restrict__ bad) {
+                    const int64_t* __restrict__ rp, double* __restrict__ val,
+                    int neq) {
   __shared__ double s_row[kAsmWaves][kAsmMaxRow];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int r = blockIdx.x * kAsmWaves + wv;
@@ -240,45 +239,28 @@ __global__ void __launch_bounds__(64 * kAsmWaves)
   wave_fence();
   const int dof = eq_dof[r], node = dof / 3, comp = dof % 3;
   const int64_t pb = inv_ptr[node], pe = inv_ptr[node + 1];
-  // elements in chunks of kAsmChunk: the chunk's index searches and Ke
-  // loads are independent and overlap; the sums then go into LDS in
-  // element order
+  // elements in chunks of kAsmChunk: their loads are independent and
+  // overlap; the sums then go into LDS in element order
   for (int64_t p0 = pb; p0 < pe; p0 += kAsmChunk) {
-    int at[kAsmChunk];
+    int pos[kAsmChunk];
     double v[kAsmChunk];
 #pragma unroll
     for (int j = 0; j < kAsmChunk; ++j) {
-      at[j] = -1;
+      pos[j] = kAsmNone;
       v[j] = 0.0;
       const int64_t p = p0 + j;
       if (p >= pe) continue;
-      const int e = inv_elem[p], b = elem_blk[e];
-      const int NN = blocks[b].nnode, T = 3 * NN;
+      const int64_t a0 = atoff[p];
+      const int T = (int)(atoff[p + 1] - a0);
       if (lane >= T) continue;
-      const int i = e - (blocks[b].felem - 1);
-      const int erow = comp * NN + inv_ln[p];
-      const int ecol = lane;
-      const int nd = conn[blocks[b].conn_off + (int64_t)i * NN + ecol % NN];
-      const int scol = eqmap[3 * nd + ecol / NN];
-      if (scol == 0) continue;
-      int lo = 0, hi = len - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (col[r0 + mid] < scol - 1)
-          lo = mid + 1;
-        else
-          hi = mid;
-      }
-      if (col[r0 + lo] != scol - 1) {
-        bad[0] = 1;
-        continue;
-      }
-      at[j] = lo;
-      v[j] = ekf[ekf_off[b] + (int64_t)i * T * T + erow * T + ecol];
+      pos[j] = at[a0 + lane];
+      if (pos[j] == kAsmNone) continue;
+      const int erow = comp * (T / 3) + inv_ln[p];
+      v[j] = ekf[ebase[p] + erow * T + lane];
     }
 #pragma unroll
     for (int j = 0; j < kAsmChunk; ++j) {
-      if (at[j] >= 0) row[at[j]] = row[at[j]] + v[j];
+      if (pos[j] != kAsmNone) row[pos[j]] = row[pos[j]] + v[j];
       wave_fence();
     }
   }
@@ -460,19 +442,16 @@ int launch_tangent_full(const DevModel& M, const WorkItem* d_work,
   return hipGetLastError() == hipSuccess ? W3D_OK : W3D_ERR_DEVICE;
 }
 
-int launch_assemble_full(const double* ekf, const int64_t* ekf_off,
-                         const BlockDesc* blocks, const int32_t* conn,
-                         const int32_t* eqmap, const int32_t* eq_dof,
-                         const int64_t* inv_ptr, const int32_t* inv_elem,
-                         const int32_t* inv_ln, const int32_t* elem_blk,
-                         const int64_t* rp, const int32_t* col, double* val,
-                         int neq, int32_t* bad, hipStream_t s) {
+int launch_assemble_full(const double* ekf, const int64_t* ebase,
+                         const int64_t* atoff, const uint16_t* at,
+                         const int32_t* eq_dof, const int64_t* inv_ptr,
+                         const int32_t* inv_ln, const int64_t* rp,
+                         double* val, int neq, hipStream_t s) {
   if (neq == 0) return W3D_OK;
   hipLaunchKernelGGL(k_assemble_full,
                      dim3((neq + kAsmWaves - 1) / kAsmWaves),
-                     dim3(64 * kAsmWaves), 0,
-                     s, ekf, ekf_off, blocks, conn, eqmap, eq_dof, inv_ptr,
-                     inv_elem, inv_ln, elem_blk, rp, col, val, neq, bad);
+                     dim3(64 * kAsmWaves), 0, s, ekf, ebase, atoff, at,
+                     eq_dof, inv_ptr, inv_ln, rp, val, neq);
   return hipGetLastError() == hipSuccess ? W3D_OK : W3D_ERR_DEVICE;
 }
 

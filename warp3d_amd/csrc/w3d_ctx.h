@@ -163,9 +163,9 @@ struct w3d_ctx {
   double* d_ekf = nullptr;
   int64_t* d_ekf_off = nullptr;
   int64_t n_ekf = 0;
-  int32_t *d_fa_eqmap = nullptr, *d_fa_eqdof = nullptr, *d_fa_elem = nullptr;
-  int32_t *d_fa_ln = nullptr, *d_fa_eblk = nullptr, *d_fa_bad = nullptr;
-  int64_t* d_fa_invptr = nullptr;
+  int32_t *d_fa_eqdof = nullptr, *d_fa_ln = nullptr;
+  int64_t *d_fa_invptr = nullptr, *d_fa_ebase = nullptr, *d_fa_atoff = nullptr;
+  uint16_t* d_fa_at = nullptr;
   int csr_maxrow = 0;  // longest row of the full CSR pattern
   bool csr_asym_ready = false;  // d_csr_val holds the asymmetric CSR
   int tangent_precision = -1;

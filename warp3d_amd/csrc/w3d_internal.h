@@ -178,13 +178,12 @@ constexpr int kAsmMaxRow = 768;  // longest CSR row the gather takes
 int bicgstab(int n, const int64_t* rp, const int32_t* col, const double* val,
              const double* b, double* x, double* work, double rtol, int maxit,
              int* iters, double* relres, hipStream_t st);
-int launch_assemble_full(const double* ekf, const int64_t* ekf_off,
-                         const BlockDesc* blocks, const int32_t* conn,
-                         const int32_t* eqmap, const int32_t* eq_dof,
-                         const int64_t* inv_ptr, const int32_t* inv_elem,
-                         const int32_t* inv_ln, const int32_t* elem_blk,
-                         const int64_t* rp, const int32_t* col, double* val,
-                         int neq, int32_t* bad, hipStream_t s);
+constexpr int kAsmNone = 0xFFFF;  // constrained column in the gather map
+int launch_assemble_full(const double* ekf, const int64_t* ebase,
+                         const int64_t* atoff, const uint16_t* at,
+                         const int32_t* eq_dof, const int64_t* inv_ptr,
+                         const int32_t* inv_ln, const int64_t* rp,
+                         double* val, int neq, hipStream_t s);
 int launch_tangent_mfma(const DevModel& M, const WorkItem* d_work,
                         const std::vector<int>& groups,
                         const std::vector<int>& group_start,
