@@ -11,7 +11,7 @@
 // inverse-incidence order, the column found by binary search).
 //
 // One 256-lane workgroup per element: the point loop stages B and [D] B of
-// one point in LDS and every lane accumulates a 4x4 (or 3x3) block of the
+// one point in LDS and every lane accumulates a 4x4 (or 2x2) block of the
 // T*T words
 // in registers.  The assembly runs one wave per equation row over a map
 // built once per pattern: for each (node, element) incidence, the Ke base of
@@ -33,9 +33,7 @@ __global__ void __launch_bounds__(256)
   constexpr int T = 3 * NN;
   // each lane owns an RB x RB block of (column, row) words: RB values of
   // B and RB of [D] B per strain component feed RB^2 products
-  constexpr int RB = (T % 4 == 0 && (T / 4) * (T / 4) <= 256 && T >= 48)
-                         ? 4
-                         : (T % 3 == 0 && (T / 3) * (T / 3) <= 256 ? 3 : 2);
+  constexpr int RB = (T % 4 == 0 && (T / 4) * (T / 4) <= 256 && T >= 48) ? 4 : 2;
   constexpr int NB = T / RB;
   static_assert(T % RB == 0 && NB * NB <= 256, "register blocking");
   __shared__ double s_ce[T];
