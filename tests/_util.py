@@ -40,7 +40,7 @@ def smooth_field(m, scale, seed=0, noise=0.2):
     u[:, 2] = scale * x[:, 2]
     u[:, 0] = -0.3 * scale * x[:, 0] + 0.2 * scale * x[:, 2]
     u[:, 1] = -0.3 * scale * x[:, 1]
-    u += noise * scale * rng.uniform(-1, 1, size=x.shape) * (m.h.mean()
-                                                            if m.h is not None
-                                                            else 1.0)
+    h = getattr(m, "h", None)  # element size (None: meshes of parts)
+    u += noise * scale * rng.uniform(-1, 1, size=x.shape) * (
+        h.mean() if h is not None else 1.0)
     return u.ravel()

@@ -68,12 +68,16 @@ def _run(arr, m, asym, amp=0.004, steps=2):
     return o
 
 
-@pytest.mark.parametrize("kind", ["hex8", "hex20", "tet10"])
+@pytest.mark.parametrize("kind", ["hex8", "hex20", "tet10", "mixed"])
 def test_symmetric_material_gives_the_packed_matrix(kind):
     mat = _abi.mm01_material(200e3, 0.3, 400.0, hprime=2000.0)
     m = {"hex8": lambda: M.box_hex8(2, jitter=0.1, seed=3),
          "hex20": lambda: M.box_hex20(2, jitter=0.05, seed=3),
-         "tet10": lambda: M.box_tet10(1, jitter=0.05, seed=3)}[kind]()
+         "tet10": lambda: M.box_tet10(1, jitter=0.05, seed=3),
+         "mixed": lambda: M.MixedMesh(
+             [(M.box_hex8(2, jitter=0.1, seed=3), 1, 1),
+              (M.box_hex20(2, jitter=0.05, seed=4), 8, 0),
+              (M.box_tet10(1, jitter=0.05, seed=5), 4, 0)])}[kind]()
     arr = m.model(mat, auto_size=8)
     oa, os_ = _run(arr, m, True, amp=0.003), _run(arr, m, False, amp=0.003)
     for b in range(1, oa.nelblk + 1):

@@ -23,7 +23,12 @@ def _case(kind):
         return m, arr, 0.004
     mat = _abi.mm01_material(200e3, 0.3, 400.0, hprime=2000.0)
     m = {"hex20": lambda: M.box_hex20(2, jitter=0.05, seed=3),
-         "tet10": lambda: M.box_tet10(2, jitter=0.05, seed=3)}[kind]()
+         "tet10": lambda: M.box_tet10(2, jitter=0.05, seed=3),
+         # blocks of three element sizes (T = 24, 60, 30) in one model
+         "mixed": lambda: M.MixedMesh(
+             [(M.box_hex8(2, jitter=0.1, seed=3), 1, 1),
+              (M.box_hex20(2, jitter=0.05, seed=4), 8, 0),
+              (M.box_tet10(1, jitter=0.05, seed=5), 4, 0)])}[kind]()
     return m, m.model(mat, auto_size=8), 0.003
 
 
@@ -46,7 +51,8 @@ def _drive(model, m, amp, cst=None):
     return ifv
 
 
-@pytest.mark.parametrize("kind", ["mm10-bbar", "mm10", "hex20", "tet10"])
+@pytest.mark.parametrize("kind", ["mm10-bbar", "mm10", "hex20", "tet10",
+                                  "mixed"])
 def test_full_matrices_and_csr_match_the_oracle(kind):
     from warp3d_amd.model import BlockModel
     m, arr, amp = _case(kind)
