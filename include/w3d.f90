@@ -3,13 +3,17 @@
 ! mnralg.f) uses this module to call the element-block hot path on the GPU:
 ! build the model once (w3d_create), then per iteration w3d_recover,
 ! w3d_tangent, w3d_assemble_vss and w3d_solve_pcg, and w3d_commit_step at
-! convergence.  Every integer index argument is 1-based as in w3d.h.
+! convergence.  The per-block form (w3d_set_displacements, then
+! w3d_recover_block / w3d_tangent_block per block, which may be called from
+! inside an !$OMP PARALLEL DO over blocks as the reference calls
+! do_nleps_block / do_nlek_block, then w3d_assemble_ifv) is bound too.
+! Every integer index argument is 1-based as in w3d.h.
 module w3d
   use, intrinsic :: iso_c_binding
   implicit none
   private :: c_int32_t, c_int64_t, c_double, c_float, c_ptr, c_char
 
-  integer(c_int32_t), parameter :: W3D_ABI_VERSION = 3
+  integer(c_int32_t), parameter :: W3D_ABI_VERSION = 4
   ! status codes
   integer(c_int32_t), parameter :: W3D_OK = 0, W3D_CUT_STEP = 1, &
        W3D_ERR_ARG = -1, W3D_ERR_UNSUPPORTED = -2, W3D_ERR_DEVICE = -3, &
@@ -115,6 +119,62 @@ module w3d
       real(c_double), intent(in) :: u(*), du(*)
       real(c_double), intent(out) :: ifv(*)
       real(c_double), intent(out) :: energy, plastic_work
+      integer(c_int) :: rc
+    end function
+    function w3d_set_displacements(ctx, u, du) &
+         bind(C, name="w3d_set_displacements") result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: u(*), du(*)
+      integer(c_int) :: rc
+    end function
+    function w3d_recover_block(ctx, blk, step, iter, extrapolated, &
+         block_energy, block_plastic_work, cut) &
+         bind(C, name="w3d_recover_block") result(rc)
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: blk, step, iter, extrapolated
+      real(c_double), intent(out) :: block_energy, block_plastic_work
+      integer(c_int32_t), intent(out) :: cut
+      integer(c_int) :: rc
+    end function
+    function w3d_recover_blocks(ctx, first_blk, nblk, step, iter, &
+         extrapolated, u, du) bind(C, name="w3d_recover_blocks") result(rc)
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: first_blk, nblk, step, iter, extrapolated
+      type(c_ptr), value :: u, du   ! c_null_ptr: the set displacements
+      integer(c_int) :: rc
+    end function
+    function w3d_assemble_ifv(ctx, ifv, sum_ifv, num_term_ifv) &
+         bind(C, name="w3d_assemble_ifv") result(rc)
+      import :: c_ptr, c_int, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: ifv(*)
+      real(c_double), intent(out) :: sum_ifv
+      integer(c_int64_t), intent(out) :: num_term_ifv
+      integer(c_int) :: rc
+    end function
+    function w3d_get_ifv_sums(ctx, sum_ifv, num_term_ifv) &
+         bind(C, name="w3d_get_ifv_sums") result(rc)
+      import :: c_ptr, c_int, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: sum_ifv
+      integer(c_int64_t), intent(out) :: num_term_ifv
+      integer(c_int) :: rc
+    end function
+    function w3d_tangent_block(ctx, blk, step, iter, precision) &
+         bind(C, name="w3d_tangent_block") result(rc)
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: blk, step, iter, precision
+      integer(c_int) :: rc
+    end function
+    function w3d_tangent_blocks(ctx, first_blk, nblk, step, iter, &
+         precision) bind(C, name="w3d_tangent_blocks") result(rc)
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: first_blk, nblk, step, iter, precision
       integer(c_int) :: rc
     end function
     function w3d_tangent(ctx, step, iter, precision) &

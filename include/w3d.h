@@ -21,9 +21,12 @@
  * Reference interfaces replaced (file:line under the reference's src/):
  *   w3d_create            model set-up in incomp.f:14 / setup.f / inelbk.f:282
  *   w3d_recover           drive_eps_sig_internal_forces.f:20
- *   w3d_recover_blocks    do_nleps_block, drive_eps_sig_internal_forces.f:399
+ *   w3d_recover_block     do_nleps_block, drive_eps_sig_internal_forces.f:399
+ *   w3d_recover_blocks    the same for a range of blocks
+ *   w3d_assemble_ifv      addifv.f:16 (the element forces into ifv)
  *   w3d_tangent           tanstf.f:17
- *   w3d_tangent_blocks    do_nlek_block, tanstf.f:119
+ *   w3d_tangent_block     do_nlek_block, tanstf.f:119
+ *   w3d_tangent_blocks    the same for a range of blocks
  *   w3d_pattern           drive_assemble_solve.f:358 (ds_setup_sparsity_local)
  *   w3d_assemble_vss      drive_assemble_solve.f:654 (ds_root_assembly ->
  *                         assemble_code.f:659 assem_by_row)
@@ -39,7 +42,7 @@
 extern "C" {
 #endif
 
-#define W3D_ABI_VERSION 3
+#define W3D_ABI_VERSION 4
 
 /* ---- status codes ---------------------------------------------------- */
 enum {
@@ -205,13 +208,57 @@ int w3d_recover_dev(w3d_ctx* ctx, int32_t step, int32_t iter,
                     double* plastic_work);
 
 /*
- * The per-block form of the recovery (do_nleps_block): processes blocks
- * first_blk .. first_blk+nblk-1 and leaves the element forces in the block's
- * einfvec array (read with w3d_get_block_state) without assembling them.
+ * The per-block interface, the counterpart of do_nleps_block /
+ * do_nlek_block called inside the reference's !$OMP PARALLEL DO over
+ * blocks (drive_eps_sig_internal_forces.f:160-180, tanstf.f:80-87):
+ *
+ *   w3d_set_displacements(ctx, u, du)           once per recovery pass
+ *   w3d_recover_block(ctx, blk, ...)            any thread, any order
+ *   w3d_assemble_ifv(ctx, ifv, &sum, &nterm)    after every block (addifv)
+ *   w3d_tangent_block(ctx, blk, ...)            any thread, any order
+ *
+ * w3d_recover_block and w3d_tangent_block may run concurrently from many
+ * host threads for distinct blocks of one context: each call works on its
+ * own block's state, flags and work items, on a stream of the context's
+ * pool, and returns when its block is done.  They must not overlap the
+ * context's other calls (whole-pass recovery, commit, rollback,
+ * w3d_set_displacements, assembly), as the reference's block loop does not
+ * overlap its update or assembly.
+ *
+ * w3d_set_displacements copies u (may be NULL: zero) and du (3*nonode) to
+ * the device after the context's earlier work has ended.
+ * w3d_recover_block recovers block blk (1-based) with those displacements
+ * and leaves its element forces in the block's einfvec; block_energy and
+ * block_plastic_work receive the block's internal energy and plastic work
+ * increments summed in element order (rstgp1_b; zero for iter = 0) and
+ * cut the block's material_cut_step (also the W3D_CUT_STEP status).  Any
+ * output pointer may be NULL.
+ * w3d_assemble_ifv gathers every block's element forces into ifv (3*nonode,
+ * may be NULL) in a fixed order, and gives addifv's sum of |element force|
+ * terms over the blocks' last recoveries (sum_ifv) and their number
+ * (num_term_ifv), the input of the convergence test on the largest residual
+ * (cvtest.f test 4).
  */
+int w3d_set_displacements(w3d_ctx* ctx, const double* u, const double* du);
+int w3d_recover_block(w3d_ctx* ctx, int32_t blk, int32_t step, int32_t iter,
+                      int32_t extrapolated, double* block_energy,
+                      double* block_plastic_work, int32_t* cut);
+int w3d_assemble_ifv(w3d_ctx* ctx, double* ifv, double* sum_ifv,
+                     int64_t* num_term_ifv);
+int w3d_tangent_block(w3d_ctx* ctx, int32_t blk, int32_t step, int32_t iter,
+                      int32_t precision);
+/* Blocks first_blk .. first_blk+nblk-1 in one call, on one stream: with du
+ * non-NULL the displacements are set first (w3d_set_displacements), with
+ * du NULL those of the last w3d_set_displacements are used.  Returns
+ * W3D_CUT_STEP when any of them asks for a cut. */
 int w3d_recover_blocks(w3d_ctx* ctx, int32_t first_blk, int32_t nblk,
                        int32_t step, int32_t iter, int32_t extrapolated,
                        const double* u, const double* du);
+/* addifv's sums of the last whole-pass recovery that returned its energy
+ * (w3d_recover / w3d_recover_dev / w3d_iteration_dev with an energy or
+ * plastic-work pointer): sum of |element force| terms and their number. */
+int w3d_get_ifv_sums(const w3d_ctx* ctx, double* sum_ifv,
+                     int64_t* num_term_ifv);
 
 /*
  * Block tangent stiffness Ke = sum_gp B^T D B w |J| for every block, from the
@@ -219,6 +266,9 @@ int w3d_recover_blocks(w3d_ctx* ctx, int32_t first_blk, int32_t nblk,
  * precision selects W3D_FP64 or W3D_FP32 (MFMA).
  */
 int w3d_tangent(w3d_ctx* ctx, int32_t step, int32_t iter, int32_t precision);
+/* blocks first_blk .. first_blk+nblk-1 in one call (the whole model: one
+ * launch per kernel group); every block of a pass takes the same
+ * precision */
 int w3d_tangent_blocks(w3d_ctx* ctx, int32_t first_blk, int32_t nblk,
                        int32_t step, int32_t iter, int32_t precision);
 

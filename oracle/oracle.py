@@ -31,6 +31,8 @@ def lib():
         L.orc_destroy.argtypes = [C.c_void_p]
         L.orc_last_error.restype = C.c_char_p
         L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_block_sums.argtypes = [C.c_void_p, C.c_int,
+                                     C.POINTER(C.c_double)]
         L.orc_tangent.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.orc_set_asymmetric.argtypes = [C.c_void_p, C.c_int]
         L.orc_assemble_csr.argtypes = [C.c_void_p, C.POINTER(C.c_int64),
@@ -379,6 +381,13 @@ class OracleModel:
             dptr(np.ascontiguousarray(u, float)),
             dptr(np.ascontiguousarray(du, float))), "recover_blocks")
 
+    def block_sums(self, blk):
+        """(energy, plastic work, sum |element force|) of block blk's last
+        recovery (do_nleps_block's block sums, addifv's sum_ifv terms)."""
+        out = np.zeros(3)
+        _chk(lib().orc_block_sums(self.h, blk, dptr(out)), "block_sums")
+        return tuple(out)
+
     def tangent(self, step, iter_, precision=0):
         return _chk(lib().orc_tangent(self.h, step, iter_), "tangent")
 
@@ -481,10 +490,18 @@ class OracleModel:
                                        data.size), "set_block_state")
 
 
+REF_LEAF_PATH = os.path.join(_HERE, "_ref", "libw3d_refleaf.so")
+
+
+def ref_leaf_built():
+    """Whether `make -C oracle ref` has built the reference's leaf
+    routines (without loading them)."""
+    return os.path.exists(REF_LEAF_PATH)
+
+
 def load_ref_leaf():
     """The reference's own leaf routines built by `make -C oracle ref`, or
     None when they are not built (no reference tree on this machine)."""
-    path = os.path.join(_HERE, "_ref", "libw3d_refleaf.so")
-    if not os.path.exists(path):
+    if not ref_leaf_built():
         return None
-    return C.CDLL(path, mode=os.RTLD_LAZY)
+    return C.CDLL(REF_LEAF_PATH, mode=os.RTLD_LAZY)

@@ -61,6 +61,7 @@ typedef struct {
   int geonl;                  /* finite strains and rotations            */
   double *rot_n, *rot_n1;     /* geonl: (9, ngp, span) [R] column-major  */
   double* ekf;                /* asymmetric: (T*T, span) full Ke         */
+  double energy, pwork;       /* the last recovery's block sums (rstgp1_b) */
 } orc_block;
 
 struct orc_model {
@@ -656,7 +657,26 @@ static int recover_block(orc_model* om, int b, int step, int iter,
     return step > 1 ? W3D_CUT_STEP : W3D_ERR_NUMERIC;
   *energy = blk_energy;
   *pwork = blk_pwork;
+  B->energy = iter > 0 ? blk_energy : 0.0;
+  B->pwork = iter > 0 ? blk_pwork : 0.0;
   return rc ? rc : (cut ? W3D_CUT_STEP : 0);
+}
+
+/* do_nleps_block's block_energy and block_plastic_work of block blk's
+   last recovery (zero for iteration 0), and addifv_standard's sum of
+   |eleifv(i,j)| over the block, j outer and i inner (addifv.f:100-105). */
+int orc_block_sums(const orc_model* om, int blk, double* out3) {
+  if (blk < 1 || blk > om->nelblk)
+    return fail(W3D_ERR_ARG, "orc_block_sums: block");
+  const orc_block* B = &om->blk[blk - 1];
+  double s = 0.0;
+  for (int j = 0; j < B->totdof; ++j)
+    for (int i = 0; i < B->span; ++i)
+      s = s + fabs(B->einfvec[(int64_t)j * B->span + i]);
+  out3[0] = B->energy;
+  out3[1] = B->pwork;
+  out3[2] = s;
+  return 0;
 }
 
 static void keep_displacements(orc_model* om, const double* u,

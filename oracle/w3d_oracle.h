@@ -118,6 +118,7 @@ int orc_recover(orc_model* om, int step, int iter, int extrapolated,
                 const double* u, const double* du, double* ifv,
                 double* energy, double* plastic_work);
 /* Recovery of blocks [first, first+n) only, element forces left in einfvec. */
+int orc_block_sums(const orc_model* om, int blk, double* out3);
 int orc_recover_blocks(orc_model* om, int first, int n, int step, int iter,
                        int extrapolated, const double* u, const double* du);
 int orc_tangent(orc_model* om, int step, int iter);

@@ -15,9 +15,11 @@ Restates the static path of mnralg.f:
   line search of mnralg_ls (mnralg.f:758-914; backtracking by rho from
   step length 1 while |s/s0| >= slack, s = -res . idu, back to 1 when r
   stops falling by 0.9 or the length falls below its minimum), residual
-  and load norm (upres.f:148-234), cvtest test 2 (cvtest.f:163-186) and
-  the divergence check (three non-decreasing residual norms, cvtest.f:
-  94-107), new element stiffnesses;
+  and load norm (upres.f:148-234), the cvtest tests (cvtest.f:103-310;
+  test 4's average nodal force from addifv's sum of |element force| and
+  upres's sum of the nonzero load terms) and the divergence check (three
+  non-decreasing residual norms, cvtest.f:94-107), new element
+  stiffnesses;
 * failure (iteration limit, divergence, a material's cut request): with
   adaptive on and step > 1, the state is restored and the increment is
   split by the adaptive stack of adapt_check.f (4 divisions, smallest
@@ -43,20 +45,89 @@ def vss_matrix(om, kd, kc):
     return (up + up.T + sp.diags(kd)).tocsc()
 
 
-def residual(cst, ifv, pbar=None):
+def residual(cst, ifv, pbar=None, sums=False):
     """upres: res = pbar - ifv on free dofs (0 on constrained ones), the
     load norm over the nonzero terms |pbar| (free) and |ifv| (constrained),
-    and the residual norm."""
+    and the residual norm; with sums also the sum and number of the
+    nonzero load terms (sum_loads, num_term_loads)."""
     p = np.zeros_like(ifv) if pbar is None else pbar
     res = np.where(cst == 0, p - ifv, 0.0)
     term = np.where(cst == 0, np.abs(p), np.abs(ifv))
-    mgload = np.sqrt(np.sum(term[term != 0.0] ** 2))
-    return res, mgload, np.sqrt(np.sum(res ** 2))
+    nz = term[term != 0.0]
+    mgload = np.sqrt(np.sum(nz ** 2))
+    out = (res, mgload, np.sqrt(np.sum(res ** 2)))
+    if sums:
+        out = out + (float(np.sum(nz)), int(nz.size))
+    return out
+
+
+def ifv_sums(om):
+    """addifv's sum of |element force| terms over the blocks and their
+    number (num_term_ifv), after a recovery."""
+    s, n = 0.0, 0
+    for b in range(1, om.nelblk + 1):
+        s = s + om.block_sums(b)[2]
+        info = om.block_info(b)
+        n += info["span"] * info["totdof"]
+    return s, n
+
+
+def cvtest(tests, it, idu, res, nres, mgload, sum_ifv, nterm_ifv, sum_loads,
+           nterm_loads, state):
+    """cvtest.f:103-310 over the selected tests ({number: tolerance}):
+    (converged, values).  state['magdu1'] keeps |idu| of the first
+    iteration of the step (set at it = 1 when test 1 or 3 is selected)."""
+    nidu = float(np.sqrt(np.sum(idu ** 2)))
+    maxidu = float(np.max(np.abs(idu)))
+    resmax = float(np.max(np.abs(res)))
+    if it == 1 and (1 in tests or 3 in tests):
+        state["magdu1"] = nidu
+    m1 = state.get("magdu1", 0.0)
+    ok, vals = True, {}
+    for t, tol in tests.items():
+        if t == 1:
+            if it == 1:
+                f = tol > 100.0
+            elif m1 == 0.0:
+                f = True
+            else:
+                vals[1] = nidu / m1 * 100.0
+                f = vals[1] < tol
+        elif t == 2:
+            if mgload == 0.0:
+                f = True
+            else:
+                vals[2] = nres / mgload * 100.0
+                f = vals[2] <= tol or (mgload <= 1e-5 and nres <= 1e-8)
+        elif t == 3:
+            if m1 == 0.0:
+                f = True
+            else:
+                vals[3] = maxidu / m1 * 100.0
+                f = vals[3] < tol
+        elif t == 4:
+            avg = (sum_ifv + sum_loads) / float(nterm_ifv + nterm_loads)
+            if avg == 0.0:
+                f = True
+            else:
+                vals[4] = resmax / avg * 100.0
+                f = vals[4] <= tol or (resmax <= 1e-8 and avg <= 1e-8)
+        elif t == 5:
+            vals[5] = nidu
+            f = nidu <= tol
+        elif t == 6:
+            vals[6] = maxidu
+            f = maxidu <= tol
+        else:
+            raise ValueError(f"cvtest: no test {t}")
+        ok = ok and f
+    return ok, vals
 
 
 @dataclass
 class NewtonOptions:
     res_tol: float = 0.01          # cvtest test 2 (%)
+    tests: dict = None             # cvtest {test: tolerance}; None: test 2
     mniter: int = 1
     mxiter: int = 15
     extrapolate: bool = True
@@ -156,6 +227,9 @@ def newton_step(om, step, cst, dmap, u, du_imposed, res_tol=0.01, mniter=1,
         ifv, _, _, rc = om.recover(step, 0, u, du, extrapolated)
         failed = rc != 0
         ratios, norms = [], []
+        tests = opts.tests if opts.tests else {2: opts.res_tol}
+        cvstate = {}
+        info["tests"] = []
         it = 0
         if not failed:
             om.tangent(step, 0)
@@ -188,12 +262,15 @@ def newton_step(om, step, cst, dmap, u, du_imposed, res_tol=0.01, mniter=1,
                 if rc != 0:
                     failed = True
                     break
-                res, mgload, nres = residual(cst, ifv, pbar)
+                res, mgload, nres, sl, nl = residual(cst, ifv, pbar,
+                                                     sums=True)
                 ratio = 0.0 if mgload == 0 else nres / mgload * 100.0
                 ratios.append(ratio)
                 norms.append(nres)
-                conv = mgload == 0 or ratio <= opts.res_tol or \
-                    (mgload <= 1e-5 and nres <= 1e-8)
+                si, ni = ifv_sums(om)
+                conv, vals = cvtest(tests, it, idu, res, nres, mgload, si,
+                                    ni, sl, nl, cvstate)
+                info["tests"].append(vals)
                 diverging = len(norms) >= 3 and norms[-1] >= norms[-2] and \
                     norms[-2] >= norms[-3]
                 if conv and it >= opts.mniter:

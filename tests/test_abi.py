@@ -30,6 +30,8 @@ def lib():
 def test_header_declares_the_boundary():
     names = declared()
     for must in ("w3d_create", "w3d_recover", "w3d_recover_blocks",
+                 "w3d_set_displacements", "w3d_recover_block",
+                 "w3d_assemble_ifv", "w3d_tangent_block",
                  "w3d_tangent", "w3d_tangent_blocks", "w3d_pattern",
                  "w3d_assemble_vss", "w3d_commit_step", "w3d_rollback",
                  "w3d_get_block_state", "w3d_last_error"):
@@ -121,3 +123,18 @@ def test_fortran_module_constants_match_the_c_header():
     for name, val in fdefs.items():
         if name in cvals:
             assert cvals[name] == val, (name, val, cvals[name])
+
+
+def test_fortran_module_binds_declared_functions():
+    """Every bind(C, name=...) of include/w3d.f90 names a function of
+    w3d.h, and the per-block interface is bound."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    text = open(os.path.join(root, "include", "w3d.f90")).read()
+    bound = set(re.findall(r'bind\(C,\s*name="(w3d_[a-z0-9_]+)"\)', text))
+    names = set(declared())
+    assert bound <= names, sorted(bound - names)
+    for must in ("w3d_set_displacements", "w3d_recover_block",
+                 "w3d_recover_blocks", "w3d_assemble_ifv",
+                 "w3d_get_ifv_sums", "w3d_tangent_block",
+                 "w3d_tangent_blocks"):
+        assert must in bound, must

@@ -171,3 +171,50 @@ def test_fortran_driver_asymmetric_mm10(tmp_path):
     got = run_drive(str(tmp_path), arr, cst, steps, _abi.W3D_FP64,
                     extrapolate=True, asymmetric=True)
     compare(got, ref)
+
+
+@pytest.mark.parametrize("tests", [{1: 1.0}, {3: 1.0}, {4: 0.01}, {5: 1e-8},
+                                   {6: 1e-9}, {1: 0.5, 2: 0.01, 4: 0.05}],
+                         ids=["t1", "t3", "t4", "t5", "t6", "t1+2+4"])
+def test_fortran_driver_convergence_tests(c1, tmp_path, tests):
+    """cvtest's tests 1-6 (cvtest.f:103-290) in the Fortran driver against
+    the oracle loop: the same iterations per step, displacements within
+    1e-8, and each test's value per iteration within 1e-6 relative where
+    it is above round-off (test 4 takes addifv's sum of |element force|
+    from the library)."""
+    m, arr, cst, steps, _ = c1
+    steps = steps[:8]
+    opts = NR.NewtonOptions(extrapolate=False, tests=tests)
+    ref = NR.run_history(arr, cst, steps, opts=opts)
+    assert all(h[0] for h in ref)
+    got = run_drive(str(tmp_path), arr, cst, steps, _abi.W3D_FP64,
+                    tests=tests)
+    compare(got, ref)
+    for g, (_, _, _, _, info) in zip(got, ref):
+        for k, vals in enumerate(info["tests"]):
+            for t, v in vals.items():
+                d = g["tests"][k][t - 1]
+                floor = 1e-6 if t <= 4 else 1e-12
+                if v > floor:
+                    assert abs(d - v) <= 1e-6 * v, (t, k, d, v)
+                else:
+                    assert d <= 10 * floor, (t, k, d, v)
+
+
+def test_fortran_driver_per_block_openmp(c1, tmp_path):
+    """The driver's per-block mode: w3d_recover_block / w3d_tangent_block
+    called from an OpenMP parallel loop over the blocks (as the reference
+    calls do_nleps_block / do_nlek_block), addifv by w3d_assemble_ifv,
+    against the oracle loop: identical iterations, displacements within
+    1e-8, and the same energies as the whole-pass driver."""
+    m, arr, cst, steps, ref = c1
+    os.environ.setdefault("OMP_NUM_THREADS", "8")
+    got = run_drive(str(tmp_path), arr, cst, steps, _abi.W3D_FP64,
+                    per_block=True)
+    compare(got, ref)
+    whole = run_drive(str(tmp_path), arr, cst, steps, _abi.W3D_FP64)
+    for a, b in zip(got, whole):
+        assert a["iters"] == b["iters"]
+        assert abs(a["energy"] - b["energy"]) <= 1e-12 * abs(b["energy"])
+        assert abs(a["plastic_work"] - b["plastic_work"]) <= \
+            1e-12 * max(abs(b["plastic_work"]), 1e-30)

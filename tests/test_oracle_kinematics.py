@@ -261,9 +261,21 @@ def test_rigid_body_modes(maker, et, order):
 
 
 # ------------------------------------- the reference's own leaf routines
-REF = O.load_ref_leaf()
-needs_ref = pytest.mark.skipif(REF is None, reason="reference leaf routines "
-                               "not built (make -C oracle ref)")
+class _RefLeaf:
+    """The reference's leaf library, loaded on the first use by a test
+    (so that a test process that runs none of these never maps it)."""
+    lib = None
+
+    def __getattr__(self, name):
+        if _RefLeaf.lib is None:
+            _RefLeaf.lib = O.load_ref_leaf()
+        return getattr(_RefLeaf.lib, name)
+
+
+REF = _RefLeaf()
+needs_ref = pytest.mark.skipif(not O.ref_leaf_built(),
+                               reason="reference leaf routines not built "
+                               "(make -C oracle ref)")
 MXVL, MXEDOF = 128, 60
 
 

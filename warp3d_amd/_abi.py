@@ -7,7 +7,7 @@ import ctypes as C
 
 import numpy as np
 
-W3D_ABI_VERSION = 3
+W3D_ABI_VERSION = 4
 
 W3D_OK = 0
 W3D_CUT_STEP = 1
@@ -230,6 +230,17 @@ PROTOS = {
     "rollback": (_I, [_P]),
     "get_block_state": (_I, [_P, _I, _I, _D, C.c_int64]),
     "set_block_state": (_I, [_P, _I, _I, _D, C.c_int64]),
+}
+
+# the library's own forms of the per-block calls (the oracle's
+# orc_tangent_blocks takes no precision) and its per-block interface
+LIB_BLOCK_PROTOS = {
+    "tangent_blocks": (_I, [_P, _I, _I, _I, _I, _I]),
+    "set_displacements": (_I, [_P, _D, _D]),
+    "recover_block": (_I, [_P, _I, _I, _I, _I, _D, _D, _IP]),
+    "assemble_ifv": (_I, [_P, _D, _D, _LP]),
+    "get_ifv_sums": (_I, [_P, _D, _LP]),
+    "tangent_block": (_I, [_P, _I, _I, _I, _I]),
 }
 
 # asymmetric assembly (the library; the oracle's counterparts take the CSR

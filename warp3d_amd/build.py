@@ -96,7 +96,7 @@ def build_fortran(verbose=False):
     os.makedirs(mod, exist_ok=True)
     os.makedirs(os.path.dirname(DRIVE), exist_ok=True)
     tmp = DRIVE + ".tmp"
-    cmd = [fc, "-O2", "-J", mod] + FORTRAN + [
+    cmd = [fc, "-O2", "-fopenmp", "-J", mod] + FORTRAN + [
         "-L", HERE, "-lw3dblk", "-Wl,-rpath,$ORIGIN/..", "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)

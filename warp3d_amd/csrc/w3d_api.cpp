@@ -122,6 +122,40 @@ void build_work(const w3d_ctx* c, bool tangent, std::vector<WorkItem>& items,
   }
 }
 
+// Each block's run of items in the recovery and tangent work lists (a
+// block's items are consecutive inside its kernel group) and in the mm10
+// list, for the per-block calls.
+void build_runs(w3d_ctx* c) {
+  c->runs.assign(c->nelblk, w3d_ctx::BlockRuns());
+  for (int pass = 0; pass < 2; ++pass) {
+    std::vector<WorkItem> items;
+    std::vector<int> g, st, n;
+    build_work(c, pass == 1, items, g, st, n);
+    for (size_t k = 0; k < g.size(); ++k)
+      for (int i = st[k]; i < st[k] + n[k]; ++i) {
+        w3d_ctx::BlockRuns& R = c->runs[items[i].blk];
+        int &key = pass ? R.tan_key : R.rec_key;
+        int &first = pass ? R.tan_first : R.rec_first;
+        int &cnt = pass ? R.tan_n : R.rec_n;
+        if (cnt == 0) {
+          key = g[k];
+          first = i;
+        }
+        cnt += 1;
+      }
+  }
+  int item = 0, idx = 0;
+  for (int b = 0; b < c->nelblk; ++b) {
+    const BlockDesc& B = c->blocks[b];
+    if (B.model != W3D_MM10) continue;
+    const int np = B.span * B.ngp;
+    c->runs[b].mm10_first = item;
+    c->runs[b].mm10_n = (np + 63) / 64;   // items of 64 points (w3d_create)
+    c->runs[b].mm10_idx = idx++;
+    item += c->runs[b].mm10_n;
+  }
+}
+
 void free_ctx(w3d_ctx* c) {
   dfree(c->d_blocks); dfree(c->d_mats); dfree(c->d_tables);
   dfree(c->d_coords); dfree(c->d_conn);
@@ -143,6 +177,10 @@ void free_ctx(w3d_ctx* c) {
   dfree(c->d_crys); dfree(c->d_uddt); dfree(c->d_work_mm10); dfree(c->d_mm10_blks);
   dfree(c->d_rot_n); dfree(c->d_rot_n1); dfree(c->d_u_t); dfree(c->d_du_t);
   dfree(c->d_counter);
+  dfree(c->d_blk_flags); dfree(c->d_blk_counter);
+  for (hipStream_t st : c->pool_all) (void)hipStreamDestroy(st);
+  c->pool_all.clear();
+  c->pool_idle.clear();
   for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
   c->graphs.clear();
   for (int ph = 0; ph < W3D_NUM_PHASES; ++ph) {
@@ -438,7 +476,7 @@ int create_impl(w3d_ctx* c, const w3d_model* m) {
               {&c->d_u, 3 * (int64_t)c->nonode},
               {&c->d_du, 3 * (int64_t)c->nonode},
               {&c->d_ifv, 3 * (int64_t)c->nonode},
-              {&c->d_blk_sums, 2 * (int64_t)c->nelblk}};
+              {&c->d_blk_sums, kBlkSums * (int64_t)c->nelblk}};
   for (auto& a : arrs) {
     if ((rc = dalloc(a.p, a.n))) return rc;
     HIPCHK(hipMemset(*a.p, 0, std::max<int64_t>(a.n, 1) * sizeof(double)));
@@ -473,7 +511,7 @@ int create_impl(w3d_ctx* c, const w3d_model* m) {
     if ((rc = upload(&c->d_work_rec, items))) return rc;
     // chunk energy partials, and each block's run of chunks (a block's
     // chunks are consecutive in the work list, in element order)
-    if ((rc = dalloc(&c->d_en_wg, 2 * items.size()))) return rc;
+    if ((rc = dalloc(&c->d_en_wg, kBlkSums * items.size()))) return rc;
     std::vector<int32_t> wf(c->nelblk, -1), wc(c->nelblk, 0);
     for (size_t k = 0; k < items.size(); ++k) {
       int b = items[k].blk;
@@ -485,6 +523,11 @@ int create_impl(w3d_ctx* c, const w3d_model* m) {
     build_work(c, true, items, c->tan_groups, c->tan_start, c->tan_count);
     if ((rc = upload(&c->d_work_tan, items))) return rc;
   }
+  if ((rc = dalloc(&c->d_blk_flags, 4 * (int64_t)c->nelblk))) return rc;
+  HIPCHK(hipMemset(c->d_blk_flags, 0, 16 * (size_t)c->nelblk));
+  if ((rc = dalloc(&c->d_blk_counter, c->nelblk))) return rc;
+  HIPCHK(hipMemset(c->d_blk_counter, 0, 4 * (size_t)c->nelblk));
+  build_runs(c);
   {
     std::vector<int64_t> off(c->inv_elem.size());
     for (size_t p = 0; p < off.size(); ++p) {
@@ -530,12 +573,68 @@ void prof_collect(w3d_ctx* c) {
   }
 }
 
-int check_flags(w3d_ctx* c, int step) {
-  int32_t f[4];
-  HIPCHK(hipMemcpyAsync(f, c->d_flags, sizeof f, hipMemcpyDeviceToHost,
-                        c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  prof_collect(c);
+// A stream for one per-block call: an idle one of the pool, or a new one
+// (default priority, non-blocking).
+int borrow_stream(w3d_ctx* c, hipStream_t* s) {
+  {
+    std::lock_guard<std::mutex> lk(c->pool_mu);
+    if (!c->pool_idle.empty()) {
+      *s = c->pool_idle.back();
+      c->pool_idle.pop_back();
+      return W3D_OK;
+    }
+  }
+  hipStream_t st;
+  HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  std::lock_guard<std::mutex> lk(c->pool_mu);
+  c->pool_all.push_back(st);
+  *s = st;
+  return W3D_OK;
+}
+
+void return_stream(w3d_ctx* c, hipStream_t s) {
+  std::lock_guard<std::mutex> lk(c->pool_mu);
+  c->pool_idle.push_back(s);
+}
+
+// The recovery of block b (0-based) on stream s, with the displacements of
+// w3d_set_displacements: the block's flags cleared, the first pass, for
+// mm10 the material update over the block's own items (its own counter),
+// grad Fe and the second pass, then the block's sums.
+int enqueue_block_recover(w3d_ctx* c, int b, hipStream_t s, int step,
+                          int iter, int extrapolated) {
+  const w3d_ctx::BlockRuns& R = c->runs[b];
+  const int nonlinear = (iter >= 1 || extrapolated) ? 1 : 0;
+  const int want_energy = iter > 0 ? 1 : 0;
+  DevModel M = c->dev();
+  M.flags = c->d_blk_flags + 4 * b;
+  const std::vector<int> g{R.rec_key}, st{R.rec_first}, n{R.rec_n};
+  int rc = launch_fill_i32(M.flags, 4, 0, s);
+  if (!rc)
+    rc = launch_recover(M, c->blocks, c->d_work_rec, g, st, n, step, iter,
+                        nonlinear, c->d_u, c->d_du, want_energy, c->d_en_wg,
+                        0, s);
+  if (!rc && R.mm10_n > 0) {
+    rc = launch_mm10(M, c->d_work_mm10 + R.mm10_first, R.mm10_n,
+                     c->d_blk_counter + b, std::min(c->mm10_grid, R.mm10_n),
+                     step, mm10_iter0_off(step, iter, extrapolated), s);
+    if (!rc)
+      rc = launch_mm10_grads(M, c->d_mm10_blks + R.mm10_idx, 1, b, b + 1,
+                             step, s);
+    if (!rc)
+      rc = launch_recover(M, c->blocks, c->d_work_rec, g, st, n, step, iter,
+                          nonlinear, c->d_u, c->d_du, want_energy,
+                          c->d_en_wg, 1, s);
+  }
+  if (!rc && want_energy)
+    rc = launch_energy_reduce(c->d_en_wg, c->d_blk_wfirst + b,
+                              c->d_blk_wcount + b, 1,
+                              c->d_blk_sums + kBlkSums * b, s);
+  return rc;
+}
+
+// The status of recovery flags: [0] fatal (mm01), [1] cut, [3] det F <= 0.
+int flags_status(const int32_t* f, int step) {
   if (f[0]) return fail(W3D_ERR_NUMERIC, "mm01: negative plastic multiplier");
   // det F <= 0 (fcomp1): a step reduction when one is possible (rstgp1.f)
   if (f[3] && step == 1)
@@ -544,47 +643,22 @@ int check_flags(w3d_ctx* c, int step) {
   return W3D_OK;
 }
 
-int recover_lists(w3d_ctx* c, int first_blk, int nblk, RecoverLists& L) {
+int check_flags(w3d_ctx* c, int step) {
+  int32_t f[4];
+  HIPCHK(hipMemcpyAsync(f, c->d_flags, sizeof f, hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  prof_collect(c);
+  return flags_status(f, step);
+}
+
+int recover_lists(w3d_ctx* c, RecoverLists& L) {
   L.d_w = c->d_work_rec;
   L.d_wm = c->d_work_mm10;
   L.nwm = c->n_work_mm10;
   L.sg = c->rec_groups;
   L.ss = c->rec_start;
   L.sn = c->rec_count;
-  if (first_blk == 1 && nblk == c->nelblk) return W3D_OK;
-  auto in = [&](const WorkItem& w) {
-    return w.blk >= first_blk - 1 && w.blk < first_blk - 1 + nblk;
-  };
-  std::vector<WorkItem> items;
-  std::vector<int> g, s, n;
-  build_work(c, false, items, g, s, n);
-  std::vector<WorkItem> sel;
-  L.sg.clear();
-  L.ss.clear();
-  L.sn.clear();
-  for (size_t k = 0; k < g.size(); ++k) {
-    int st = (int)sel.size();
-    for (int i = s[k]; i < s[k] + n[k]; ++i)
-      if (in(items[i])) sel.push_back(items[i]);
-    if ((int)sel.size() > st) {
-      L.sg.push_back(g[k]);
-      L.ss.push_back(st);
-      L.sn.push_back((int)sel.size() - st);
-    }
-  }
-  int rc;
-  if ((rc = upload(&L.d_sel, sel))) return rc;
-  L.d_w = L.d_sel;
-  if (c->has_mm10) {
-    std::vector<WorkItem> all(c->n_work_mm10), selm;
-    HIPCHK(hipMemcpy(all.data(), c->d_work_mm10, all.size() * sizeof(WorkItem),
-                     hipMemcpyDeviceToHost));
-    for (const WorkItem& w : all)
-      if (in(w)) selm.push_back(w);
-    if ((rc = upload(&L.d_selm, selm))) return rc;
-    L.d_wm = L.d_selm;
-    L.nwm = (int)selm.size();
-  }
   return W3D_OK;
 }
 
@@ -703,17 +777,20 @@ int enqueue_csr_values(w3d_ctx* c, const double* d_diag,
 // order), then the flags.
 int finish_recover(w3d_ctx* c, int step, int iter, double* energy,
                    double* plastic_work) {
-  double en = 0.0, pw = 0.0;
+  double en = 0.0, pw = 0.0, fa = 0.0;
   if ((energy || plastic_work) && iter > 0) {
-    std::vector<double> sums(2 * c->nelblk);
+    std::vector<double> sums(kBlkSums * (size_t)c->nelblk);
     HIPCHK(hipMemcpyAsync(sums.data(), c->d_blk_sums,
                           sums.size() * sizeof(double),
                           hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     for (int b = 0; b < c->nelblk; ++b) {
-      en = en + sums[2 * b];
-      pw = pw + sums[2 * b + 1];
+      en = en + sums[kBlkSums * b];
+      pw = pw + sums[kBlkSums * b + 1];
+      fa = fa + sums[kBlkSums * b + 2];
     }
+    c->sum_ifv = fa;
+    c->num_term_ifv = c->nifv;
   }
   if (energy) *energy = en;
   if (plastic_work) *plastic_work = pw;
@@ -726,15 +803,10 @@ int recover_impl(w3d_ctx* c, int step, int iter, int extrapolated,
                  int nblk) {
   HIPCHK(hipSetDevice(c->device));
   RecoverLists L;
-  int rc = recover_lists(c, first_blk, nblk, L);
+  int rc = recover_lists(c, L);
   if (!rc)
     rc = enqueue_recover(c, L, step, iter, extrapolated, d_u, d_du, d_ifv,
                          energy || plastic_work, first_blk, nblk);
-  if (L.d_sel || L.d_selm) {
-    hipStreamSynchronize(c->stream);
-    dfree(L.d_sel);
-    dfree(L.d_selm);
-  }
   if (rc) return rc;
   return finish_recover(c, step, iter, energy, plastic_work);
 }
@@ -915,36 +987,130 @@ int w3d_recover(w3d_ctx* c, int32_t step, int32_t iter, int32_t extrapolated,
   return rc;
 }
 
+int w3d_set_displacements(w3d_ctx* c, const double* u, const double* du) {
+  if (!c || !du)
+    return fail(W3D_ERR_ARG, "w3d_set_displacements: null argument");
+  const size_t nb = 3 * (size_t)c->nonode * sizeof(double);
+  HIPCHK(hipSetDevice(c->device));
+  // earlier work of the context (a commit or rollback) ends first: the
+  // per-block calls run on other streams
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (u)
+    HIPCHK(hipMemcpy(c->d_u, u, nb, hipMemcpyHostToDevice));
+  else
+    HIPCHK(hipMemset(c->d_u, 0, nb));
+  HIPCHK(hipMemcpy(c->d_du, du, nb, hipMemcpyHostToDevice));
+  if (c->has_geonl) {  // the tangent's configuration X + u + du
+    HIPCHK(hipMemcpy(c->d_u_t, c->d_u, nb, hipMemcpyDeviceToDevice));
+    HIPCHK(hipMemcpy(c->d_du_t, c->d_du, nb, hipMemcpyDeviceToDevice));
+  }
+  c->disp_set = true;
+  return W3D_OK;
+}
+
+int w3d_recover_block(w3d_ctx* c, int32_t blk, int32_t step, int32_t iter,
+                      int32_t extrapolated, double* block_energy,
+                      double* block_plastic_work, int32_t* cut) {
+  if (!c) return fail(W3D_ERR_ARG, "w3d_recover_block: null context");
+  if (blk < 1 || blk > c->nelblk)
+    return fail(W3D_ERR_ARG, "w3d_recover_block: block number");
+  if (step < 1 || iter < 0)
+    return fail(W3D_ERR_ARG, "w3d_recover_block: step/iter");
+  if (!c->disp_set)
+    return fail(W3D_ERR_ARG,
+                "w3d_recover_block: no w3d_set_displacements before");
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s;
+  int rc = borrow_stream(c, &s);
+  if (rc) return rc;
+  rc = enqueue_block_recover(c, blk - 1, s, step, iter, extrapolated);
+  int32_t f[4] = {0, 0, 0, 0};
+  double sums[kBlkSums] = {0.0, 0.0, 0.0};
+  if (!rc &&
+      (hipMemcpyAsync(f, c->d_blk_flags + 4 * (blk - 1), sizeof f,
+                      hipMemcpyDeviceToHost, s) != hipSuccess ||
+       (iter > 0 &&
+        hipMemcpyAsync(sums, c->d_blk_sums + kBlkSums * (blk - 1),
+                       sizeof sums, hipMemcpyDeviceToHost, s) != hipSuccess)))
+    rc = W3D_ERR_DEVICE;
+  if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = W3D_ERR_DEVICE;
+  return_stream(c, s);
+  if (rc) return fail(rc, "w3d_recover_block: kernel launch failed");
+  if (block_energy) *block_energy = sums[0];
+  if (block_plastic_work) *block_plastic_work = sums[1];
+  rc = flags_status(f, step);
+  if (cut) *cut = rc == W3D_CUT_STEP ? 1 : 0;
+  return rc;
+}
+
 int w3d_recover_blocks(w3d_ctx* c, int32_t first_blk, int32_t nblk,
                        int32_t step, int32_t iter, int32_t extrapolated,
                        const double* u, const double* du) {
-  if (!c || !du) return fail(W3D_ERR_ARG, "w3d_recover_blocks: null argument");
+  if (!c) return fail(W3D_ERR_ARG, "w3d_recover_blocks: null context");
   if (first_blk < 1 || nblk < 1 || first_blk - 1 + nblk > c->nelblk)
     return fail(W3D_ERR_ARG, "w3d_recover_blocks: block range");
-  size_t nb = 3 * (size_t)c->nonode * sizeof(double);
+  if (step < 1 || iter < 0)
+    return fail(W3D_ERR_ARG, "w3d_recover_blocks: step/iter");
+  int rc;
+  if (du && (rc = w3d_set_displacements(c, u, du))) return rc;
+  if (!c->disp_set)
+    return fail(W3D_ERR_ARG, "w3d_recover_blocks: no displacements");
   HIPCHK(hipSetDevice(c->device));
-  if (u)
-    HIPCHK(hipMemcpyAsync(c->d_u, u, nb, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(c->d_du, du, nb, hipMemcpyHostToDevice, c->stream));
-  return recover_impl(c, step, iter, extrapolated, u ? c->d_u : nullptr,
-                      c->d_du, nullptr, nullptr, nullptr, first_blk, nblk);
+  hipStream_t s;
+  if ((rc = borrow_stream(c, &s))) return rc;
+  for (int b = first_blk - 1; b < first_blk - 1 + nblk && !rc; ++b)
+    rc = enqueue_block_recover(c, b, s, step, iter, extrapolated);
+  std::vector<int32_t> f(4 * (size_t)nblk, 0);
+  if (!rc && hipMemcpyAsync(f.data(), c->d_blk_flags + 4 * (first_blk - 1),
+                            f.size() * 4, hipMemcpyDeviceToHost,
+                            s) != hipSuccess)
+    rc = W3D_ERR_DEVICE;
+  if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = W3D_ERR_DEVICE;
+  return_stream(c, s);
+  if (rc) return fail(rc, "w3d_recover_blocks: kernel launch failed");
+  int32_t any[4] = {0, 0, 0, 0};
+  for (int b = 0; b < nblk; ++b)
+    for (int k = 0; k < 4; ++k) any[k] |= f[4 * b + k];
+  return flags_status(any, step);
+}
+
+int w3d_assemble_ifv(w3d_ctx* c, double* ifv, double* sum_ifv,
+                     int64_t* num_term_ifv) {
+  if (!c) return fail(W3D_ERR_ARG, "w3d_assemble_ifv: null context");
+  HIPCHK(hipSetDevice(c->device));
+  int rc = launch_ifv_gather(c->d_einfvec, c->d_inv_ptr, c->d_inv_off,
+                             c->nonode, c->d_ifv, c->stream);
+  if (rc) return fail(rc, "w3d_assemble_ifv: launch failed");
+  if (ifv)
+    HIPCHK(hipMemcpyAsync(ifv, c->d_ifv, 3 * (size_t)c->nonode * 8,
+                          hipMemcpyDeviceToHost, c->stream));
+  std::vector<double> sums;
+  if (sum_ifv) {  // the blocks' sums of the last per-block recoveries
+    sums.resize(kBlkSums * (size_t)c->nelblk);
+    HIPCHK(hipMemcpyAsync(sums.data(), c->d_blk_sums, sums.size() * 8,
+                          hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (sum_ifv) {
+    double fa = 0.0;
+    for (int b = 0; b < c->nelblk; ++b) fa = fa + sums[kBlkSums * b + 2];
+    *sum_ifv = fa;
+  }
+  if (num_term_ifv) *num_term_ifv = c->nifv;
+  return W3D_OK;
+}
+
+int w3d_get_ifv_sums(const w3d_ctx* c, double* sum_ifv,
+                     int64_t* num_term_ifv) {
+  if (!c) return fail(W3D_ERR_ARG, "w3d_get_ifv_sums: null context");
+  if (sum_ifv) *sum_ifv = c->sum_ifv;
+  if (num_term_ifv) *num_term_ifv = c->num_term_ifv;
+  return W3D_OK;
 }
 
 // bdbt_asym for every element (asymmetric assembly), FP64
 static int tangent_full(w3d_ctx* c) {
   int rc;
-  if (!c->d_ekf) {
-    std::vector<int64_t> off(c->nelblk + 1, 0);
-    for (int b = 0; b < c->nelblk; ++b) {
-      const BlockDesc& B = c->blocks[b];
-      off[b + 1] = off[b] + (int64_t)B.span * B.totdof * B.totdof;
-    }
-    c->n_ekf = off[c->nelblk];
-    const int64_t bytes0 = t_dalloc_bytes;
-    if ((rc = upload(&c->d_ekf_off, off))) return rc;
-    if ((rc = dalloc(&c->d_ekf, c->n_ekf))) return rc;
-    c->dev_bytes += t_dalloc_bytes - bytes0;
-  }
   prof_begin(c, W3D_PHASE_TANGENT);
   rc = launch_tangent_full(c->dev(), c->d_work_tan, c->tan_groups,
                            c->tan_start, c->tan_count, c->d_ekf_off, c->d_ekf,
@@ -959,54 +1125,84 @@ static int tangent_full(w3d_ctx* c) {
   return W3D_OK;
 }
 
+// The tangents of blocks [first, first + nblk) (0-based) on stream s.
+static int enqueue_block_tangents(w3d_ctx* c, int first, int nblk,
+                                  int precision, hipStream_t s) {
+  DevModel M = c->dev();
+  int rc = W3D_OK;
+  for (int b = first; b < first + nblk && !rc; ++b) {
+    const w3d_ctx::BlockRuns& R = c->runs[b];
+    const std::vector<int> g{R.tan_key}, st{R.tan_first}, n{R.tan_n};
+    if (c->asym)
+      rc = launch_tangent_full(M, c->d_work_tan, g, st, n, c->d_ekf_off,
+                               c->d_ekf, s);
+    else
+      rc = launch_tangent_mfma(M, c->d_work_tan, g, st, n, precision, s);
+  }
+  return rc;
+}
+
+static int tangent_args(w3d_ctx* c, int precision, bool whole) {
+  if (precision != W3D_FP64 && precision != W3D_FP32)
+    return fail(W3D_ERR_ARG, "w3d_tangent: precision");
+  if (c->asym && precision != W3D_FP64)
+    return fail(W3D_ERR_UNSUPPORTED,
+                "w3d_tangent: asymmetric assembly takes the FP64 tangent");
+  (void)whole;
+  return W3D_OK;
+}
+
+int w3d_tangent_block(w3d_ctx* c, int32_t blk, int32_t step, int32_t iter,
+                      int32_t precision) {
+  (void)step;
+  (void)iter;
+  if (!c) return fail(W3D_ERR_ARG, "w3d_tangent_block: null context");
+  if (blk < 1 || blk > c->nelblk)
+    return fail(W3D_ERR_ARG, "w3d_tangent_block: block number");
+  int rc = tangent_args(c, precision, false);
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s;
+  if ((rc = borrow_stream(c, &s))) return rc;
+  {
+    // the context records which Ke array the blocks' tangents are in
+    std::lock_guard<std::mutex> lk(c->pool_mu);
+    c->tangent_precision = precision;
+    if (c->asym) {
+      c->asym_tangent = true;
+      c->csr_asym_ready = false;
+    }
+  }
+  rc = enqueue_block_tangents(c, blk - 1, 1, precision, s);
+  if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = W3D_ERR_DEVICE;
+  return_stream(c, s);
+  if (rc) return fail(rc, "w3d_tangent_block: kernel launch failed");
+  return W3D_OK;
+}
+
 int w3d_tangent_blocks(w3d_ctx* c, int32_t first_blk, int32_t nblk,
                        int32_t step, int32_t iter, int32_t precision) {
   (void)step;
   (void)iter;
   if (!c) return fail(W3D_ERR_ARG, "w3d_tangent: null context");
-  if (precision != W3D_FP64 && precision != W3D_FP32)
-    return fail(W3D_ERR_ARG, "w3d_tangent: precision");
   if (first_blk < 1 || nblk < 1 || first_blk - 1 + nblk > c->nelblk)
     return fail(W3D_ERR_ARG, "w3d_tangent_blocks: block range");
+  int rc = tangent_args(c, precision, first_blk == 1 && nblk == c->nelblk);
+  if (rc) return rc;
   HIPCHK(hipSetDevice(c->device));
-  if (c->asym) {
-    if (precision != W3D_FP64)
-      return fail(W3D_ERR_UNSUPPORTED,
-                  "w3d_tangent: asymmetric assembly takes the FP64 tangent");
-    if (first_blk != 1 || nblk != c->nelblk)
-      return fail(W3D_ERR_UNSUPPORTED,
-                  "w3d_tangent_blocks: asymmetric assembly forms every block");
-    return tangent_full(c);
-  }
-  DevModel M = c->dev();
-  int rc;
-  if (first_blk == 1 && nblk == c->nelblk) {
+  if (first_blk == 1 && nblk == c->nelblk) {  // one launch per group
+    if (c->asym) return tangent_full(c);
     prof_begin(c, W3D_PHASE_TANGENT);
-    rc = launch_tangent_mfma(M, c->d_work_tan, c->tan_groups, c->tan_start,
-                             c->tan_count, precision, c->stream);
+    rc = launch_tangent_mfma(c->dev(), c->d_work_tan, c->tan_groups,
+                             c->tan_start, c->tan_count, precision,
+                             c->stream);
     prof_end(c, W3D_PHASE_TANGENT);
   } else {
-    std::vector<WorkItem> items;
-    std::vector<int> g, s, n;
-    build_work(c, true, items, g, s, n);
-    std::vector<WorkItem> sel;
-    std::vector<int> sg, ss, sn;
-    for (size_t k = 0; k < g.size(); ++k) {
-      int st = (int)sel.size();
-      for (int i = s[k]; i < s[k] + n[k]; ++i)
-        if (items[i].blk >= first_blk - 1 && items[i].blk < first_blk - 1 + nblk)
-          sel.push_back(items[i]);
-      if ((int)sel.size() > st) {
-        sg.push_back(g[k]);
-        ss.push_back(st);
-        sn.push_back((int)sel.size() - st);
-      }
+    if (c->asym) {
+      c->asym_tangent = true;
+      c->csr_asym_ready = false;
     }
-    WorkItem* d_sel = nullptr;
-    if ((rc = upload(&d_sel, sel))) return rc;
-    rc = launch_tangent_mfma(M, d_sel, sg, ss, sn, precision, c->stream);
-    hipStreamSynchronize(c->stream);
-    dfree(d_sel);
+    rc = enqueue_block_tangents(c, first_blk - 1, nblk, precision, c->stream);
   }
   if (rc) return fail(rc, "w3d_tangent: kernel launch failed");
   c->tangent_precision = precision;
@@ -1355,6 +1551,20 @@ int w3d_set_asymmetric(w3d_ctx* c, int32_t on) {
       if (B.geonl)
         return fail(W3D_ERR_UNSUPPORTED,
                     "w3d_set_asymmetric: finite-strain blocks");
+  if (on && !c->d_ekf) {  // the full Ke of every element
+    std::vector<int64_t> off(c->nelblk + 1, 0);
+    for (int b = 0; b < c->nelblk; ++b) {
+      const BlockDesc& B = c->blocks[b];
+      off[b + 1] = off[b] + (int64_t)B.span * B.totdof * B.totdof;
+    }
+    c->n_ekf = off[c->nelblk];
+    const int64_t bytes0 = t_dalloc_bytes;
+    int rc;
+    HIPCHK(hipSetDevice(c->device));
+    if ((rc = upload(&c->d_ekf_off, off))) return rc;
+    if ((rc = dalloc(&c->d_ekf, c->n_ekf))) return rc;
+    c->dev_bytes += t_dalloc_bytes - bytes0;
+  }
   c->asym = on != 0;
   c->asym_tangent = false;
   c->csr_asym_ready = false;
@@ -1847,7 +2057,7 @@ int w3d_iteration_dev(w3d_ctx* c, int32_t step, int32_t iter,
       c->graphs.erase(c->graphs.begin());
     }
     RecoverLists L;
-    int rc = recover_lists(c, 1, c->nelblk, L);  // the resident lists
+    int rc = recover_lists(c, L);  // the resident lists
     if (rc) return rc;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     c->capturing = true;
@@ -1985,6 +2195,7 @@ int w3d_autotune(w3d_ctx* c, int32_t precision, int32_t* choices,
   build_work(c, true, items, c->tan_groups, c->tan_start, c->tan_count);
   dfree(c->d_work_tan);
   if ((rc = upload(&c->d_work_tan, items))) return rc;
+  build_runs(c);
   for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
   c->graphs.clear();
   if (choices)

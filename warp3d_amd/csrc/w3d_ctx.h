@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -135,7 +136,27 @@ struct w3d_ctx {
   std::vector<int> tan_groups, tan_start, tan_count;
   int64_t *d_inv_ptr = nullptr, *d_inv_off = nullptr;
   int32_t *d_blk_wfirst = nullptr, *d_blk_wcount = nullptr;
+  // per block: internal energy, plastic work and the sum of |element
+  // force| of the last recovery (kBlkSums words)
   double* d_blk_sums = nullptr;
+  // the last whole-pass recovery's addifv sums (sum_ifv, num_term_ifv)
+  double sum_ifv = 0.0;
+  int64_t num_term_ifv = 0;
+  // per-block calls (w3d_recover_block, w3d_tangent_block), safe to run
+  // concurrently for distinct blocks: each block's run of work items in
+  // the recovery, tangent and mm10 lists, its own flags and mm10 counter,
+  // and a pool of streams the calls borrow
+  struct BlockRuns {
+    int rec_key = 0, rec_first = 0, rec_n = 0;
+    int tan_key = 0, tan_first = 0, tan_n = 0;
+    int mm10_first = 0, mm10_n = 0, mm10_idx = -1;
+  };
+  std::vector<BlockRuns> runs;
+  int32_t* d_blk_flags = nullptr;    // 4 words per block
+  int32_t* d_blk_counter = nullptr;  // one word per block
+  std::mutex pool_mu;
+  std::vector<hipStream_t> pool_idle, pool_all;
+  bool disp_set = false;             // w3d_set_displacements was called
   double *d_u = nullptr, *d_du = nullptr, *d_ifv = nullptr;
   // sparsity + assembly map
   int neqns = 0;
@@ -211,17 +232,24 @@ struct w3d_ctx {
 
 namespace w3d {
 
-// Work lists of one recovery: every block, or the blocks
-// [first_blk, first_blk + nblk) (device copies owned by the caller).
+// Work lists of one whole-pass recovery (the context's resident lists).
 struct RecoverLists {
   const WorkItem* d_w = nullptr;
   const WorkItem* d_wm = nullptr;
   int nwm = 0;
   std::vector<int> sg, ss, sn;
-  WorkItem *d_sel = nullptr, *d_selm = nullptr;
 };
 
-int recover_lists(w3d_ctx* c, int first_blk, int nblk, RecoverLists& L);
+int recover_lists(w3d_ctx* c, RecoverLists& L);
+// Per-block calls: a stream of the context's pool, its return, the
+// recovery of one block (0-based) enqueued on a stream, and the status of
+// a block's four recovery flags.
+int borrow_stream(w3d_ctx* c, hipStream_t* s);
+void return_stream(w3d_ctx* c, hipStream_t s);
+int enqueue_block_recover(w3d_ctx* c, int b, hipStream_t s, int step,
+                          int iter, int extrapolated);
+int flags_status(const int32_t* f, int step);
+void build_runs(w3d_ctx* c);
 // The stages of one recovery on the context's stream (no host
 // synchronisation, no memsets): stage 0 clears the flags and runs the
 // first recovery pass, stage 1 the mm10 update, stage 2 its grad Fe words,

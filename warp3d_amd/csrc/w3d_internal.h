@@ -30,6 +30,9 @@ constexpr int kMM01Hist = 11;
 // mm10 history of one point: 87 common words + 71 per crystal (mm10_d.f)
 constexpr int kMM10Common = 87;
 constexpr int kMM10Crystal = 71;
+// words per recovery chunk and per block of the energy sums: internal
+// energy, plastic work, sum of |element force| (addifv's sum_ifv)
+constexpr int kBlkSums = 3;
 
 // One element block as the kernels see it.
 struct BlockDesc {

@@ -270,6 +270,7 @@ __global__ void __launch_bounds__(256)
   __shared__ double s_buf[NA > NB ? NA : NB];
   __shared__ double s_gam[E][G][10];
   __shared__ double s_vol[BBAR ? E : 1][25];
+  __shared__ double s_red[4];
   double(*s_ce)[T] = reinterpret_cast<double(*)[T]>(s_buf);
   double(*s_du)[T] = reinterpret_cast<double(*)[T]>(s_buf + E * T);
   double(*s_u)[T] = reinterpret_cast<double(*)[T]>(s_buf + 2 * E * T);
@@ -564,8 +565,8 @@ __global__ void __launch_bounds__(256)
       b = b + __shfl_xor(b, off);
     }
     if (tid == 0) {
-      en_wg[2 * (slot0 + blockIdx.x)] = a;
-      en_wg[2 * (slot0 + blockIdx.x) + 1] = b;
+      en_wg[kBlkSums * (slot0 + blockIdx.x)] = a;
+      en_wg[kBlkSums * (slot0 + blockIdx.x) + 1] = b;
     }
   }
 
@@ -573,6 +574,7 @@ __global__ void __launch_bounds__(256)
   // stored node-major inside the element: word 3 j + c is the force of
   // component c at local node j (the reference's dof c*NN + j)
   const double scalar = bd.etype == W3D_TET10 ? 1.0 / 6.0 : 1.0;
+  double fabs_sum = 0.0;
   for (int k = tid; k < cnt * T; k += 256) {
     int ee = k / T, r = k % T;
     int j = r / 3, c = r % 3;
@@ -615,6 +617,17 @@ __global__ void __launch_bounds__(256)
       }
     }
     M.einfvec[bd.ifv_off + (int64_t)(wi.first + ee) * T + r] = f;
+    fabs_sum = fabs_sum + fabs(f);
+  }
+  if (want_energy) {  // uniform: addifv's sum of |element force| terms
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1)
+      fabs_sum = fabs_sum + __shfl_xor(fabs_sum, off);
+    if ((tid & 63) == 0) s_red[tid >> 6] = fabs_sum;
+    __syncthreads();
+    if (tid == 0)
+      en_wg[kBlkSums * (slot0 + blockIdx.x) + 2] =
+          ((s_red[0] + s_red[1]) + s_red[2]) + s_red[3];
   }
   for (int k = tid; k < cnt; k += 256) {
     double v = 0.0;
@@ -644,21 +657,23 @@ __global__ void k_ifv_gather(const double* __restrict__ einfvec,
   ifv[3 * n + 2] = fz;
 }
 
-// Per-block internal energy and plastic work: the block's chunk sums in
-// element order (one thread per block).
+// Per-block internal energy, plastic work and sum of |element force|: the
+// block's chunk sums in element order (one thread per block).
 __global__ void k_energy_reduce(const double* __restrict__ en_wg,
                                 const int32_t* __restrict__ wfirst,
                                 const int32_t* __restrict__ wcount, int nblk,
                                 double* __restrict__ sums) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nblk) return;
-  double en = 0.0, pw = 0.0;
+  double en = 0.0, pw = 0.0, fa = 0.0;
   for (int w = wfirst[b]; w < wfirst[b] + wcount[b]; ++w) {
-    en = en + en_wg[2 * w];
-    pw = pw + en_wg[2 * w + 1];
+    en = en + en_wg[kBlkSums * w];
+    pw = pw + en_wg[kBlkSums * w + 1];
+    fa = fa + en_wg[kBlkSums * w + 2];
   }
-  sums[2 * b] = en;
-  sums[2 * b + 1] = pw;
+  sums[kBlkSums * b] = en;
+  sums[kBlkSums * b + 1] = pw;
+  sums[kBlkSums * b + 2] = fa;
 }
 
 // Row-by-row assembly (assem_a_row, assemble_code.f:785) over the segment

@@ -469,7 +469,7 @@ int w3d_batch_advance(w3d_batch* b, w3d_problem* const* ps, int32_t n,
   b->mark(0, 0);
   std::vector<RecoverLists> lists(nr);
   for (int k = 0; k < nr && !rc; ++k)
-    rc = recover_lists(run[k]->c, 1, run[k]->c->nelblk, lists[k]);
+    rc = recover_lists(run[k]->c, lists[k]);
   for (int k = 0; k < nr && !rc; ++k)
     rc = enqueue_recover_stage(run[k]->c, lists[k], 0, run[k]->step,
                                run[k]->iter, run[k]->extrap, run[k]->d_u,
@@ -681,12 +681,6 @@ int w3d_batch_advance(w3d_batch* b, w3d_problem* const* ps, int32_t n,
     if (b->ev_used[ph] &&
         hipEventElapsedTime(&t, b->ev[ph][0], b->ev[ph][1]) == hipSuccess)
       b->phase_ms[ph] += t;
-  }
-  for (int k = 0; k < nr; ++k) {
-    if (lists[k].d_sel || lists[k].d_selm) {
-      dfree(lists[k].d_sel);
-      dfree(lists[k].d_selm);
-    }
   }
   if (status)
     for (int k = 0; k < n; ++k) status[k] = ps[k]->status;

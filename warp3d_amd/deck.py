@@ -23,14 +23,20 @@ def write_deck(path, arrays, cstmap, du_steps, precision=0, res_tol=0.01,
                dt=1.0, extrapolate=True, line_search=False, adaptive=False,
                divergence_check=True, ls_rho=0.7, ls_slack=0.5, ls_min=0.01,
                ls_max=1.0, dloads=None, mfs=None, mf_changes=None,
-               asymmetric=False):
+               asymmetric=False, tests=None, per_block=False):
     """arrays: warp3d_amd._abi.ModelArrays (single-type connectivity);
     du_steps: (nsteps, 3*nonode) imposed increments (zero off the
     constrained dofs); dloads: external load increments per step (default
     none); mfs: load multipliers of the steps (default 1: proportional
-    steps); mf_changes: steps whose load pattern changes.  The Newton
-    options default to the reference's (mnralg: extrapolation on, line
-    search and adaptive off, divergence check on)."""
+    steps); mf_changes: steps whose load pattern changes; tests: the
+    cvtest tests as {number: tolerance} (default {2: res_tol});
+    per_block: recoveries and tangents through the per-block calls in an
+    OpenMP loop over the blocks.  The Newton options default to the
+    reference's (mnralg: extrapolation on, line search and adaptive off,
+    divergence check on)."""
+    tests = {2: res_tol} if tests is None else dict(tests)
+    convrg = [int(k in tests) for k in range(1, 7)]
+    tols = [float(tests.get(k, 0.0)) for k in range(1, 7)]
     nnpe = int(np.diff(arrays.incmap).max()) if arrays.noelem > 1 else \
         arrays.incid.size
     if not np.array_equal(arrays.incmap,
@@ -49,9 +55,9 @@ def write_deck(path, arrays, cstmap, du_steps, precision=0, res_tol=0.01,
                   len(arrays.materials), nsteps, precision, mniter, mxiter,
                   arrays.auto_size, pcg_maxit, int(extrapolate),
                   int(line_search), int(adaptive), int(divergence_check),
-                  int(asymmetric)],
+                  int(asymmetric)] + convrg + [int(per_block)],
                  np.int32).tofile(f)
-        np.array([res_tol, pcg_rtol, dt, ls_rho, ls_slack, ls_min, ls_max],
+        np.array(tols + [pcg_rtol, dt, ls_rho, ls_slack, ls_min, ls_max],
                  np.float64).tofile(f)
         arrays.coords.astype(np.float64).tofile(f)
         for a in (arrays.incid, arrays.etype, arrays.int_order, arrays.matnum,
@@ -80,8 +86,8 @@ def write_deck(path, arrays, cstmap, du_steps, precision=0, res_tol=0.01,
 
 def read_result(path):
     """List of per-step dicts (status, converged, iters (all attempts),
-    last_iters, cuts, pcg_iters and ratio of the last sub-increment's
-    iterations, energy, plastic_work, u)."""
+    last_iters, cuts, pcg_iters, ratio and the six cvtest values (tests)
+    of the last sub-increment's iterations, energy, plastic_work, u)."""
     steps = []
     with open(path, "rb") as f:
         nsteps, nodof = np.fromfile(f, np.int32, 2)
@@ -92,11 +98,13 @@ def read_result(path):
             status, conv, iters, last, cuts = (int(x) for x in head)
             pcg = np.fromfile(f, np.int32, last)
             ratio = np.fromfile(f, np.float64, last)
+            tests = np.fromfile(f, np.float64, 6 * last).reshape(last, 6)
             en, pw = np.fromfile(f, np.float64, 2)
             u = np.fromfile(f, np.float64, nodof)
             steps.append(dict(status=status, converged=bool(conv),
                               iters=iters, last_iters=last, cuts=cuts,
-                              pcg_iters=pcg, ratio=ratio, energy=en,
+                              pcg_iters=pcg, ratio=ratio, tests=tests,
+                              energy=en,
                               plastic_work=pw, u=u))
     return steps
 

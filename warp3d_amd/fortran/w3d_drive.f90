@@ -5,9 +5,12 @@
 !
 ! Deck (stream, native byte order): int32 nonode, noelem, nnpe, nmat,
 ! nsteps, precision, mniter, mxiter, auto_size, pcg_maxit, extrapolate,
-! line_search, adaptive, divergence_check, asymmetric; double res_tol,
-! pcg_rtol (of the PCG, or of the BiCGStab when asymmetric), dt,
-! ls_rho, ls_slack, ls_min, ls_max; double coords(3*nonode); int32
+! line_search, adaptive, divergence_check, asymmetric, convrg(6) (the
+! cvtest tests selected), per_block (recoveries and tangents through the
+! per-block calls in an OpenMP loop); double tol(6) (the tests'
+! tolerances), pcg_rtol (of the PCG, or of the BiCGStab when
+! asymmetric), dt, ls_rho, ls_slack, ls_min, ls_max; double
+! coords(3*nonode); int32
 ! incid(nnpe*noelem), etype(noelem), int_order(noelem), matnum(noelem),
 ! bbar(noelem); per material: int32 model, linear_elastic, slip_type,
 ! elastic_type, miter, alter_mode, angle_type, ncrystals; real32 e, nu,
@@ -20,7 +23,8 @@
 !
 ! Result (stream): int32 nsteps, nodof; per step int32 status, converged,
 ! iters (all attempts), last_iters, cuts, pcg_iters(last_iters); double
-! ratio(last_iters), energy, plastic_work, u(nodof) after the step.
+! ratio(last_iters), the six cvtest values of each iteration
+! (6, last_iters), energy, plastic_work, u(nodof) after the step.
 program w3d_drive
   use, intrinsic :: iso_c_binding
   use w3d
@@ -33,7 +37,7 @@ program w3d_drive
   integer(c_int64_t) :: ncoeff
   real(c_float) :: fval(5)
   real(c_double) :: dval(14), dt, ls(4), mfv(2), mf_prev, sload
-  integer(c_int32_t) :: flags(5)
+  integer(c_int32_t) :: flags(5), tests(6), per_block
   real(c_double), allocatable, target :: coords(:), angles(:), u(:), dui(:), &
        du(:), load_n(:), dload(:)
   integer(c_int32_t), allocatable, target :: incid(:), incmap(:), etype(:), &
@@ -55,13 +59,16 @@ program w3d_drive
        status='old', iostat=ios)
   if (ios /= 0) stop 'w3d_drive: cannot open the deck'
   read(fu) nonode, noelem, nnpe, nmat, nsteps, opts%precision, &
-       opts%mniter, opts%mxiter, model%auto_size, opts%pcg_maxit, flags
+       opts%mniter, opts%mxiter, model%auto_size, opts%pcg_maxit, flags, &
+       tests, per_block
+  opts%convrg = tests /= 0
+  opts%per_block = per_block /= 0
   opts%extrapolate = flags(1) /= 0
   opts%line_search = flags(2) /= 0
   opts%adaptive = flags(3) /= 0
   opts%divergence_check = flags(4) /= 0
   opts%asymmetric = flags(5) /= 0
-  read(fu) opts%res_tol, opts%pcg_rtol, dt, ls
+  read(fu) opts%tol, opts%pcg_rtol, dt, ls
   opts%ls_rho = ls(1)
   opts%ls_slack = ls(2)
   opts%ls_min = ls(3)
@@ -161,6 +168,7 @@ program w3d_drive
          int(rep%last_iters, c_int32_t), int(rep%cuts, c_int32_t)
     write(fo) (int(rep%pcg_iters(k), c_int32_t), k = 1, rep%last_iters)
     write(fo) (rep%ratio(k), k = 1, rep%last_iters)
+    write(fo) (rep%ratios(:, k), k = 1, rep%last_iters)
     write(fo) rep%energy, rep%plastic_work
     write(fo) u
     write(*, '(a,i4,a,i3,a,i2,a,l2,a,es10.3)') ' step', step, &

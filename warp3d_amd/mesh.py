@@ -218,26 +218,49 @@ def uniaxial_bc(mesh, strain, size=1.0):
 
 
 class MixedMesh:
-    """Meshes of different element types side by side along x with disjoint
-    node sets, elements numbered part after part (blocks are contiguous
-    element ranges of one type).  parts: (mesh, int_order, bbar)."""
+    """Meshes of different element types side by side along x, elements
+    numbered part after part (blocks are contiguous element ranges of one
+    type).  parts: (mesh, int_order, bbar).  With glue, the parts touch
+    (no gap) and the nodes of a part that coincide with nodes of earlier
+    parts become those nodes, so elements of different types share nodes
+    (the mesh need not be conforming); otherwise the node sets are
+    disjoint."""
 
-    def __init__(self, parts, gap=0.1):
+    def __init__(self, parts, gap=0.1, glue=False):
         coords, incid, incmap, etype, order, bbar = [], [], [], [], [], []
         noff, x0, pos = 0, 0.0, 1
+        known = {}
         for mesh, int_order, bb in parts:
             c = mesh.coords.copy()
             c[:, 0] += x0 - c[:, 0].min()
-            x0 = c[:, 0].max() + gap
-            coords.append(c)
+            x0 = c[:, 0].max() + (0.0 if glue else gap)
             ne, nn = mesh.conn.shape
-            incid.append((mesh.conn + noff).ravel())
+            ids = np.arange(1, mesh.nonode + 1, dtype=np.int64)
+            if glue:
+                # lattice-rounded positions identify coincident nodes
+                key = np.round(c / 1e-9).astype(np.int64)
+                new = []
+                for k in range(mesh.nonode):
+                    t = tuple(key[k])
+                    if t in known:
+                        ids[k] = known[t]
+                    else:
+                        new.append(k)
+                        ids[k] = noff + len(new)
+                        known[t] = ids[k]
+                c = c[new]
+                ids_conn = ids[mesh.conn - 1]
+                noff += len(new)
+            else:
+                ids_conn = mesh.conn + noff
+                noff += mesh.nonode
+            coords.append(c)
+            incid.append(np.asarray(ids_conn).ravel())
             incmap.append(pos + nn * np.arange(ne, dtype=np.int32))
             pos += ne * nn
             etype.append(np.full(ne, mesh.etype, np.int32))
             order.append(np.full(ne, int_order, np.int32))
             bbar.append(np.full(ne, bb, np.int32))
-            noff += mesh.nonode
         self.coords = np.concatenate(coords)
         self.incid = np.concatenate(incid).astype(np.int32)
         self.incmap = np.concatenate(incmap).astype(np.int32)

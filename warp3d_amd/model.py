@@ -11,8 +11,8 @@ import os
 
 import numpy as np
 
-from ._abi import (ASYM_PROTOS, PROTOS, SERVE_PROTOS, W3D_CUT_STEP, W3DModel, dptr,
-                   iptr)
+from ._abi import (ASYM_PROTOS, LIB_BLOCK_PROTOS, PROTOS, SERVE_PROTOS,
+                   W3D_CUT_STEP, W3DModel, dptr, iptr)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libw3dblk.so")
@@ -31,8 +31,8 @@ def lib():
             raise W3DError(f"{LIB_PATH} is not built: run "
                            "python -m warp3d_amd.build")
         L = C.CDLL(LIB_PATH)
-        for name, (res, args) in {**PROTOS, **SERVE_PROTOS,
-                                  **ASYM_PROTOS}.items():
+        for name, (res, args) in {**PROTOS, **SERVE_PROTOS, **ASYM_PROTOS,
+                                  **LIB_BLOCK_PROTOS}.items():
             f = getattr(L, "w3d_" + name)
             f.restype = res
             f.argtypes = args
@@ -170,6 +170,46 @@ class BlockModel:
     def tangent_blocks(self, first, n, step, iter_, precision=0):
         return _chk(lib().w3d_tangent_blocks(self.h, first, n, step, iter_,
                                              precision), "w3d_tangent_blocks")
+
+    # the per-block interface (do_nleps_block / do_nlek_block / addifv);
+    # recover_block and tangent_block may run in several threads at once
+    # for distinct blocks (ctypes releases the GIL during the call)
+    def set_displacements(self, u, du):
+        _chk(lib().w3d_set_displacements(
+            self.h, dptr(np.ascontiguousarray(u, float)),
+            dptr(np.ascontiguousarray(du, float))), "w3d_set_displacements")
+
+    def recover_block(self, blk, step, iter_, extrapolated=False):
+        """Returns (status, block energy, block plastic work, cut)."""
+        en, pw, cut = C.c_double(), C.c_double(), C.c_int32()
+        rc = _chk(lib().w3d_recover_block(
+            self.h, blk, step, iter_, int(extrapolated),
+            C.cast(C.byref(en), C.POINTER(C.c_double)),
+            C.cast(C.byref(pw), C.POINTER(C.c_double)), C.byref(cut)),
+            "w3d_recover_block")
+        return rc, en.value, pw.value, cut.value
+
+    def assemble_ifv(self):
+        """Returns (ifv, sum_ifv, num_term_ifv)."""
+        ifv = np.zeros(3 * self.nonode)
+        s, n = C.c_double(), C.c_int64()
+        _chk(lib().w3d_assemble_ifv(
+            self.h, dptr(ifv), C.cast(C.byref(s), C.POINTER(C.c_double)),
+            C.byref(n)), "w3d_assemble_ifv")
+        return ifv, s.value, n.value
+
+    def ifv_sums(self):
+        """addifv's (sum_ifv, num_term_ifv) of the last whole-pass
+        recovery that returned its energy."""
+        s, n = C.c_double(), C.c_int64()
+        _chk(lib().w3d_get_ifv_sums(
+            self.h, C.cast(C.byref(s), C.POINTER(C.c_double)), C.byref(n)),
+            "w3d_get_ifv_sums")
+        return s.value, n.value
+
+    def tangent_block(self, blk, step, iter_, precision=0):
+        return _chk(lib().w3d_tangent_block(self.h, blk, step, iter_,
+                                            precision), "w3d_tangent_block")
 
     # -- assembly --------------------------------------------------------
     def pattern(self, cstmap):
