@@ -325,7 +325,9 @@ int w3d_get_block_ek_full(const w3d_ctx* ctx, int32_t blk, double* out,
  * Pardiso mtype 1, iparm(12) = 0: A x = b with A the rows as assembled):
  * Jacobi-preconditioned BiCGStab from x = 0, FP64, to |r| <= rtol |b|.
  * iters/relres as w3d_solve_pcg; W3D_ERR_NUMERIC after maxit iterations or
- * on a breakdown, x holding the last iterate. */
+ * on a breakdown (a zero (rh, r) or (rh, v)), x holding the last iterate.
+ * Its work vectors are allocated with the CSR gather map, once per
+ * pattern. */
 int w3d_solve_bicgstab(w3d_ctx* ctx, const double* rhs, double* x,
                        double rtol, int32_t maxit, int32_t* iters,
                        double* relres);

@@ -170,7 +170,7 @@ void free_ctx(w3d_ctx* c) {
   dfree(c->d_blk_sums); dfree(c->d_u); dfree(c->d_du); dfree(c->d_ifv);
   dfree(c->d_asm_hdr); dfree(c->d_asm_src); dfree(c->d_asm_wbase); dfree(c->d_kout);
   dfree(c->d_csr_rp); dfree(c->d_csr_col); dfree(c->d_csr_vidx);
-  dfree(c->d_csr_val); dfree(c->d_pcg);
+  dfree(c->d_csr_val); dfree(c->d_pcg); dfree(c->d_bicg);
   dfree(c->d_ekf); dfree(c->d_ekf_off);
   dfree(c->d_fa_eqdof); dfree(c->d_fa_ln); dfree(c->d_fa_invptr);
   dfree(c->d_fa_ebase); dfree(c->d_fa_atoff); dfree(c->d_fa_at);
@@ -1511,6 +1511,7 @@ int w3d_pattern(w3d_ctx* c, const int32_t* cstmap, int32_t* neqns_out,
     dfree(c->d_csr_vidx);
     dfree(c->d_csr_val);
     dfree(c->d_pcg);
+    dfree(c->d_bicg);
     dfree(c->d_fa_eqdof);
     if ((rc = upload(&c->d_csr_rp, rp))) return rc;
     if ((rc = upload(&c->d_csr_col, col))) return rc;
@@ -1668,6 +1669,10 @@ int w3d_assemble_csr_dev(w3d_ctx* c, double* d_val) {
     if ((rc = upload(&c->d_fa_ebase, ebase))) return rc;
     if ((rc = upload(&c->d_fa_atoff, atoff))) return rc;
     if ((rc = upload(&c->d_fa_at, at))) return rc;
+    // the BiCGStab vectors of this pattern: b, x and the work array
+    dfree(c->d_bicg);
+    if ((rc = dalloc(&c->d_bicg, 2 * (int64_t)neq + bicgstab_words(neq))))
+      return rc;
     c->dev_bytes += t_dalloc_bytes - bytes0;
   }
   prof_begin(c, W3D_PHASE_ASSEMBLE);
@@ -1705,9 +1710,11 @@ int w3d_solve_bicgstab(w3d_ctx* c, const double* rhs, double* x, double rtol,
     return fail(W3D_ERR_ARG, "w3d_solve_bicgstab: rtol > 0 and maxit >= 1");
   HIPCHK(hipSetDevice(c->device));
   const int n = c->neqns;
-  double* d = nullptr;
+  if (!c->d_bicg)
+    return fail(W3D_ERR_ARG, "w3d_solve_bicgstab: no workspace (assemble "
+                             "the CSR first)");
+  double* d = c->d_bicg;
   int rc;
-  if ((rc = dalloc(&d, 12 * (int64_t)n + 8 + solver_npart(n)))) return rc;
   double* d_b = d;
   double* d_x = d + n;
   HIPCHK(hipMemcpyAsync(d_b, rhs, 8 * (size_t)n, hipMemcpyHostToDevice,
@@ -1720,12 +1727,12 @@ int w3d_solve_bicgstab(w3d_ctx* c, const double* rhs, double* x, double rtol,
                      c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess)
     rc = W3D_ERR_DEVICE;
-  dfree(d);
   if (iters) *iters = it;
   if (relres) *relres = rr;
   if (rc == W3D_ERR_NUMERIC)
     return fail(rc, "w3d_solve_bicgstab: no convergence in " +
-                        std::to_string(it) + " iterations (relres " +
+                        std::to_string(it) +
+                        " iterations or a breakdown (relres " +
                         std::to_string(rr) + ")");
   if (rc) return fail(rc, "w3d_solve_bicgstab: device failure");
   return W3D_OK;

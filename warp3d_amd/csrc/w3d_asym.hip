@@ -286,70 +286,103 @@ __global__ void k_csr_diag_inv(const int64_t* __restrict__ rp,
   dinv[r] = d != 0.0 ? 1.0 / d : 1.0;
 }
 
-// p = r + beta (p - omega v), y = p / diag
+// The scalars of one BiCGStab iteration live in sc (device): [0] rho =
+// (rh, r), [1] rho of the last iteration, [2] alpha of the last, [3] omega
+// of the last, [4] (rh, v), [5] (t, s), [6] (t, t), [7] (r, r), [8] the
+// breakdown flag, [9] beta, [10] alpha, [11] omega.  One thread forms them
+// between the dot products; a breakdown (a zero rho, (rh, v), last rho or
+// last omega) raises the flag, after which no vector kernel changes x or r,
+// so x keeps the last iterate.
+__global__ void k_bicg_beta(double* __restrict__ sc, int first) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (sc[0] == 0.0 || (!first && (sc[1] == 0.0 || sc[3] == 0.0))) {
+    sc[8] = 1.0;
+    sc[9] = 0.0;
+    return;
+  }
+  sc[9] = first ? 0.0 : (sc[0] / sc[1]) * (sc[2] / sc[3]);
+}
+
+__global__ void k_bicg_alpha(double* __restrict__ sc) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (sc[8] != 0.0 || sc[4] == 0.0) {
+    sc[8] = 1.0;
+    sc[10] = 0.0;
+    return;
+  }
+  sc[10] = sc[0] / sc[4];
+}
+
+// omega = (t, s) / (t, t); (t, t) = 0: s is already the residual of x (a
+// converged half step), omega = 0
+__global__ void k_bicg_omega(double* __restrict__ sc) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  sc[11] = sc[6] != 0.0 ? sc[5] / sc[6] : 0.0;
+}
+
+// p = r + beta (p - omega_last v), y = p / diag
 __global__ void k_bicg_p(int n, const double* __restrict__ sc, int first,
                          const double* __restrict__ r,
                          const double* __restrict__ v,
                          const double* __restrict__ dinv,
                          double* __restrict__ p, double* __restrict__ y) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || sc[8] != 0.0) return;
   double pi;
-  if (first) {
+  if (first)
     pi = r[i];
-  } else {
-    const double beta = (sc[0] / sc[1]) * (sc[2] / sc[3]);
-    pi = r[i] + beta * (p[i] - sc[3] * v[i]);
-  }
+  else
+    pi = r[i] + sc[9] * (p[i] - sc[3] * v[i]);
   p[i] = pi;
   y[i] = pi * dinv[i];
 }
 
-// alpha = rho / (rh, v); s = r - alpha v; x += alpha y; z = s / diag
+// s = r - alpha v; z = s / diag (x takes alpha y with omega z in k_bicg_x)
 __global__ void k_bicg_s(int n, const double* __restrict__ sc,
                          const double* __restrict__ r,
                          const double* __restrict__ v,
-                         const double* __restrict__ y,
                          const double* __restrict__ dinv,
-                         double* __restrict__ s, double* __restrict__ x,
-                         double* __restrict__ z) {
+                         double* __restrict__ s, double* __restrict__ z) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double alpha = sc[0] / sc[4];
-  const double si = r[i] - alpha * v[i];
+  if (i >= n || sc[8] != 0.0) return;
+  const double si = r[i] - sc[10] * v[i];
   s[i] = si;
-  x[i] = x[i] + alpha * y[i];
   z[i] = si * dinv[i];
 }
 
-// omega = (t, s) / (t, t); x += omega z; r = s - omega t
+// x += alpha y + omega z (alpha y first); r = s - omega t
 __global__ void k_bicg_x(int n, const double* __restrict__ sc,
                          const double* __restrict__ s,
                          const double* __restrict__ t,
+                         const double* __restrict__ y,
                          const double* __restrict__ z,
                          double* __restrict__ x, double* __restrict__ r) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  // (t, t) = 0: s is already the residual of x (converged half step)
-  const double omega = sc[6] != 0.0 ? sc[5] / sc[6] : 0.0;
-  x[i] = x[i] + omega * z[i];
+  if (i >= n || sc[8] != 0.0) return;
+  const double omega = sc[11];
+  x[i] = (x[i] + sc[10] * y[i]) + omega * z[i];
   r[i] = s[i] - omega * t[i];
 }
 
-// the next iteration's rho_prev, alpha, omega
+// the next iteration's rho_last, alpha_last, omega_last
 __global__ void k_bicg_shift(double* __restrict__ sc) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const double alpha = sc[0] / sc[4];
-  const double omega = sc[6] != 0.0 ? sc[5] / sc[6] : 0.0;
   sc[1] = sc[0];
-  sc[2] = alpha;
-  sc[3] = omega;
+  sc[2] = sc[10];
+  sc[3] = sc[11];
+}
+
+__global__ void k_fill_f64(double* __restrict__ d, int n, double v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = v;
 }
 
 #define ACHK(x)                                  \
   do {                                           \
     if ((x) != hipSuccess) return W3D_ERR_DEVICE; \
   } while (0)
+
+int bicgstab_words(int n) { return 9 * n + kBicgScalars + solver_npart(n); }
 
 int bicgstab(int n, const int64_t* rp, const int32_t* col, const double* val,
              const double* b, double* x, double* work, double rtol, int maxit,
@@ -364,13 +397,17 @@ int bicgstab(int n, const int64_t* rp, const int32_t* col, const double* val,
   double* z = y + n;
   double* dinv = z + n;
   double* sc = dinv + n;
-  double* part = sc + 8;
+  double* part = sc + kBicgScalars;
   const dim3 gb((n + 255) / 256), tb(256);
+  // fills by kernels, not memsets: the work array may be a static device
+  // array (w3d_solve_bicgstab_noalloc), and no memset node enters a graph
   hipLaunchKernelGGL(k_csr_diag_inv, gb, tb, 0, st, rp, col, val, n, dinv);
   ACHK(hipMemcpyAsync(r, b, 8 * (size_t)n, hipMemcpyDeviceToDevice, st));
   ACHK(hipMemcpyAsync(rh, b, 8 * (size_t)n, hipMemcpyDeviceToDevice, st));
-  ACHK(hipMemsetAsync(x, 0, 8 * (size_t)n, st));
-  ACHK(hipMemsetAsync(v, 0, 8 * (size_t)n, st));
+  hipLaunchKernelGGL(k_fill_f64, gb, tb, 0, st, x, n, 0.0);
+  hipLaunchKernelGGL(k_fill_f64, gb, tb, 0, st, v, n, 0.0);
+  hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(64), 0, st, sc, kBicgScalars,
+                     0.0);
   int rc;
   if ((rc = solver_dot(n, b, b, part, sc + 7, st))) return rc;
   double bb = 0.0;
@@ -381,25 +418,29 @@ int bicgstab(int n, const int64_t* rp, const int32_t* col, const double* val,
   *relres = 0.0;
   if (bnorm == 0.0) return W3D_OK;
   for (int it = 1; it <= maxit; ++it) {
+    const int first = it == 1 ? 1 : 0;
     if ((rc = solver_dot(n, rh, r, part, sc + 0, st))) return rc;
-    hipLaunchKernelGGL(k_bicg_p, gb, tb, 0, st, n, sc, it == 1 ? 1 : 0, r, v,
-                       dinv, p, y);
+    hipLaunchKernelGGL(k_bicg_beta, dim3(1), dim3(64), 0, st, sc, first);
+    hipLaunchKernelGGL(k_bicg_p, gb, tb, 0, st, n, sc, first, r, v, dinv, p,
+                       y);
     if ((rc = solver_spmv(n, rp, col, val, y, v, st))) return rc;
     if ((rc = solver_dot(n, rh, v, part, sc + 4, st))) return rc;
-    hipLaunchKernelGGL(k_bicg_s, gb, tb, 0, st, n, sc, r, v, y, dinv, s, x,
-                       z);
+    hipLaunchKernelGGL(k_bicg_alpha, dim3(1), dim3(64), 0, st, sc);
+    hipLaunchKernelGGL(k_bicg_s, gb, tb, 0, st, n, sc, r, v, dinv, s, z);
     if ((rc = solver_spmv(n, rp, col, val, z, t, st))) return rc;
     if ((rc = solver_dot(n, t, s, part, sc + 5, st))) return rc;
     if ((rc = solver_dot(n, t, t, part, sc + 6, st))) return rc;
-    hipLaunchKernelGGL(k_bicg_x, gb, tb, 0, st, n, sc, s, t, z, x, r);
+    hipLaunchKernelGGL(k_bicg_omega, dim3(1), dim3(64), 0, st, sc);
+    hipLaunchKernelGGL(k_bicg_x, gb, tb, 0, st, n, sc, s, t, y, z, x, r);
     hipLaunchKernelGGL(k_bicg_shift, dim3(1), dim3(64), 0, st, sc);
     if ((rc = solver_dot(n, r, r, part, sc + 7, st))) return rc;
-    double rr = 0.0;
-    ACHK(hipMemcpyAsync(&rr, sc + 7, 8, hipMemcpyDeviceToHost, st));
+    double rr[2] = {0.0, 0.0};
+    ACHK(hipMemcpyAsync(rr, sc + 7, 16, hipMemcpyDeviceToHost, st));
     ACHK(hipStreamSynchronize(st));
+    *relres = std::sqrt(rr[0]) / bnorm;
+    if (rr[1] != 0.0) return W3D_ERR_NUMERIC;  // breakdown: x as it was
     *iters = it;
-    *relres = std::sqrt(rr) / bnorm;
-    if (!std::isfinite(rr)) return W3D_ERR_NUMERIC;  // breakdown
+    if (!std::isfinite(rr[0])) return W3D_ERR_NUMERIC;
     if (*relres <= rtol) return W3D_OK;
   }
   return W3D_ERR_NUMERIC;

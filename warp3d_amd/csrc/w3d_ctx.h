@@ -177,6 +177,9 @@ struct w3d_ctx {
   uint32_t* d_csr_vidx = nullptr;
   double* d_csr_val = nullptr;
   double* d_pcg = nullptr;   // solver work vectors and scalars
+  // BiCGStab of the asymmetric CSR: b, x and the work vectors, allocated
+  // with the gather map, once per pattern
+  double* d_bicg = nullptr;
   // asymmetric assembly (w3d_set_asymmetric): the full Ke of every element
   // (block b from d_ekf_off[b]), whether the last tangent formed it, and
   // the gather tables of the CSR assembly (rebuilt after w3d_pattern)

@@ -175,9 +175,13 @@ int launch_tangent_full(const DevModel& M, const WorkItem* d_work,
                         const int64_t* d_ekf_off, double* d_ekf,
                         hipStream_t s);
 constexpr int kAsmMaxRow = 768;  // longest CSR row the gather takes
-// Jacobi-BiCGStab on full CSR rows from x = 0; work holds 10 n + 8 +
-// solver_npart(n) doubles.  W3D_ERR_NUMERIC after maxit iterations or on a
-// breakdown (x the last iterate).
+// Jacobi-BiCGStab on full CSR rows from x = 0; work holds
+// bicgstab_words(n) = 9 n + kBicgScalars + solver_npart(n) doubles.
+// W3D_ERR_NUMERIC after maxit iterations or on a breakdown (a zero rho or
+// (rh, v)), x then the last iterate.  Clears its work array with kernels
+// (no memsets).
+constexpr int kBicgScalars = 16;
+int bicgstab_words(int n);
 int bicgstab(int n, const int64_t* rp, const int32_t* col, const double* val,
              const double* b, double* x, double* work, double rtol, int maxit,
              int* iters, double* relres, hipStream_t st);
