@@ -59,6 +59,8 @@ module w3d
     type(c_ptr) :: elblks
     integer(c_int32_t) :: auto_size = 128
     real(c_double) :: eps_bbar = 0.0d0
+    ! skewed constraints (trn, trnmat column-major 3x3 per node); NULL: none
+    type(c_ptr) :: trn = c_null_ptr, trnmat = c_null_ptr
   end type w3d_model
 
   ! w3d_newton_opts: controls of a served problem (w3d_problem_create)

@@ -154,6 +154,16 @@ typedef struct w3d_model {
   const int32_t* elblks;    /* (4*nelblk) span, felem, owner, unused        */
   int32_t auto_size;        /* automatic block size (inelbk.f:219)          */
   double eps_bbar;          /* B-bar stabilisation, indypm.f:713 (0.0)      */
+  /* Skewed constraints, the reference's trn / trnmat (main_data): node n
+     with trn[n-1] != 0 has its displacements, forces and stiffness rows in
+     the constraint-compatible system u_c = T u_g, T = trnmat(n)%mat stored
+     column-major at trnmat[9(n-1) + r + 3c] (row r, column c).  The
+     library rotates element displacements to global coordinates (trnvec,
+     dupstr.f:153-160), element forces back (trnvecs, rknifv.f:75-82) and
+     element stiffnesses to T Ke T^T (trnmtx, rktstf.f:178-186).  Both NULL:
+     no transformations. */
+  const int32_t* trn;       /* (nonode) 0/1                                 */
+  const double* trnmat;     /* (9*nonode)                                   */
 } w3d_model;
 
 typedef struct w3d_ctx w3d_ctx;

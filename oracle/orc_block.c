@@ -73,6 +73,10 @@ struct orc_model {
   w3d_material* mats;
   double eps_bbar;
   int32_t* elblks; /* (4, nelblk) */
+  /* skewed constraints: trn (nonode) and trnmat (9 per node, column-major
+     mat(r, c) at r + 3c), NULL when no node has one */
+  int32_t* trn;
+  double* trnmat;
   double dt;       /* time increment of the load step */
   int asym;        /* asymmetric assembly                */
   orc_block* blk;
@@ -166,6 +170,12 @@ orc_model* orc_create(const w3d_model* m) {
   om->geonl = (int32_t*)malloc(sizeof(int32_t) * m->noelem);
   om->u_last = (double*)calloc(3 * (size_t)m->nonode, sizeof(double));
   om->du_last = (double*)calloc(3 * (size_t)m->nonode, sizeof(double));
+  if (m->trn && m->trnmat) {
+    om->trn = (int32_t*)malloc(sizeof(int32_t) * m->nonode);
+    om->trnmat = (double*)malloc(sizeof(double) * 9 * m->nonode);
+    memcpy(om->trn, m->trn, sizeof(int32_t) * m->nonode);
+    memcpy(om->trnmat, m->trnmat, sizeof(double) * 9 * m->nonode);
+  }
   om->incmap = (int32_t*)malloc(sizeof(int32_t) * m->noelem);
   int64_t ninc = 0;
   for (int e = 0; e < m->noelem; ++e) {
@@ -349,6 +359,7 @@ void orc_destroy(orc_model* om) {
   free(om->blk); free(om->elem_blk); free(om->c); free(om->incid);
   free(om->incmap); free(om->etype); free(om->order); free(om->matnum);
   free(om->bbar); free(om->geonl); free(om->u_last); free(om->du_last);
+  free(om->trn); free(om->trnmat);
   free(om->mats); free(om->elblks); free(om->inv_ptr);
   free(om->inv_list); free(om->dof_eqn_map); free(om->eqn_node_map);
   free(om->k_ptrs); free(om->k_indexes); free(om->row_start);
@@ -393,6 +404,77 @@ static void gather_coords(const orc_model* om, const orc_block* B, int i,
                           double* ce) {
   const int32_t* ed = B->edest + (int64_t)i * B->totdof;
   for (int k = 0; k < B->totdof; ++k) ce[k] = om->c[ed[k] - 1];
+}
+
+/* Skewed constraints of element i's nodes: mats[j] = trnmat of local node
+   j (column-major 3x3) or NULL (duptrans, tanstf.f:733); returns whether
+   any node has one (trn_e_flags). */
+static int elem_trn(const orc_model* om, const orc_block* B, int i,
+                    const double** mats) {
+  int any = 0;
+  const int32_t* ed = B->edest + (int64_t)i * B->totdof;
+  for (int j = 0; j < B->nnode; ++j) {
+    mats[j] = NULL;
+    if (!om->trn) continue;
+    const int node = (ed[j] - 1) / 3; /* edest of component x of node j */
+    if (om->trn[node]) {
+      mats[j] = om->trnmat + 9 * (int64_t)node;
+      any = 1;
+    }
+  }
+  return any;
+}
+
+/* trnvec / trnvecs (trnvec.f:17,109) on an element vector in the local
+   dof order (component c of node j at c N + j): switch 1 v = T v (global
+   to constraint compatible), switch 2 v = T^T v. */
+static void trnvec(double* v, int N, const double** mats, int sw) {
+  for (int j = 0; j < N; ++j) {
+    const double* t = mats[j];
+    if (!t) continue;
+    double x = v[j], y = v[N + j], z = v[2 * N + j], r[3];
+    for (int a = 0; a < 3; ++a) {
+      if (sw == 1)
+        r[a] = t[a + 0] * x + t[a + 3] * y + t[a + 6] * z;
+      else
+        r[a] = t[0 + 3 * a] * x + t[1 + 3 * a] * y + t[2 + 3 * a] * z;
+    }
+    v[j] = r[0];
+    v[N + j] = r[1];
+    v[2 * N + j] = r[2];
+  }
+}
+
+/* trnmtx (trnmtx.f:19): Ke = T Ke T^T on the full T x T matrix a (column
+   major, a[r + T c]), rows first, then columns, node by node. */
+static void trnmtx(double* a, int N, const double** mats) {
+  const int T = 3 * N;
+  for (int j = 0; j < N; ++j) { /* rows j, N+j, 2N+j <- mat . rows */
+    const double* t = mats[j];
+    if (!t) continue;
+    for (int col = 0; col < T; ++col) {
+      double r0 = a[j + T * col], r1 = a[N + j + T * col],
+             r2 = a[2 * N + j + T * col], s3[3];
+      for (int r = 0; r < 3; ++r)
+        s3[r] = t[r + 0] * r0 + t[r + 3] * r1 + t[r + 6] * r2;
+      a[j + T * col] = s3[0];
+      a[N + j + T * col] = s3[1];
+      a[2 * N + j + T * col] = s3[2];
+    }
+  }
+  for (int j = 0; j < N; ++j) { /* columns j, N+j, 2N+j <- cols . mat^T */
+    const double* t = mats[j];
+    if (!t) continue;
+    for (int row = 0; row < T; ++row) {
+      double c0 = a[row + T * j], c1 = a[row + T * (N + j)],
+             c2 = a[row + T * (2 * N + j)], s3[3];
+      for (int r = 0; r < 3; ++r)
+        s3[r] = t[r + 0] * c0 + t[r + 3] * c1 + t[r + 6] * c2;
+      a[row + T * j] = s3[0];
+      a[row + T * (N + j)] = s3[1];
+      a[row + T * (2 * N + j)] = s3[2];
+    }
+  }
 }
 
 /* Jacobians at every point of one element, then (hex8 B-bar) the
@@ -466,15 +548,19 @@ static int recover_block(orc_model* om, int b, int step, int iter,
   for (int i = 0; i < S; ++i) {
     gather_coords(om, B, i, ce + i * T);
     const int32_t* ed = B->edest + (int64_t)i * T;
+    const double* tm[MXND];
+    const int trn_e = elem_trn(om, B, i, tm);
     for (int k = 0; k < T; ++k) due[i * T + k] = du[ed[k] - 1];
+    /* dupstr: displacements of skewed nodes to global coordinates */
+    if (trn_e) trnvec(due + i * T, N, tm, 2);
     if (geonl) {
       /* rknstr_geonl: Jacobians and B-bar volume terms at the mid-step
          configuration ce_0 + u + du/2 (dupstr_blocked) */
       double cm[MXDOF];
-      for (int k = 0; k < T; ++k) {
-        ue[i * T + k] = u[ed[k] - 1];
+      for (int k = 0; k < T; ++k) ue[i * T + k] = u[ed[k] - 1];
+      if (trn_e) trnvec(ue + i * T, N, tm, 2);
+      for (int k = 0; k < T; ++k)
         cm[k] = ce[i * T + k] + ue[i * T + k] + 0.5 * due[i * T + k];
-      }
       elem_kinematics(B, &t, cm, dj[i], gama[i], vol[i]);
     } else {
       elem_kinematics(B, &t, ce + i * T, dj[i], gama[i], vol[i]);
@@ -643,6 +729,16 @@ static int recover_block(orc_model* om, int b, int step, int iter,
       B->evol[i] = B->evol[i] + w * d;
     }
   }
+  /* rknifv: element forces of skewed nodes to constraint-compatible
+     coordinates (trnvecs, switch 1) */
+  for (int i = 0; i < S; ++i) {
+    const double* tm[MXND];
+    if (!elem_trn(om, B, i, tm)) continue;
+    double f[MXDOF];
+    for (int k = 0; k < T; ++k) f[k] = B->einfvec[(int64_t)k * S + i];
+    trnvec(f, N, tm, 1);
+    for (int k = 0; k < T; ++k) B->einfvec[(int64_t)k * S + i] = f[k];
+  }
   /* rplstr: the gathered history of iter <= 0 is not scattered back; the
      n+1 history array keeps its previous contents.  [R n+1] (column-major
      records) is scattered for iter > 0 only. */
@@ -781,10 +877,21 @@ static void tangent_block(orc_model* om, int b) {
     }
   for (int i = 0; i < S; ++i) {
     gather_coords(om, B, i, ce);
-    if (B->geonl) { /* dptstf: coordinates updated to ce_0 + u + du */
+    const double* tm[MXND];
+    const int trn_e = elem_trn(om, B, i, tm);
+    if (B->geonl) { /* dptstf: coordinates updated to ce_0 + u + du, the
+                       displacements of skewed nodes rotated to global */
       const int32_t* ed = B->edest + (int64_t)i * T;
-      for (int k = 0; k < T; ++k)
-        ce[k] = ce[k] + om->u_last[ed[k] - 1] + om->du_last[ed[k] - 1];
+      double ue[MXDOF], due[MXDOF];
+      for (int k = 0; k < T; ++k) {
+        ue[k] = om->u_last[ed[k] - 1];
+        due[k] = om->du_last[ed[k] - 1];
+      }
+      if (trn_e) {
+        trnvec(ue, N, tm, 2);
+        trnvec(due, N, tm, 2);
+      }
+      for (int k = 0; k < T; ++k) ce[k] = ce[k] + ue[k] + due[k];
     }
     elem_kinematics(B, &t, ce, dj, gama, vol);
     double* ek = B->ek + (int64_t)i * U; /* transposed layout (utsz,span) */
@@ -859,6 +966,27 @@ static void tangent_block(orc_model* om, int b) {
       if (B->geonl) /* kgstiff: initial-stress stiffness (no 1/6 for tets) */
         orc_kgstiff(N, gama[g], t.nxi[g], t.neta[g], t.nzeta[g],
                     B->bbar ? vol : NULL, cs, dj[g], t.w[g], ek);
+    }
+    if (trn_e) { /* rktstf: trnmtx on the element's matrix */
+      double full[MXDOF * MXDOF];
+      if (ef) {
+        memcpy(full, ef, sizeof(double) * T * T);
+      } else {
+        for (int j = 1; j <= T; ++j)
+          for (int r = 1; r <= j; ++r) {
+            double v = ek[(j - 1) * j / 2 + r - 1];
+            full[(r - 1) + T * (j - 1)] = v;
+            full[(j - 1) + T * (r - 1)] = v;
+          }
+      }
+      trnmtx(full, N, tm);
+      if (ef) {
+        memcpy(ef, full, sizeof(double) * T * T);
+      } else {
+        for (int j = 1; j <= T; ++j)
+          for (int r = 1; r <= j; ++r)
+            ek[(j - 1) * j / 2 + r - 1] = full[(r - 1) + T * (j - 1)];
+      }
     }
   }
   free(icp_r);

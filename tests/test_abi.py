@@ -73,6 +73,7 @@ int main(void) {
   P(w3d_model, angle_stride);
   P(w3d_model, nmat); P(w3d_model, mats); P(w3d_model, nelblk);
   P(w3d_model, elblks); P(w3d_model, auto_size); P(w3d_model, eps_bbar);
+  P(w3d_model, trn); P(w3d_model, trnmat);
   printf("w3d_model.size %zu\n", sizeof(w3d_model));
   P(w3d_newton_opts, nsteps); P(w3d_newton_opts, precision);
   P(w3d_newton_opts, pcg_maxit); P(w3d_newton_opts, adaptive);

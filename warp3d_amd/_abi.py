@@ -69,6 +69,8 @@ class W3DModel(C.Structure):
         ("elblks", C.POINTER(C.c_int32)),
         ("auto_size", C.c_int32),
         ("eps_bbar", C.c_double),
+        ("trn", C.POINTER(C.c_int32)),
+        ("trnmat", C.POINTER(C.c_double)),
     ]
 
 
@@ -143,7 +145,9 @@ class ModelArrays:
 
     def __init__(self, coords, incid, incmap, etype, int_order, matnum,
                  bbar, materials, elblks=None, auto_size=128, eps_bbar=0.0,
-                 angles=None, angle_stride=1, geonl=None):
+                 angles=None, angle_stride=1, geonl=None, trnmat=None):
+        """trnmat: None, or a dict {node (1-based): 3x3 T} of skewed
+        constraint systems (u_c = T u_g)."""
         self.coords = np.ascontiguousarray(coords, dtype=np.float64).ravel()
         self.incid = np.ascontiguousarray(incid, dtype=np.int32).ravel()
         self.incmap = np.ascontiguousarray(incmap, dtype=np.int32)
@@ -164,6 +168,14 @@ class ModelArrays:
         self.eps_bbar = eps_bbar
         self.nonode = self.coords.size // 3
         self.noelem = self.etype.size
+        self.trn = self.trnmat = None
+        if trnmat:
+            self.trn = np.zeros(self.nonode, np.int32)
+            self.trnmat = np.zeros(9 * self.nonode)
+            for n, t in trnmat.items():
+                self.trn[n - 1] = 1
+                self.trnmat[9 * (n - 1):9 * n] = np.asarray(
+                    t, float).reshape(3, 3).ravel(order="F")
 
     def struct(self):
         m = W3DModel()
@@ -192,6 +204,12 @@ class ModelArrays:
             m.elblks = C.POINTER(C.c_int32)()
         m.auto_size = self.auto_size
         m.eps_bbar = self.eps_bbar
+        if self.trn is not None:
+            m.trn = iptr(self.trn)
+            m.trnmat = dptr(self.trnmat)
+        else:
+            m.trn = C.POINTER(C.c_int32)()
+            m.trnmat = C.POINTER(C.c_double)()
         return m
 
 

@@ -91,7 +91,8 @@ void build_work(const w3d_ctx* c, bool tangent, std::vector<WorkItem>& items,
   for (int b = 0; b < c->nelblk; ++b) {
     const BlockDesc& B = c->blocks[b];
     int key = group_key(B.nnode, B.ngp, B.bbar,
-                        !tangent && B.model == W3D_MM10, B.geonl);
+                        !tangent && B.model == W3D_MM10, B.geonl,
+                        tangent && B.trn);
     int E = tangent ? tan_mfma_chunk(B.nnode) : rec_elems(B.ngp);
     if (tangent)
       for (const auto& kc : c->tan_chunk)
@@ -178,6 +179,7 @@ void free_ctx(w3d_ctx* c) {
   dfree(c->d_rot_n); dfree(c->d_rot_n1); dfree(c->d_u_t); dfree(c->d_du_t);
   dfree(c->d_counter);
   dfree(c->d_blk_flags); dfree(c->d_blk_counter);
+  dfree(c->d_trn_idx); dfree(c->d_trn_mat);
   for (hipStream_t st : c->pool_all) (void)hipStreamDestroy(st);
   c->pool_all.clear();
   c->pool_idle.clear();
@@ -370,6 +372,22 @@ int create_impl(w3d_ctx* c, const w3d_model* m) {
       }
     }
   }
+  // skewed constraints (duptrans, tanstf.f:733): node -> matrix index
+  if (m->trn && m->trnmat) {
+    c->trn_idx.assign(c->nonode, -1);
+    for (int n = 0; n < c->nonode; ++n)
+      if (m->trn[n]) {
+        c->trn_idx[n] = (int32_t)(c->trn_mat.size() / 9);
+        c->trn_mat.insert(c->trn_mat.end(), m->trnmat + 9 * (int64_t)n,
+                          m->trnmat + 9 * (int64_t)n + 9);
+      }
+    for (int b = 0; b < c->nelblk; ++b) {
+      BlockDesc& B = c->blocks[b];
+      for (int64_t k = 0; k < (int64_t)B.span * B.nnode && !B.trn; ++k)
+        B.trn = c->trn_idx[c->conn[B.conn_off + k]] >= 0 ? 1 : 0;
+    }
+    if (c->trn_mat.empty()) c->trn_idx.clear();
+  }
   // inverse incidences (element order; repeated nodes listed once)
   c->inv_ptr.assign(c->nonode + 1, 0);
   for (int b = 0; b < c->nelblk; ++b) {
@@ -432,6 +450,10 @@ int create_impl(w3d_ctx* c, const w3d_model* m) {
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   int rc;
   if ((rc = upload(&c->d_blocks, c->blocks))) return rc;
+  if (!c->trn_idx.empty()) {
+    if ((rc = upload(&c->d_trn_idx, c->trn_idx))) return rc;
+    if ((rc = upload(&c->d_trn_mat, c->trn_mat))) return rc;
+  }
   if ((rc = upload(&c->d_mats, c->mats))) return rc;
   if ((rc = upload(&c->d_tables, c->tables))) return rc;
   if ((rc = upload(&c->d_conn, c->conn))) return rc;
@@ -1548,10 +1570,14 @@ int w3d_get_pattern(const w3d_ctx* c, int32_t* dof_eqn_map,
 int w3d_set_asymmetric(w3d_ctx* c, int32_t on) {
   if (!c) return fail(W3D_ERR_ARG, "w3d_set_asymmetric: null context");
   if (on)
-    for (const BlockDesc& B : c->blocks)
+    for (const BlockDesc& B : c->blocks) {
       if (B.geonl)
         return fail(W3D_ERR_UNSUPPORTED,
                     "w3d_set_asymmetric: finite-strain blocks");
+      if (B.trn)
+        return fail(W3D_ERR_UNSUPPORTED,
+                    "w3d_set_asymmetric: skewed constraints");
+    }
   if (on && !c->d_ekf) {  // the full Ke of every element
     std::vector<int64_t> off(c->nelblk + 1, 0);
     for (int b = 0; b < c->nelblk; ++b) {

@@ -471,7 +471,7 @@ int launch_tangent_full(const DevModel& M, const WorkItem* d_work,
     const WorkItem* w = d_work + gstart[k];
     bool done = false;
 #define X(NN, G, BB)                                                      \
-  if (!done && (groups[k] >> 2) == (group_key(NN, G, BB, 0, 0) >> 2)) {  \
+  if (!done && (groups[k] >> 3) == (group_key(NN, G, BB, 0, 0) >> 3)) {  \
     hipLaunchKernelGGL((k_tangent_full<NN, G, BB>), dim3(gcount[k]),      \
                        dim3(256), 0, s, M, w, d_ekf_off, d_ekf);          \
     done = true;                                                          \

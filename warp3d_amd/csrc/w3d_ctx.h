@@ -83,6 +83,12 @@ struct w3d_ctx {
   double *d_rot_n = nullptr, *d_rot_n1 = nullptr;
   double *d_u_t = nullptr, *d_du_t = nullptr;
   std::vector<int32_t> elblks;
+  // skewed constraints: per node the index of its matrix (-1: none), and
+  // the matrices (column-major 3x3)
+  std::vector<int32_t> trn_idx;
+  std::vector<double> trn_mat;
+  int32_t* d_trn_idx = nullptr;
+  double* d_trn_mat = nullptr;
   std::vector<BlockDesc> blocks;
   std::vector<MatDesc> mats;
   std::vector<GaussTable> tables;
@@ -229,6 +235,8 @@ struct w3d_ctx {
     M.u_t = d_u_t;
     M.du_t = d_du_t;
     M.asym = asym ? 1 : 0;
+    M.trn_idx = d_trn_idx;
+    M.trn_mat = d_trn_mat;
     return M;
   }
 };

@@ -41,7 +41,7 @@ struct BlockDesc {
   int32_t table;     // index of the Gauss/derivative table
   int32_t ncry;      // mm10: crystals per point
   int32_t geonl;     // finite strains and rotations (lprops(18))
-  int32_t pad_;
+  int32_t trn;       // some node of the block has a skewed constraint
   int64_t q_off;     // first Gauss point of the block (global numbering)
   int64_t hist_off;  // first history word of the block
   int64_t ek_off;    // first Ke word of the block
@@ -127,7 +127,28 @@ struct DevModel {
   const double* u_t;
   const double* du_t;
   int32_t asym;           // asymmetric assembly: mm10 tangent unsymmetrised
+  // skewed constraints: per node the index of its transformation (-1:
+  // none) and the matrices, mat(r, c) of index t at trn_mat[9 t + r + 3 c]
+  // (u_c = mat u_g)
+  const int32_t* trn_idx;
+  const double* trn_mat;
 };
+
+// Rotations of one node's 3-vector by its skewed-constraint matrix t
+// (column-major): trnvec's switch 1 (v = T v, global to constraint
+// compatible) and switch 2 (v = T^T v).
+__host__ __device__ inline void trn_apply(const double* t, double* v) {
+  const double x = v[0], y = v[1], z = v[2];
+  v[0] = t[0] * x + t[3] * y + t[6] * z;
+  v[1] = t[1] * x + t[4] * y + t[7] * z;
+  v[2] = t[2] * x + t[5] * y + t[8] * z;
+}
+__host__ __device__ inline void trn_apply_t(const double* t, double* v) {
+  const double x = v[0], y = v[1], z = v[2];
+  v[0] = t[0] * x + t[1] * y + t[2] * z;
+  v[1] = t[3] * x + t[4] * y + t[5] * z;
+  v[2] = t[6] * x + t[7] * y + t[8] * z;
+}
 
 // Kernel launchers (w3d_kernels.hip, w3d_tangent.hip).
 // Recovery passes: pass 0 runs mm01 blocks in full and the strain
@@ -196,13 +217,22 @@ int launch_tangent_mfma(const DevModel& M, const WorkItem* d_work,
                         const std::vector<int>& group_start,
                         const std::vector<int>& group_count, int precision,
                         hipStream_t s);
-// MFMA tangent: waves per workgroup (one element per wave at a time) and
-// elements per workgroup (each wave takes every tan_mfma_waves-th one)
+// MFMA tangent: waves per workgroup and elements per workgroup (each wave
+// takes batches of tan_batch consecutive elements)
 __host__ __device__ constexpr int tan_mfma_waves(int nnode) {
   return (void)nnode, 4;
 }
 __host__ __device__ constexpr int tan_mfma_chunk(int nnode) {
   return nnode == 20 ? 16 : 32;
+}
+// Elements of a tangent wave's batch: as many as fill the 64 lanes with
+// (element, point) pairs, at most 16; for the FP64 contraction at most 4,
+// and one for rules with more than 8 points (a larger batch's LDS would
+// cut the waves per SIMD).
+__host__ __device__ constexpr int tan_batch(int ngp, int tbytes) {
+  return tbytes == 8
+             ? (ngp > 8 ? 1 : (64 / ngp > 4 ? 4 : 64 / ngp))
+             : (64 / ngp < 1 ? 1 : (64 / ngp > 16 ? 16 : 64 / ngp));
 }
 
 // Jacobi-PCG building blocks (w3d_solver.hip)
@@ -311,9 +341,12 @@ bool mm10_crystal_setup(const w3d_material& m, const double* angles,
                         CrystalDev* cr);
 
 // Kernel-group keys: one instantiation per (nnode, ngp, bbar), per
-// material family for the recovery (mm10 blocks run in two passes) and
-// per kinematics (finite strains).  Bit 0: geonl; bit 1: mm10.
-int group_key(int nnode, int ngp, int bbar, int mm10 = 0, int geonl = 0);
+// material family for the recovery (mm10 blocks run in two passes), per
+// kinematics (finite strains) and, for the tangent, by whether a node has
+// a skewed constraint.  Bit 0: geonl; bit 1: mm10; bit 2: skewed
+// constraints; key >> 3 the element (nodes, points, B-bar).
+int group_key(int nnode, int ngp, int bbar, int mm10 = 0, int geonl = 0,
+              int trn = 0);
 
 // Lower-triangle index of (row, col) in the 21-word [D] storage
 // (rstgp1_store_cep order (1,1),(2,1),(2,2),(3,1),...).

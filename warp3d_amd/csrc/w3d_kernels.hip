@@ -305,6 +305,23 @@ __global__ void __launch_bounds__(256)
       s_u[e][NN + j] = u[3 * node + 1];
       s_u[e][2 * NN + j] = u[3 * node + 2];
     }
+    // dupstr: displacements of a skewed node to global coordinates
+    // (trnvec, switch 2)
+    const int ti = bd.trn ? M.trn_idx[node] : -1;
+    if (ti >= 0) {
+      const double* t = M.trn_mat + 9 * ti;
+      double v[3];
+      if (MODE != 2) {
+        v[0] = s_du[e][j]; v[1] = s_du[e][NN + j]; v[2] = s_du[e][2 * NN + j];
+        trn_apply_t(t, v);
+        s_du[e][j] = v[0]; s_du[e][NN + j] = v[1]; s_du[e][2 * NN + j] = v[2];
+      }
+      if (GEONL) {
+        v[0] = s_u[e][j]; v[1] = s_u[e][NN + j]; v[2] = s_u[e][2 * NN + j];
+        trn_apply_t(t, v);
+        s_u[e][j] = v[0]; s_u[e][NN + j] = v[1]; s_u[e][2 * NN + j] = v[2];
+      }
+    }
   }
   __syncthreads();
 
@@ -575,10 +592,10 @@ __global__ void __launch_bounds__(256)
   // component c at local node j (the reference's dof c*NN + j)
   const double scalar = bd.etype == W3D_TET10 ? 1.0 / 6.0 : 1.0;
   double fabs_sum = 0.0;
-  for (int k = tid; k < cnt * T; k += 256) {
-    int ee = k / T, r = k % T;
-    int j = r / 3, c = r % 3;
-    double f = 0.0;
+  // lanes = (element, node): the node's three force components
+  for (int k = tid; k < cnt * NN; k += 256) {
+    const int ee = k / NN, j = k % NN;
+    double f[3] = {0.0, 0.0, 0.0};
     for (int gg = 0; gg < G; ++gg) {
       const double* ga = s_gam[ee][gg];
       double nx = s_dn[0][gg][j], ny = s_dn[1][gg][j], nz = s_dn[2][gg][j];
@@ -587,37 +604,51 @@ __global__ void __launch_bounds__(256)
       double bz = ga[6] * nx + ga[7] * ny + ga[8] * nz;
       const double* s = s_sig[ee][gg];
       double w = s_w[gg] * scalar, d = ga[9];
-      if (c == 0) {
+      {  // component x
         double b0 = bx, b1 = 0.0;
         if (BBAR) {
           double vx = s_vol[ee][3 * j];
           b0 = alpha * bx + beta_b * vx;
           b1 = beta_b * (vx - bx);
         }
-        f = f + b0 * s[0] * w * d + b1 * s[1] * w * d + b1 * s[2] * w * d +
-            by * s[3] * w * d + bz * s[5] * w * d;
-      } else if (c == 1) {
+        f[0] = f[0] + b0 * s[0] * w * d + b1 * s[1] * w * d +
+               b1 * s[2] * w * d + by * s[3] * w * d + bz * s[5] * w * d;
+      }
+      {  // component y
         double b1 = by, b0 = 0.0;
         if (BBAR) {
           double vy = s_vol[ee][3 * j + 1];
           b0 = beta_b * (vy - by);
           b1 = alpha * by + beta_b * vy;
         }
-        f = f + b0 * s[0] * w * d + b1 * s[1] * w * d + b0 * s[2] * w * d +
-            bx * s[3] * w * d + bz * s[4] * w * d;
-      } else {
+        f[1] = f[1] + b0 * s[0] * w * d + b1 * s[1] * w * d +
+               b0 * s[2] * w * d + bx * s[3] * w * d + bz * s[4] * w * d;
+      }
+      {  // component z
         double b2 = bz, b0 = 0.0;
         if (BBAR) {
           double vz = s_vol[ee][3 * j + 2];
           b0 = beta_b * (vz - bz);
           b2 = alpha * bz + beta_b * vz;
         }
-        f = f + b0 * s[0] * w * d + b0 * s[1] * w * d + b2 * s[2] * w * d +
-            by * s[4] * w * d + bx * s[5] * w * d;
+        f[2] = f[2] + b0 * s[0] * w * d + b0 * s[1] * w * d +
+               b2 * s[2] * w * d + by * s[4] * w * d + bx * s[5] * w * d;
       }
     }
-    M.einfvec[bd.ifv_off + (int64_t)(wi.first + ee) * T + r] = f;
-    fabs_sum = fabs_sum + fabs(f);
+    // rknifv: forces of a skewed node to constraint-compatible
+    // coordinates (trnvecs, switch 1)
+    if (bd.trn) {
+      const int node = M.conn[bd.conn_off + (int64_t)(wi.first + ee) * NN + j];
+      const int ti = M.trn_idx[node];
+      if (ti >= 0) trn_apply(M.trn_mat + 9 * ti, f);
+    }
+    double* out = M.einfvec + bd.ifv_off + (int64_t)(wi.first + ee) * T + 3 * j;
+    out[0] = f[0];
+    out[1] = f[1];
+    out[2] = f[2];
+    fabs_sum = fabs_sum + fabs(f[0]);
+    fabs_sum = fabs_sum + fabs(f[1]);
+    fabs_sum = fabs_sum + fabs(f[2]);
   }
   if (want_energy) {  // uniform: addifv's sum of |element force| terms
 #pragma unroll
@@ -732,8 +763,9 @@ __global__ void __launch_bounds__(256)
 // ---------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------
-int group_key(int nnode, int ngp, int bbar, int mm10, int geonl) {
-  return (((nnode * 100 + ngp) * 2 + (bbar ? 1 : 0)) * 2 + (mm10 ? 1 : 0)) *
+int group_key(int nnode, int ngp, int bbar, int mm10, int geonl, int trn) {
+  return ((((nnode * 100 + ngp) * 2 + (bbar ? 1 : 0)) * 2 + (trn ? 1 : 0)) *
+              2 + (mm10 ? 1 : 0)) *
              2 + (geonl ? 1 : 0);
 }
 
@@ -765,7 +797,7 @@ int launch_recover(const DevModel& M, const std::vector<BlockDesc>& hblocks,
     const WorkItem* w = d_work + gstart[k];
     bool done = false;
 #define X(NN, G, BB)                                                        \
-  if (!done && (groups[k] >> 2) == (group_key(NN, G, BB) >> 2)) {          \
+  if (!done && (groups[k] >> 3) == (group_key(NN, G, BB) >> 3)) {          \
     if (geonl)                                                              \
       hipLaunchKernelGGL((k_recover<NN, G, BB, 0, true>), dim3(gcount[k]),  \
                          dim3(256), 0, s, M, w, step, iter, nonlinear, d_u,  \

@@ -18,8 +18,11 @@
 // (3*rank + component), so the accumulators store directly into the ranked
 // row-major element layout (ek_word) in runs of 16 words.
 //
-// One wave owns one element at a time; the next element's coordinates and
-// [D] are fetched while the current one is contracted.
+// A wave takes a batch of elements (tan_batch(G) of them, lanes = element x
+// point for the set-up): it gathers their nodes and coordinates, forms the
+// Jacobians, inverses and B-bar terms of all their points at once and
+// loads their [D] words coalesced (one point per lane), then contracts the
+// elements one after the other on the matrix cores.
 //
 // Finite strains (GEONL; gptns1.f:307 gptns1_a, :1735 ctran1, kg1.f:16
 // kgstiff): the configuration is X + u + du of the last recovery; [D] is
@@ -63,35 +66,63 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int NN, int G, bool BBAR, typename T, bool GEONL = false>
+// The B columns of a node's three components (x, y, z) from its gradient
+// (bx, by, bz) and, for B-bar, its volume-averaged gradient vol[3 j + c]:
+// bc[c][r] is row r (xx, yy, zz, xy, yz, xz) of component c's column.
+template <bool BBAR, typename T>
+__device__ inline void column_xyz(T bx, T by, T bz, const T* vol, int j,
+                                  T alpha, T beta_b, T (&bc)[3][6]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 6; ++r) bc[c][r] = 0;
+  bc[0][0] = bx; bc[0][3] = by; bc[0][5] = bz;
+  bc[1][1] = by; bc[1][3] = bx; bc[1][4] = bz;
+  bc[2][2] = bz; bc[2][4] = by; bc[2][5] = bx;
+  if (BBAR) {
+    const T vx = vol[3 * j], vy = vol[3 * j + 1], vz = vol[3 * j + 2];
+    bc[0][0] = alpha * bx + beta_b * vx;
+    bc[0][1] = bc[0][2] = beta_b * (vx - bx);
+    bc[1][0] = bc[1][2] = beta_b * (vy - by);
+    bc[1][1] = alpha * by + beta_b * vy;
+    bc[2][0] = bc[2][1] = beta_b * (vz - bz);
+    bc[2][2] = alpha * bz + beta_b * vz;
+  }
+}
+
+template <int NN, int G, bool BBAR, typename T, bool GEONL = false,
+          bool TRN = false>
 __global__ void __launch_bounds__(64 * tan_mfma_waves(NN)) __attribute__((amdgpu_waves_per_eu(2)))
     k_tangent_mfma(DevModel M, const WorkItem* __restrict__ work,
                    T* __restrict__ ek_out) {
   constexpr int W = tan_mfma_waves(NN);   // waves per workgroup
+  constexpr int EB = tan_batch(G, (int)sizeof(T));  // elements of a batch
   constexpr int TD = 3 * NN;              // element dofs
   constexpr int NTL = (TD + 15) / 16;     // 16-wide tiles per side
   constexpr int NUP = NTL * (NTL + 1) / 2;  // upper tiles
   constexpr int NPG = (G + 3) / 4;        // 4-point groups (one k step each)
-  constexpr int ND = (21 * G + 63) / 64;  // [D] words per lane
   using MF = Mfma<T>;
   using v4 = typename MF::v4;
 
-  __shared__ double s_dn64[3][G][NN];     // Jacobians (FP64)
-  __shared__ T s_dn[3][G][NN];            // gradients (MFMA type)
+  // shape-function derivatives: FP64 for the Jacobians, and in the MFMA
+  // type for the gradients (the same array when that type is FP64)
+  constexpr bool F64 = sizeof(T) == 8;
+  __shared__ double s_dn64[3][G][NN];
+  __shared__ T s_dnt[F64 ? 1 : 3][G][NN];
+  T(*s_dn)[G][NN] = F64 ? reinterpret_cast<T(*)[G][NN]>(s_dn64) : s_dnt;
   __shared__ double s_w[G];
-  __shared__ double s_ce[W][3][NN];
-  __shared__ double s_detw[W][G];
-  __shared__ T s_gam[W][G][9];
-  __shared__ T s_vol[W][3 * NN];
-  __shared__ T s_d[W][G][21];
-  __shared__ int s_node[W][NN];
-  __shared__ int s_inv[W][NN];
-  // finite strains: w|J|-scaled [D] in FP64 (then the geometric stiffness
-  // of the node pairs in the same words) and the Cauchy stresses
+  __shared__ double s_ce[W][EB][3][NN];
+  __shared__ double s_detw[W][EB][G];
+  __shared__ T s_gam[W][EB][G][9];
+  __shared__ T s_vol[W][BBAR ? EB : 1][3 * NN];
+  __shared__ T s_d[W][EB][G][21];
+  __shared__ int s_node[W][EB][NN];
+  __shared__ int s_inv[W][EB][NN];
+  // finite strains: the Cauchy stresses of the batch's points, and the
+  // geometric stiffness of the current element's node pairs
   constexpr int GW = GEONL ? W : 1;
-  constexpr int GX = GEONL ? (21 * G > NN * NN ? 21 * G : NN * NN) : 1;
-  __shared__ double s_gx[GW][GX];
-  __shared__ T s_cs[GW][GEONL ? G : 1][6];
+  __shared__ T s_cs[GW][GEONL ? EB : 1][GEONL ? G : 1][6];
+  __shared__ double s_gx[GW][GEONL ? NN * NN : 1];
 
   const WorkItem wi = work[blockIdx.x];
   const BlockDesc bd = M.blocks[wi.blk];
@@ -102,78 +133,79 @@ __global__ void __launch_bounds__(64 * tan_mfma_waves(NN)) __attribute__((amdgpu
   const int q = l >> 4;      // point of the lane inside a 4-point group
   const int cres = l & 15;   // column residue inside a tile
   const bool tet = bd.etype == W3D_TET10;
-  const T third = (T)(1.0 / 3.0);
   const T alpha = (T)((2.0 + M.eps_bbar) * (1.0 / 3.0));
   const T beta_b = (T)((1.0 - M.eps_bbar) * (1.0 / 3.0));
-  (void)third;
   const int64_t SG = (int64_t)bd.span * G;
   const double* cep_b = M.cep + 21 * bd.q_off;
+  // the point of this lane in the batch set-up: element pe, point pg
+  const int pe = l / G, pgp = l % G;
 
   for (int k = tid; k < 3 * G * NN; k += 64 * W) {
     const int d = k / (G * NN), rem = k % (G * NN);
     const double v = gt->dn[d][rem / NN][rem % NN];
     s_dn64[d][rem / NN][rem % NN] = v;
-    s_dn[d][rem / NN][rem % NN] = (T)v;
+    if (!F64) s_dnt[d][rem / NN][rem % NN] = (T)v;
   }
   if (tid < G) s_w[tid] = gt->w[tid];
   __syncthreads();
 
-  // element data in flight: coordinates of node l (lanes < NN) and this
-  // lane's share of the stored [D] (word k = l + 64 i is component k / G
-  // of point k % G)
-  double pc[3] = {0, 0, 0}, pd[ND];
-  int pnode = 0;
-  auto fetch = [&](int e) {
-    const int ie = wi.first + e;
-    if (l < NN) {
-      const int node = M.conn[bd.conn_off + (int64_t)ie * NN + l];
-      pnode = node;
-      pc[0] = M.coords[3 * node];
-      pc[1] = M.coords[3 * node + 1];
-      pc[2] = M.coords[3 * node + 2];
-      if (GEONL) {  // dptstf: X + u + du
+  // each wave takes batches of EB consecutive elements of the work item
+  for (int b0 = wv * EB; b0 < wi.count; b0 += W * EB) {
+    const int nb = wi.count - b0 < EB ? wi.count - b0 : EB;
+    const int ie0 = wi.first + b0;
+    wave_sync();
+    // S1: nodes and coordinates of the batch (lanes = element x node)
+    for (int k = l; k < nb * NN; k += 64) {
+      const int e = k / NN, j = k % NN;
+      const int node = M.conn[bd.conn_off + (int64_t)(ie0 + e) * NN + j];
+      s_node[wv][e][j] = node;
+      double x[3] = {M.coords[3 * node], M.coords[3 * node + 1],
+                     M.coords[3 * node + 2]};
+      if (GEONL) {  // dptstf: X + u + du (skewed nodes' rotated to global)
+        double uu[3], dd[3];
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
-          pc[k] = pc[k] + M.u_t[3 * node + k] + M.du_t[3 * node + k];
+        for (int c = 0; c < 3; ++c) {
+          uu[c] = M.u_t[3 * node + c];
+          dd[c] = M.du_t[3 * node + c];
+        }
+        const int ti = TRN ? M.trn_idx[node] : -1;
+        if (ti >= 0) {
+          trn_apply_t(M.trn_mat + 9 * ti, uu);
+          trn_apply_t(M.trn_mat + 9 * ti, dd);
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) x[c] = x[c] + uu[c] + dd[c];
       }
-    }
 #pragma unroll
-    for (int i = 0; i < ND; ++i) {
-      const int k = l + 64 * i;
-      pd[i] = k < 21 * G ? cep_b[(k / G) * SG + (int64_t)ie * G + k % G] : 0.0;
+      for (int c = 0; c < 3; ++c) s_ce[wv][e][c][j] = x[c];
     }
-  };
-
-  int e = wv;
-  if (e < wi.count) fetch(e);
-  for (; e < wi.count; e += W) {
-    const int ie = wi.first + e;
+    // this lane's point: the 21 words of its stored [D], coalesced over
+    // the batch (word k of point (i, g) at k S G + i G + g)
+    const bool pact = l < nb * G;
+    double pd[21];
+#pragma unroll
+    for (int k = 0; k < 21; ++k)
+      pd[k] = pact ? cep_b[k * SG + (int64_t)ie0 * G + l] : 0.0;
     wave_sync();
-    if (l < NN) {
-      s_node[wv][l] = pnode;
-      s_ce[wv][0][l] = pc[0];
-      s_ce[wv][1][l] = pc[1];
-      s_ce[wv][2][l] = pc[2];
-    }
-    wave_sync();
-    // P1: node ranks; Jacobian at every point (jacob1), its inverse
-    if (l < NN) {
-      const int mine = s_node[wv][l];
+    // S2: node ranks; the Jacobian at every point (jacob1) and its inverse
+    for (int k = l; k < nb * NN; k += 64) {
+      const int e = k / NN, j = k % NN;
+      const int mine = s_node[wv][e][j];
       int r = 0;
 #pragma unroll
-      for (int b = 0; b < NN; ++b) {
-        const int o = s_node[wv][b];
-        r += (o < mine || (o == mine && b < l)) ? 1 : 0;
+      for (int bb = 0; bb < NN; ++bb) {
+        const int o = s_node[wv][e][bb];
+        r += (o < mine || (o == mine && bb < j)) ? 1 : 0;
       }
-      s_inv[wv][r] = l;
+      s_inv[wv][e][r] = j;
     }
-    if (l < G) {
-      const int g = l;
+    if (pact) {
+      const int e = pe, g = pgp;
       double j11 = 0, j12 = 0, j13 = 0, j21 = 0, j22 = 0, j23 = 0, j31 = 0,
              j32 = 0, j33 = 0;
       for (int j = 0; j < NN; ++j) {
-        const double x = s_ce[wv][0][j], y = s_ce[wv][1][j],
-                     z = s_ce[wv][2][j];
+        const double x = s_ce[wv][e][0][j], y = s_ce[wv][e][1][j],
+                     z = s_ce[wv][e][2][j];
         const double a = s_dn64[0][g][j], b = s_dn64[1][g][j],
                      c = s_dn64[2][g][j];
         j11 = j11 + a * x; j12 = j12 + a * y; j13 = j13 + a * z;
@@ -184,7 +216,7 @@ __global__ void __launch_bounds__(64 * tan_mfma_waves(NN)) __attribute__((amdgpu
       const double c2 = j21 * j33 - j23 * j31;
       const double c3 = j21 * j32 - j22 * j31;
       const double dj = j11 * c1 - j12 * c2 + j13 * c3;
-      T* ga = s_gam[wv][g];
+      T* ga = s_gam[wv][e][g];
       ga[0] = (T)(c1 / dj);
       ga[3] = (T)(-c2 / dj);
       ga[6] = (T)(c3 / dj);
@@ -194,234 +226,274 @@ __global__ void __launch_bounds__(64 * tan_mfma_waves(NN)) __attribute__((amdgpu
       ga[2] = (T)((j12 * j23 - j13 * j22) / dj);
       ga[5] = (T)((j13 * j21 - j11 * j23) / dj);
       ga[8] = (T)((j11 * j22 - j12 * j21) / dj);
-      s_detw[wv][g] = dj;
+      s_detw[wv][e][g] = dj;
     }
     wave_sync();
-    // P2: B-bar volume terms; [D] scaled by w |J| (extract_D_symmetric)
-    // and by 1/6 for tetrahedra (adjust_cnst)
-    if (BBAR && l < NN) {
-      const int j = l;
-      T vx = 0, vy = 0, vz = 0;
-      double volume = 0;
-      for (int g = 0; g < G; ++g) {
-        const T* a = s_gam[wv][g];
-        const T d = (T)s_detw[wv][g];
-        const T nx = s_dn[0][g][j], ny = s_dn[1][g][j], nz = s_dn[2][g][j];
-        vx = vx + (a[0] * nx + a[1] * ny + a[2] * nz) * d;
-        vy = vy + (a[3] * nx + a[4] * ny + a[5] * nz) * d;
-        vz = vz + (a[6] * nx + a[7] * ny + a[8] * nz) * d;
-        volume = volume + s_detw[wv][g];
-      }
-      const T voli = (T)(1.0 / volume);
-      s_vol[wv][3 * j] = vx * voli;
-      s_vol[wv][3 * j + 1] = vy * voli;
-      s_vol[wv][3 * j + 2] = vz * voli;
-    }
-#pragma unroll
-    for (int i = 0; i < ND; ++i) {
-      const int k = l + 64 * i;
-      if (k < 21 * G) {
-        const int g = k % G;
-        double v = (s_w[g] * s_detw[wv][g]) * pd[i];
-        if (GEONL) {
-          s_gx[wv][21 * g + k / G] = v;
-        } else {
-          if (tet) v = (1.0 / 6.0) * v;
-          s_d[wv][g][k / G] = (T)v;
+    // S3: B-bar volume terms (lanes = element x node); [D] scaled by
+    // w |J| (extract_D_symmetric) and by 1/6 for tetrahedra (adjust_cnst)
+    if (BBAR)
+      for (int k = l; k < nb * NN; k += 64) {
+        const int e = k / NN, j = k % NN;
+        T vx = 0, vy = 0, vz = 0;
+        double volume = 0;
+        for (int g = 0; g < G; ++g) {
+          const T* a = s_gam[wv][e][g];
+          const T d = (T)s_detw[wv][e][g];
+          const T nx = s_dn[0][g][j], ny = s_dn[1][g][j], nz = s_dn[2][g][j];
+          vx = vx + (a[0] * nx + a[1] * ny + a[2] * nz) * d;
+          vy = vy + (a[3] * nx + a[4] * ny + a[5] * nz) * d;
+          vz = vz + (a[6] * nx + a[7] * ny + a[8] * nz) * d;
+          volume = volume + s_detw[wv][e][g];
         }
+        const T voli = (T)(1.0 / volume);
+        s_vol[wv][e][3 * j] = vx * voli;
+        s_vol[wv][e][3 * j + 1] = vy * voli;
+        s_vol[wv][e][3 * j + 2] = vz * voli;
       }
-    }
-    if (GEONL) {
-      // ctran1 (+ Q-bar) with [qn1] of the stored [R n+1] and the Cauchy
-      // stress {T} = [qn1] {t n+1}; then adjust_cnst
-      wave_sync();
-      if (l < G) {
-        const int g = l;
-        const int64_t SGb = (int64_t)bd.span * G;
-        const int64_t qp = (int64_t)ie * G + g;
+    if (pact) {
+      const int e = pe, g = pgp;
+      const double wdj = s_w[g] * s_detw[wv][e][g];
+      if (GEONL) {
+        // ctran1 (+ Q-bar) with [qn1] of the stored [R n+1] and the
+        // Cauchy stress {T} = [qn1] {t n+1}; then adjust_cnst
+        const int64_t qp = (int64_t)(ie0 + e) * G + g;
         const double* rr = M.rot_n1 + 9 * bd.q_off + qp;
         const double* sn = M.stress_n1 + 9 * bd.q_off + qp;
         double r9[9], sg[6], q2[36], cs[6], dm[36];
 #pragma unroll
         for (int a = 0; a < 3; ++a)
 #pragma unroll
-          for (int c = 0; c < 3; ++c) r9[3 * a + c] = rr[(a + 3 * c) * SGb];
+          for (int c = 0; c < 3; ++c) r9[3 * a + c] = rr[(a + 3 * c) * SG];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) sg[k] = sn[k * SGb];
+        for (int k = 0; k < 6; ++k) sg[k] = sn[k * SG];
         geonl_q(r9, 2, q2);
         geonl_qmul(q2, sg, cs);
 #pragma unroll
         for (int r = 0; r < 6; ++r)
 #pragma unroll
-          for (int c = 0; c < 6; ++c) dm[6 * r + c] = s_gx[wv][21 * g + tri(r, c)];
-        geonl_ctran(dm, q2, cs, s_detw[wv][g] * s_w[g]);
+          for (int c = 0; c < 6; ++c) dm[6 * r + c] = wdj * pd[tri(r, c)];
+        geonl_ctran(dm, q2, cs, wdj);
 #pragma unroll
         for (int r = 0; r < 6; ++r)
 #pragma unroll
           for (int c = 0; c <= r; ++c) {
             double v = dm[6 * r + c];
             if (tet) v = (1.0 / 6.0) * v;
-            s_d[wv][g][tri(r, c)] = (T)v;
+            s_d[wv][e][g][tri(r, c)] = (T)v;
           }
 #pragma unroll
-        for (int k = 0; k < 6; ++k) s_cs[wv][g][k] = (T)cs[k];
+        for (int k = 0; k < 6; ++k) s_cs[wv][e][g][k] = (T)cs[k];
+      } else {
+#pragma unroll
+        for (int k = 0; k < 21; ++k) {
+          double v = wdj * pd[k];
+          if (tet) v = (1.0 / 6.0) * v;
+          s_d[wv][e][g][k] = (T)v;
+        }
       }
     }
-    // the next element's data travels while this one is contracted
-    if (e + W < wi.count) fetch(e + W);
     wave_sync();
-    // the lane's dof column of every tile: node (local index) and component
-    int jn[NTL], cc[NTL];
-#pragma unroll
-    for (int t = 0; t < NTL; ++t) {
-      const int dp = 16 * t + cres;
-      jn[t] = dp < TD ? s_inv[wv][dp / 3] : -1;
-      cc[t] = dp % 3;
-    }
-    // P3: the contraction on the matrix cores, four points per k step
-    v4 acc[NUP];
-#pragma unroll
-    for (int u = 0; u < NUP; ++u) acc[u] = v4{0, 0, 0, 0};
-#pragma unroll 1
-    for (int pg = 0; pg < NPG; ++pg) {
-      const int p = 4 * pg + q;
-      const int pp = p < G ? p : G - 1;
-      const T* ga = s_gam[wv][pp];
-      const T g0 = ga[0], g1 = ga[1], g2 = ga[2], g3 = ga[3], g4 = ga[4],
-              g5 = ga[5], g6 = ga[6], g7 = ga[7], g8 = ga[8];
-      T col[NTL][6];
+    // S4: the elements of the batch one by one on the matrix cores
+    for (int e = 0; e < nb; ++e) {
+      const int ie = ie0 + e;
+      // the lane's dof column of every tile: node (local index), component
+      int jn[NTL], cc[NTL];
 #pragma unroll
       for (int t = 0; t < NTL; ++t) {
-        const int j = jn[t] < 0 ? 0 : jn[t];
-        const T nx = s_dn[0][pp][j], ny = s_dn[1][pp][j], nz = s_dn[2][pp][j];
-        const T bx = g0 * nx + g1 * ny + g2 * nz;
-        const T by = g3 * nx + g4 * ny + g5 * nz;
-        const T bz = g6 * nx + g7 * ny + g8 * nz;
-        T c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0, c5 = 0;
-        if (cc[t] == 0) {
-          c0 = bx; c3 = by; c5 = bz;
-          if (BBAR) {
-            const T vx = s_vol[wv][3 * j];
-            c0 = alpha * bx + beta_b * vx;
-            c1 = c2 = beta_b * (vx - bx);
-          }
-        } else if (cc[t] == 1) {
-          c1 = by; c3 = bx; c4 = bz;
-          if (BBAR) {
-            const T vy = s_vol[wv][3 * j + 1];
-            c0 = c2 = beta_b * (vy - by);
-            c1 = alpha * by + beta_b * vy;
-          }
-        } else {
-          c2 = bz; c4 = by; c5 = bx;
-          if (BBAR) {
-            const T vz = s_vol[wv][3 * j + 2];
-            c0 = c1 = beta_b * (vz - bz);
-            c2 = alpha * bz + beta_b * vz;
-          }
-        }
-        const bool live = p < G && jn[t] >= 0;
-        col[t][0] = live ? c0 : (T)0;
-        col[t][1] = live ? c1 : (T)0;
-        col[t][2] = live ? c2 : (T)0;
-        col[t][3] = live ? c3 : (T)0;
-        col[t][4] = live ? c4 : (T)0;
-        col[t][5] = live ? c5 : (T)0;
+        const int dp = 16 * t + cres;
+        jn[t] = dp < TD ? s_inv[wv][e][dp / 3] : -1;
+        cc[t] = dp % 3;
       }
-      const T* dm = s_d[wv][pp];
+      // the contraction, four points per k step
+      v4 acc[NUP];
 #pragma unroll
-      for (int r = 0; r < 6; ++r) {
-        // BD(r, dof) = sum_m D(m, r) B(m, dof), bdbtgen order
-        const T d0 = dm[tri(0, r)], d1 = dm[tri(1, r)], d2 = dm[tri(2, r)],
-                d3 = dm[tri(3, r)], d4 = dm[tri(4, r)], d5 = dm[tri(5, r)];
-        T bdv[NTL];
-#pragma unroll
-        for (int t = 0; t < NTL; ++t)
-          bdv[t] = d0 * col[t][0] + d1 * col[t][1] + d2 * col[t][2] +
-                   d3 * col[t][3] + d4 * col[t][4] + d5 * col[t][5];
-#pragma unroll
-        for (int u = 0; u < NUP; ++u)
-          acc[u] = MF::mma(bdv[tile_i(u)], col[tile_j(u)][r], acc[u]);
-      }
-    }
-    if (GEONL) {
-      // kgstiff on the matrix cores: one 4-k step per point, k = the
-      // three gradient components (and a zero row); lane (component
-      // q, node 16 t + cres)
-      constexpr int NTG = (NN + 15) / 16;
-      constexpr int NUG = NTG * (NTG + 1) / 2;
-      v4 accg[NUG];
-#pragma unroll
-      for (int u = 0; u < NUG; ++u) accg[u] = v4{0, 0, 0, 0};
+      for (int u = 0; u < NUP; ++u) acc[u] = v4{0, 0, 0, 0};
 #pragma unroll 1
-      for (int p = 0; p < G; ++p) {
-        const T* ga = s_gam[wv][p];
-        const T* sp = s_cs[wv][p];
-        const T wdj = (T)(s_w[p] * s_detw[wv][p]);
-        T av[NTG], bv[NTG];
+      for (int pg = 0; pg < NPG; ++pg) {
+        const int p = 4 * pg + q;
+        const int pp = p < G ? p : G - 1;
+        const T* ga = s_gam[wv][e][pp];
+        const T g0 = ga[0], g1 = ga[1], g2 = ga[2], g3 = ga[3], g4 = ga[4],
+                g5 = ga[5], g6 = ga[6], g7 = ga[7], g8 = ga[8];
+        T col[NTL][6];
 #pragma unroll
-        for (int t = 0; t < NTG; ++t) {
-          const int a = 16 * t + cres;
-          T gx = 0, gy = 0, gz = 0;
-          if (a < NN) {
+        for (int t = 0; t < NTL; ++t) {
+          const int j = jn[t] < 0 ? 0 : jn[t];
+          const T nx = s_dn[0][pp][j], ny = s_dn[1][pp][j],
+                  nz = s_dn[2][pp][j];
+          const T bx = g0 * nx + g1 * ny + g2 * nz;
+          const T by = g3 * nx + g4 * ny + g5 * nz;
+          const T bz = g6 * nx + g7 * ny + g8 * nz;
+          T c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0, c5 = 0;
+          if (TRN) {
+            // skewed constraints: the column of B T^T, sum_c' T(c, c')
+            // B(:, c') over the node's three component columns (trnmtx's
+            // T Ke T^T formed inside the contraction)
+            T bc[3][6];
+            column_xyz<BBAR, T>(bx, by, bz, s_vol[wv][BBAR ? e : 0], j,
+                                alpha, beta_b, bc);
+            // the component's row of the node's matrix (identity row for
+            // a node without one)
+            const int ti = M.trn_idx[s_node[wv][e][j]];
+            const double* tm = M.trn_mat + 9 * (ti >= 0 ? ti : 0);
+            const int ct = cc[t];
+            const T r0 = ti >= 0 ? (T)tm[ct] : (T)(ct == 0 ? 1 : 0);
+            const T r1 = ti >= 0 ? (T)tm[ct + 3] : (T)(ct == 1 ? 1 : 0);
+            const T r2 = ti >= 0 ? (T)tm[ct + 6] : (T)(ct == 2 ? 1 : 0);
+            c0 = r0 * bc[0][0] + r1 * bc[1][0] + r2 * bc[2][0];
+            c1 = r0 * bc[0][1] + r1 * bc[1][1] + r2 * bc[2][1];
+            c2 = r0 * bc[0][2] + r1 * bc[1][2] + r2 * bc[2][2];
+            c3 = r0 * bc[0][3] + r1 * bc[1][3] + r2 * bc[2][3];
+            c4 = r0 * bc[0][4] + r1 * bc[1][4] + r2 * bc[2][4];
+            c5 = r0 * bc[0][5] + r1 * bc[1][5] + r2 * bc[2][5];
+          } else if (cc[t] == 0) {
+            c0 = bx; c3 = by; c5 = bz;
             if (BBAR) {
-              gx = s_vol[wv][3 * a];
-              gy = s_vol[wv][3 * a + 1];
-              gz = s_vol[wv][3 * a + 2];
-            } else {
-              const T nx = s_dn[0][p][a], ny = s_dn[1][p][a], nz = s_dn[2][p][a];
-              gx = ga[0] * nx + ga[1] * ny + ga[2] * nz;
-              gy = ga[3] * nx + ga[4] * ny + ga[5] * nz;
-              gz = ga[6] * nx + ga[7] * ny + ga[8] * nz;
+              const T vx = s_vol[wv][e][3 * j];
+              c0 = alpha * bx + beta_b * vx;
+              c1 = c2 = beta_b * (vx - bx);
+            }
+          } else if (cc[t] == 1) {
+            c1 = by; c3 = bx; c4 = bz;
+            if (BBAR) {
+              const T vy = s_vol[wv][e][3 * j + 1];
+              c0 = c2 = beta_b * (vy - by);
+              c1 = alpha * by + beta_b * vy;
+            }
+          } else {
+            c2 = bz; c4 = by; c5 = bx;
+            if (BBAR) {
+              const T vz = s_vol[wv][e][3 * j + 2];
+              c0 = c1 = beta_b * (vz - bz);
+              c2 = alpha * bz + beta_b * vz;
             }
           }
-          // (S g)_q with S = (xx, yy, zz, xy, yz, xz)
-          const T s0 = q == 0 ? sp[0] : q == 1 ? sp[3] : sp[5];
-          const T s1 = q == 0 ? sp[3] : q == 1 ? sp[1] : sp[4];
-          const T s2 = q == 0 ? sp[5] : q == 1 ? sp[4] : sp[2];
-          const T gq = q == 0 ? gx : q == 1 ? gy : gz;
-          av[t] = q < 3 ? (s0 * gx + s1 * gy + s2 * gz) * wdj : (T)0;
-          bv[t] = q < 3 ? gq : (T)0;
+          const bool live = p < G && jn[t] >= 0;
+          col[t][0] = live ? c0 : (T)0;
+          col[t][1] = live ? c1 : (T)0;
+          col[t][2] = live ? c2 : (T)0;
+          col[t][3] = live ? c3 : (T)0;
+          col[t][4] = live ? c4 : (T)0;
+          col[t][5] = live ? c5 : (T)0;
         }
+        const T* dm = s_d[wv][e][pp];
 #pragma unroll
-        for (int u = 0; u < NUG; ++u)
-          accg[u] = MF::mma(av[tile_i(u)], bv[tile_j(u)], accg[u]);
+        for (int r = 0; r < 6; ++r) {
+          // BD(r, dof) = sum_m D(m, r) B(m, dof), bdbtgen order
+          const T d0 = dm[tri(0, r)], d1 = dm[tri(1, r)], d2 = dm[tri(2, r)],
+                  d3 = dm[tri(3, r)], d4 = dm[tri(4, r)], d5 = dm[tri(5, r)];
+          T bdv[NTL];
+#pragma unroll
+          for (int t = 0; t < NTL; ++t)
+            bdv[t] = d0 * col[t][0] + d1 * col[t][1] + d2 * col[t][2] +
+                     d3 * col[t][3] + d4 * col[t][4] + d5 * col[t][5];
+#pragma unroll
+          for (int u = 0; u < NUP; ++u)
+            acc[u] = MF::mma(bdv[tile_i(u)], col[tile_j(u)][r], acc[u]);
+        }
       }
-      // the symmetric NN x NN result into the wave's FP64 slab
+      if (GEONL) {
+        // kgstiff on the matrix cores: one 4-k step per point, k = the
+        // three gradient components (and a zero row); lane (component
+        // q, node 16 t + cres)
+        constexpr int NTG = (NN + 15) / 16;
+        constexpr int NUG = NTG * (NTG + 1) / 2;
+        v4 accg[NUG];
 #pragma unroll
-      for (int u = 0; u < NUG; ++u) {
-        const int I = tile_i(u), J = tile_j(u);
-        const int b = 16 * J + cres;
+        for (int u = 0; u < NUG; ++u) accg[u] = v4{0, 0, 0, 0};
+#pragma unroll 1
+        for (int p = 0; p < G; ++p) {
+          const T* ga = s_gam[wv][e][p];
+          const T* sp = s_cs[wv][e][p];
+          const T wdj = (T)(s_w[p] * s_detw[wv][e][p]);
+          T av[NTG], bv[NTG];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int a = 16 * I + MF::row(l, v);
-          if (a < NN && b < NN) {
-            s_gx[wv][a * NN + b] = (double)accg[u][v];
-            s_gx[wv][b * NN + a] = (double)accg[u][v];
+          for (int t = 0; t < NTG; ++t) {
+            const int a = 16 * t + cres;
+            T gx = 0, gy = 0, gz = 0;
+            if (a < NN) {
+              if (BBAR) {
+                gx = s_vol[wv][e][3 * a];
+                gy = s_vol[wv][e][3 * a + 1];
+                gz = s_vol[wv][e][3 * a + 2];
+              } else {
+                const T nx = s_dn[0][p][a], ny = s_dn[1][p][a],
+                        nz = s_dn[2][p][a];
+                gx = ga[0] * nx + ga[1] * ny + ga[2] * nz;
+                gy = ga[3] * nx + ga[4] * ny + ga[5] * nz;
+                gz = ga[6] * nx + ga[7] * ny + ga[8] * nz;
+              }
+            }
+            // (S g)_q with S = (xx, yy, zz, xy, yz, xz)
+            const T s0 = q == 0 ? sp[0] : q == 1 ? sp[3] : sp[5];
+            const T s1 = q == 0 ? sp[3] : q == 1 ? sp[1] : sp[4];
+            const T s2 = q == 0 ? sp[5] : q == 1 ? sp[4] : sp[2];
+            const T gq = q == 0 ? gx : q == 1 ? gy : gz;
+            av[t] = q < 3 ? (s0 * gx + s1 * gy + s2 * gz) * wdj : (T)0;
+            bv[t] = q < 3 ? gq : (T)0;
+          }
+#pragma unroll
+          for (int u = 0; u < NUG; ++u)
+            accg[u] = MF::mma(av[tile_i(u)], bv[tile_j(u)], accg[u]);
+        }
+        // the symmetric NN x NN result into the wave's FP64 slab
+        wave_sync();
+#pragma unroll
+        for (int u = 0; u < NUG; ++u) {
+          const int I = tile_i(u), J = tile_j(u);
+          const int bcol = 16 * J + cres;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int a = 16 * I + MF::row(l, v);
+            if (a < NN && bcol < NN) {
+              s_gx[wv][a * NN + bcol] = (double)accg[u][v];
+              s_gx[wv][bcol * NN + a] = (double)accg[u][v];
+            }
           }
         }
+        wave_sync();
       }
-      wave_sync();
-    }
-    // P4: the upper triangle straight from the accumulators, ranked
-    // row-major: each store writes four runs of 16 consecutive words
-    T* out = ek_out + bd.ek_off + (int64_t)ie * (TD * (TD + 1) / 2);
-    // the store offsets depend on the lane only; formed here, per element,
-    // rather than held in registers across the element loop
-    int ls = l;
-    asm volatile("" : "+v"(ls));
+      // the upper triangle straight from the accumulators, ranked
+      // row-major: each store writes four runs of 16 consecutive words
+      T* out = ek_out + bd.ek_off + (int64_t)ie * (TD * (TD + 1) / 2);
+      // the store offsets depend on the lane only; formed here, per
+      // element, rather than held in registers across the element loop
+      int ls = l;
+      asm volatile("" : "+v"(ls));
 #pragma unroll
-    for (int u = 0; u < NUP; ++u) {
-      const int I = tile_i(u), J = tile_j(u);
-      const int cl = 16 * J + (ls & 15);
+      for (int u = 0; u < NUP; ++u) {
+        const int I = tile_i(u), J = tile_j(u);
+        const int cl = 16 * J + (ls & 15);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int rw = 16 * I + MF::row(ls, v);
-        if (rw <= cl && cl < TD) {
-          T val = acc[u][v];
-          if (GEONL && rw % 3 == cl % 3)
-            val = val + (T)s_gx[wv][s_inv[wv][rw / 3] * NN + s_inv[wv][cl / 3]];
-          out[ek_word(TD, rw, cl)] = val;
+        for (int v = 0; v < 4; ++v) {
+          const int rw = 16 * I + MF::row(ls, v);
+          if (rw <= cl && cl < TD) {
+            T val = acc[u][v];
+            if (GEONL) {
+              const int ja = s_inv[wv][e][rw / 3], jb = s_inv[wv][e][cl / 3];
+              const double gab = s_gx[wv][ja * NN + jb];
+              if (TRN) {
+                // the geometric stiffness g_ab I of the node pair rotated
+                // to T_a (g_ab I) T_b^T
+                const int ta = M.trn_idx[s_node[wv][e][ja]];
+                const int tb = M.trn_idx[s_node[wv][e][jb]];
+                const int ca = rw % 3, cb = cl % 3;
+                double dot = 0.0;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                  const double xa = ta >= 0 ? M.trn_mat[9 * ta + ca + 3 * k]
+                                            : (k == ca ? 1.0 : 0.0);
+                  const double xb = tb >= 0 ? M.trn_mat[9 * tb + cb + 3 * k]
+                                            : (k == cb ? 1.0 : 0.0);
+                  dot = dot + xa * xb;
+                }
+                val = val + (T)(gab * dot);
+              } else if (rw % 3 == cl % 3) {
+                val = val + (T)gab;
+              }
+            }
+            out[ek_word(TD, rw, cl)] = val;
+          }
         }
       }
     }
@@ -448,27 +520,35 @@ int launch_tangent_mfma(const DevModel& M, const WorkItem* d_work,
     if (gcount[k] == 0) continue;
     const WorkItem* w = d_work + gstart[k];
     const bool geonl = groups[k] & 1;
+    const bool trn = (groups[k] >> 2) & 1;
     const dim3 grid(gcount[k]);
     bool done = false;
-#define X(NN, G, BB)                                                         \
-  if (!done && (groups[k] >> 2) == (group_key(NN, G, BB) >> 2)) {           \
-    const dim3 blk(64 * tan_mfma_waves(NN));                                 \
-    if (precision == W3D_FP32 && geonl)                                      \
-      hipLaunchKernelGGL((k_tangent_mfma<NN, G, BB, float, true>), grid, blk, \
-                         0, s, M, w, M.ek32);                                \
-    else if (precision == W3D_FP32)                                          \
-      hipLaunchKernelGGL((k_tangent_mfma<NN, G, BB, float>), grid, blk, 0, s, \
-                         M, w, M.ek32);                                      \
-    else if (geonl)                                                          \
-      hipLaunchKernelGGL((k_tangent_mfma<NN, G, BB, double, true>), grid,    \
-                         blk, 0, s, M, w, M.ek64);                           \
-    else                                                                     \
-      hipLaunchKernelGGL((k_tangent_mfma<NN, G, BB, double>), grid, blk, 0,  \
-                         s, M, w, M.ek64);                                   \
-    done = true;                                                             \
+#define LAUNCH(NN, G, BB, T, GE, TR)                                          \
+  hipLaunchKernelGGL((k_tangent_mfma<NN, G, BB, T, GE, TR>), grid,            \
+                     dim3(64 * tan_mfma_waves(NN)), 0, s, M, w,               \
+                     precision == W3D_FP32 ? (T*)M.ek32 : (T*)M.ek64)
+#define BY_TRN(NN, G, BB, T, GE)          \
+  if (trn)                                \
+    LAUNCH(NN, G, BB, T, GE, true);       \
+  else                                    \
+    LAUNCH(NN, G, BB, T, GE, false)
+#define X(NN, G, BB)                                                     \
+  if (!done && (groups[k] >> 3) == (group_key(NN, G, BB) >> 3)) {       \
+    if (precision == W3D_FP32 && geonl) {                                \
+      BY_TRN(NN, G, BB, float, true);                                    \
+    } else if (precision == W3D_FP32) {                                  \
+      BY_TRN(NN, G, BB, float, false);                                   \
+    } else if (geonl) {                                                  \
+      BY_TRN(NN, G, BB, double, true);                                   \
+    } else {                                                             \
+      BY_TRN(NN, G, BB, double, false);                                  \
+    }                                                                    \
+    done = true;                                                         \
   }
     W3D_TAN_GROUPS(X)
 #undef X
+#undef BY_TRN
+#undef LAUNCH
     if (!done) return W3D_ERR_UNSUPPORTED;
   }
   return hipGetLastError() == hipSuccess ? W3D_OK : W3D_ERR_DEVICE;

@@ -60,8 +60,10 @@ def write_deck(path, arrays, cstmap, du_steps, precision=0, res_tol=0.01,
         np.array(tols + [pcg_rtol, dt, ls_rho, ls_slack, ls_min, ls_max],
                  np.float64).tofile(f)
         arrays.coords.astype(np.float64).tofile(f)
+        geonl = (np.zeros(arrays.noelem, np.int32) if arrays.geonl is None
+                 else arrays.geonl)
         for a in (arrays.incid, arrays.etype, arrays.int_order, arrays.matnum,
-                  arrays.bbar):
+                  arrays.bbar, geonl):
             np.asarray(a, np.int32).tofile(f)
         for m in arrays.materials:
             np.array([m.model, m.linear_elastic, m.slip_type, m.elastic_type,
@@ -77,6 +79,11 @@ def write_deck(path, arrays, cstmap, du_steps, precision=0, res_tol=0.01,
         np.array([int(has), arrays.angle_stride], np.int32).tofile(f)
         if has:
             arrays.angles.astype(np.float64).tofile(f)
+        has_trn = arrays.trn is not None
+        np.array([int(has_trn)], np.int32).tofile(f)
+        if has_trn:
+            np.asarray(arrays.trn, np.int32).tofile(f)
+            np.asarray(arrays.trnmat, np.float64).tofile(f)
         np.asarray(cstmap, np.int32).tofile(f)
         for k in range(nsteps):
             np.array([mfs[k], chg[k]], np.float64).tofile(f)

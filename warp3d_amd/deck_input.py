@@ -1,6 +1,7 @@
 """A subset of the reference's input language, enough to drive the
-library from the reference's own verification decks (e.g.
-verification/test82/voche_model/warp3d_voche.inp).
+library from the reference's own verification decks
+(verification/test82/voche_model/warp3d_voche.inp, the Voce crystal case;
+verification/test24/test_24, a hollow sphere with skewed constraints).
 
 Commands read (main_program.f:121-369 dispatches them in the reference):
   structure, crystal (incrystal.f: slip_type, elastic_type isotropic,
@@ -8,10 +9,14 @@ Commands read (main_program.f:121-369 dispatches them in the reference):
   material ... properties cp (inmat.f: angle_convention, angle_type,
   n_crystals 1, crystal_input single, crystal_type, orientation_input
   single angles), number of nodes/elements, elements (inelem.f: type
-  l3disop, linear, material, order 2x2x2, bbar, center_output, short),
-  coordinates, incidences, blocking automatic size = N (inelbk.f:208),
-  loading <pattern> with element face pressures (inlod.f), constraints
-  with "plane x|y|z=0 symmetry" (incon.f), loading <name> nonlinear with
+  l3disop, linear or nonlinear (finite strains), material, order 2x2x2,
+  bbar, center_output, short), material ... properties bilinear (inmat.f:
+  e, nu, yld_pt, tan_e, beta), coordinates, incidences, blocking
+  automatic [display] [size = N] (inelbk.f:208), loading <pattern> with
+  element face pressures or nodal loads force_x/y/z (inlod.f),
+  constraints (incon.f) with "plane x|y|z=0 symmetry", node lists with
+  u/v/w values and "transformation matrix <nodes> row_1 .. row_2 ..
+  row_3 .." (skewed constraint systems), loading <name> nonlinear with
   "step a-b <pattern> <factor>", initial conditions temperatures (read and
   ignored: the build is isothermal), nonlinear analysis parameters
   (indypm.f: time step, maximum/minimum iterations, divergence check,
@@ -23,7 +28,9 @@ Anything else raises DeckError.
 Face pressures become consistent nodal forces (elem_load_b.f: the
 pressure acts along the inward normal of the face, eqnrmvh; integrated
 with the face's bilinear shape functions at 2x2 points); the face node
-lists of the 8-node brick are those of eqelfn (elem_load_c.f:886).
+lists of the 8-node brick are those of eqelfn (elem_load_c.f:886).  Nodal
+loads at nodes with a skewed constraint system are rotated into it
+(rotate_loads, eqivld.f:788).
 """
 import re
 from dataclasses import dataclass, field
@@ -53,7 +60,10 @@ class Deck:
     elem_opts: dict = field(default_factory=dict)
     auto_size: int = 128
     patterns: dict = field(default_factory=dict)   # name -> [(e, face, p)]
+    nodal: dict = field(default_factory=dict)      # name -> {node: f(3)}
     symmetry: list = field(default_factory=list)   # planes: 'x', 'y', 'z'
+    trn: dict = field(default_factory=dict)        # node -> 3x3 rows
+    fixed: dict = field(default_factory=dict)      # (node, dof) -> value
     steps: dict = field(default_factory=dict)      # step -> [(pat, fac)]
     nonlinear: dict = field(default_factory=dict)
     compute_to: int = 0
@@ -65,7 +75,8 @@ def _logical_lines(text):
     out, cur = [], ""
     for raw in text.splitlines():
         s = raw.split("!")[0].split("$")[0].rstrip()
-        if not s.strip() or s.lstrip().startswith(("c ", "C ", "*echo")):
+        if not s.strip() or s.strip() in ("c", "C") or \
+                s.lstrip().startswith(("c ", "C ", "*echo")):
             continue
         cur = (cur + " " + s.strip()) if cur else s.strip()
         if cur.endswith(","):
@@ -121,17 +132,23 @@ def read(path):
             name = ln.split()[1]
             props = lines[i].lower().replace("properties", "").split()
             i += 1
-            if not props or props[0] != "cp":
-                raise DeckError("only crystal-plasticity (cp) materials")
+            if not props or props[0] not in ("cp", "bilinear"):
+                raise DeckError("only cp and bilinear materials")
             mat_names[name.lower()] = len(mat_names) + 1
-            dk.materials[name.lower()] = props[1:]
+            dk.materials[name.lower()] = props
             continue
-        if low.startswith("number of nodes"):
-            dk.nonode = int(tok[3])
-            dk.noelem = int(tok[5])
-            dk.coords = np.zeros((dk.nonode, 3))
-            dk.incid = np.zeros((dk.noelem, 8), np.int32)
-            dk.elem_material = np.zeros(dk.noelem, np.int32)
+        if low.startswith(("number of nodes", "number of elements")):
+            # "number of nodes N elements M", or one per line
+            for k in range(len(tok) - 1):
+                if tok[k] == "nodes":
+                    dk.nonode = int(tok[k + 1])
+                if tok[k] == "elements":
+                    dk.noelem = int(tok[k + 1])
+            if dk.nonode and dk.coords is None:
+                dk.coords = np.zeros((dk.nonode, 3))
+            if dk.noelem and dk.incid is None:
+                dk.incid = np.zeros((dk.noelem, 8), np.int32)
+                dk.elem_material = np.zeros(dk.noelem, np.int32)
             continue
         if low == "elements":
             mode = "elements"
@@ -143,7 +160,8 @@ def read(path):
             mode = "incidences"
             continue
         if low.startswith("blocking automatic"):
-            dk.auto_size = int(tok[-1])
+            if "size" in tok:
+                dk.auto_size = int(tok[-1])
             mode = None
             continue
         if low.startswith("loading ") and len(tok) == 2:
@@ -154,6 +172,10 @@ def read(path):
                 i += 1
                 mode = "element_loads"
                 dk.patterns[cur_pattern] = []
+            elif nxt == "nodal loads":
+                i += 1
+                mode = "nodal_loads"
+                dk.nodal[cur_pattern] = {}
             elif nxt == "nonlinear":
                 i += 1
                 mode = "steps"
@@ -194,7 +216,8 @@ def read(path):
                                 if "order" in rest else "2x2x2",
                                 bbar="bbar" in rest,
                                 center_output="center_output" in rest,
-                                linear="linear" in rest)
+                                linear="linear" in rest,
+                                geonl="nonlinear" in rest)
             del opts
             continue
         if mode == "coordinates":
@@ -213,11 +236,41 @@ def read(path):
             dk.patterns[cur_pattern].append((int(v[0]), int(v[2]),
                                              float(v[4])))
             continue
+        if mode == "nodal_loads":
+            v = low.split()
+            f = dk.nodal[cur_pattern].setdefault(int(v[0]), np.zeros(3))
+            for k in range(1, len(v) - 1, 2):
+                if v[k] not in ("force_x", "force_y", "force_z"):
+                    raise DeckError(f"nodal loads: {ln}")
+                f["xyz".index(v[k][-1])] += float(v[k + 1])
+            continue
         if mode == "constraints":
             m = re.match(r"plane\s+([xyz])\s*=\s*0(\.0*)?\s+symmetry", low)
-            if not m:
+            if m:
+                dk.symmetry.append(m.group(1))
+                continue
+            if low.startswith("transformation matrix"):
+                v = low.split()[2:]
+                k = v.index("row_1")
+                nodes = [int(x) for x in v[:k]]
+                rows = []
+                for r in ("row_1", "row_2", "row_3"):
+                    j = v.index(r)
+                    rows.append([float(x) for x in v[j + 1:j + 4]])
+                for n in nodes:
+                    dk.trn[n] = np.array(rows)
+                continue
+            v = low.split()
+            k = 0
+            while k < len(v) and v[k].isdigit():
+                k += 1
+            if k == 0 or (len(v) - k) % 2:
                 raise DeckError(f"constraints: {ln}")
-            dk.symmetry.append(m.group(1))
+            for n in (int(x) for x in v[:k]):
+                for j in range(k, len(v), 2):
+                    if v[j] not in ("u", "v", "w"):
+                        raise DeckError(f"constraints: {ln}")
+                    dk.fixed[(n, "uvw".index(v[j]))] = float(v[j + 1])
             continue
         if mode == "steps":
             m = re.match(r"step\s+(\d+)\s*(-\s*(\d+))?\s+(\S+)\s+(\S+)", low)
@@ -238,10 +291,26 @@ def read(path):
     return dk
 
 
-def _crystal_material(dk):
+def _bilinear_material(props):
+    """material ... properties bilinear (inmat.f): e, nu, yld_pt, tan_e,
+    beta (1: isotropic hardening)."""
+    d = _kv(props[1:])
+    return _abi.mm01_material(float(d["e"]), float(d["nu"]),
+                              float(d["yld_pt"]), tan_e=float(d["tan_e"]),
+                              beta=float(d.get("beta", 1.0)))
+
+
+def _material(dk):
+    """The deck's one material: (w3d_material, Euler angles or None)."""
     if len(dk.materials) != 1:
         raise DeckError("one material")
     (props,) = dk.materials.values()
+    if props[0] == "bilinear":
+        return _bilinear_material(props), None
+    return _crystal_material(props[1:], dk)
+
+
+def _crystal_material(props, dk):
     p = props
     d = {}
     k = 0
@@ -306,10 +375,18 @@ class Problem:
     asymmetric: bool = False    # assembly asymmetric on
 
 
+def nodal_loads(dk, pattern):
+    """The nodal forces (3*nonode) of a pattern, global components."""
+    f = np.zeros((dk.nonode, 3))
+    for n, v in dk.nodal[pattern].items():
+        f[n - 1] += v
+    return f.ravel()
+
+
 def build(dk):
     """Model arrays, constraints, the steps' load increments and the
     Newton options of a deck."""
-    mat, angles = _crystal_material(dk)
+    mat, angles = _material(dk)
     if dk.elem_opts.get("order") != "2x2x2":
         raise DeckError("2x2x2 integration")
     ne = dk.noelem
@@ -333,9 +410,9 @@ def build(dk):
         elif k.startswith("adaptive"):
             newton["adaptive"] = t[1] == "on"
         elif k.startswith("extrapolate"):
-            newton["extrapolate"] = t[-1] == "on"
+            newton["extrapolate"] = t[1] == "on"
         elif k.startswith("line search"):
-            newton["line_search"] = t[-1] == "on"
+            newton["line_search"] = t[2] == "on"
         elif k.startswith("assembly asymmetric"):
             asym = t[-1] == "on"
         elif k.startswith("bbar stabilization factor"):
@@ -346,31 +423,59 @@ def build(dk):
         dk.elem_material, np.full(ne, 1 if dk.elem_opts.get("bbar") else 0,
                                   np.int32),
         [mat], auto_size=dk.auto_size, eps_bbar=eps_bbar,
-        angles=np.tile(np.asarray(angles, float), (ne, 1)))
+        angles=(None if angles is None else
+                np.tile(np.asarray(angles, float), (ne, 1))),
+        geonl=np.full(ne, 1 if dk.elem_opts.get("geonl") else 0, np.int32),
+        trnmat=dk.trn or None)
     cst = np.zeros(3 * dk.nonode, np.int32)
     for ax in dk.symmetry:
         c = "xyz".index(ax)
         cst[c::3][dk.coords[:, c] == 0.0] = 1
+    for (n, c), val in dk.fixed.items():
+        if val != 0.0:
+            raise DeckError("nonzero imposed displacements")
+        cst[3 * (n - 1) + c] = 1
     nsteps = dk.compute_to or max(dk.steps)
     dloads, mfs = [], []
     for s in range(1, nsteps + 1):
         d = np.zeros(3 * dk.nonode)
         mf = 0.0
         for pat, fac in dk.steps[s]:
-            d += fac * face_loads(dk, pat)
+            if pat in dk.nodal:
+                d += fac * nodal_loads(dk, pat)
+            else:
+                d += fac * face_loads(dk, pat)
             mf += fac
+        # rotate_loads: forces at skewed nodes in their constraint system
+        for n, t in dk.trn.items():
+            d[3 * (n - 1):3 * n] = t @ d[3 * (n - 1):3 * n]
         d[cst != 0] = 0.0
         dloads.append(d)
         mfs.append(mf)
     return Problem(arr, cst, dloads, mfs, nsteps, dt, newton, asym)
 
 
-def center_stress(model, elem, auto_size_blocks):
+def center_stress(model, elem, auto_size_blocks, geonl=False):
     """The element-centre stresses of an element (the mean of its 8 point
     values, oumkcv.f), from the model's n-state; auto_size_blocks: the
-    model's blocking (rows of span, first element)."""
+    model's blocking (rows of span, first element).  With finite strains
+    the point values are the Cauchy stresses {T} = [qn1] {t} of the
+    unrotated ones with [R n+1] (ougts1.f:117-126), here R t R^T."""
     for b, (span, first) in enumerate(auto_size_blocks, start=1):
         if first <= elem < first + span:
-            s = model.get_block_state(b, 0).reshape(span, 8, 9)
-            return s[elem - first].mean(axis=0)[:6]
+            s = model.get_block_state(b, _abi.STRESS_N).reshape(span, 8, 9)
+            s = s[elem - first][:, :6]
+            if not geonl:
+                return s.mean(axis=0)
+            r = model.get_block_state(b, _abi.ROT_N1).reshape(span, 8, 9)
+            out = np.zeros((8, 6))
+            for g in range(8):
+                rr = r[elem - first, g].reshape(3, 3, order="F")
+                t = s[g]
+                tm = np.array([[t[0], t[3], t[5]], [t[3], t[1], t[4]],
+                               [t[5], t[4], t[2]]])
+                c = rr @ tm @ rr.T
+                out[g] = [c[0, 0], c[1, 1], c[2, 2], c[0, 1], c[1, 2],
+                          c[0, 2]]
+            return out.mean(axis=0)
     raise DeckError(f"element {elem} not found")

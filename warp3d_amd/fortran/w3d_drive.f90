@@ -12,12 +12,15 @@
 ! asymmetric), dt, ls_rho, ls_slack, ls_min, ls_max; double
 ! coords(3*nonode); int32
 ! incid(nnpe*noelem), etype(noelem), int_order(noelem), matnum(noelem),
-! bbar(noelem); per material: int32 model, linear_elastic, slip_type,
+! bbar(noelem), geonl(noelem); per material: int32 model, linear_elastic,
+! slip_type,
 ! elastic_type, miter, alter_mode, angle_type, ncrystals; real32 e, nu,
 ! beta, tan_e, yld_pt; double cp_e, cp_nu, cp_mu, harden_n, theta_0, tau_y,
 ! tau_v, voce_m, iD_v, eps_dot_0_y, atol, atol1, rtol, rtol1; int32
 ! has_angles, angle_stride and (if has_angles) double
-! angles(3*angle_stride*noelem); int32 cstmap(3*nonode); then per step
+! angles(3*angle_stride*noelem); int32 has_trn and (if has_trn) int32
+! trn(nonode), double trnmat(9*nonode) (skewed constraint systems); int32
+! cstmap(3*nonode); then per step
 ! double mf (the step's load multiplier), mf_change (1: the load pattern
 ! changed), du_imposed(3*nonode), dload(3*nonode).
 !
@@ -33,15 +36,15 @@ program w3d_drive
   character(len=4096) :: deck, result, arg
   integer :: device, fu, fo, ios, i, k
   integer(c_int32_t) :: nonode, noelem, nnpe, nmat, nsteps, nodof, step
-  integer(c_int32_t) :: has_angles, neqns, rc, ival(8)
+  integer(c_int32_t) :: has_angles, has_trn, neqns, rc, ival(8)
   integer(c_int64_t) :: ncoeff
   real(c_float) :: fval(5)
   real(c_double) :: dval(14), dt, ls(4), mfv(2), mf_prev, sload
   integer(c_int32_t) :: flags(5), tests(6), per_block
   real(c_double), allocatable, target :: coords(:), angles(:), u(:), dui(:), &
-       du(:), load_n(:), dload(:)
+       du(:), load_n(:), dload(:), trnmat(:)
   integer(c_int32_t), allocatable, target :: incid(:), incmap(:), etype(:), &
-       order(:), matnum(:), bbar(:), cstmap(:), dmap(:)
+       order(:), matnum(:), bbar(:), geonl(:), cstmap(:), dmap(:), trn(:)
   type(w3d_material), allocatable, target :: mats(:)
   type(w3d_model) :: model
   type(newton_options) :: opts
@@ -76,9 +79,9 @@ program w3d_drive
   nodof = 3 * nonode
   allocate(coords(nodof), incid(nnpe * noelem), incmap(noelem), &
            etype(noelem), order(noelem), matnum(noelem), bbar(noelem), &
-           mats(nmat), cstmap(nodof), dmap(nodof), u(nodof), dui(nodof), &
-           du(nodof), load_n(nodof), dload(nodof))
-  read(fu) coords, incid, etype, order, matnum, bbar
+           geonl(noelem), mats(nmat), cstmap(nodof), dmap(nodof), &
+           u(nodof), dui(nodof), du(nodof), load_n(nodof), dload(nodof))
+  read(fu) coords, incid, etype, order, matnum, bbar, geonl
   do k = 1, nmat
     read(fu) ival, fval, dval
     mats(k)%model = ival(1)
@@ -105,6 +108,11 @@ program w3d_drive
     allocate(angles(3 * model%angle_stride * noelem))
     read(fu) angles
   end if
+  read(fu) has_trn
+  if (has_trn /= 0) then
+    allocate(trn(nonode), trnmat(9 * nonode))
+    read(fu) trn, trnmat
+  end if
   read(fu) cstmap
   do i = 1, noelem
     incmap(i) = (i - 1) * nnpe + 1
@@ -119,6 +127,11 @@ program w3d_drive
   model%int_order = c_loc(order)
   model%matnum = c_loc(matnum)
   model%bbar = c_loc(bbar)
+  model%geonl = c_loc(geonl)
+  if (has_trn /= 0) then
+    model%trn = c_loc(trn)
+    model%trnmat = c_loc(trnmat)
+  end if
   model%angles = c_null_ptr
   if (has_angles /= 0) model%angles = c_loc(angles)
   model%nmat = nmat
