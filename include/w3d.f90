@@ -13,7 +13,8 @@ module w3d
   implicit none
   private :: c_int32_t, c_int64_t, c_double, c_float, c_ptr, c_char
 
-  integer(c_int32_t), parameter :: W3D_ABI_VERSION = 4
+  integer(c_int32_t), parameter :: W3D_ABI_VERSION = 5
+  integer(c_int32_t), parameter :: W3D_MAX_CURVES = 8
   ! status codes
   integer(c_int32_t), parameter :: W3D_OK = 0, W3D_CUT_STEP = 1, &
        W3D_ERR_ARG = -1, W3D_ERR_UNSUPPORTED = -2, W3D_ERR_DEVICE = -3, &
@@ -42,6 +43,17 @@ module w3d
          rtol1 = 1.0d-5
     integer(c_int32_t) :: miter = 30, alter_mode = 0, angle_type = 1, &
          angle_convention = 1, ncrystals = 1, pad_ = 0
+    ! thermal expansion (props 9, 13, 34-37) and alpha_zero (matprp 250)
+    real(c_float) :: alpha(6) = 0, alpha_zero = 0
+    ! mm01 segmental curve set: 0 none, 1 temperature independent,
+    ! 2 temperature dependent (ncurves curves at curve_temp)
+    integer(c_int32_t) :: curve_type = 0, ncurves = 0
+    real(c_double) :: curve_eps(2) = 0
+    real(c_double) :: curve_temp(W3D_MAX_CURVES) = 0, &
+         curve_e(W3D_MAX_CURVES) = 0, curve_nu(W3D_MAX_CURVES) = 0, &
+         curve_alpha(W3D_MAX_CURVES) = 0
+    ! stresses of curve points 1, 2 (C: curve_sig[curve][point])
+    real(c_double) :: curve_sig(2, W3D_MAX_CURVES) = 0
   end type w3d_material
 
   ! struct w3d_model: pointers to arrays the caller keeps alive for the
@@ -111,6 +123,17 @@ module w3d
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx
       real(c_double), value :: dt
+      integer(c_int) :: rc
+    end function
+    function w3d_set_temperatures(ctx, temper_nodes, dtemp_nodes, &
+         temper_elems, dtemp_elems, incremental) &
+         bind(C, name="w3d_set_temperatures") result(rc)
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: ctx
+      ! each a c_loc of a double precision array, or c_null_ptr
+      type(c_ptr), value :: temper_nodes, dtemp_nodes, temper_elems, &
+           dtemp_elems
+      integer(c_int32_t), value :: incremental
       integer(c_int) :: rc
     end function
     function w3d_recover(ctx, step, iter, extrapolated, u, du, ifv, energy, &

@@ -42,7 +42,10 @@
 extern "C" {
 #endif
 
-#define W3D_ABI_VERSION 4
+#define W3D_ABI_VERSION 5
+
+/* curves of one mm01 segmental curve set (temperature-dependent form) */
+#define W3D_MAX_CURVES 8
 
 /* ---- status codes ---------------------------------------------------- */
 enum {
@@ -125,6 +128,32 @@ typedef struct w3d_material {
                                accepts, rknstr.f:1927)                     */
   int32_t ncrystals;        /* crystals per point, Taylor-averaged         */
   int32_t pad_;
+  /* thermal expansion (both models): props 9, 13, 34-37 of the element
+     property table (REAL): alpha xx, yy, zz, xy, yz, xz (rknstr.f:1046-1053)
+     and alpha_zero, matprp(250) (rknstr.f:150) */
+  float alpha[6];
+  float alpha_zero;
+  /* mm01 segmental stress-strain curve set (the material's curve
+     property; segmental_curves module, set_up_segmental,
+     setup_mat_props.f:67): curve_type 0 none (H' from tan_e, yield
+     yld_pt); 1 one temperature-independent curve, the reference's curve
+     type 0 (curve 0 below; e, nu and alpha stay the properties above); 2
+     temperature dependent, the reference's curve type 1: ncurves curves
+     (at most W3D_MAX_CURVES) at the ascending
+     temperatures curve_temp, each with its own e, nu and isotropic alpha,
+     interpolated linearly at the point's temperatures at n and n+1
+     (linear_interpolate, setup_mat_props.f:361, constant outside the
+     range).  mm01 uses the first two points of the curve at the point's
+     temperature (set_up_h_prime, setup_mat_props.f:406): H' = (sig_2 -
+     sig_1) / (eps_2 - eps_1), yield stress sig_1. */
+  int32_t curve_type;
+  int32_t ncurves;
+  double curve_eps[2];                 /* plastic strains of points 1, 2 */
+  double curve_temp[W3D_MAX_CURVES];   /* seg_curves_value              */
+  double curve_e[W3D_MAX_CURVES];      /* seg_curves_ym                 */
+  double curve_nu[W3D_MAX_CURVES];     /* seg_curves_nu                 */
+  double curve_alpha[W3D_MAX_CURVES];  /* seg_curves_alpha              */
+  double curve_sig[W3D_MAX_CURVES][2]; /* stresses of points 1, 2       */
 } w3d_material;
 
 /*
@@ -195,6 +224,29 @@ int w3d_block_info(const w3d_ctx* ctx, int32_t blk, int32_t* info5);
  * default is 1.0.
  */
 int w3d_set_time_increment(w3d_ctx* ctx, double dt);
+
+/*
+ * Temperatures of the current load step, the reference's main_data
+ * temper_nodes / temper_elems (totals at the start of the step) and
+ * dtemp_nodes / dtemp_elems (the step's imposed changes); any of the four
+ * may be NULL (zero).  Each element gathers node value + element value
+ * (gadtemps, gatemps, gatemps.f:63,117), every Gauss point interpolates them
+ * with the shape functions (gauss_pt_temps, gp_temp_eps.f:17), and, when
+ * incremental != 0 (the reference's `temperatures` flag: temperature
+ * changes imposed in this step), the thermal strain increment
+ * alpha(T n+1) (T n+1 - alpha_zero) - alpha(T n) (T n - alpha_zero)
+ * (gp_temp_eps, gp_temp_eps.f:163) is taken from the strain increment the
+ * material sees (drive_01_update rstgp1.f:804-819, drive_10_update
+ * :4400-4447); the stored strains stay those of the displacements.
+ * Temperature-dependent mm01 curves read the point temperatures whether or
+ * not incremental is set.  Replaces drive_eps_sig_internal_forces.f:
+ * 1756-1790 (the gathers of dupstr_blocked).  The values are copied; they
+ * hold until the next call.
+ */
+int w3d_set_temperatures(w3d_ctx* ctx, const double* temper_nodes,
+                         const double* dtemp_nodes,
+                         const double* temper_elems,
+                         const double* dtemp_elems, int32_t incremental);
 
 /*
  * Strain/stress recovery, material update and internal forces for every

@@ -54,6 +54,8 @@ def lib():
                                   C.c_double, C.c_double, C.c_int,
                                   C.c_double] + \
             [C.POINTER(C.c_double)] * 5 + [C.c_double, C.POINTER(C.c_double)]
+        L.orc_shapef.argtypes = [C.c_int, C.c_double, C.c_double,
+                                 C.c_double, C.POINTER(C.c_double)]
         L.orc_hprime.restype = C.c_float
         L.orc_hprime.argtypes = [C.c_float, C.c_float]
         _D = C.POINTER(C.c_double)
@@ -130,6 +132,12 @@ def derivs(etype, xi, eta, zeta, nnode):
     a = [np.zeros(nnode) for _ in range(3)]
     lib().orc_derivs(etype, xi, eta, zeta, *[dptr(x) for x in a])
     return a
+
+
+def shapef(etype, xi, eta, zeta, nnode):
+    n = np.zeros(nnode)
+    lib().orc_shapef(etype, xi, eta, zeta, dptr(n))
+    return n
 
 
 def jacob1(nxi, neta, nzeta, ce):
@@ -355,6 +363,25 @@ class OracleModel:
 
     def set_time_increment(self, dt):
         lib().orc_set_time_increment(self.h, dt)
+
+    def set_temperatures(self, temper_nodes=None, dtemp_nodes=None,
+                         temper_elems=None, dtemp_elems=None,
+                         incremental=True):
+        a = [None if v is None else np.ascontiguousarray(v, float)
+             for v in (temper_nodes, dtemp_nodes, temper_elems, dtemp_elems)]
+        _chk(lib().orc_set_temperatures(
+            self.h, *[dptr(v) if v is not None else None for v in a],
+            int(bool(incremental))), "set_temperatures")
+
+    def ifv_sums(self):
+        """addifv's (sum_ifv, num_term_ifv) of the last recovery, summed
+        block by block."""
+        s, n = 0.0, 0
+        for b in range(1, self.nelblk + 1):
+            s = s + self.block_sums(b)[2]
+            info = self.block_info(b)
+            n += info["span"] * info["totdof"]
+        return s, n
 
     def block_info(self, blk):
         info = np.zeros(5, np.int32)

@@ -78,6 +78,10 @@ struct orc_model {
   int32_t* trn;
   double* trnmat;
   double dt;       /* time increment of the load step */
+  /* temperatures of the load step (main_data temper_nodes, dtemp_nodes,
+     temper_elems, dtemp_elems; NULL = zero) and the `temperatures` flag */
+  double *tn, *dtn, *te, *dte;
+  int temps_incr;
   int asym;        /* asymmetric assembly                */
   orc_block* blk;
   int32_t* elem_blk; /* (noelem) 0-based block of each element */
@@ -94,6 +98,25 @@ static int g_threads = 0;
 void orc_set_threads(int n) { g_threads = n; }
 
 void orc_set_time_increment(orc_model* om, double dt) { om->dt = dt; }
+
+static double* copy_or_null(double* old, const double* src, int64_t n) {
+  free(old);
+  if (!src) return NULL;
+  double* d = (double*)malloc(sizeof(double) * n);
+  memcpy(d, src, sizeof(double) * n);
+  return d;
+}
+
+int orc_set_temperatures(orc_model* om, const double* temper_nodes,
+                         const double* dtemp_nodes, const double* temper_elems,
+                         const double* dtemp_elems, int incremental) {
+  om->tn = copy_or_null(om->tn, temper_nodes, om->nonode);
+  om->dtn = copy_or_null(om->dtn, dtemp_nodes, om->nonode);
+  om->te = copy_or_null(om->te, temper_elems, om->noelem);
+  om->dte = copy_or_null(om->dte, dtemp_elems, om->noelem);
+  om->temps_incr = incremental != 0;
+  return 0;
+}
 
 static int ncry_of(const w3d_material* m) {
   return m->ncrystals > 0 ? m->ncrystals : 1;
@@ -360,6 +383,7 @@ void orc_destroy(orc_model* om) {
   free(om->incmap); free(om->etype); free(om->order); free(om->matnum);
   free(om->bbar); free(om->geonl); free(om->u_last); free(om->du_last);
   free(om->trn); free(om->trnmat);
+  free(om->tn); free(om->dtn); free(om->te); free(om->dte);
   free(om->mats); free(om->elblks); free(om->inv_ptr);
   free(om->inv_list); free(om->dof_eqn_map); free(om->eqn_node_map);
   free(om->k_ptrs); free(om->k_indexes); free(om->row_start);
@@ -389,6 +413,7 @@ int orc_block_info(const orc_model* om, int blk, int32_t* info5) {
 typedef struct {
   double w[MXGP];
   double nxi[MXGP][MXND], neta[MXGP][MXND], nzeta[MXGP][MXND];
+  double sf[MXGP][MXND]; /* shape functions (gauss_pt_temps) */
 } gp_table;
 
 static void make_table(const orc_block* B, gp_table* t) {
@@ -396,6 +421,100 @@ static void make_table(const orc_block* B, gp_table* t) {
     double xi, eta, zeta;
     orc_getgpts(B->etype, B->order, g + 1, &xi, &eta, &zeta, &t->w[g]);
     orc_derivs(B->etype, xi, eta, zeta, t->nxi[g], t->neta[g], t->nzeta[g]);
+    orc_shapef(B->etype, xi, eta, zeta, t->sf[g]);
+  }
+}
+
+/* Temperatures of element i's nodes (dupstr_blocked,
+   drive_eps_sig_internal_forces.f:1756-1790): gadtemps (gatemps.f:63)
+   node + element increments when the step imposes temperature changes,
+   else zero; gatemps (:117) totals at the end of the step. */
+static void elem_temps(const orc_model* om, const orc_block* B, int i,
+                       double* dtn, double* tn1) {
+  const int elem = B->felem + i; /* 1-based */
+  const int32_t* inc = om->incid + om->incmap[elem - 1] - 1;
+  for (int j = 0; j < B->nnode; ++j) {
+    const int node = inc[j];
+    dtn[j] = om->temps_incr ? (om->dtn ? om->dtn[node - 1] : 0.0) +
+                                  (om->dte ? om->dte[elem - 1] : 0.0)
+                            : 0.0;
+    tn1[j] = (om->tn ? om->tn[node - 1] : 0.0) +
+             (om->te ? om->te[elem - 1] : 0.0) + dtn[j];
+  }
+}
+
+/* gauss_pt_temps (gp_temp_eps.f:17): sum of shape function x nodal value,
+   from zero, in node order */
+static double gp_interp(int nnode, const double* sf, const double* v) {
+  double s = 0.0;
+  for (int j = 0; j < nnode; ++j) s = s + sf[j] * v[j];
+  return s;
+}
+
+/* linear_interpolate (setup_mat_props.f:361) */
+static double lin_interp(double x, int n, const double* xs,
+                         const double* ys, int ystride) {
+  if (x <= xs[0]) return ys[0];
+  if (x >= xs[n - 1]) return ys[(n - 1) * ystride];
+  for (int p = 1; p < n; ++p) {
+    if (x > xs[p]) continue;
+    double x1 = xs[p - 1], x2 = xs[p], y1 = ys[(p - 1) * ystride],
+           y2 = ys[p * ystride];
+    return y1 + (x - x1) * (y2 - y1) / (x2 - x1);
+  }
+  return ys[(n - 1) * ystride];
+}
+
+/* mm01 properties of one point: setup_mm01_rknstr (rknstr.f:1018; REAL
+   props, H' in REAL), replaced by the segmental curve set where the
+   material has one: set_up_segmental (setup_mat_props.f:67; e, nu and the
+   isotropic alpha at the point's temperatures at n+1 and n, the curve's
+   stresses at n+1) and set_up_h_prime (:406). */
+typedef struct {
+  double e, nu, e_n, nu_n, hp, yld, alpha[6], alpha_n[6];
+} pt_props;
+
+static void point_props(const w3d_material* mat, double t_n1, double dtemp,
+                        pt_props* p) {
+  p->e = p->e_n = (double)mat->e;
+  p->nu = p->nu_n = (double)mat->nu;
+  p->hp = (double)orc_hprime(mat->e, mat->tan_e);
+  p->yld = (double)mat->yld_pt;
+  for (int k = 0; k < 6; ++k) p->alpha[k] = p->alpha_n[k] = mat->alpha[k];
+  if (mat->curve_type == 0) return;
+  double s1 = mat->curve_sig[0][0], s2 = mat->curve_sig[0][1];
+  if (mat->curve_type == 2) {
+    const int n = mat->ncurves;
+    const double* ts = mat->curve_temp;
+    const double t_n = t_n1 - dtemp;
+    p->e = lin_interp(t_n1, n, ts, mat->curve_e, 1);
+    p->e_n = lin_interp(t_n, n, ts, mat->curve_e, 1);
+    p->nu = lin_interp(t_n1, n, ts, mat->curve_nu, 1);
+    p->nu_n = lin_interp(t_n, n, ts, mat->curve_nu, 1);
+    double a = lin_interp(t_n1, n, ts, mat->curve_alpha, 1);
+    p->alpha[0] = p->alpha[1] = p->alpha[2] = a;
+    a = lin_interp(t_n, n, ts, mat->curve_alpha, 1);
+    p->alpha_n[0] = p->alpha_n[1] = p->alpha_n[2] = a;
+    s1 = lin_interp(t_n1, n, ts, &mat->curve_sig[0][0], 2);
+    s2 = lin_interp(t_n1, n, ts, &mat->curve_sig[0][1], 2);
+  }
+  double dsigma = s2 - s1;
+  double depspls = mat->curve_eps[1] - mat->curve_eps[0];
+  p->hp = dsigma / depspls;
+  p->yld = s1;
+}
+
+/* gp_temp_eps (gp_temp_eps.f:163): the (negative) thermal strain
+   increment added to the displacement increment deps */
+static void thermal_strain(const double* alpha_n1, const double* alpha_n,
+                           double alpha_zero, double t_n1, double dtemp,
+                           double* deps) {
+  double temps_n = t_n1 - dtemp;
+  double tn = temps_n - alpha_zero, tn1 = t_n1 - alpha_zero;
+  for (int k = 0; k < 6; ++k) {
+    double th = alpha_n1[k] * tn1 - alpha_n[k] * tn;
+    double ut = 0.0 - th;
+    deps[k] = deps[k] + ut;
   }
 }
 
@@ -521,12 +640,11 @@ static int recover_block(orc_model* om, int b, int step, int iter,
   const int S = B->span, G = B->ngp, N = B->nnode, T = B->totdof,
             H = B->hist;
   const double scalar = B->etype == W3D_TET10 ? 1.0 / 6.0 : 1.0;
-  /* material property vectors: setup_mm01_rknstr (rknstr.f:1018) */
-  const double e = (double)mat->e, nu = (double)mat->nu,
-               beta = (double)mat->beta,
-               hp = (double)orc_hprime(mat->e, mat->tan_e),
-               yld = (double)mat->yld_pt;
+  /* material property vectors: setup_mm01_rknstr (rknstr.f:1018), per
+     point where temperature-dependent curves interpolate them */
+  const double beta = (double)mat->beta;
   const int lnelas = mat->linear_elastic;
+  const double alpha_zero = (double)mat->alpha_zero;
   const int nonlinear = iter >= 1 || extrapolated;
   double* ce = (double*)malloc(sizeof(double) * S * T);
   double* due = (double*)malloc(sizeof(double) * S * T);
@@ -619,6 +737,20 @@ static int recover_block(orc_model* om, int b, int step, int iter,
       /* rstgp1_update_strains */
       for (int k = 0; k < 6; ++k)
         B->eps_n1[q * 6 + k] = B->eps_n[q * 6 + k] + deps[k];
+      /* gauss_pt_temps, the point's properties, and the thermal strain
+         increment taken from the increment the material sees
+         (drive_01_update rstgp1.f:770-819, drive_10_update :4386-4447) */
+      double dtn[MXND], tn1[MXND];
+      elem_temps(om, B, i, dtn, tn1);
+      const double gp_dtemps = om->temps_incr ? gp_interp(N, t.sf[g], dtn)
+                                              : 0.0;
+      const double gp_temps = gp_interp(N, t.sf[g], tn1);
+      pt_props pp;
+      point_props(mat, gp_temps, gp_dtemps, &pp);
+      if (om->temps_incr)
+        thermal_strain(pp.alpha, pp.alpha_n, alpha_zero, gp_temps,
+                       gp_dtemps, deps);
+      const double e = pp.e, nu = pp.nu;
       double* cgn = sloc + q * 9;
       double* cgn1 = B->urcs_n1 + q * 9;
       double cep[36];
@@ -639,8 +771,9 @@ static int recover_block(orc_model* om, int b, int step, int iter,
       } else if (nonlinear) {
         if (step == 1) first_hist11 = 0.0;
         double h1[11];
-        if (orc_mm01_gp(step, e, nu, beta, hp, lnelas, yld, cgn, cgn1, deps,
-                        hloc + q * H, h1, first_hist11, cep) != 0)
+        if (orc_mm01_gp_t(step, e, nu, pp.e_n, pp.nu_n, beta, pp.hp, lnelas,
+                          pp.yld, cgn, cgn1, deps, hloc + q * H, h1,
+                          first_hist11, cep) != 0)
           rc = W3D_ERR_NUMERIC;
         /* rplstr: history is scattered for iter > 0 only */
         if (iter > 0) memcpy(B->hist_n1 + q * H, h1, sizeof h1);

@@ -315,6 +315,64 @@ void orc_derivs(int etype, double xi, double eta, double zeta, double* nxi,
   }
 }
 
+/* Shape-function values (shapef.f:15): shape1 (:169, 20-node brick; the
+   corner functions as -(1-+r)(1-+s)(...)(1-+t)/8 with the factors in that
+   order), shape2 (:223, 8-node brick), shape6 (:447, 10-node tetrahedron
+   in (s2, s3, s4), s1 = 1 - s2 - s3 - s4).  Node numbering as derivs. */
+void orc_shapef(int etype, double xi, double eta, double zeta, double* n) {
+  const double one = 1.0, two = 2.0, four = 4.0, fourth = 0.25,
+               eighth = 0.125;
+  if (etype == W3D_L3DISOP) {
+    double xp = one + xi, ep = one + eta, zp = one + zeta;
+    double xm = one - xi, em = one - eta, zm = one - zeta;
+    n[0] = xm * em * zp * eighth;
+    n[1] = xm * em * zm * eighth;
+    n[2] = xm * ep * zm * eighth;
+    n[3] = xm * ep * zp * eighth;
+    n[4] = xp * em * zp * eighth;
+    n[5] = xp * em * zm * eighth;
+    n[6] = xp * ep * zm * eighth;
+    n[7] = xp * ep * zp * eighth;
+  } else if (etype == W3D_Q3DISOP) {
+    double rp = one + xi, rm = one - xi, sp = one + eta, sm = one - eta,
+           tp = one + zeta, tm = one - zeta, t = zeta;
+    /* corners: sign pattern of (r, s, t) per node, the mid-side nodes
+       per the edge they sit on */
+    n[0] = -rm * sm * (rp + sp - t) * tp * eighth;
+    n[1] = -rm * sm * (rp + sp + t) * tm * eighth;
+    n[2] = -rm * sp * (rp + sm + t) * tm * eighth;
+    n[3] = -rm * sp * (rp + sm - t) * tp * eighth;
+    n[4] = -rp * sm * (rm + sp - t) * tp * eighth;
+    n[5] = -rp * sm * (rm + sp + t) * tm * eighth;
+    n[6] = -rp * sp * (rm + sm + t) * tm * eighth;
+    n[7] = -rp * sp * (rm + sm - t) * tp * eighth;
+    n[8] = rm * sm * tm * tp * fourth;
+    n[9] = rm * sm * sp * tm * fourth;
+    n[10] = rm * sp * tm * tp * fourth;
+    n[11] = rm * sm * sp * tp * fourth;
+    n[12] = rp * sm * tm * tp * fourth;
+    n[13] = rp * sm * sp * tm * fourth;
+    n[14] = rp * sp * tm * tp * fourth;
+    n[15] = rp * sm * sp * tp * fourth;
+    n[16] = rm * rp * sm * tp * fourth;
+    n[17] = rm * rp * sm * tm * fourth;
+    n[18] = rm * rp * sp * tm * fourth;
+    n[19] = rm * rp * sp * tp * fourth;
+  } else if (etype == W3D_TET10) {
+    double s2 = xi, s3 = eta, s4 = zeta, s1 = one - s2 - s3 - s4;
+    n[0] = two * s1 * s1 - s1;
+    n[1] = two * s2 * s2 - s2;
+    n[2] = two * s3 * s3 - s3;
+    n[3] = two * s4 * s4 - s4;
+    n[4] = four * s1 * s2;
+    n[5] = four * s2 * s3;
+    n[6] = four * s1 * s3;
+    n[7] = four * s1 * s4;
+    n[8] = four * s2 * s4;
+    n[9] = four * s3 * s4;
+  }
+}
+
 /* jacob1, gtmat1.f:300 (3-D solid branch): J(r,c) = sum_j dN_j/dxi_r x_j,c;
    determinant by cofactors of row 1; gama = J^-1 by the adjugate.  jac and
    gama are row-major [r][c]. */

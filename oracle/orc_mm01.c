@@ -4,9 +4,10 @@
  *
  * TEST INFRASTRUCTURE ONLY (see w3d_oracle.h).  Restates mm01.f:24 (mm01),
  * :235 (mm01_set_history), :308 (mm01_init), :776 (mm01_general), :600
- * (mm01_sig_final), :524 (mm01_wrapup) and :1196 (cnst1) for the isothermal
- * case (no temperature loading, so ym_n = ym_n1 and nu_n = nu_n1), with the
- * reference's own truncated constants.
+ * (mm01_sig_final), :524 (mm01_wrapup) and :1196 (cnst1), with the
+ * reference's own truncated constants.  e, nu are the point's moduli at n+1
+ * and e_n, nu_n those at n (they differ for temperature-dependent curves;
+ * the elastic strain at n uses the latter, mm01.f:374-383).
  *
  * History words (1-based as in mm01_set_history): 1 lambda*dt, 2 k (yield
  * radius), 3 accumulated plastic strain, 4 state (an integer kept in the low
@@ -41,6 +42,15 @@ int orc_mm01_gp(int step, double e, double nu, double beta, double hprime,
                 int lnelas, double yld, double* cgn, double* cgn1,
                 const double* deps, double* history, double* history1,
                 double first_hist11, double* cep) {
+  return orc_mm01_gp_t(step, e, nu, e, nu, beta, hprime, lnelas, yld, cgn,
+                       cgn1, deps, history, history1, first_hist11, cep);
+}
+
+int orc_mm01_gp_t(int step, double e, double nu, double e_nv, double nu_nv,
+                  double beta, double hprime, int lnelas, double yld,
+                  double* cgn, double* cgn1, const double* deps,
+                  double* history, double* history1, double first_hist11,
+                  double* cep) {
   const double zero = 0.0, one = 1.0, two = 2.0, three = 3.0, half = 0.5,
                six = 6.0, four = 4.0;
   double* h = history;   /* 11 words, history(i,1..11) */
@@ -65,7 +75,7 @@ int orc_mm01_gp(int step, double e, double nu, double beta, double hprime,
   double de1 = deps[0] - eps_mean, de2 = deps[1] - eps_mean,
          de3 = deps[2] - eps_mean, de4 = deps[3], de5 = deps[4],
          de6 = deps[5];
-  double e_n = e, nu_n = nu;
+  double e_n = e_nv, nu_n = nu_nv;
   double g_n = e_n / two / (one + nu_n);
   double een1 = (cgn[0] - nu_n * (cgn[1] + cgn[2])) / e_n;
   double een2 = (cgn[1] - nu_n * (cgn[0] + cgn[2])) / e_n;

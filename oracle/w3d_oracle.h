@@ -47,6 +47,17 @@ int orc_mm01_gp(int step, double e, double nu, double beta, double hprime,
                 int lnelas, double yld, double* cgn, double* cgn1,
                 const double* deps, double* hist, double* hist1,
                 double first_hist11, double* cep);
+/* The same with the moduli at n (e_n, nu_n) apart from those at n+1. */
+int orc_mm01_gp_t(int step, double e, double nu, double e_n, double nu_n,
+                  double beta, double hprime, int lnelas, double yld,
+                  double* cgn, double* cgn1, const double* deps, double* hist,
+                  double* hist1, double first_hist11, double* cep);
+/* Shape-function values of an element type at a parametric point
+   (shapef.f:15), node numbering as orc_derivs. */
+void orc_shapef(int etype, double xi, double eta, double zeta, double* n);
+int orc_set_temperatures(orc_model* om, const double* temper_nodes,
+                         const double* dtemp_nodes, const double* temper_elems,
+                         const double* dtemp_elems, int incremental);
 /* H' = e*et/(e-et) in REAL arithmetic (elprp.f:872). */
 float orc_hprime(float e, float et);
 

@@ -9,7 +9,8 @@ import pytest
 
 from oracle import oracle as O
 from tests._util import RTOL_FP64, assert_close, smooth_field
-from tests.test_oracle_trn import CASES, _rotations
+from tests.test_gpu_geonl import large_field
+from tests.test_oracle_trn import CASES, _rotations, _tg
 from warp3d_amd import _abi
 from warp3d_amd import mesh as M
 
@@ -27,10 +28,12 @@ def test_skewed_constraints_match_the_oracle(kind, geonl):
     nodes = rng.choice(np.arange(3, m.nonode + 1), size=m.nonode // 3,
                        replace=False)
     base = m.model(mat, int_order=mk[1], auto_size=7, geonl=geonl)
+    trn = _rotations(nodes, 5)
     arr = _abi.ModelArrays(
         base.coords, base.incid, base.incmap, base.etype, base.int_order,
         base.matnum, base.bbar, base.materials, auto_size=7,
-        geonl=base.geonl, trnmat=_rotations(nodes, 5))
+        geonl=base.geonl, trnmat=trn)
+    tg = _tg(m.nonode, trn)
     dev, orc = BlockModel(arr), O.OracleModel(arr)
     cst = np.zeros(3 * m.nonode, np.int32)
     cst[:3] = 1
@@ -39,7 +42,13 @@ def test_skewed_constraints_match_the_oracle(kind, geonl):
     orc.pattern(cst)
     u = np.zeros(3 * m.nonode)
     for step in (1, 2):
-        du = smooth_field(m, 0.004, seed=step)
+        # finite strains of a few % with rotation (test_gpu_geonl's field):
+        # at a few 1e-3 the polar decomposition's conditioning, not the
+        # device, limits agreement
+        # (the finite-strain field is given globally and handed over in
+        # the nodes' constraint-compatible systems, u_c = T u_g)
+        du = (tg @ large_field(m, 0.08, seed=step) if geonl
+              else smooth_field(m, 0.004, seed=step))
         for it in (0, 1):
             tag = f"{kind} geonl={geonl} s{step} i{it}"
             fd, ed, _, rcd = dev.recover(step, it, u, du)

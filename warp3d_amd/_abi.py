@@ -7,7 +7,8 @@ import ctypes as C
 
 import numpy as np
 
-W3D_ABI_VERSION = 4
+W3D_ABI_VERSION = 5
+W3D_MAX_CURVES = 8
 
 W3D_OK = 0
 W3D_CUT_STEP = 1
@@ -45,7 +46,53 @@ class W3DMaterial(C.Structure):
         ("miter", C.c_int32), ("alter_mode", C.c_int32),
         ("angle_type", C.c_int32), ("angle_convention", C.c_int32),
         ("ncrystals", C.c_int32), ("pad_", C.c_int32),
+        ("alpha", C.c_float * 6), ("alpha_zero", C.c_float),
+        ("curve_type", C.c_int32), ("ncurves", C.c_int32),
+        ("curve_eps", C.c_double * 2),
+        ("curve_temp", C.c_double * W3D_MAX_CURVES),
+        ("curve_e", C.c_double * W3D_MAX_CURVES),
+        ("curve_nu", C.c_double * W3D_MAX_CURVES),
+        ("curve_alpha", C.c_double * W3D_MAX_CURVES),
+        ("curve_sig", (C.c_double * 2) * W3D_MAX_CURVES),
     ]
+
+
+def set_thermal(m, alpha, alpha_zero=0.0):
+    """Thermal expansion of a material: alpha a scalar (isotropic: xx, yy,
+    zz; shear terms zero) or the six props 9, 13, 34-37 (xx, yy, zz, xy,
+    yz, xz), rounded through REAL like the reference's property table."""
+    a = np.broadcast_to(np.asarray(alpha, float), (3,)) if np.ndim(alpha) == 0 \
+        else np.asarray(alpha, float)
+    vals = list(a) + [0.0] * (6 - len(a))
+    for k in range(6):
+        m.alpha[k] = vals[k]
+    m.alpha_zero = alpha_zero
+    return m
+
+
+def set_curves(m, eps, curves):
+    """A segmental curve set on an mm01 material: eps = (eps_1, eps_2), the
+    plastic strains of the curve points; curves = [(sig_1, sig_2)] (one
+    temperature-independent curve) or [(temp, e, nu, alpha, sig_1, sig_2)]
+    with ascending temperatures (temperature dependent)."""
+    m.curve_eps[0], m.curve_eps[1] = eps
+    if len(curves[0]) == 2:
+        m.curve_type = 1
+        m.ncurves = 1
+        m.curve_sig[0][0], m.curve_sig[0][1] = curves[0]
+        return m
+    if len(curves) > W3D_MAX_CURVES:
+        raise ValueError("too many curves")
+    m.curve_type = 2
+    m.ncurves = len(curves)
+    for k, (t, e, nu, al, s1, s2) in enumerate(curves):
+        m.curve_temp[k] = t
+        m.curve_e[k] = e
+        m.curve_nu[k] = nu
+        m.curve_alpha[k] = al
+        m.curve_sig[k][0] = s1
+        m.curve_sig[k][1] = s2
+    return m
 
 
 class W3DModel(C.Structure):
@@ -248,6 +295,7 @@ PROTOS = {
     "rollback": (_I, [_P]),
     "get_block_state": (_I, [_P, _I, _I, _D, C.c_int64]),
     "set_block_state": (_I, [_P, _I, _I, _D, C.c_int64]),
+    "set_temperatures": (_I, [_P, _D, _D, _D, _D, _I]),
 }
 
 # the library's own forms of the per-block calls (the oracle's

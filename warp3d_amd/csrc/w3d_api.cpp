@@ -180,6 +180,7 @@ void free_ctx(w3d_ctx* c) {
   dfree(c->d_counter);
   dfree(c->d_blk_flags); dfree(c->d_blk_counter);
   dfree(c->d_trn_idx); dfree(c->d_trn_mat);
+  dfree(c->d_therm); dfree(c->d_temps);
   for (hipStream_t st : c->pool_all) (void)hipStreamDestroy(st);
   c->pool_all.clear();
   c->pool_idle.clear();
@@ -227,11 +228,43 @@ int create_impl(w3d_ctx* c, const w3d_model* m) {
   }
   // materials (setup_mm01_rknstr: REAL props widened to FP64)
   c->mats.resize(m->nmat);
+  c->therm.resize(m->nmat);
   for (int k = 0; k < m->nmat; ++k) {
     const w3d_material& s = m->mats[k];
     MatDesc& d = c->mats[k];
     std::memset(&d, 0, sizeof d);
     d.model = s.model;
+    // thermal expansion (props 9, 13, 34-37, matprp 250) and the segmental
+    // curve set
+    w3d::ThermDesc& td = c->therm[k];
+    std::memset(&td, 0, sizeof td);
+    for (int j = 0; j < 6; ++j) td.alpha[j] = (double)s.alpha[j];
+    td.alpha_zero = (double)s.alpha_zero;
+    td.curve_type = s.curve_type;
+    if (s.curve_type != 0) {
+      const int nc = s.curve_type == 1 ? 1 : s.ncurves;
+      bool ok = s.model == W3D_MM01 &&
+                (s.curve_type == 1 || s.curve_type == 2) && nc >= 1 &&
+                nc <= w3d::kMaxCurves &&
+                s.curve_eps[1] != s.curve_eps[0];
+      for (int j = 1; ok && s.curve_type == 2 && j < nc; ++j)
+        ok = s.curve_temp[j] > s.curve_temp[j - 1];
+      if (!ok)
+        return fail(W3D_ERR_ARG, "w3d_create: segmental curves of material " +
+                                     std::to_string(k + 1));
+      td.ncurves = nc;
+      td.curve_eps[0] = s.curve_eps[0];
+      td.curve_eps[1] = s.curve_eps[1];
+      for (int j = 0; j < nc; ++j) {
+        td.temp[j] = s.curve_temp[j];
+        td.e[j] = s.curve_e[j];
+        td.nu[j] = s.curve_nu[j];
+        td.alpha_c[j] = s.curve_alpha[j];
+        td.sig[j][0] = s.curve_sig[j][0];
+        td.sig[j][1] = s.curve_sig[j][1];
+      }
+      c->curves = true;
+    }
     if (s.model == W3D_MM01) {
       if (!(s.e > s.tan_e) || s.e <= 0.0f || s.nu <= -1.0f || s.nu >= 0.5f)
         return fail(W3D_ERR_ARG, "w3d_create: mm01 properties of material " +
@@ -455,6 +488,7 @@ int create_impl(w3d_ctx* c, const w3d_model* m) {
     if ((rc = upload(&c->d_trn_mat, c->trn_mat))) return rc;
   }
   if ((rc = upload(&c->d_mats, c->mats))) return rc;
+  if ((rc = upload(&c->d_therm, c->therm))) return rc;
   if ((rc = upload(&c->d_tables, c->tables))) return rc;
   if ((rc = upload(&c->d_conn, c->conn))) return rc;
   if (c->has_mm10) {
@@ -2011,6 +2045,35 @@ int w3d_set_time_increment(w3d_ctx* c, double dt) {
   if (!(dt > 0.0))
     return fail(W3D_ERR_ARG, "w3d_set_time_increment: dt must be > 0");
   c->dt = dt;
+  return W3D_OK;
+}
+
+int w3d_set_temperatures(w3d_ctx* c, const double* temper_nodes,
+                         const double* dtemp_nodes, const double* temper_elems,
+                         const double* dtemp_elems, int32_t incremental) {
+  if (!c) return fail(W3D_ERR_ARG, "w3d_set_temperatures: null context");
+  const int64_t nn = c->nonode, ne = c->noelem;
+  std::vector<double> h(2 * (nn + ne), 0.0);
+  if (temper_nodes) std::memcpy(h.data(), temper_nodes, nn * 8);
+  if (dtemp_nodes) std::memcpy(h.data() + nn, dtemp_nodes, nn * 8);
+  if (temper_elems) std::memcpy(h.data() + 2 * nn, temper_elems, ne * 8);
+  if (dtemp_elems) std::memcpy(h.data() + 2 * nn + ne, dtemp_elems, ne * 8);
+  HIPCHK(hipSetDevice(c->device));
+  // ordered after the work already queued on the context's stream
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const bool fresh = !c->d_temps;
+  if (fresh) {
+    int rc = dalloc(&c->d_temps, (int64_t)h.size());
+    if (rc) return rc;
+  }
+  HIPCHK(hipMemcpy(c->d_temps, h.data(), h.size() * 8,
+                   hipMemcpyHostToDevice));
+  // captured iterations hold the temperature pointer and flag by value
+  if (fresh || c->temps_incr != (incremental != 0)) {
+    for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+    c->graphs.clear();
+  }
+  c->temps_incr = incremental != 0;
   return W3D_OK;
 }
 

@@ -91,6 +91,13 @@ struct w3d_ctx {
   double* d_trn_mat = nullptr;
   std::vector<BlockDesc> blocks;
   std::vector<MatDesc> mats;
+  // thermal expansion and segmental curves per material; the step's
+  // temperatures (nodes, node changes, elements, element changes) once set
+  std::vector<w3d::ThermDesc> therm;
+  w3d::ThermDesc* d_therm = nullptr;
+  bool curves = false;
+  double* d_temps = nullptr;
+  int temps_incr = 0;
   std::vector<GaussTable> tables;
   std::vector<int32_t> conn;      // 0-based nodes, block order
   std::vector<int32_t> elem_blk;  // block of each element (0-based)
@@ -237,6 +244,12 @@ struct w3d_ctx {
     M.asym = asym ? 1 : 0;
     M.trn_idx = d_trn_idx;
     M.trn_mat = d_trn_mat;
+    M.temps = d_temps;
+    M.therm = d_therm;
+    M.nonode = nonode;
+    M.noelem = noelem;
+    M.temps_incr = temps_incr;
+    M.curves = curves ? 1 : 0;
     return M;
   }
 };

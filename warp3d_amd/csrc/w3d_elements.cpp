@@ -167,6 +167,48 @@ static void derivs(int etype, const double p[3], double* nx, double* ny,
   std::memcpy(nz, d4, sizeof d4);
 }
 
+// Shape-function values at p (shapef.f:15: shape2 :223, shape1 :169,
+// shape6 :447), node numbering as derivs.
+static void shapes(int etype, const double p[3], double* sf) {
+  if (etype == W3D_L3DISOP) {
+    static const int nat[8][3] = {{-1, -1, 1}, {-1, -1, -1}, {-1, 1, -1},
+                                  {-1, 1, 1},  {1, -1, 1},   {1, -1, -1},
+                                  {1, 1, -1},  {1, 1, 1}};
+    for (int j = 0; j < 8; ++j)
+      sf[j] = (1.0 + nat[j][0] * p[0]) * (1.0 + nat[j][1] * p[1]) *
+              (1.0 + nat[j][2] * p[2]) * 0.125;
+    return;
+  }
+  if (etype == W3D_Q3DISOP) {
+    static const int nat[20][3] = {
+        {-1, -1, 1}, {-1, -1, -1}, {-1, 1, -1}, {-1, 1, 1}, {1, -1, 1},
+        {1, -1, -1}, {1, 1, -1},   {1, 1, 1},   {-1, -1, 0}, {-1, 0, -1},
+        {-1, 1, 0},  {-1, 0, 1},   {1, -1, 0},  {1, 0, -1}, {1, 1, 0},
+        {1, 0, 1},   {0, -1, 1},   {0, -1, -1}, {0, 1, -1}, {0, 1, 1}};
+    const double r = p[0], s = p[1], t = p[2];
+    for (int j = 0; j < 20; ++j) {
+      int ri = nat[j][0], si = nat[j][1], ti = nat[j][2];
+      double a = 1.0 + ri * r, b = 1.0 + si * s, c = 1.0 + ti * t;
+      if (ri != 0 && si != 0 && ti != 0)
+        sf[j] = 0.125 * a * b * c * (ri * r + si * s + ti * t - 2.0);
+      else if (ri == 0)
+        sf[j] = 0.25 * (1.0 - r * r) * b * c;
+      else if (si == 0)
+        sf[j] = 0.25 * (1.0 - s * s) * a * c;
+      else
+        sf[j] = 0.25 * (1.0 - t * t) * a * b;
+    }
+    return;
+  }
+  const double s2 = p[0], s3 = p[1], s4 = p[2], s1 = 1.0 - s2 - s3 - s4;
+  const double v[10] = {s1 * (2 * s1 - 1), s2 * (2 * s2 - 1),
+                        s3 * (2 * s3 - 1), s4 * (2 * s4 - 1),
+                        4 * s1 * s2,       4 * s2 * s3,
+                        4 * s1 * s3,       4 * s1 * s4,
+                        4 * s2 * s4,       4 * s3 * s4};
+  std::memcpy(sf, v, sizeof v);
+}
+
 // Least-squares solution x (4) of min |A x - b|, A (m x 4) column-major,
 // by Householder QR and back substitution (the steps of LAPACK's DGELS for
 // a full-rank A); for m < 4 the minimum-norm solution A^T (A A^T)^-1 b of
@@ -224,6 +266,7 @@ bool fill_gauss_table(int etype, int order, GaussTable* t) {
     double p[3];
     point(etype, order, k, p, &t->w[k]);
     derivs(etype, p, t->dn[0][k], t->dn[1][k], t->dn[2][k]);
+    shapes(etype, p, t->sf[k]);
     amat[k] = p[0];
     amat[g + k] = p[1];
     amat[2 * g + k] = p[2];

@@ -66,6 +66,18 @@ struct MatDesc {
   CpDesc cp;
 };
 
+// Thermal expansion and the mm01 segmental curve set of a material
+// (rknstr.f:150,1046-1053; set_up_segmental / set_up_h_prime).  curve_type
+// 0 none, 1 one temperature-independent curve, 2 temperature dependent.
+constexpr int kMaxCurves = 8;
+struct ThermDesc {
+  double alpha[6], alpha_zero;
+  int32_t curve_type, ncurves;
+  double curve_eps[2];
+  double temp[kMaxCurves], e[kMaxCurves], nu[kMaxCurves],
+      alpha_c[kMaxCurves], sig[kMaxCurves][2];
+};
+
 // One crystal of one element (setup_mm10_rknstr): stiffness rotated into
 // the reference frame ((i,j) at C[i + 6j]), Schmid tensors (Voigt,
 // engineering shear) and skew vectors of the 12 slip systems, the crystal
@@ -83,6 +95,7 @@ struct GaussTable {
   int32_t etype, order, ngp, nnode;
   double w[kMaxGp];
   double dn[3][kMaxGp][kMaxNodes];  // d/dxi, d/deta, d/dzeta
+  double sf[kMaxGp][kMaxNodes];     // shape functions (temperatures)
   // slopes of the least-squares linear fit c1 xi + c2 eta + c3 zeta + c4
   // of values at the points: c_d = sum_p lsq[d][p] v_p (d = 0..2)
   double lsq[3][kMaxGp];
@@ -132,6 +145,14 @@ struct DevModel {
   // (u_c = mat u_g)
   const int32_t* trn_idx;
   const double* trn_mat;
+  // temperatures of the load step: temper_nodes, dtemp_nodes (nonode
+  // each), temper_elems, dtemp_elems (noelem each), back to back; NULL
+  // when none were set (all zero)
+  const double* temps;
+  const ThermDesc* therm;  // per material
+  int32_t nonode, noelem;
+  int32_t temps_incr;      // the step imposes temperature changes
+  int32_t curves;          // some material has a segmental curve set
 };
 
 // Rotations of one node's 3-vector by its skewed-constraint matrix t

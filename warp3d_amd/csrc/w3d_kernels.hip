@@ -18,27 +18,124 @@
 namespace w3d {
 
 // ---------------------------------------------------------------------
-// mm01 (bilinear Mises, isothermal) + cnst1 for one Gauss point.
+// Material properties of one point: the block's (setup_mm01_rknstr,
+// rknstr.f:1018: REAL props widened, H' formed in REAL) unless the
+// material has a segmental curve set: set_up_segmental
+// (setup_mat_props.f:67) interpolates e, nu and the isotropic alpha at the
+// point's temperatures at n+1 and n and the curve stresses at n+1,
+// set_up_h_prime (:406) takes H' and the yield stress from the curve's
+// first two points.
+// ---------------------------------------------------------------------
+struct PointProps {
+  double e, nu, e_n, nu_n, hprime, yld;
+  double alpha[6], alpha_n[6];
+};
+
+// linear_interpolate (setup_mat_props.f:361): constant outside the table
+__device__ inline double lin_interp(double x, int n, const double* xs,
+                                    const double* ys, int ystride) {
+  if (x <= xs[0]) return ys[0];
+  if (x >= xs[n - 1]) return ys[(n - 1) * ystride];
+  int p = 1;
+  while (p < n - 1 && x > xs[p]) ++p;
+  const double x1 = xs[p - 1], x2 = xs[p], y1 = ys[(p - 1) * ystride],
+               y2 = ys[p * ystride];
+  return y1 + (x - x1) * (y2 - y1) / (x2 - x1);
+}
+
+__device__ inline void point_props(const MatDesc& md, const ThermDesc* td,
+                                   double t_n1, double dtemp,
+                                   PointProps& p) {
+  p.e = p.e_n = md.e;
+  p.nu = p.nu_n = md.nu;
+  p.hprime = md.hprime;
+  p.yld = md.yld;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) p.alpha[k] = p.alpha_n[k] = td->alpha[k];
+  if (td->curve_type == 0) return;
+  double s1 = td->sig[0][0], s2 = td->sig[0][1];
+  if (td->curve_type == 2) {
+    const int n = td->ncurves;
+    const double t_n = t_n1 - dtemp;
+    p.e = lin_interp(t_n1, n, td->temp, td->e, 1);
+    p.e_n = lin_interp(t_n, n, td->temp, td->e, 1);
+    p.nu = lin_interp(t_n1, n, td->temp, td->nu, 1);
+    p.nu_n = lin_interp(t_n, n, td->temp, td->nu, 1);
+    const double a1 = lin_interp(t_n1, n, td->temp, td->alpha_c, 1);
+    const double a0 = lin_interp(t_n, n, td->temp, td->alpha_c, 1);
+    p.alpha[0] = p.alpha[1] = p.alpha[2] = a1;
+    p.alpha_n[0] = p.alpha_n[1] = p.alpha_n[2] = a0;
+    s1 = lin_interp(t_n1, n, td->temp, &td->sig[0][0], 2);
+    s2 = lin_interp(t_n1, n, td->temp, &td->sig[0][1], 2);
+  }
+  p.hprime = (s2 - s1) / (td->curve_eps[1] - td->curve_eps[0]);
+  p.yld = s1;
+}
+
+// gauss_pt_temps (gp_temp_eps.f:17) of lane (element le of the item, point
+// g): the step's temperature change (zero unless the step imposes one) and
+// the temperature at n+1, from the element's node and element values
+// (gadtemps / gatemps, gatemps.f:63,117)
+template <int NN>
+__device__ inline void point_temps(const DevModel& M, const BlockDesc& bd,
+                                   const GaussTable* gt, int elem_in_blk,
+                                   int g, double& t_n1, double& dtemp) {
+  const int elem = bd.felem - 1 + elem_in_blk;  // 0-based
+  const double* tn = M.temps;
+  const double* dtn = M.temps + M.nonode;
+  const double te = M.temps[2 * M.nonode + elem];
+  const double dte = M.temps[2 * M.nonode + M.noelem + elem];
+  const int32_t* cn = M.conn + bd.conn_off + (int64_t)elem_in_blk * NN;
+  double a = 0.0, b = 0.0;
+#pragma unroll 4
+  for (int j = 0; j < NN; ++j) {
+    const int node = cn[j];
+    const double sf = gt->sf[g][j];
+    const double d = M.temps_incr ? dtn[node] + dte : 0.0;
+    a = a + sf * d;
+    b = b + sf * (tn[node] + te + d);
+  }
+  dtemp = M.temps_incr ? a : 0.0;
+  t_n1 = b;
+}
+
+// gp_temp_eps (gp_temp_eps.f:163): adds the (negative) thermal strain
+// increment to the displacement strain increment deps
+__device__ inline void thermal_strain(const double* alpha_n1,
+                                      const double* alpha_n,
+                                      double alpha_zero, double t_n1,
+                                      double dtemp, double* deps) {
+  const double tn = (t_n1 - dtemp) - alpha_zero, tn1 = t_n1 - alpha_zero;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const double th = alpha_n1[k] * tn1 - alpha_n[k] * tn;
+    deps[k] = deps[k] + (0.0 - th);
+  }
+}
+
+// ---------------------------------------------------------------------
+// mm01 (bilinear Mises) + cnst1 for one Gauss point.
 // cgn: 9 stress words at n (words 8, 9 may be reset at step 1); h: 11
-// history words at n.  The results go straight to the point's slots
-// (stride SG): stresses to sn1, history to hn1 (when store_hist), the
-// 21-word [D] to cp; sig receives stress words 1-8 for the caller.
-// Returns false on a negative plastic multiplier (die_abort in
-// mm01_general).
+// history words at n; pt: the point's moduli at n+1 and n, H' and yield
+// stress.  The results go straight to the point's slots (stride SG):
+// stresses to sn1, history to hn1 (when store_hist), the 21-word [D] to
+// cp; sig receives stress words 1-8 for the caller.  Returns false on a
+// negative plastic multiplier (die_abort in mm01_general).
 // ---------------------------------------------------------------------
 __device__ inline double state_word(int st) {
   return __longlong_as_double((long long)(unsigned)st);
 }
 
-__device__ bool mm01_point(int step, const MatDesc& md, double* cgn,
-                           const double* deps, double* h,
+__device__ bool mm01_point(int step, const MatDesc& md, const PointProps& pt,
+                           double* cgn, const double* deps, double* h,
                            double first_hist11, double* __restrict__ sn1,
                            double* __restrict__ hn1, double* __restrict__ cp,
                            int64_t SG, bool store_hist, double* sig) {
   const double zero = 0.0, one = 1.0, two = 2.0, three = 3.0, half = 0.5,
                six = 6.0, four = 4.0;
-  const double e = md.e, nu = md.nu, beta = md.beta, hprime = md.hprime,
-               yld = md.yld;
+  const double e = pt.e, nu = pt.nu, beta = md.beta, hprime = pt.hprime,
+               yld = pt.yld;
+  const double e_n = pt.e_n, nu_n = pt.nu_n;
   const double root3 = sqrt(3.0), root2 = sqrt(2.0);
   if (step == 1) {  // mm01_set_history
     double kn = yld / root3;
@@ -54,10 +151,10 @@ __device__ bool mm01_point(int step, const MatDesc& md, double* cgn,
   const double htol = 0.000001, yld_tol = 0.0000001;
   double deps_vol = deps[0] + deps[1] + deps[2];
   double eps_mean = deps_vol / three;
-  double g_n = e / two / (one + nu);
-  double een1 = (cgn[0] - nu * (cgn[1] + cgn[2])) / e;
-  double een2 = (cgn[1] - nu * (cgn[0] + cgn[2])) / e;
-  double een3 = (cgn[2] - nu * (cgn[0] + cgn[1])) / e;
+  double g_n = e_n / two / (one + nu_n);
+  double een1 = (cgn[0] - nu_n * (cgn[1] + cgn[2])) / e_n;
+  double een2 = (cgn[1] - nu_n * (cgn[0] + cgn[2])) / e_n;
+  double een3 = (cgn[2] - nu_n * (cgn[0] + cgn[1])) / e_n;
   double eps_vol_n1 = een1 + een2 + een3 + deps_vol;
   double eps_mean_n = (een1 + een2 + een3) / three;
   double shear_mod = e / (two * (one + nu));
@@ -470,6 +567,24 @@ __global__ void __launch_bounds__(256)
     double* cp = M.cep + 21 * bd.q_off + q;
 #pragma unroll
     for (int k = 0; k < 6; ++k) en1[k * SG] = en[k * SG] + deps[k];
+    // the point's temperatures and properties; the thermal strain
+    // increment comes off the increment the material sees (the stored
+    // strains above are the displacements')
+    const MatDesc md = M.mats[bd.mat];
+    PointProps pt;
+    pt.e = pt.e_n = md.e;
+    pt.nu = pt.nu_n = md.nu;
+    pt.hprime = md.hprime;
+    pt.yld = md.yld;
+    if (M.temps || M.curves) {
+      const ThermDesc* td = M.therm + bd.mat;
+      double gp_t = 0.0, gp_dt = 0.0;
+      if (M.temps) point_temps<NN>(M, bd, gt, wi.first + e, g, gp_t, gp_dt);
+      point_props(md, td, gp_t, gp_dt, pt);
+      if (M.temps && M.temps_incr)
+        thermal_strain(pt.alpha, pt.alpha_n, td->alpha_zero, gp_t, gp_dt,
+                       deps);
+    }
     if (MODE == 1) {
       double* ud = M.uddt + 6 * bd.q_off + q;
 #pragma unroll
@@ -478,7 +593,6 @@ __global__ void __launch_bounds__(256)
     double cgn[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) cgn[k] = sn[k * SG];
-    const MatDesc md = M.mats[bd.mat];
     if (MODE == 1) {
       // the material update is the mm10 kernel's
     } else if (nonlinear) {
@@ -492,12 +606,12 @@ __global__ void __launch_bounds__(256)
         f11 = M.hist_n[bd.hist_off + 10 * SG + g];  // history(1,11)
       }
       // rplstr scatters history only for iter > 0
-      if (!mm01_point(step, md, cgn, deps, h, f11, sn1, hn1, cp, SG,
+      if (!mm01_point(step, md, pt, cgn, deps, h, f11, sn1, hn1, cp, SG,
                       iter > 0, sig))
         atomicOr(&M.flags[0], 1);
     } else {
       // drive_01_update_a / _b: linear-elastic [D]
-      const double one = 1.0, two = 2.0, e_ = md.e, nu = md.nu;
+      const double one = 1.0, two = 2.0, e_ = pt.e, nu = pt.nu;
       double c1 = (e_ / ((one + nu) * (one - two * nu)));
       double c2 = (one - nu) * c1;
       double c3 = ((one - two * nu) / two) * c1;

@@ -127,6 +127,17 @@ class BlockModel:
         _chk(lib().w3d_set_time_increment(self.h, dt),
              "w3d_set_time_increment")
 
+    def set_temperatures(self, temper_nodes=None, dtemp_nodes=None,
+                         temper_elems=None, dtemp_elems=None,
+                         incremental=True):
+        """The load step's temperatures (totals at n and the step's
+        changes, nodal and element parts; None = zero)."""
+        a = [None if v is None else np.ascontiguousarray(v, float)
+             for v in (temper_nodes, dtemp_nodes, temper_elems, dtemp_elems)]
+        _chk(lib().w3d_set_temperatures(
+            self.h, *[dptr(v) if v is not None else None for v in a],
+            int(bool(incremental))), "w3d_set_temperatures")
+
     def block_info(self, blk):
         info = np.zeros(5, np.int32)
         _chk(lib().w3d_block_info(self.h, blk, iptr(info)), "block_info")
