@@ -414,6 +414,23 @@ int w3d_solve_pcg_dev(w3d_ctx* ctx, const double* d_k_diag,
                       double* d_x, double rtol, int32_t maxit,
                       int32_t* iters, double* relres);
 
+/*
+ * w3d_solve_pcg_dev for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * It has no counterpart in the reference.  Same arguments and results as
+ * w3d_solve_pcg_dev, but its work vectors live in a static device array of
+ * the library (4 * 2^19 + 1032 doubles, about 16.8 MB per device, held
+ * until the process ends) instead of the context's pattern-time workspace;
+ * it calls no allocator.  Limit: at most 2^19 = 524,288 equations
+ * (W3D_ERR_ARG beyond).  Every call on one device shares that array:
+ * calls that overlap on one device (from several threads or contexts)
+ * race and must be serialised by the caller.
+ */
+int w3d_solve_pcg_static(w3d_ctx* ctx, const double* d_k_diag,
+                         const double* d_k_coeffs, const double* d_rhs,
+                         double* d_x, double rtol, int32_t maxit,
+                         int32_t* iters, double* relres);
+
 /* n+1 -> n roll-over of stresses, strains and history (update.f:15). */
 int w3d_commit_step(w3d_ctx* ctx);
 /* Discard the n+1 state (adaptive step restart): n+1 <- n. */

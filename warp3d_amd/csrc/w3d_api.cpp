@@ -868,17 +868,19 @@ int recover_impl(w3d_ctx* c, int step, int iter, int extrapolated,
 }
 
 // Jacobi-PCG on an assembled matrix (diag, coef device arrays); b and x
-// device arrays of neqns (x: initial guess in, solution out).
+// device arrays of neqns (x: initial guess in, solution out).  work: the
+// vectors r, z, p, q (neqns each) back to back; sc: 8 scalars followed by
+// solver_npart(neqns) partial sums.
 int pcg_impl(w3d_ctx* c, const double* d_diag, const double* d_coef,
              const double* d_b, double* d_x, double rtol, int maxit,
-             int32_t* iters, double* relres) {
+             int32_t* iters, double* relres, double* work, double* sc) {
   const int n = c->neqns;
   hipStream_t s = c->stream;
-  double* r = c->d_pcg;
+  double* r = work;
   double* z = r + n;
   double* p = z + n;
   double* q = p + n;
-  double* sc = q + 3 * (int64_t)n;  // rz_old, rz, pq, rr, bb
+  // sc: rz_old, rz, pq, rr, bb
   double* part = sc + 8;
   int rc = solver_csr_values(d_diag, d_coef, n, c->d_csr_vidx, c->csr_nnz,
                              c->d_csr_val, s);
@@ -2011,7 +2013,30 @@ int w3d_solve_pcg_dev(w3d_ctx* c, const double* d_k_diag,
     return fail(W3D_ERR_ARG, "w3d_solve_pcg: rtol > 0 and maxit >= 1");
   HIPCHK(hipSetDevice(c->device));
   return pcg_impl(c, d_k_diag, d_k_coeffs, d_rhs, d_x, rtol, maxit, iters,
-                  relres);
+                  relres, c->d_pcg, c->d_pcg + 6 * (int64_t)c->neqns);
+}
+
+int w3d_solve_pcg_static(w3d_ctx* c, const double* d_k_diag,
+                         const double* d_k_coeffs, const double* d_rhs,
+                         double* d_x, double rtol, int32_t maxit,
+                         int32_t* iters, double* relres) {
+  if (!c || !c->d_csr_rp)
+    return fail(W3D_ERR_ARG, "w3d_solve_pcg_static: no pattern");
+  if (!d_k_diag || !d_rhs || !d_x || (!d_k_coeffs && c->ncoeff))
+    return fail(W3D_ERR_ARG, "w3d_solve_pcg_static: null argument");
+  if (!(rtol > 0.0) || maxit < 1)
+    return fail(W3D_ERR_ARG, "w3d_solve_pcg_static: rtol > 0 and maxit >= 1");
+  if (c->neqns > w3d::kStaticMaxEq)
+    return fail(W3D_ERR_ARG,
+                "w3d_solve_pcg_static: " + std::to_string(c->neqns) +
+                    " equations, more than the static work memory's " +
+                    std::to_string(w3d::kStaticMaxEq));
+  HIPCHK(hipSetDevice(c->device));
+  double* work = nullptr;
+  int rc = w3d::solver_static_scratch(c->device, &work);
+  if (rc) return fail(rc, "w3d_solve_pcg_static: static work memory");
+  return pcg_impl(c, d_k_diag, d_k_coeffs, d_rhs, d_x, rtol, maxit, iters,
+                  relres, work, work + 4 * (int64_t)w3d::kStaticMaxEq);
 }
 
 int w3d_solve_pcg(w3d_ctx* c, const double* rhs, double* x, double rtol,
@@ -2032,7 +2057,7 @@ int w3d_solve_pcg(w3d_ctx* c, const double* rhs, double* x, double rtol,
   HIPCHK(hipMemcpyAsync(xd, x, n * sizeof(double), hipMemcpyHostToDevice,
                         c->stream));
   int rc = pcg_impl(c, c->d_kout, c->d_kout + n, b, xd, rtol, maxit, iters,
-                    relres);
+                    relres, c->d_pcg, c->d_pcg + 6 * (int64_t)n);
   if (rc < 0 && rc != W3D_ERR_NUMERIC) return rc;
   HIPCHK(hipMemcpyAsync(x, xd, n * sizeof(double), hipMemcpyDeviceToHost,
                         c->stream));

@@ -263,6 +263,12 @@ int solver_csr_values(const double* diag, const double* coef, int neq,
 int solver_spmv(int n, const int64_t* rp, const int32_t* col,
                 const double* val, const double* x, double* y, hipStream_t s);
 int solver_npart(int n);
+// w3d_solve_pcg_static's work memory: a file-scope device array of
+// 4 kStaticMaxEq + 8 + 1024 doubles, its address on `device` (the caller
+// has made that device current)
+constexpr int kStaticMaxEq = 1 << 19;
+constexpr int kMaxDevices = 64;
+int solver_static_scratch(int device, double** out);
 int solver_dot(int n, const double* a, const double* b, double* part,
                double* out, hipStream_t s);
 int solver_jacobi(int n, const double* diag, const double* r, double* z,

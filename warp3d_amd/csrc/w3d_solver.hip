@@ -10,11 +10,39 @@
 // then one block over the partials), so a solve is bitwise repeatable.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "w3d_internal.h"
 
 namespace w3d {
 
 constexpr int kRedBlock = 256;
+
+// Work memory of w3d_solve_pcg_static, which allocates nothing: the vectors
+// r, z, p, q of up to kStaticMaxEq equations, then 8 scalars and the dot
+// products' partial sums (solver_npart <= 1024).  Each device holds its own
+// copy for the life of the process; concurrent calls on one device share
+// it.
+constexpr int64_t kStaticWords = 4LL * kStaticMaxEq + 8 + 1024;
+static __device__ __align__(256) double scratch[kStaticWords];
+
+// The device address of this device's scratch (hipGetSymbolAddress once
+// per device, with that device current; the host-side name is not a
+// device address).
+int solver_static_scratch(int device, double** out) {
+  static std::mutex mu;
+  static double* addr[kMaxDevices] = {};
+  if (device < 0 || device >= kMaxDevices) return W3D_ERR_ARG;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!addr[device]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+      return W3D_ERR_DEVICE;
+    addr[device] = static_cast<double*>(p);
+  }
+  *out = addr[device];
+  return W3D_OK;
+}
 
 // vals[k] = the assembled word vidx[k] (diagonal words first, then the
 // upper-triangle coefficients): the full CSR values of an assembly

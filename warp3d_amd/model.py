@@ -85,6 +85,8 @@ def lib():
                                         C.c_int32, C.POINTER(C.c_int32),
                                         C.POINTER(C.c_double)]
         L.w3d_solve_pcg_dev.restype = C.c_int32
+        L.w3d_solve_pcg_static.argtypes = L.w3d_solve_pcg_dev.argtypes
+        L.w3d_solve_pcg_static.restype = C.c_int32
         _LIB = L
     return _LIB
 
@@ -345,6 +347,17 @@ class BlockModel:
         _chk(lib().w3d_solve_pcg_dev(self.h, d_kdiag, d_kcoeffs, d_rhs, d_x,
                                      rtol, maxit, C.byref(it), C.byref(rr)),
              "w3d_solve_pcg_dev")
+        return it.value, rr.value
+
+    def solve_pcg_static(self, d_kdiag, d_kcoeffs, d_rhs, d_x, rtol=1e-10,
+                         maxit=10000):
+        """solve_pcg_dev with the library's static work memory (no
+        device allocation)."""
+        it = C.c_int32()
+        rr = C.c_double()
+        _chk(lib().w3d_solve_pcg_static(self.h, d_kdiag, d_kcoeffs, d_rhs,
+                                        d_x, rtol, maxit, C.byref(it),
+                                        C.byref(rr)), "w3d_solve_pcg_static")
         return it.value, rr.value
 
     # -- state -----------------------------------------------------------
