@@ -80,7 +80,29 @@ static const int FCC_N[NSL][3] = {{1, 1, 1},  {1, 1, 1},   {1, 1, 1},
 /* small dense linear algebra (column-major, a(i,j) at a[i + n*j])      */
 /* ------------------------------------------------------------------ */
 /* DGESV: LU with partial pivoting (dgetf2 order), then dgetrs. */
+/* Work counters of the local solves (test and measurement instrument):
+   [0] crystal updates, [1] sub-step solves, [2] formR1 evaluations (also
+   inside formR), [3] formR, [4] formJ11 (also inside formJ), [5] formJ,
+   [6] 6x6 LU solves, [7] 7x7 LU solves; and a histogram of formR1
+   evaluations per point (bin 63 = 63 or more). */
+static long long g_cnt[8];
+static long long g_hist[64];
+static __thread long long t_evals;
+#define CNT(i) __atomic_fetch_add(&g_cnt[i], 1, __ATOMIC_RELAXED)
+void orc_mm10_counters(long long* out8, long long* hist64, int reset) {
+  for (int i = 0; i < 8; ++i) {
+    if (out8) out8[i] = __atomic_load_n(&g_cnt[i], __ATOMIC_RELAXED);
+    if (reset) __atomic_store_n(&g_cnt[i], 0, __ATOMIC_RELAXED);
+  }
+  for (int i = 0; i < 64; ++i) {
+    if (hist64) hist64[i] = __atomic_load_n(&g_hist[i], __ATOMIC_RELAXED);
+    if (reset) __atomic_store_n(&g_hist[i], 0, __ATOMIC_RELAXED);
+  }
+}
+
 static int lu_solve(int n, double* a, int nrhs, double* b) {
+  if (n == 6 && nrhs == 1) CNT(6);
+  if (n == 7) CNT(7);
   int ipiv[8];
   for (int j = 0; j < n; ++j) {
     int p = j;
@@ -414,6 +436,7 @@ typedef struct {
 
 static double sgn(double x) { return x >= 0.0 ? one : -one; }
 
+
 /* mm10_setup_np1 (mm10_a.f:1689); ed/ep keep their previous contents in the
    reference and are left untouched here too */
 static void setup_np1(const double* Rur, const double* dstrain, double dt,
@@ -528,6 +551,8 @@ static double h_voche(const cprops* p, const cstate* np1, const cstate* n,
 static void formR1(const cprops* p, const cstate* np1, const cstate* n,
                    const double* s, double tt, double* R1) {
   double dbarp[6], wp[3], symTW[6], w1[6], w2[6];
+  CNT(2);
+  ++t_evals;
   form_dbarp(p, np1, s, tt, dbarp);
   form_w(p, np1, s, tt, np1->qc, wp);
   symsw(s, wp, symTW);
@@ -541,6 +566,7 @@ static void formR1(const cprops* p, const cstate* np1, const cstate* n,
    hardening rate */
 static void formR(const cprops* p, cstate* np1, const cstate* n,
                   const double* x, double* R) {
+  CNT(3);
   formR1(p, np1, n, x, x[6], R);
   double h = h_voche(p, np1, n, x, x[6]);
   R[6] = x[6] - h;
@@ -551,6 +577,7 @@ static void formR(const cprops* p, cstate* np1, const cstate* n,
 static void formJ11(const cprops* p, const cstate* np1, const double* s,
                     double tt, double* J11) {
   double symtq[NSL][6], dgdt[NSL];
+  CNT(4);
   memset(J11, 0, sizeof(double) * 36);
   for (int i = 0; i < NSL; ++i) symsw(s, np1->qc[i], symtq[i]);
   for (int i = 0; i < NSL; ++i) { /* mm10_dgdt_voche */
@@ -600,6 +627,7 @@ static void formJ(const cprops* p, const cstate* np1, const double* x,
   const double* s = x;
   double tt = x[6];
   double J11[36];
+  CNT(5);
   formJ11(p, np1, s, tt, J11);
   /* J12 (mm10_b.f:281) */
   double J12[6] = {0, 0, 0, 0, 0, 0};
@@ -655,6 +683,7 @@ static int any_nan(const double* x, int n) {
 static void solve(const cprops* p, cstate* np1, const cstate* n,
                   double* stress, double* tt, int* fail, double dtinc,
                   double* J) {
+  CNT(1);
   const double c = 1.0e-4, red = 0.5;
   const int mls = 10, mmin = 1;
   double x[7], inR1;
@@ -1104,7 +1133,13 @@ int orc_mm10_gp(const w3d_material* mat, const orc_crystal* crys, int step,
     double J[49], pst[6];
     int fail = 0;
     memset(J, 0, sizeof J);
+    t_evals = 0;
     int no_load = solve_strup(&p, &np1, &n, &fail, iter0_off, J);
+    if (!(iter0_off || no_load)) {
+      CNT(0);
+      __atomic_fetch_add(&g_hist[t_evals < 63 ? t_evals : 63], 1,
+                         __ATOMIC_RELAXED);
+    }
     if (fail) return 1;
     if (!(iter0_off || no_load)) {
       tangent(&p, &np1, J);

@@ -90,6 +90,12 @@ void orc_mm10_residual(const w3d_material* mat, const orc_crystal* cr,
                        double dt, const double* x, double* R, double* J);
 void orc_mm10_rotation_matrix(const double angles[3], int angle_type,
                               double r[9]);
+/* Work counters of the mm10 local solves since the last reset: [0] crystal
+   updates, [1] sub-step solves, [2] formR1 evaluations (also those inside
+   formR), [3] formR, [4] formJ11 (also inside formJ), [5] formJ, [6] 6x6 and
+   [7] 7x7 LU solves; hist64: points by formR1 evaluations (bin 63: 63 or
+   more).  Either output may be NULL. */
+void orc_mm10_counters(long long* out8, long long* hist64, int reset);
 /* finite strains (orc_geonl.c); 3x3 matrices row-major */
 void orc_tcomp1(int nnode, const double gama[9], const double* nxi,
                 const double* neta, const double* nzeta, const double* ue,
