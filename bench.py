@@ -184,6 +184,9 @@ def run_device(arr, cst, du, steps, warmup, precision, asym=False):
     ifv = d_ifv.download()
     info = {"neqns": neq, "ncoeff": nco, "nnz": nnz,
             "ifv_norm": float(np.linalg.norm(ifv))}
+    if not asym:
+        info["k_coeffs"] = d_kc.download()
+        info["k_diag"] = d_kd.download()
     return (t1 - t0) * 1e3 / steps, phases, info
 
 
@@ -203,6 +206,33 @@ def contrib_counts(arr, cst):
     return total
 
 
+# FP64 flops of the mm10 local solves, per unit of work counted by the
+# oracle's instrumented restatement (orc_mm10_counters): the adds,
+# multiplies and divides of each routine as the reference forms them (an
+# FMA counts 2); pow is counted apart.  formR1: the 12 resolved shears and
+# slip increments (180), form_dbarp (180), form_wp (348), symsw (18), C w1
+# (66), the residual (30); formR adds the Voce sum (34); formJ11: the slips
+# (180) and per system dgdt, symsw, 2 sym(sigma q) + C ms and the rank-one
+# update (2,340 in all), form_wp (348), the spin and identity terms (24);
+# formJ adds J12 (1,680), J21 and J22 (360); the 6 x 6 and 7 x 7 LU
+# solves with J^T R (270, 400).
+MM10_FLOPS = {"formR1": 822, "formR_extra": 34, "formJ11": 2892,
+              "formJ_extra": 2040, "lu6": 270, "lu7": 400}
+MM10_POW = {"formR1": 12, "formR_extra": 1, "formJ11": 25,
+            "formJ_extra": 26}
+
+
+def mm10_work(cnt):
+    """(flops, pows) of the local solves from the oracle's counters."""
+    r1, r, j11, j, lu6, lu7 = cnt[2], cnt[3], cnt[4], cnt[5], cnt[6], cnt[7]
+    f = (r1 * MM10_FLOPS["formR1"] + r * MM10_FLOPS["formR_extra"] +
+         j11 * MM10_FLOPS["formJ11"] + j * MM10_FLOPS["formJ_extra"] +
+         lu6 * MM10_FLOPS["lu6"] + lu7 * MM10_FLOPS["lu7"])
+    pw = (r1 * MM10_POW["formR1"] + r * MM10_POW["formR_extra"] +
+          j11 * MM10_POW["formJ11"] + j * MM10_POW["formJ_extra"])
+    return f, pw
+
+
 def cpu_baseline(arr, du, ngp, budget_s=10.0):
     from oracle import oracle as O
     threads = len(os.sched_getaffinity(0))
@@ -214,21 +244,66 @@ def cpu_baseline(arr, du, ngp, budget_s=10.0):
     u = du.copy()
     om.recover(2, 0, u, du)
     om.tangent(2, 0)
+    import ctypes as C
+    L = O.lib()
+    L.orc_mm10_counters.argtypes = [C.POINTER(C.c_longlong),
+                                    C.POINTER(C.c_longlong), C.c_int]
+    cnt = (C.c_longlong * 8)()
+    L.orc_mm10_counters(cnt, None, 1)
     it = 0
     t0 = time.perf_counter()
     while True:
         it += 1
         om.recover(2, it, u, du)
         om.tangent(2, it)
-        if time.perf_counter() - t0 > budget_s and it >= 2:
+        if time.perf_counter() - t0 > budget_s and (it >= 2 or
+                                                    budget_s <= 0):
             break
     t1 = time.perf_counter()
-    return {"value": ngp * it / (t1 - t0), "unit": "GP updates/s",
-            "cores": threads, "kind": "port",
-            "sample": f"oracle (C, OpenMP over blocks of 128) recovery + "
-                      f"material update + element forces + FP64 tangent on "
-                      f"the full workload at load step 2, {it} Newton "
-                      f"iterations in {t1 - t0:.1f} s"}
+    L.orc_mm10_counters(cnt, None, 1)
+    out = {"value": ngp * it / (t1 - t0), "unit": "GP updates/s",
+           "cores": threads, "kind": "port",
+           "sample": f"oracle (C, OpenMP over blocks of 128) recovery + "
+                     f"material update + element forces + FP64 tangent on "
+                     f"the full workload at load step 2, {it} Newton "
+                     f"iterations in {t1 - t0:.1f} s"}
+    if cnt[0]:
+        out["mm10_counts_per_iteration"] = [c / it for c in cnt]
+    return out
+
+
+def c1_precision(tmp):
+    """The FP32 tangent's cost in Newton iterations: the C1 load history
+    (10^3 hex8, 20 steps to 2 %) through the Fortran driver with the FP64
+    and the FP32 tangent, both converged to a relative residual of 1e-7."""
+    from tests._util import c1_model
+    from warp3d_amd import _abi
+    from warp3d_amd import deck as D
+    from warp3d_amd import mesh as M
+    m, arr = c1_model(10)
+    cst, top, val = M.uniaxial_bc(m, 0.02 / 20)
+    du = np.zeros(3 * m.nonode)
+    du[top] = val
+    steps = np.array([du] * 20)
+    res = {}
+    for prec in (_abi.W3D_FP64, _abi.W3D_FP32):
+        deck = os.path.join(tmp, f"c1_{prec}.deck")
+        out = os.path.join(tmp, f"c1_{prec}.res")
+        D.write_deck(deck, arr, cst, steps, precision=prec, pcg_rtol=1e-13,
+                     res_tol=1e-5, extrapolate=False)
+        _, res[prec] = D.run(deck, out)
+    r64, r32 = res[_abi.W3D_FP64], res[_abi.W3D_FP32]
+    extra = [b["iters"] - a["iters"] for a, b in zip(r64, r32)]
+    du_rel = max(float(np.abs(b["u"] - a["u"]).max() / np.abs(a["u"]).max())
+                 for a, b in zip(r64, r32))
+    return {"config": "C1: 10^3 hex8 B-bar, mm01, 20 steps to 2 %, "
+                      "relative residual 1e-7",
+            "newton_iters_fp64": int(sum(a["iters"] for a in r64)),
+            "newton_iters_fp32": int(sum(b["iters"] for b in r32)),
+            "extra_iters_per_step_max": int(max(extra)),
+            "u_rel_diff_max": du_rel,
+            "all_converged": all(a["converged"] and b["converged"]
+                                 for a, b in zip(r64, r32))}
 
 
 def run_rve(args):
@@ -343,6 +418,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="hex8")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the hex20 line and the C1 Newton precision "
+                         "runs that the hex8 line carries")
     ap.add_argument("--asym", action="store_true",
                     help="time the asymmetric-assembly iteration (full FP64 "
                          "Ke and CSR rows) on the workload instead")
@@ -477,14 +555,108 @@ def main():
         "roofline": roof,
         "phases": phase_report,
     }
+    # the FP32 tangent against the FP64 one on this workload (assembled
+    # matrix, norm-wise), and the bound on the FP32/FP64 ratio: only the
+    # tangent (FP32 MFMA peak 157.3 against FP64 78.6 TF/s on MI355X) and
+    # the Ke words the assembly reads (4 against 8 bytes) change
+    i64 = res["fp64"][2]
+    kd_err = float(np.abs(info["k_diag"] - i64["k_diag"]).max() /
+                   np.abs(i64["k_diag"]).max())
+    kc_err = float(np.abs(info["k_coeffs"] - i64["k_coeffs"]).max() /
+                   np.abs(i64["k_coeffs"]).max())
+    out["fp32_tangent"] = {"k_diag_rel_err": kd_err,
+                           "k_coeffs_rel_err": kc_err}
+    out["speedup_bound_fp32_tangent"] = round(
+        precision_bound(ph64, alg, parts, m.nonode, ncontrib, nout, ninv), 3)
+    if args.workload == "hex8" and not args.no_extra:
+        out["fp32_tangent"].update(c1_precision_safe())
+        out["hex20"] = extra_workload("hex20", args)
+    mm = ph32.get("mm10")
     if not args.no_cpu:
         t = time.perf_counter()
         out["cpu_baseline"] = cpu_baseline(arr, du, ngp)
         log(f"cpu baseline done ({time.perf_counter() - t:.1f} s)")
+        cpi = out["cpu_baseline"].pop("mm10_counts_per_iteration", None)
+        if mm and mm[0] and cpi:
+            f, pw = mm10_work(cpi)
+            ms = mm[1] / mm[0]
+            out["mm10"] = {
+                "ms": round(ms, 3),
+                "fp64_flops_per_launch": round(f),
+                "pow_per_launch": round(pw),
+                "fp64_TFLOP/s": round(f / (ms * 1e-3) / 1e12, 3),
+                "frac_fp64_peak": round(f / (ms * 1e-3) / 1e12 /
+                                        FP64_PEAK_TFS, 4),
+                "local_solve_counts_per_point": {
+                    k: round(cpi[i] / cpi[0], 3) for i, k in enumerate(
+                        ("crystal_updates", "substeps", "formR1", "formR",
+                         "formJ11", "formJ", "lu6", "lu7")) if i},
+                "flops_per_point": round(f / cpi[0])}
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
     return 0
+
+
+def precision_bound(ph64, alg, parts, nonode, ncontrib, nout, ninv):
+    """ms64 over the best FP32 path: the tangent at the FP32/FP64 MFMA peak
+    ratio and the assembly at its 4-byte/8-byte Ke traffic ratio, the
+    other phases unchanged."""
+    tot = sum(v[1] / v[0] for v in ph64.values() if v[0])
+    t = {k: v[1] / v[0] for k, v in ph64.items() if v[0]}
+    a64 = algorithmic(parts, nonode, ncontrib, nout, ninv, 8)["assemble"][1]
+    a32 = algorithmic(parts, nonode, ncontrib, nout, ninv, 4)["assemble"][1]
+    best = (tot - t.get("tangent", 0) - t.get("assemble", 0) +
+            t.get("tangent", 0) * FP64_PEAK_TFS / FP32_MFMA_PEAK_TFS +
+            t.get("assemble", 0) * a32 / a64)
+    return tot / best
+
+
+def c1_precision_safe():
+    import tempfile
+    try:
+        with tempfile.TemporaryDirectory() as tmp:
+            return c1_precision(tmp)
+    except Exception as e:  # reported, not fatal to the line
+        return {"c1_error": str(e)[:300]}
+
+
+def extra_workload(name, args):
+    """Another workload of the same metric in the same run (FP64 and FP32
+    paths), for the line's extra key."""
+    from warp3d_amd import _abi
+    m, arr, cst, du, parts = build_workload(name)
+    ngp = sum(ne * g for ne, _, g, _ in parts)
+    r = {}
+    for prec, tag in ((_abi.W3D_FP64, "fp64"), (_abi.W3D_FP32, "fp32")):
+        ms, phases, info = run_device(arr, cst, du, args.steps, args.warmup,
+                                      prec)
+        r[tag] = (ms, phases, info)
+        log(f"{name} {tag}: {ms:.3f} ms/step")
+    ms32, ph32, info = r["fp32"]
+    ms64, ph64, _ = r["fp64"]
+    ninv = int(arr.incid.size)
+    ncontrib = contrib_counts(arr, cst)
+    nout = info["neqns"] + info["ncoeff"]
+    alg = algorithmic(parts, m.nonode, ncontrib, nout, ninv, 4)
+    tan = ph32.get("tangent")
+    out = {"workload": WORKLOADS[name], "gauss_points": ngp,
+           "value": round(ngp / (ms32 * 1e-3), 1),
+           "ms_per_step": round(ms32, 4),
+           "plain_fp64": {"value": round(ngp / (ms64 * 1e-3), 1),
+                          "ms_per_step": round(ms64, 4)},
+           "speedup_vs_fp64": round(ms64 / ms32, 3),
+           "speedup_bound_fp32_tangent": round(precision_bound(
+               ph64, alg, parts, m.nonode, ncontrib, nout, ninv), 3),
+           "k_coeffs_rel_err_fp32": float(
+               np.abs(info["k_coeffs"] - r["fp64"][2]["k_coeffs"]).max() /
+               np.abs(r["fp64"][2]["k_coeffs"]).max())}
+    if tan and tan[0]:
+        d = tan[1] / tan[0] * 1e-3
+        out["tangent_TFLOP/s"] = round(alg["tangent"][1] / d / 1e12, 2)
+        out["tangent_frac_fp32_mfma"] = round(
+            alg["tangent"][1] / d / 1e12 / FP32_MFMA_PEAK_TFS, 4)
+    return out
 
 
 if __name__ == "__main__":
