@@ -66,10 +66,13 @@ def lib():
         L.orc_mm10_gp.restype = C.c_int
         L.orc_mm10_gp.argtypes = [C.POINTER(W3DMaterial), C.POINTER(Crystal),
                                   C.c_int, C.c_int, C.c_double, _D, _D, _D,
-                                  _D, _D, C.c_int]
+                                  _D, _D, C.c_int, _D]
         L.orc_mm10_residual.argtypes = [C.POINTER(W3DMaterial),
                                         C.POINTER(Crystal), _D, _D,
-                                        C.c_double, _D, _D, _D]
+                                        C.c_double, _D, _D, _D, _D]
+        L.orc_mm10_calc_grads_geonl.restype = C.c_int
+        L.orc_mm10_calc_grads_geonl.argtypes = [C.c_int, C.c_int, C.c_int,
+                                                C.c_int, _D, _D, _D, _D]
         L.orc_mm10_rotation_matrix.argtypes = [_D, C.c_int, _D]
         L.orc_mm10_rt2rve.argtypes = [_D, _D]
         L.orc_mm10_rt2rvw.argtypes = [_D, _D]
@@ -197,30 +200,55 @@ def mm10_crystal(mat, angles):
 
 
 def mm10_gp(mat, crystals, step, iter0_off, dt, uddt, urcs_n, hist_n,
-            asym=False):
-    """One mm10 point: returns (rc, urcs_n1, hist_n1, hist_n as used)."""
+            asym=False, rn1=None):
+    """One mm10 point: returns (rc, urcs_n1, hist_n1, hist_n as used).
+    rn1: [R n+1] as a 3x3 array (finite strains), None for identity."""
     ncry = len(crystals)
     arr = (Crystal * ncry)(*crystals)
     hn = np.array(hist_n, dtype=np.float64)
     hn1 = np.zeros(87 + 71 * ncry)
     u1 = np.zeros(9)
+    rcm = (None if rn1 is None else
+           np.ascontiguousarray(np.asarray(rn1, float).ravel(order="F")))
     rc = lib().orc_mm10_gp(C.byref(mat), C.cast(arr, C.POINTER(Crystal)),
                            step, int(iter0_off), dt,
                            dptr(np.ascontiguousarray(uddt, dtype=float)),
                            dptr(np.ascontiguousarray(urcs_n, dtype=float)),
-                           dptr(u1), dptr(hn), dptr(hn1), int(asym))
+                           dptr(u1), dptr(hn), dptr(hn1), int(asym),
+                           None if rcm is None else dptr(rcm))
     return rc, u1, hn1, hn
 
 
-def mm10_residual(mat, crystal, hist_crystal, uddt, dt, x):
+def mm10_residual(mat, crystal, hist_crystal, uddt, dt, x, rn1=None):
+    """Residual (7) and Jacobian (7x7) of the local problem; rn1: [R n+1]
+    as a 3x3 array, None for identity."""
     R = np.zeros(7)
     J = np.zeros(49)
-    lib().orc_mm10_residual(C.byref(mat), C.byref(crystal),
-                            dptr(np.ascontiguousarray(hist_crystal, float)),
-                            dptr(np.ascontiguousarray(uddt, float)), dt,
-                            dptr(np.ascontiguousarray(x, float)), dptr(R),
-                            dptr(J))
+    hc = np.ascontiguousarray(hist_crystal, float)
+    ud = np.ascontiguousarray(uddt, float)
+    xx = np.ascontiguousarray(x, float)
+    rcm = (None if rn1 is None else
+           np.ascontiguousarray(np.asarray(rn1, float).ravel(order="F")))
+    lib().orc_mm10_residual(C.byref(mat), C.byref(crystal), dptr(hc),
+                            dptr(ud), dt, dptr(xx), dptr(R), dptr(J),
+                            None if rcm is None else dptr(rcm))
     return R, J.reshape(7, 7, order="F")
+
+
+def mm10_calc_grads_geonl(etype, order, hel, rot, jac):
+    """grad Fe words (27) of one element of a finite-strain block: hel
+    the element's n-history (ngp, hist), rot its [R n+1] (ngp, 3, 3), jac
+    its n = 0 Jacobian at the last point (3x3).  Returns (bad, words)."""
+    hel = np.ascontiguousarray(hel, float)
+    ngp, hist = hel.shape
+    rot = np.ascontiguousarray(
+        np.stack([np.asarray(r, float).ravel(order="F") for r in rot]))
+    jac = np.ascontiguousarray(np.asarray(jac, float).ravel())
+    out = np.zeros(27)
+    bad = lib().orc_mm10_calc_grads_geonl(etype, order, ngp, hist,
+                                          dptr(hel), dptr(rot), dptr(jac),
+                                          dptr(out))
+    return bad, out
 
 
 def mm10_rotation_matrix(angles, angle_type=1):

@@ -241,7 +241,6 @@ orc_model* orc_create(const w3d_model* m) {
            (mt->angle_type == 1 || mt->angle_type == 2) &&
            ncry_of(mt) <= (m->angle_stride > 0 ? m->angle_stride : 1) &&
            m->angles != NULL;
-    if (mt->model == W3D_MM10 && om->geonl[e]) ok = 0;
     if (!ok) {
       fail(W3D_ERR_UNSUPPORTED, "orc_create: material model");
       orc_destroy(om);
@@ -658,6 +657,7 @@ static int recover_block(orc_model* om, int b, int step, int iter,
   double* rloc = geonl ? (double*)malloc(sizeof(double) * S * G * 9) : NULL;
   double* dfn1 = geonl ? (double*)malloc(sizeof(double) * S * G) : NULL;
   double* dj0 = geonl ? (double*)malloc(sizeof(double) * S * G) : NULL;
+  double* jlast = geonl ? (double*)malloc(sizeof(double) * S * 9) : NULL;
   /* local n-state copies (gathered, then possibly reset at step 1) */
   double* hloc = (double*)malloc(sizeof(double) * S * G * H);
   double* sloc = (double*)malloc(sizeof(double) * S * G * 9);
@@ -706,6 +706,8 @@ static int recover_block(orc_model* om, int b, int step, int iter,
         double jac0[9], g0[9], th[9], fm[9], df, rn[9];
         orc_jacob1(N, t.nxi[g], t.neta[g], t.nzeta[g], ce + i * T, jac0,
                    &dj0[q], g0);
+        /* the Jacobian left in local_work%jac after the last point */
+        if (g == G - 1) memcpy(jlast + 9 * i, jac0, sizeof jac0);
         double un[MXDOF];
         for (int k = 0; k < T; ++k) un[k] = ue[i * T + k] + d[k];
         orc_tcomp1(N, g0, t.nxi[g], t.neta[g], t.nzeta[g], un, th);
@@ -760,9 +762,15 @@ static int recover_block(orc_model* om, int b, int step, int iter,
            after step 1 (rstgp1.f:4453-4457); history always scattered */
         int iter0_off = iter == 0 && (!extrapolated || step == 1);
         double* h1 = B->hist_n1 + q * H;
+        /* finite strains: [R n+1] of the point (rot_blk_n1 layout) */
+        double rcm[9];
+        if (geonl)
+          for (int a = 0; a < 3; ++a)
+            for (int c2 = 0; c2 < 3; ++c2)
+              rcm[a + 3 * c2] = rloc[9 * q + 3 * a + c2];
         if (orc_mm10_gp(mat, B->crys + (int64_t)i * B->ncry, step, iter0_off,
                         om->dt, deps, cgn, cgn1, hloc + q * H, h1,
-                        om->asym))
+                        om->asym, geonl ? rcm : NULL))
           cut = 1;
         /* [D] as stored for the tangent: history words 1-36 (6x6,
            column-major, drive_10_cnst) */
@@ -817,7 +825,7 @@ static int recover_block(orc_model* om, int b, int step, int iter,
   /* rknstr_finish_cp: grad Fe (history words 37-63) of every element of
      a single-crystal block from the gathered n-history as mm10 left it
      (re-initialised in place at step 1); zero with several crystals */
-  if (B->model == W3D_MM10 && !cut) {
+  if (B->model == W3D_MM10 && !cut && !geonl) {
     double gv[27];
     if (B->ncry == 1)
       orc_mm10_calc_grads(B->etype, B->order, G, S, H, hloc, gv);
@@ -826,6 +834,25 @@ static int recover_block(orc_model* om, int b, int step, int iter,
     for (int64_t q = 0; q < (int64_t)S * G; ++q)
       memcpy(B->hist_n1 + q * H + 36, gv, sizeof gv);
   }
+  /* finite strains: per element, Rp R^T fitted and taken to the current
+     axes with the element's last n = 0 Jacobian (mm10_calc_grads) */
+  if (B->model == W3D_MM10 && !cut && geonl)
+    for (int i = 0; i < S; ++i) {
+      double gv[27], rcm[9 * MXGP];
+      memset(gv, 0, sizeof gv);
+      if (B->ncry == 1) {
+        for (int g = 0; g < G; ++g)
+          for (int a = 0; a < 3; ++a)
+            for (int c2 = 0; c2 < 3; ++c2)
+              rcm[9 * g + a + 3 * c2] =
+                  rloc[9 * ((int64_t)i * G + g) + 3 * a + c2];
+        orc_mm10_calc_grads_geonl(B->etype, B->order, G, H,
+                                  hloc + (int64_t)i * G * H, rcm,
+                                  jlast + 9 * i, gv);
+      }
+      for (int g = 0; g < G; ++g)
+        memcpy(B->hist_n1 + ((int64_t)i * G + g) * H + 36, gv, sizeof gv);
+    }
   /* rknifv_std: element internal forces with the updated stresses (the
      coordinates at n+1 equal the initial ones in the small-displacement
      form, so the Jacobians above are reused) */
@@ -881,7 +908,7 @@ static int recover_block(orc_model* om, int b, int step, int iter,
         for (int c2 = 0; c2 < 3; ++c2)
           B->rot_n1[9 * q + a + 3 * c2] = rloc[9 * q + 3 * a + c2];
   free(ce); free(due); free(dj); free(gama); free(vol); free(hloc);
-  free(sloc); free(ue); free(rloc); free(dfn1); free(dj0);
+  free(sloc); free(ue); free(rloc); free(dfn1); free(dj0); free(jlast);
   if (cut == 2) /* det F <= 0: adaptive step reduction (rstgp1.f) */
     return step > 1 ? W3D_CUT_STEP : W3D_ERR_NUMERIC;
   *energy = blk_energy;

@@ -1073,9 +1073,10 @@ int orc_mm10_hist_size(int ncry) { return NCOMMON + NCRYS * ncry; }
 int orc_mm10_gp(const w3d_material* mat, const orc_crystal* crys, int step,
                 int iter0_off, double dt, const double* uddt,
                 const double* urcs_n, double* urcs_n1, double* hn,
-                double* hn1, int asym) {
+                double* hn1, int asym, const double* rn1) {
   const int ncry = mat->ncrystals > 0 ? mat->ncrystals : 1;
   static const double I3[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  const double* Rur = rn1 ? rn1 : I3;
   if (step == 1) { /* mm10_init_general_hist, _uout_hist, _slip_hist */
     for (int k = 0; k < 63; ++k) hn[k] = zero;
     for (int k = 0; k < 9; ++k) hn[63 + k] = I3[k];
@@ -1127,8 +1128,8 @@ int orc_mm10_gp(const w3d_material* mat, const orc_crystal* crys, int step,
     n.tt_rate = h[58];
     for (int k = 0; k < 6; ++k) n.ep[k] = h[59 + k];
     for (int k = 0; k < 6; ++k) n.ed[k] = h[65 + k];
-    /* mm10_setup_np1 with R_n+1 = I (small strain, drive_10_update) */
-    setup_np1(I3, uddt, dt, &np1);
+    /* mm10_setup_np1 with R n+1 (identity for small strains) */
+    setup_np1(Rur, uddt, dt, &np1);
     /* mm10_solve_crystal */
     double J[49], pst[6];
     int fail = 0;
@@ -1180,7 +1181,7 @@ int orc_mm10_gp(const w3d_material* mat, const orc_crystal* crys, int step,
   for (int k = 0; k < 6; ++k) urcs_n1[k] = sig_avg[k];
   for (int k = 0; k < 36; ++k) hn1[k] = tang_avg[k];
   for (int k = 36; k < 63; ++k) hn1[k] = zero;
-  for (int k = 0; k < 9; ++k) hn1[63 + k] = I3[k];
+  for (int k = 0; k < 9; ++k) hn1[63 + k] = Rur[k];
   for (int k = 0; k < NSL; ++k) hn1[75 + k] = hn[75 + k] + slip_avg[k];
   urcs_n1[6] = urcs_n[6] + t_work_inc;
   urcs_n1[7] = urcs_n[7] + p_work_inc;
@@ -1196,7 +1197,8 @@ int orc_mm10_gp(const w3d_material* mat, const orc_crystal* crys, int step,
    exported for the finite-difference pins of formJ. */
 void orc_mm10_residual(const w3d_material* mat, const orc_crystal* cr,
                        const double* hist_crystal, const double* uddt,
-                       double dt, const double* x, double* R, double* J) {
+                       double dt, const double* x, double* R, double* J,
+                       const double* rn1) {
   cprops p;
   p.cr = cr;
   p.rate_n = mat->harden_n;
@@ -1214,7 +1216,7 @@ void orc_mm10_residual(const w3d_material* mat, const orc_crystal* cr,
   for (int k = 0; k < 6; ++k) n.stress[k] = hist_crystal[k];
   for (int k = 0; k < 9; ++k) n.Rp[k] = hist_crystal[9 + k];
   n.tau_tilde = hist_crystal[42];
-  setup_np1(I3, uddt, dt, &np1);
+  setup_np1(rn1 ? rn1 : I3, uddt, dt, &np1);
   setup(&p, &np1, &n);
   double xx[7];
   memcpy(xx, x, sizeof xx);
@@ -1280,6 +1282,67 @@ void orc_lsq4(int m, const double* a_in, const double* b_in, double* x) {
    over the Gauss points (DGELS) give grads(a,b,1:3).  hloc is the block's
    n-history in the (hist, ngp, span) layout; grads_vec receives grads in
    Fortran order (a fastest). */
+int orc_mm10_calc_grads_geonl(int etype, int order, int ngp, int hist,
+                              const double* hel, const double* rot,
+                              const double jac[9], double* grads_vec) {
+  /* mm10_a_invert_33 of the element's Jacobian */
+  const double* a = jac;
+#define A3(i, j) a[3 * ((i) - 1) + (j) - 1]
+  double det = A3(1, 1) * A3(2, 2) * A3(3, 3) - A3(1, 1) * A3(2, 3) * A3(3, 2) -
+               A3(1, 2) * A3(2, 1) * A3(3, 3) + A3(1, 2) * A3(2, 3) * A3(3, 1) +
+               A3(1, 3) * A3(2, 1) * A3(3, 2) - A3(1, 3) * A3(2, 2) * A3(3, 1);
+  double ji[9]; /* jacinv, row-major */
+  int bad = fabs(det) <= 1.0e-10;
+  if (bad) {
+    for (int k = 0; k < 9; ++k) ji[k] = zero;
+  } else {
+    double cf[3][3];
+    cf[0][0] = +(A3(2, 2) * A3(3, 3) - A3(2, 3) * A3(3, 2));
+    cf[0][1] = -(A3(2, 1) * A3(3, 3) - A3(2, 3) * A3(3, 1));
+    cf[0][2] = +(A3(2, 1) * A3(3, 2) - A3(2, 2) * A3(3, 1));
+    cf[1][0] = -(A3(1, 2) * A3(3, 3) - A3(1, 3) * A3(3, 2));
+    cf[1][1] = +(A3(1, 1) * A3(3, 3) - A3(1, 3) * A3(3, 1));
+    cf[1][2] = -(A3(1, 1) * A3(3, 2) - A3(1, 2) * A3(3, 1));
+    cf[2][0] = +(A3(1, 2) * A3(2, 3) - A3(1, 3) * A3(2, 2));
+    cf[2][1] = -(A3(1, 1) * A3(2, 3) - A3(1, 3) * A3(2, 1));
+    cf[2][2] = +(A3(1, 1) * A3(2, 2) - A3(1, 2) * A3(2, 1));
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) ji[3 * i + j] = cf[j][i] / det;
+  }
+#undef A3
+  /* Rt3d(:, :, p) = Rp R^T (mm10_a_mult_type_3t), column-major */
+  double rt[9 * MXGP];
+  for (int p = 0; p < ngp; ++p) {
+    const double* b = hel + (int64_t)p * hist + NCOMMON + 9; /* Rp, crys 1 */
+    const double* c = rot + 9 * p;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        rt[9 * p + i + 3 * j] = M3(b, i, 0) * M3(c, j, 0) +
+                                M3(b, i, 1) * M3(c, j, 1) +
+                                M3(b, i, 2) * M3(c, j, 2);
+  }
+  double amat[4 * MXGP];
+  for (int p = 0; p < ngp; ++p) {
+    double xi, eta, zeta, w;
+    orc_getgpts(etype, order, p + 1, &xi, &eta, &zeta, &w);
+    amat[p] = xi;
+    amat[ngp + p] = eta;
+    amat[2 * ngp + p] = zeta;
+    amat[3 * ngp + p] = 1.0;
+  }
+  for (int ia = 0; ia < 3; ++ia)
+    for (int ib = 0; ib < 3; ++ib) {
+      double rhs[MXGP], x[4];
+      for (int p = 0; p < ngp; ++p) rhs[p] = rt[9 * p + ia + 3 * ib];
+      orc_lsq4(ngp, amat, rhs, x);
+      /* take to the current coordinates: jacinv (slopes) */
+      for (int d = 0; d < 3; ++d)
+        grads_vec[ia + 3 * ib + 9 * d] =
+            ji[3 * d] * x[0] + ji[3 * d + 1] * x[1] + ji[3 * d + 2] * x[2];
+    }
+  return bad;
+}
+
 void orc_mm10_calc_grads(int etype, int order, int ngp, int span, int hist,
                          const double* hloc, double* grads_vec) {
   double amat[4 * MXGP];

@@ -80,14 +80,17 @@ int orc_mm10_hist_size(int ncrystals);
    n-history (re-initialised in place at step 1), hn1 receives the n+1
    history, urcs 9 words.  Returns 0, or 1 when the local solve fails (the
    reference's material_cut_step).  asym: the tangent is not symmetrised
-   (asymmetric assembly, mm10_a.f:1165). */
+   (asymmetric assembly, mm10_a.f:1165).  rn1: the point's [R n+1] from
+   the polar decomposition (9 words, column-major like rot_blk_n1), NULL
+   for small strains (identity, drive_10_update rstgp1.f:4405-4417). */
 int orc_mm10_gp(const w3d_material* mat, const orc_crystal* crys, int step,
                 int iter0_off, double dt, const double* uddt,
                 const double* urcs_n, double* urcs_n1, double* hn,
-                double* hn1, int asym);
+                double* hn1, int asym, const double* rn1);
 void orc_mm10_residual(const w3d_material* mat, const orc_crystal* cr,
                        const double* hist_crystal, const double* uddt,
-                       double dt, const double* x, double* R, double* J);
+                       double dt, const double* x, double* R, double* J,
+                       const double* rn1);
 void orc_mm10_rotation_matrix(const double angles[3], int angle_type,
                               double r[9]);
 /* Work counters of the mm10 local solves since the last reset: [0] crystal
@@ -118,6 +121,16 @@ void orc_lsq4(int m, const double* a, const double* b, double* x);
    n-history hloc (hist, ngp, span). */
 void orc_mm10_calc_grads(int etype, int order, int ngp, int span, int hist,
                          const double* hloc, double* grads_vec);
+/* The same for one element of a finite-strain block (mm10_calc_grads'
+   geonl branch): the fitted components are those of Rp R^T at each point
+   (Rp of crystal 1 from the element's n-history hel (hist, ngp), R =
+   [R n+1] from rot (9 column-major words a point)), and the slopes are
+   taken to the current axes by the inverse of jac (the element's n = 0
+   coordinate Jacobian at its last point, row-major).  Returns 1 when jac
+   is singular (the reference leaves the inverse zero). */
+int orc_mm10_calc_grads_geonl(int etype, int order, int ngp, int hist,
+                              const double* hel, const double* rot,
+                              const double jac[9], double* grads_vec);
 void orc_mm10_rt2rve(const double* rt, double* rv);
 void orc_mm10_rt2rvw(const double* rt, double* rv);
 void orc_mm10_expmw(const double wv[3], double a[9]);

@@ -1,22 +1,26 @@
 """A subset of the reference's input language, enough to drive the
 library from the reference's own verification decks
 (verification/test82/voche_model/warp3d_voche.inp, the Voce crystal case;
-verification/test24/test_24, a hollow sphere with skewed constraints).
+verification/test24/test_24, a hollow sphere with skewed constraints;
+verification/test77/test_77, a 20-node brick cantilever at large
+rotations).
 
 Commands read (main_program.f:121-369 dispatches them in the reference):
   structure, crystal (incrystal.f: slip_type, elastic_type isotropic,
   e, nu, hardening voce, harden_n, voce_m, theta_0, tau_y, tau_v),
+  list (named lists for output, read and ignored),
   material ... properties cp (inmat.f: angle_convention, angle_type,
   n_crystals 1, crystal_input single, crystal_type, orientation_input
   single angles), number of nodes/elements, elements (inelem.f: type
-  l3disop, linear or nonlinear (finite strains), material, order 2x2x2,
-  bbar, center_output, short), material ... properties bilinear (inmat.f:
-  e, nu, yld_pt, tan_e, beta), coordinates, incidences, blocking
-  automatic [display] [size = N] (inelbk.f:208), loading <pattern> with
-  element face pressures or nodal loads force_x/y/z (inlod.f),
-  constraints (incon.f) with "plane x|y|z=0 symmetry", node lists with
-  u/v/w values and "transformation matrix <nodes> row_1 .. row_2 ..
-  row_3 .." (skewed constraint systems), loading <name> nonlinear with
+  l3disop or q3disop, one type per deck, linear or nonlinear (finite
+  strains), material, order 2x2x2, bbar, center_output, short), material
+  ... properties bilinear (inmat.f: e, nu, yld_pt, tan_e, beta),
+  coordinates, incidences, blocking automatic [display] [size = N]
+  (inelbk.f:208), loading <pattern> with element face pressures or nodal
+  loads force_x/y/z (inlod.f), constraints (incon.f) with "plane x|y|z=0
+  symmetry|fixed", node lists with u/v/w values and "transformation
+  matrix <nodes> row_1 .. row_2 .. row_3 .." (skewed constraint systems),
+  loading <name> nonlinear with
   "step a-b <pattern> <factor>", initial conditions temperatures (read and
   ignored: the build is isothermal), nonlinear analysis parameters
   (indypm.f: time step, maximum/minimum iterations, divergence check,
@@ -53,7 +57,9 @@ class Deck:
     nonode: int = 0
     noelem: int = 0
     coords: np.ndarray = None
-    incid: np.ndarray = None           # (noelem, 8) 1-based
+    incid: np.ndarray = None           # (noelem, nnode) 1-based
+    etype: int = _abi.W3D_L3DISOP
+    nnode: int = 8
     crystals: dict = field(default_factory=dict)
     materials: dict = field(default_factory=dict)
     elem_material: np.ndarray = None   # material name index per element
@@ -62,6 +68,7 @@ class Deck:
     patterns: dict = field(default_factory=dict)   # name -> [(e, face, p)]
     nodal: dict = field(default_factory=dict)      # name -> {node: f(3)}
     symmetry: list = field(default_factory=list)   # planes: 'x', 'y', 'z'
+    fixed_planes: list = field(default_factory=list)  # planes held in u, v, w
     trn: dict = field(default_factory=dict)        # node -> 3x3 rows
     fixed: dict = field(default_factory=dict)      # (node, dof) -> value
     steps: dict = field(default_factory=dict)      # step -> [(pat, fac)]
@@ -115,7 +122,8 @@ def read(path):
         i += 1
         # the parameter block ends at the next command
         if mode == "nlparams" and not low.startswith(
-                ("compute", "output", "stop", "loading", "constraints")):
+                ("compute", "output", "stop", "loading", "constraints",
+                 "list ")):
             dk.nonlinear[low] = tok
             continue
         if low.startswith("structure"):
@@ -147,7 +155,8 @@ def read(path):
             if dk.nonode and dk.coords is None:
                 dk.coords = np.zeros((dk.nonode, 3))
             if dk.noelem and dk.incid is None:
-                dk.incid = np.zeros((dk.noelem, 8), np.int32)
+                # sized for the largest element; trimmed at the end
+                dk.incid = np.zeros((dk.noelem, 20), np.int32)
                 dk.elem_material = np.zeros(dk.noelem, np.int32)
             continue
         if low == "elements":
@@ -199,6 +208,10 @@ def read(path):
             dk.outputs.append(low)
             mode = None
             continue
+        if low.startswith("list "):
+            # a named node or element list for output commands (inlist)
+            mode = None
+            continue
         if low == "stop":
             break
         # data lines of the current mode
@@ -208,8 +221,11 @@ def read(path):
                 raise DeckError(f"elements: {ln}")
             a, b, rest = int(m.group(1)), int(m.group(2)), m.group(3).split()
             opts = dict(zip(rest[0::1], rest[1:] + [""]))
-            if rest[:2] != ["type", "l3disop"]:
-                raise DeckError("only l3disop elements")
+            if rest[:2] not in (["type", "l3disop"], ["type", "q3disop"]):
+                raise DeckError("only l3disop and q3disop elements")
+            dk.etype = (_abi.W3D_L3DISOP if rest[1] == "l3disop"
+                        else _abi.W3D_Q3DISOP)
+            dk.nnode = 8 if rest[1] == "l3disop" else 20
             mname = rest[rest.index("material") + 1]
             dk.elem_material[a - 1:b] = mat_names[mname]
             dk.elem_opts = dict(order=rest[rest.index("order") + 1]
@@ -226,7 +242,9 @@ def read(path):
             continue
         if mode == "incidences":
             v = [int(x) for x in ln.split()]
-            dk.incid[v[0] - 1] = v[1:9]
+            if len(v) != dk.nnode + 1:
+                raise DeckError(f"incidences: {ln}")
+            dk.incid[v[0] - 1, :dk.nnode] = v[1:]
             continue
         if mode == "element_loads":
             v = low.split()
@@ -245,9 +263,15 @@ def read(path):
                 f["xyz".index(v[k][-1])] += float(v[k + 1])
             continue
         if mode == "constraints":
-            m = re.match(r"plane\s+([xyz])\s*=\s*0(\.0*)?\s+symmetry", low)
+            m = re.match(r"plane\s+([xyz])\s*=\s*0(\.0*)?\s+"
+                         r"(symmetry|fixed)$", low)
             if m:
-                dk.symmetry.append(m.group(1))
+                # symmetry: the plane's normal component held; fixed: all
+                # three components of the plane's nodes (incon.f)
+                if m.group(3) == "symmetry":
+                    dk.symmetry.append(m.group(1))
+                else:
+                    dk.fixed_planes.append(m.group(1))
                 continue
             if low.startswith("transformation matrix"):
                 v = low.split()[2:]
@@ -288,6 +312,8 @@ def read(path):
         if mode == "ignore":
             continue
         raise DeckError(f"unsupported line: {ln}")
+    if dk.incid is not None:
+        dk.incid = np.ascontiguousarray(dk.incid[:, :dk.nnode])
     return dk
 
 
@@ -296,7 +322,8 @@ def _bilinear_material(props):
     beta (1: isotropic hardening)."""
     d = _kv(props[1:])
     return _abi.mm01_material(float(d["e"]), float(d["nu"]),
-                              float(d["yld_pt"]), tan_e=float(d["tan_e"]),
+                              float(d["yld_pt"]),
+                              tan_e=float(d.get("tan_e", 0.0)),
                               beta=float(d.get("beta", 1.0)))
 
 
@@ -390,6 +417,9 @@ def build(dk):
     if dk.elem_opts.get("order") != "2x2x2":
         raise DeckError("2x2x2 integration")
     ne = dk.noelem
+    nn = dk.nnode
+    # 2x2x2 is order 1 of the 8-node brick and order 8 of the 20-node one
+    order = 1 if dk.etype == _abi.W3D_L3DISOP else 8
     eps_bbar = 0.0
     nl = dk.nonlinear
     newton = dict(res_tol=0.01, mxiter=5, mniter=1, extrapolate=True,
@@ -418,8 +448,8 @@ def build(dk):
         elif k.startswith("bbar stabilization factor"):
             eps_bbar = float(t[-1])
     arr = _abi.ModelArrays(
-        dk.coords, dk.incid.ravel(), np.arange(ne, dtype=np.int32) * 8 + 1,
-        np.full(ne, _abi.W3D_L3DISOP, np.int32), np.ones(ne, np.int32),
+        dk.coords, dk.incid.ravel(), np.arange(ne, dtype=np.int32) * nn + 1,
+        np.full(ne, dk.etype, np.int32), np.full(ne, order, np.int32),
         dk.elem_material, np.full(ne, 1 if dk.elem_opts.get("bbar") else 0,
                                   np.int32),
         [mat], auto_size=dk.auto_size, eps_bbar=eps_bbar,
@@ -431,6 +461,10 @@ def build(dk):
     for ax in dk.symmetry:
         c = "xyz".index(ax)
         cst[c::3][dk.coords[:, c] == 0.0] = 1
+    for ax in dk.fixed_planes:
+        on = dk.coords[:, "xyz".index(ax)] == 0.0
+        for c in range(3):
+            cst[c::3][on] = 1
     for (n, c), val in dk.fixed.items():
         if val != 0.0:
             raise DeckError("nonzero imposed displacements")
@@ -478,4 +512,28 @@ def center_stress(model, elem, auto_size_blocks, geonl=False):
                 out[g] = [c[0, 0], c[1, 1], c[2, 2], c[0, 1], c[1, 2],
                           c[0, 2]]
             return out.mean(axis=0)
+    raise DeckError(f"element {elem} not found")
+
+
+def point_strain(model, elem, gp, auto_size_blocks, ngp, geonl=False):
+    """The strains the reference prints for a Gauss point (ougts1.f:51-103):
+    eps_xx, eps_yy, eps_zz, gam_xy, gam_yz, gam_xz from the model's
+    n-state; with finite strains the accumulated unrotated strains d are
+    turned to the current spatial axes, D = R d R^T with [R n+1]
+    (ou_get_spatial_from_material, ougts1.f:306), shears kept as
+    engineering strains.  gp is 1-based."""
+    for b, (span, first) in enumerate(auto_size_blocks, start=1):
+        if first <= elem < first + span:
+            e = model.get_block_state(b, _abi.STRAIN_N).reshape(span, ngp, 6)
+            d = e[elem - first, gp - 1].copy()
+            if not geonl:
+                return d
+            r = model.get_block_state(b, _abi.ROT_N1).reshape(span, ngp, 9)
+            rr = r[elem - first, gp - 1].reshape(3, 3, order="F")
+            t = np.array([[d[0], 0.5 * d[3], 0.5 * d[5]],
+                          [0.5 * d[3], d[1], 0.5 * d[4]],
+                          [0.5 * d[5], 0.5 * d[4], d[2]]])
+            c = rr @ t @ rr.T
+            return np.array([c[0, 0], c[1, 1], c[2, 2], 2.0 * c[0, 1],
+                             2.0 * c[1, 2], 2.0 * c[0, 2]])
     raise DeckError(f"element {elem} not found")
