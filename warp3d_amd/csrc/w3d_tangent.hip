@@ -90,9 +90,18 @@ __device__ inline void column_xyz(T bx, T by, T bz, const T* vol, int j,
   }
 }
 
+// Waves per SIMD the register allocation aims at: four for the 8-node
+// brick's small-strain tangent, whose LDS (under 40 KB a workgroup of
+// four waves) lets four workgroups share a CU; two for the others, whose
+// LDS holds the CU to fewer.
+__host__ __device__ constexpr int tan_waves_per_eu(int nnode, bool geonl) {
+  return nnode == 8 && !geonl ? 4 : 2;
+}
+
 template <int NN, int G, bool BBAR, typename T, bool GEONL = false,
           bool TRN = false>
-__global__ void __launch_bounds__(64 * tan_mfma_waves(NN)) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ void __launch_bounds__(64 * tan_mfma_waves(NN))
+    __attribute__((amdgpu_waves_per_eu(tan_waves_per_eu(NN, GEONL))))
     k_tangent_mfma(DevModel M, const WorkItem* __restrict__ work,
                    T* __restrict__ ek_out) {
   constexpr int W = tan_mfma_waves(NN);   // waves per workgroup
@@ -111,11 +120,16 @@ __global__ void __launch_bounds__(64 * tan_mfma_waves(NN)) __attribute__((amdgpu
   __shared__ T s_dnt[F64 ? 1 : 3][G][NN];
   T(*s_dn)[G][NN] = F64 ? reinterpret_cast<T(*)[G][NN]>(s_dn64) : s_dnt;
   __shared__ double s_w[G];
-  __shared__ double s_ce[W][EB][3][NN];
   __shared__ double s_detw[W][EB][G];
   __shared__ T s_gam[W][EB][G][9];
   __shared__ T s_vol[W][BBAR ? EB : 1][3 * NN];
-  __shared__ T s_d[W][EB][G][21];
+  // the batch's scaled [D] words (S3 on) share each wave's slab with the
+  // nodal coordinates (S1-S2 only): one workgroup more per CU.  The slabs
+  // are per wave, as the waves run their stages independently.
+  constexpr size_t kDBytes = sizeof(T) * EB * G * 21;
+  constexpr size_t kCeBytes = sizeof(double) * EB * 3 * NN;
+  constexpr size_t kSlab = (kDBytes > kCeBytes ? kDBytes : kCeBytes) / 8;
+  __shared__ double s_dce[W][kSlab];
   __shared__ int s_node[W][EB][NN];
   __shared__ int s_inv[W][EB][NN];
   // finite strains: the Cauchy stresses of the batch's points, and the
@@ -129,6 +143,9 @@ __global__ void __launch_bounds__(64 * tan_mfma_waves(NN)) __attribute__((amdgpu
   const GaussTable* gt = M.tables + bd.table;
   const int tid = threadIdx.x;
   const int wv = tid >> 6;   // wave of this lane
+  // this wave's slab as [D] words and as coordinates
+  T(*s_dw)[G][21] = reinterpret_cast<T(*)[G][21]>(&s_dce[wv][0]);
+  double(*s_cew)[3][NN] = reinterpret_cast<double(*)[3][NN]>(&s_dce[wv][0]);
   const int l = tid & 63;    // lane in the wave
   const int q = l >> 4;      // point of the lane inside a 4-point group
   const int cres = l & 15;   // column residue inside a tile
@@ -139,6 +156,27 @@ __global__ void __launch_bounds__(64 * tan_mfma_waves(NN)) __attribute__((amdgpu
   const double* cep_b = M.cep + 21 * bd.q_off;
   // the point of this lane in the batch set-up: element pe, point pg
   const int pe = l / G, pgp = l % G;
+
+  // the lane's component in each tile's dof column
+  int cmp[NTL];
+#pragma unroll
+  for (int t = 0; t < NTL; ++t) cmp[t] = (16 * t + cres) % 3;
+  // small elements (at most 3 upper tiles): the word of each accumulator
+  // value in the ranked upper triangle, or -1 below the diagonal or past
+  // the last dof
+  constexpr bool kPreOff = NUP <= 3;
+  int soff[kPreOff ? 4 * NUP : 1];
+  if (kPreOff) {
+#pragma unroll
+    for (int u = 0; u < NUP; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int rw = 16 * tile_i(u) + MF::row(l, v);
+        const int cl = 16 * tile_j(u) + cres;
+        soff[kPreOff ? 4 * u + v : 0] =
+            (rw <= cl && cl < TD) ? ek_word(TD, rw, cl) : -1;
+      }
+  }
 
   for (int k = tid; k < 3 * G * NN; k += 64 * W) {
     const int d = k / (G * NN), rem = k % (G * NN);
@@ -177,16 +215,10 @@ __global__ void __launch_bounds__(64 * tan_mfma_waves(NN)) __attribute__((amdgpu
         for (int c = 0; c < 3; ++c) x[c] = x[c] + uu[c] + dd[c];
       }
 #pragma unroll
-      for (int c = 0; c < 3; ++c) s_ce[wv][e][c][j] = x[c];
+      for (int c = 0; c < 3; ++c) s_cew[e][c][j] = x[c];
     }
-    // this lane's point: the 21 words of its stored [D], coalesced over
-    // the batch (word k of point (i, g) at k S G + i G + g)
-    const bool pact = l < nb * G;
-    double pd[21];
-#pragma unroll
-    for (int k = 0; k < 21; ++k)
-      pd[k] = pact ? cep_b[k * SG + (int64_t)ie0 * G + l] : 0.0;
     wave_sync();
+    const bool pact = l < nb * G;   // the lane has a point of the batch
     // S2: node ranks; the Jacobian at every point (jacob1) and its inverse
     for (int k = l; k < nb * NN; k += 64) {
       const int e = k / NN, j = k % NN;
@@ -204,8 +236,8 @@ __global__ void __launch_bounds__(64 * tan_mfma_waves(NN)) __attribute__((amdgpu
       double j11 = 0, j12 = 0, j13 = 0, j21 = 0, j22 = 0, j23 = 0, j31 = 0,
              j32 = 0, j33 = 0;
       for (int j = 0; j < NN; ++j) {
-        const double x = s_ce[wv][e][0][j], y = s_ce[wv][e][1][j],
-                     z = s_ce[wv][e][2][j];
+        const double x = s_cew[e][0][j], y = s_cew[e][1][j],
+                     z = s_cew[e][2][j];
         const double a = s_dn64[0][g][j], b = s_dn64[1][g][j],
                      c = s_dn64[2][g][j];
         j11 = j11 + a * x; j12 = j12 + a * y; j13 = j13 + a * z;
@@ -216,19 +248,31 @@ __global__ void __launch_bounds__(64 * tan_mfma_waves(NN)) __attribute__((amdgpu
       const double c2 = j21 * j33 - j23 * j31;
       const double c3 = j21 * j32 - j22 * j31;
       const double dj = j11 * c1 - j12 * c2 + j13 * c3;
+      // one reciprocal for the nine cofactors (jacob1 divides each; the
+      // difference is one rounding of the inverse, far below the FP64
+      // tangent's tolerance)
+      const double rdj = 1.0 / dj;
       T* ga = s_gam[wv][e][g];
-      ga[0] = (T)(c1 / dj);
-      ga[3] = (T)(-c2 / dj);
-      ga[6] = (T)(c3 / dj);
-      ga[1] = (T)((j32 * j13 - j12 * j33) / dj);
-      ga[4] = (T)((j11 * j33 - j31 * j13) / dj);
-      ga[7] = (T)((j12 * j31 - j11 * j32) / dj);
-      ga[2] = (T)((j12 * j23 - j13 * j22) / dj);
-      ga[5] = (T)((j13 * j21 - j11 * j23) / dj);
-      ga[8] = (T)((j11 * j22 - j12 * j21) / dj);
+      ga[0] = (T)(c1 * rdj);
+      ga[3] = (T)(-c2 * rdj);
+      ga[6] = (T)(c3 * rdj);
+      ga[1] = (T)((j32 * j13 - j12 * j33) * rdj);
+      ga[4] = (T)((j11 * j33 - j31 * j13) * rdj);
+      ga[7] = (T)((j12 * j31 - j11 * j32) * rdj);
+      ga[2] = (T)((j12 * j23 - j13 * j22) * rdj);
+      ga[5] = (T)((j13 * j21 - j11 * j23) * rdj);
+      ga[8] = (T)((j11 * j22 - j12 * j21) * rdj);
       s_detw[wv][e][g] = dj;
     }
     wave_sync();
+    // this lane's point: the 21 words of its stored [D], coalesced over
+    // the batch (word k of point (i, g) at k S G + i G + g); issued after
+    // the Jacobians, which need many registers, and in flight during the
+    // B-bar sums; lanes past the batch read its last point (in bounds) and drop it
+    const int64_t pq = (int64_t)ie0 * G + (pact ? l : nb * G - 1);
+    double pd[21];
+#pragma unroll
+    for (int k = 0; k < 21; ++k) pd[k] = cep_b[k * SG + pq];
     // S3: B-bar volume terms (lanes = element x node); [D] scaled by
     // w |J| (extract_D_symmetric) and by 1/6 for tetrahedra (adjust_cnst)
     if (BBAR)
@@ -279,7 +323,7 @@ __global__ void __launch_bounds__(64 * tan_mfma_waves(NN)) __attribute__((amdgpu
           for (int c = 0; c <= r; ++c) {
             double v = dm[6 * r + c];
             if (tet) v = (1.0 / 6.0) * v;
-            s_d[wv][e][g][tri(r, c)] = (T)v;
+            s_dw[e][g][tri(r, c)] = (T)v;
           }
 #pragma unroll
         for (int k = 0; k < 6; ++k) s_cs[wv][e][g][k] = (T)cs[k];
@@ -288,7 +332,7 @@ __global__ void __launch_bounds__(64 * tan_mfma_waves(NN)) __attribute__((amdgpu
         for (int k = 0; k < 21; ++k) {
           double v = wdj * pd[k];
           if (tet) v = (1.0 / 6.0) * v;
-          s_d[wv][e][g][k] = (T)v;
+          s_dw[e][g][k] = (T)v;
         }
       }
     }
@@ -296,13 +340,13 @@ __global__ void __launch_bounds__(64 * tan_mfma_waves(NN)) __attribute__((amdgpu
     // S4: the elements of the batch one by one on the matrix cores
     for (int e = 0; e < nb; ++e) {
       const int ie = ie0 + e;
-      // the lane's dof column of every tile: node (local index), component
-      int jn[NTL], cc[NTL];
+      // the lane's dof column of every tile: node (local index); its
+      // component is the per-lane constant cmp[t]
+      int jn[NTL];
 #pragma unroll
       for (int t = 0; t < NTL; ++t) {
         const int dp = 16 * t + cres;
         jn[t] = dp < TD ? s_inv[wv][e][dp / 3] : -1;
-        cc[t] = dp % 3;
       }
       // the contraction, four points per k step
       v4 acc[NUP];
@@ -336,7 +380,7 @@ __global__ void __launch_bounds__(64 * tan_mfma_waves(NN)) __attribute__((amdgpu
             // a node without one)
             const int ti = M.trn_idx[s_node[wv][e][j]];
             const double* tm = M.trn_mat + 9 * (ti >= 0 ? ti : 0);
-            const int ct = cc[t];
+            const int ct = cmp[t];
             const T r0 = ti >= 0 ? (T)tm[ct] : (T)(ct == 0 ? 1 : 0);
             const T r1 = ti >= 0 ? (T)tm[ct + 3] : (T)(ct == 1 ? 1 : 0);
             const T r2 = ti >= 0 ? (T)tm[ct + 6] : (T)(ct == 2 ? 1 : 0);
@@ -346,29 +390,33 @@ __global__ void __launch_bounds__(64 * tan_mfma_waves(NN)) __attribute__((amdgpu
             c3 = r0 * bc[0][3] + r1 * bc[1][3] + r2 * bc[2][3];
             c4 = r0 * bc[0][4] + r1 * bc[1][4] + r2 * bc[2][4];
             c5 = r0 * bc[0][5] + r1 * bc[1][5] + r2 * bc[2][5];
-          } else if (cc[t] == 0) {
-            c0 = bx; c3 = by; c5 = bz;
-            if (BBAR) {
-              const T vx = s_vol[wv][e][3 * j];
-              c0 = alpha * bx + beta_b * vx;
-              c1 = c2 = beta_b * (vx - bx);
-            }
-          } else if (cc[t] == 1) {
-            c1 = by; c3 = bx; c4 = bz;
-            if (BBAR) {
-              const T vy = s_vol[wv][e][3 * j + 1];
-              c0 = c2 = beta_b * (vy - by);
-              c1 = alpha * by + beta_b * vy;
-            }
           } else {
-            c2 = bz; c4 = by; c5 = bx;
+            // the column of component cmp[t] (x: bx 0 0 by 0 bz, y: 0 by
+            // 0 bx bz 0, z: 0 0 bz 0 by bx), formed by selects on the
+            // lane's fixed component rather than by diverging branches
+            const bool x = cmp[t] == 0, y = cmp[t] == 1, z = cmp[t] == 2;
+            c3 = x ? by : (y ? bx : (T)0);
+            c4 = y ? bz : (z ? by : (T)0);
+            c5 = x ? bz : (z ? bx : (T)0);
+            const T bo = x ? bx : (y ? by : bz);
             if (BBAR) {
-              const T vz = s_vol[wv][e][3 * j + 2];
-              c0 = c1 = beta_b * (vz - bz);
-              c2 = alpha * bz + beta_b * vz;
+              // the volumetric rows: the mean dilatation term of the own
+              // gradient on the diagonal row, beta (v - b) on the other two
+              const T vo = s_vol[wv][e][3 * j + cmp[t]];
+              const T dg = alpha * bo + beta_b * vo;
+              const T of = beta_b * (vo - bo);
+              c0 = x ? dg : of;
+              c1 = y ? dg : of;
+              c2 = z ? dg : of;
+            } else {
+              c0 = x ? bo : (T)0;
+              c1 = y ? bo : (T)0;
+              c2 = z ? bo : (T)0;
             }
           }
-          const bool live = p < G && jn[t] >= 0;
+          // points past the rule (the last 4-point group of a rule whose
+          // size is not a multiple of 4) contribute nothing
+          const bool live = (G % 4 == 0) || p < G;
           col[t][0] = live ? c0 : (T)0;
           col[t][1] = live ? c1 : (T)0;
           col[t][2] = live ? c2 : (T)0;
@@ -376,7 +424,7 @@ __global__ void __launch_bounds__(64 * tan_mfma_waves(NN)) __attribute__((amdgpu
           col[t][4] = live ? c4 : (T)0;
           col[t][5] = live ? c5 : (T)0;
         }
-        const T* dm = s_d[wv][e][pp];
+        const T* dm = s_dw[e][pp];
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
           // BD(r, dof) = sum_m D(m, r) B(m, dof), bdbtgen order
@@ -456,8 +504,17 @@ __global__ void __launch_bounds__(64 * tan_mfma_waves(NN)) __attribute__((amdgpu
       // the upper triangle straight from the accumulators, ranked
       // row-major: each store writes four runs of 16 consecutive words
       T* out = ek_out + bd.ek_off + (int64_t)ie * (TD * (TD + 1) / 2);
-      // the store offsets depend on the lane only; formed here, per
-      // element, rather than held in registers across the element loop
+      if (kPreOff && !GEONL) {
+#pragma unroll
+        for (int u = 0; u < NUP; ++u)
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            if (soff[4 * u + v] >= 0) out[soff[4 * u + v]] = acc[u][v];
+        continue;
+      }
+      // larger elements: the store offsets depend on the lane only and
+      // are formed here, per element, rather than held in registers
+      // across the element loop
       int ls = l;
       asm volatile("" : "+v"(ls));
 #pragma unroll
