@@ -28,7 +28,15 @@ def _case(kind):
          "mixed": lambda: M.MixedMesh(
              [(M.box_hex8(2, jitter=0.1, seed=3), 1, 1),
               (M.box_hex20(2, jitter=0.05, seed=4), 8, 0),
-              (M.box_tet10(1, jitter=0.05, seed=5), 4, 0)])}[kind]()
+              (M.box_tet10(1, jitter=0.05, seed=5), 4, 0)]),
+         # the same types glued where they touch: CSR rows of the shared
+         # interface nodes gather Ke words of elements with T = 24, 60 and
+         # 30 (boundary nodes are not jittered, so the lattices meet)
+         "mixed_glued": lambda: M.MixedMesh(
+             [(M.box_hex8(2, jitter=0.1, seed=3), 1, 1),
+              (M.box_hex20(1, jitter=0.05, seed=4), 8, 0),
+              (M.box_tet10(1, jitter=0.05, seed=5), 4, 0)],
+             glue=True)}[kind]()
     return m, m.model(mat, auto_size=8), 0.003
 
 
@@ -52,7 +60,7 @@ def _drive(model, m, amp, cst=None):
 
 
 @pytest.mark.parametrize("kind", ["mm10-bbar", "mm10", "hex20", "tet10",
-                                  "mixed"])
+                                  "mixed", "mixed_glued"])
 def test_full_matrices_and_csr_match_the_oracle(kind):
     from warp3d_amd.model import BlockModel
     m, arr, amp = _case(kind)

@@ -139,13 +139,18 @@ def test_geonl_assembly_and_rigid_rotation(W):
     assert np.abs(f).max() < 1e-6
 
 
-def test_geonl_with_mm10_is_rejected(W):
+def test_geonl_with_asymmetric_assembly_is_rejected(W):
+    """The unsymmetrised crystal tangent is assembled only for small
+    strains: finite-strain blocks refuse asymmetric assembly (mm10 itself
+    runs with finite strains, test_gpu_mm10_geonl)."""
     m = M.box_hex8(1)
     mat = _abi.mm10_material(tau_y=50.0, tau_v=100.0, theta_0=250.0,
                              e=100e3, nu=0.3, mu=80e3, elastic="cubic")
     arr = m.model(mat, geonl=True, angles=np.zeros((1, 3)))
+    dev = W.BlockModel(arr)
     with pytest.raises(W.W3DError):
-        W.BlockModel(arr)
+        dev.set_asymmetric(True)
+    dev.close()
 
 
 def test_small_strain_finite_kinematics_within_their_conditioning(W):

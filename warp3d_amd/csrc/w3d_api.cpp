@@ -175,7 +175,7 @@ void free_ctx(w3d_ctx* c) {
   dfree(c->d_ekf); dfree(c->d_ekf_off);
   dfree(c->d_fa_eqdof); dfree(c->d_fa_ln); dfree(c->d_fa_invptr);
   dfree(c->d_fa_ebase); dfree(c->d_fa_atoff); dfree(c->d_fa_at);
-  dfree(c->d_crys); dfree(c->d_uddt); dfree(c->d_work_mm10); dfree(c->d_mm10_blks);
+  dfree(c->d_crys); dfree(c->d_uddt); dfree(c->d_rot_tmp); dfree(c->d_jac_tmp); dfree(c->d_work_mm10); dfree(c->d_mm10_blks);
   dfree(c->d_rot_n); dfree(c->d_rot_n1); dfree(c->d_u_t); dfree(c->d_du_t);
   dfree(c->d_counter);
   dfree(c->d_blk_flags); dfree(c->d_blk_counter);
@@ -366,10 +366,6 @@ int create_impl(w3d_ctx* c, const w3d_model* m) {
                                  : kMM10Common + kMM10Crystal * B.ncry;
     B.bbar = c->bbar[fe];
     B.geonl = c->geonl[fe];
-    if (B.geonl && B.model == W3D_MM10)
-      return fail(W3D_ERR_UNSUPPORTED,
-                  "w3d_create: finite strains with mm10 (block " +
-                      std::to_string(b + 1) + ")");
     B.table = table_of(B.etype, B.order);
     B.q_off = q;
     B.hist_off = h;
@@ -494,6 +490,13 @@ int create_impl(w3d_ctx* c, const w3d_model* m) {
   if (c->has_mm10) {
     if ((rc = upload(&c->d_crys, crys))) return rc;
     if ((rc = dalloc(&c->d_uddt, 6 * q))) return rc;
+    bool mm10_geonl = false;
+    for (const BlockDesc& B : c->blocks)
+      mm10_geonl = mm10_geonl || (B.model == W3D_MM10 && B.geonl);
+    if (mm10_geonl) {
+      if ((rc = dalloc(&c->d_rot_tmp, 9 * q))) return rc;
+      if ((rc = dalloc(&c->d_jac_tmp, 9 * (int64_t)c->noelem))) return rc;
+    }
     // one work item per 64 consecutive points of an mm10 block
     std::vector<WorkItem> mw;
     for (int b = 0; b < c->nelblk; ++b) {
@@ -1274,6 +1277,9 @@ int w3d_tangent(w3d_ctx* c, int32_t step, int32_t iter, int32_t precision) {
   return w3d_tangent_blocks(c, 1, c->nelblk, step, iter, precision);
 }
 
+static int finish_pattern(w3d_ctx* c, int32_t* neqns_out,
+                          int64_t* ncoeff_out, int64_t bytes0);
+
 int w3d_pattern(w3d_ctx* c, const int32_t* cstmap, int32_t* neqns_out,
                 int64_t* ncoeff_out) {
   if (!c || !cstmap) return fail(W3D_ERR_ARG, "w3d_pattern: null argument");
@@ -1323,6 +1329,12 @@ int w3d_pattern(w3d_ctx* c, const int32_t* cstmap, int32_t* neqns_out,
       std::memcpy(&c->k_indexes[c->row_start[r]], rows[r].data(),
                   rows[r].size() * sizeof(int32_t));
   rows.clear();
+  HIPCHK(hipSetDevice(c->device));
+  dfree(c->d_asm_hdr);
+  dfree(c->d_asm_src);
+  dfree(c->d_asm_wbase);
+  dfree(c->d_kout);
+  c->asm_ready = false;
   // assembly map (assem_by_row / assem_a_row, assemble_code.f:659,785).
   // Outputs: the diagonal of row r at t = r, coefficient k of the upper
   // triangle at t = neqns + k.  They are cut into segments: one per
@@ -1518,17 +1530,22 @@ int w3d_pattern(w3d_ctx* c, const int32_t* cstmap, int32_t* neqns_out,
     if (oo != nout)
       return fail(W3D_ERR_NUMERIC, "w3d_pattern: assembly map output count");
   }
-  HIPCHK(hipSetDevice(c->device));
-  dfree(c->d_asm_hdr);
-  dfree(c->d_asm_src);
-  dfree(c->d_asm_wbase);
-  dfree(c->d_kout);
   c->asm_nseg = nseg;
   int rc;
   if ((rc = upload(&c->d_asm_hdr, hdr))) return rc;
   if ((rc = upload(&c->d_asm_src, src))) return rc;
   if ((rc = upload(&c->d_asm_wbase, wbase))) return rc;
   if ((rc = dalloc(&c->d_kout, nout))) return rc;
+  c->asm_ready = true;
+  return finish_pattern(c, neqns_out, ncoeff_out, bytes0);
+}
+
+// The rest of w3d_pattern once the assembly map is built: the solver's
+// full symmetric CSR and the outputs.
+static int finish_pattern(w3d_ctx* c, int32_t* neqns_out,
+                          int64_t* ncoeff_out, int64_t bytes0) {
+  const int neq = c->neqns;
+  int rc;
   // full symmetric CSR for the solver: row j = the transposed upper entries
   // (i, j) of rows i < j (ascending i), the diagonal, then row j's upper
   // entries (ascending columns)
@@ -1820,7 +1837,7 @@ int w3d_get_block_ek_full(const w3d_ctx* c, int32_t blk, double* out,
 }
 
 int w3d_assemble_vss_dev(w3d_ctx* c, double* d_k_diag, double* d_k_coeffs) {
-  if (!c || !c->d_asm_hdr) return fail(W3D_ERR_ARG, "w3d_assemble: no pattern");
+  if (!c || !c->asm_ready) return fail(W3D_ERR_ARG, "w3d_assemble: no pattern");
   if (c->asym)
     return fail(W3D_ERR_ARG, "w3d_assemble_vss: asymmetric assembly is on");
   if (c->tangent_precision < 0)
@@ -1829,10 +1846,8 @@ int w3d_assemble_vss_dev(w3d_ctx* c, double* d_k_diag, double* d_k_coeffs) {
     return fail(W3D_ERR_ARG, "w3d_assemble_vss_dev: null output");
   HIPCHK(hipSetDevice(c->device));
   prof_begin(c, W3D_PHASE_ASSEMBLE);
-  int rc = launch_assemble(c->d_ek64, c->d_ek32,
-                           c->tangent_precision == W3D_FP32, c->d_asm_hdr,
-                           c->d_asm_src, c->d_asm_wbase, c->asm_nseg,
-                           c->neqns, d_k_diag, d_k_coeffs, c->stream);
+  int rc = c->enqueue_assemble(c->tangent_precision == W3D_FP32, d_k_diag,
+                               d_k_coeffs, c->stream);
   prof_end(c, W3D_PHASE_ASSEMBLE);
   if (rc) return fail(rc, "w3d_assemble: kernel launch failed");
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1841,18 +1856,15 @@ int w3d_assemble_vss_dev(w3d_ctx* c, double* d_k_diag, double* d_k_coeffs) {
 }
 
 int w3d_assemble_vss(w3d_ctx* c, double* k_diag, double* k_coeffs) {
-  if (!c || !c->d_asm_hdr) return fail(W3D_ERR_ARG, "w3d_assemble: no pattern");
+  if (!c || !c->asm_ready) return fail(W3D_ERR_ARG, "w3d_assemble: no pattern");
   if (c->asym)
     return fail(W3D_ERR_ARG, "w3d_assemble_vss: asymmetric assembly is on");
   if (c->tangent_precision < 0)
     return fail(W3D_ERR_ARG, "w3d_assemble: no tangent computed");
   HIPCHK(hipSetDevice(c->device));
   prof_begin(c, W3D_PHASE_ASSEMBLE);
-  int rc = launch_assemble(c->d_ek64, c->d_ek32,
-                           c->tangent_precision == W3D_FP32, c->d_asm_hdr,
-                           c->d_asm_src, c->d_asm_wbase, c->asm_nseg,
-                           c->neqns, c->d_kout, c->d_kout + c->neqns,
-                           c->stream);
+  int rc = c->enqueue_assemble(c->tangent_precision == W3D_FP32, c->d_kout,
+                               c->d_kout + c->neqns, c->stream);
   prof_end(c, W3D_PHASE_ASSEMBLE);
   if (rc) return fail(rc, "w3d_assemble: kernel launch failed");
   c->assembled += 1;
@@ -2155,7 +2167,7 @@ int w3d_iteration_dev(w3d_ctx* c, int32_t step, int32_t iter,
     return fail(W3D_ERR_ARG, "w3d_iteration_dev: step/iter");
   if (precision != W3D_FP64 && precision != W3D_FP32)
     return fail(W3D_ERR_ARG, "w3d_iteration_dev: precision");
-  if (!c->d_asm_hdr) return fail(W3D_ERR_ARG, "w3d_iteration_dev: no pattern");
+  if (!c->asm_ready) return fail(W3D_ERR_ARG, "w3d_iteration_dev: no pattern");
   if (!d_k_diag || (!d_k_coeffs && c->ncoeff))
     return fail(W3D_ERR_ARG, "w3d_iteration_dev: null matrix output");
   HIPCHK(hipSetDevice(c->device));
@@ -2189,10 +2201,8 @@ int w3d_iteration_dev(w3d_ctx* c, int32_t step, int32_t iter,
                                c->tan_start, c->tan_count, precision,
                                c->stream);
     if (!rc)
-      rc = launch_assemble(c->d_ek64, c->d_ek32, precision == W3D_FP32,
-                           c->d_asm_hdr, c->d_asm_src, c->d_asm_wbase,
-                           c->asm_nseg, c->neqns, d_k_diag, d_k_coeffs,
-                           c->stream);
+      rc = c->enqueue_assemble(precision == W3D_FP32, d_k_diag, d_k_coeffs,
+                               c->stream);
     c->capturing = false;
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(c->stream, &graph);

@@ -122,6 +122,8 @@ struct w3d_ctx {
   // mm10: crystal records, strain increments, per-point work items
   CrystalDev* d_crys = nullptr;
   double* d_uddt = nullptr;
+  double* d_rot_tmp = nullptr;  // mm10 with finite strains only
+  double* d_jac_tmp = nullptr;
   WorkItem* d_work_mm10 = nullptr;
   int n_work_mm10 = 0;
   int32_t* d_mm10_blks = nullptr;  // indices of the mm10 blocks
@@ -180,6 +182,13 @@ struct w3d_ctx {
   uint32_t* d_asm_src = nullptr;
   int64_t* d_asm_wbase = nullptr;
   int64_t asm_nseg = 0;
+  bool asm_ready = false;
+  // the symmetric assembly (assem_by_row) into (diag, coef) on stream s
+  int enqueue_assemble(int fp32, double* diag, double* coef,
+                       hipStream_t s) const {
+    return w3d::launch_assemble(d_ek64, d_ek32, fp32, d_asm_hdr, d_asm_src,
+                                d_asm_wbase, asm_nseg, neqns, diag, coef, s);
+  }
   double* d_kout = nullptr;  // neqns diagonal then ncoeff coefficients
   int64_t assembled = 0;     // serial of the last assembly
   // full symmetric CSR of the assembled matrix for the solver: row
@@ -236,6 +245,8 @@ struct w3d_ctx {
     M.flags = d_flags;
     M.crys = d_crys;
     M.uddt = d_uddt;
+    M.rot_tmp = d_rot_tmp;
+    M.jac_tmp = d_jac_tmp;
     M.eps_bbar = eps_bbar;
     M.dt = dt;
     M.rot_n1 = d_rot_n1;

@@ -130,6 +130,10 @@ struct DevModel {
   int32_t* flags;         // [0] fatal numeric, [1] cut step, [2] det J <= 0
   const CrystalDev* crys; // mm10 crystal records
   double* uddt;           // mm10: strain increments (6, points) per block
+  double* rot_tmp;        // mm10 finite strains: [R n+1] of this recovery
+                          // (9, points) per block, column-major
+  double* jac_tmp;        // mm10 finite strains: each element's n = 0
+                          // Jacobian at its last point (9, elements)
   double eps_bbar;
   double dt;              // time increment of the load step
   // finite strains: [R n+1] of every point (9 words, column-major, SoA
@@ -420,6 +424,19 @@ __host__ __device__ constexpr int tile_j(int t) {
 // Elements per workgroup of the recovery (lanes = element x point).
 __host__ __device__ constexpr int rec_elems(int ngp) {
   return 256 / ngp < 40 ? 256 / ngp : 40;
+}
+
+// MI355X dispatches workgroup b of a launch to XCD b % 8, and each XCD has
+// its own L2.  For a kernel whose neighbouring workgroups read the same
+// lines (the force gather's element vectors), workgroup b takes the
+// logical index that makes every XCD run one contiguous range of the
+// work: XCD x gets the x-th of eight nearly equal ranges, in order.  A
+// bijection on [0, nb) for any nb.
+constexpr int kNumXcd = 8;
+__device__ inline int64_t xcd_block(int64_t b, int64_t nb) {
+  const int64_t x = b % kNumXcd, i = b / kNumXcd;
+  const int64_t q = nb / kNumXcd, r = nb % kNumXcd;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
 }
 
 

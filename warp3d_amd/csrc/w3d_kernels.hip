@@ -345,11 +345,14 @@ struct BCol {
 // MODE 0: mm01 blocks, everything fused.  MODE 1: mm10 blocks, phases 1-3
 // up to the strain increment (left in uddt for the mm10 kernel).  MODE 2:
 // mm10 blocks, energy and element forces from the updated stresses.
-// GEONL (MODE 0, finite strains; rstgp1.f:19, gtmat1.f:17, rknifv.f:141):
-// the strain increment is formed with the mid-step configuration
-// X + u + du/2 and rotated to the unrotated frame by [R n+1/2]; [R n+1]
-// and det F n+1 come from F of the n = 0 configuration; the forces use the
-// n+1 configuration and the Cauchy stress [R n+1] t [R n+1]^T.
+// GEONL (finite strains; rstgp1.f:19, gtmat1.f:17, rknifv.f:141): the
+// strain increment is formed with the mid-step configuration X + u + du/2
+// and rotated to the unrotated frame by [R n+1/2]; [R n+1] and det F n+1
+// come from F of the n = 0 configuration; the forces use the n+1
+// configuration and the Cauchy stress [R n+1] t [R n+1]^T.  With mm10
+// (MODE 1) [R n+1] and the element's last n = 0 Jacobian are left for the
+// mm10 and grad Fe kernels; MODE 2 forms [R n+1] again from the same
+// inputs.
 // ---------------------------------------------------------------------
 template <int NN, int G, bool BBAR, int MODE, bool GEONL = false>
 __global__ void __launch_bounds__(256)
@@ -392,7 +395,8 @@ __global__ void __launch_bounds__(256)
     s_ce[e][j] = M.coords[3 * node];
     s_ce[e][NN + j] = M.coords[3 * node + 1];
     s_ce[e][2 * NN + j] = M.coords[3 * node + 2];
-    if (MODE != 2) {
+    // (the mm10 force pass needs the increments only for F n+1)
+    if (MODE != 2 || GEONL) {
       s_du[e][j] = du[3 * node];
       s_du[e][NN + j] = du[3 * node + 1];
       s_du[e][2 * NN + j] = du[3 * node + 2];
@@ -408,7 +412,7 @@ __global__ void __launch_bounds__(256)
     if (ti >= 0) {
       const double* t = M.trn_mat + 9 * ti;
       double v[3];
-      if (MODE != 2) {
+      if (MODE != 2 || GEONL) {
         v[0] = s_du[e][j]; v[1] = s_du[e][NN + j]; v[2] = s_du[e][2 * NN + j];
         trn_apply_t(t, v);
         s_du[e][j] = v[0]; s_du[e][NN + j] = v[1]; s_du[e][2 * NN + j] = v[2];
@@ -424,8 +428,9 @@ __global__ void __launch_bounds__(256)
 
   const int e = tid / G, g = tid % G;
   const bool active = e < cnt;
-  // jacob1 of lane (e, g) on the configuration X + a u + b du
-  auto jacobian = [&](double a_u, double b_du) {
+  // jacob1 of lane (e, g) on the configuration X + a u + b du; jout (when
+  // given) receives the Jacobian itself, row-major
+  auto jacobian = [&](double a_u, double b_du, double* jout) {
     double j11 = 0, j12 = 0, j13 = 0, j21 = 0, j22 = 0, j23 = 0, j31 = 0,
            j32 = 0, j33 = 0;
 #pragma unroll 2
@@ -440,6 +445,11 @@ __global__ void __launch_bounds__(256)
       j11 = j11 + a * x; j12 = j12 + a * y; j13 = j13 + a * z;
       j21 = j21 + b * x; j22 = j22 + b * y; j23 = j23 + b * z;
       j31 = j31 + c * x; j32 = j32 + c * y; j33 = j33 + c * z;
+    }
+    if (jout) {
+      jout[0] = j11; jout[1] = j12; jout[2] = j13;
+      jout[3] = j21; jout[4] = j22; jout[5] = j23;
+      jout[6] = j31; jout[7] = j32; jout[8] = j33;
     }
     double c1 = j22 * j33 - j23 * j32;
     double c2 = j21 * j33 - j23 * j31;
@@ -462,7 +472,15 @@ __global__ void __launch_bounds__(256)
   // the lane's point (gtmat1: F from the n = 0 Jacobian)
   double rn1[9], rnh[9], dfn1 = 0.0, dj0 = 0.0;
   if (GEONL && active) {
-    jacobian(0.0, 0.0);
+    double j0[9];
+    jacobian(0.0, 0.0, j0);
+    // mm10 finite strains: the Jacobian left in local_work%jac after the
+    // element's last point, for the grad Fe fit (mm10_calc_grads)
+    if (MODE == 1 && g == G - 1) {
+      double* jt = M.jac_tmp + 9 * ((int64_t)bd.felem - 1 + wi.first + e);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) jt[k] = j0[k];
+    }
     const double* ga = s_gam[e][g];
     dj0 = ga[9];
     double g0[9];
@@ -478,11 +496,20 @@ __global__ void __launch_bounds__(256)
     bad = bad || dfh <= 1.0e-20;
     geonl_rotation(f, rnh);
     if (bad) atomicOr(&M.flags[3], 1);
+    if (MODE == 1) {  // [R n+1] of the point for the mm10 kernel
+      const int64_t SG = (int64_t)bd.span * G;
+      const int64_t q = (int64_t)(wi.first + e) * G + g;
+      double* rr = M.rot_tmp + 9 * bd.q_off + q;
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) rr[(a + 3 * c) * SG] = rn1[3 * a + c];
+    }
   }
   if (GEONL) __syncthreads();  // the n = 0 Jacobians are replaced below
   // phase 1: coordinate Jacobian, determinant and inverse (jacob1), on the
   // mid-step configuration for finite strains (rknstr_geonl)
-  if (active) jacobian(GEONL ? 1.0 : 0.0, 0.5);
+  if (active) jacobian(GEONL ? 1.0 : 0.0, 0.5, nullptr);
   __syncthreads();
   // phase 2: mean-dilatation volume terms (vol_terms + vol_avg)
   auto vol_terms = [&]() {
@@ -639,7 +666,7 @@ __global__ void __launch_bounds__(256)
   __syncthreads();
   if (GEONL) {
     // rknifv_std: Jacobians and B-bar terms of the n+1 configuration
-    if (active) jacobian(1.0, 1.0);
+    if (active) jacobian(1.0, 1.0, nullptr);
     __syncthreads();
     if (BBAR) {
       vol_terms();
@@ -788,7 +815,8 @@ __global__ void k_ifv_gather(const double* __restrict__ einfvec,
                              const int64_t* __restrict__ inv_ptr,
                              const int64_t* __restrict__ inv_off, int nonode,
                              double* __restrict__ ifv) {
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = (int)xcd_block(blockIdx.x, gridDim.x) * blockDim.x +
+                threadIdx.x;
   if (n >= nonode) return;
   double fx = 0.0, fy = 0.0, fz = 0.0;
   for (int64_t p = inv_ptr[n]; p < inv_ptr[n + 1]; ++p) {
@@ -907,13 +935,20 @@ int launch_recover(const DevModel& M, const std::vector<BlockDesc>& hblocks,
     const bool geonl = groups[k] & 1;
     if (pass == 1 && !mm10) continue;
     const int mode = mm10 ? (pass == 0 ? 1 : 2) : 0;
-    if (geonl && mode != 0) return W3D_ERR_UNSUPPORTED;
     const WorkItem* w = d_work + gstart[k];
     bool done = false;
 #define X(NN, G, BB)                                                        \
   if (!done && (groups[k] >> 3) == (group_key(NN, G, BB) >> 3)) {          \
-    if (geonl)                                                              \
+    if (geonl && mode == 0)                                                 \
       hipLaunchKernelGGL((k_recover<NN, G, BB, 0, true>), dim3(gcount[k]),  \
+                         dim3(256), 0, s, M, w, step, iter, nonlinear, d_u,  \
+                         d_du, want_energy, en_wg, gstart[k]);              \
+    else if (geonl && mode == 1)                                            \
+      hipLaunchKernelGGL((k_recover<NN, G, BB, 1, true>), dim3(gcount[k]),  \
+                         dim3(256), 0, s, M, w, step, iter, nonlinear, d_u,  \
+                         d_du, want_energy, en_wg, gstart[k]);              \
+    else if (geonl)                                                         \
+      hipLaunchKernelGGL((k_recover<NN, G, BB, 2, true>), dim3(gcount[k]),  \
                          dim3(256), 0, s, M, w, step, iter, nonlinear, d_u,  \
                          d_du, want_energy, en_wg, gstart[k]);              \
     else if (mode == 0)                                                     \

@@ -2,8 +2,9 @@
 // (Voce hardening, 12 fcc or bcc slip systems), one thread per Gauss point.
 //
 // Reference routines restated (file:line under the reference's src/):
-//   rstgp1.f:4327 drive_10_update (small strain: R = I; the iteration-0
-//   "extrapolate off" linear-elastic update), mm10_a.f:29 mm10 (crystal
+//   rstgp1.f:4327 drive_10_update ([R n+1] of the polar decomposition for
+//   finite strains, R = I for small strains; the iteration-0 "extrapolate
+//   off" linear-elastic update), mm10_a.f:29 mm10 (crystal
 //   loop, averages, store), :480 mm10_init_cc_hist0, :1777/:1808/:2605
 //   step-1 history initialisation, :2493 mm10_copy_cc_hist, :1689
 //   mm10_setup_np1, :852 mm10_setup, :2086 mm10_setup_voche, :1102
@@ -41,8 +42,8 @@ struct Props {
 };
 
 // crystal_state of one crystal (the Voce subset).  The slip tensors
-// (mm10_setup's ms, qs; qc equals qs bitwise for R = I) depend only on the
-// plastic rotation at n, so every sub-step shares one copy.
+// (mm10_setup's ms, qc) depend only on the plastic rotation at n and R n+1,
+// so every sub-step shares one copy.
 // The rotated ms live in LDS, one column per lane (word w of lane t at
 // g[w * kLanes + t]), so that the local problem's registers stay free; qs
 // is formed where it is used from the crystal's reference vectors and the
@@ -55,9 +56,10 @@ struct CState {
   double tt_rate, tinc, dg;
   const double* g;           // 72 ms words, lane-strided
   const double (*q0)[3];     // reference qs of the 12 systems
-  double rw[9];              // column-major
+  double rw[9];              // RT2RVW(R Rp_n^T), column-major
   __device__ double ms(int i, int k) const { return g[(6 * i + k) * kLanes]; }
-  // rw q0[i] (mv3)
+  // the lattice-spin vector of system i: rw q0[i] (mv3); qc of the
+  // reference, or qs for a state whose rw is RT2RVW(Rp_n^T)
   __device__ void qs3(int i, double* q) const {
 #pragma unroll
     for (int k = 0; k < 3; ++k)
@@ -249,16 +251,39 @@ __device__ inline void setup(const Props& p, CState& s) {
   if (p.alter_mode) s.dg = p.eps_dot_0_y * s.tinc;
 }
 
-// mm10_setup: the slip tensors rotated by the plastic rotation at n
+// mm10_a_mult_type_1: a = b c (3x3 column-major), the reference's order
+__device__ inline void mul33(double* a, const double* b, const double* c) {
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      a[i + 3 * j] = b[i] * c[3 * j] + b[i + 3] * c[1 + 3 * j] +
+                     b[i + 6] * c[2 + 3 * j];
+}
+
+// mm10_setup: the slip tensors rotated by the plastic rotation at n.  The
+// rotation kept in the state (rw) is the one of qc, RT2RVW(R Rp_n^T)
+// (mm10_a.f:891-892); with small strains (rot null, R = I) R Rp_n^T is
+// Rp_n^T exactly and qc equals qs.  rot: the point's [R n+1], column-major
+// at rot[k * SG].
 __device__ inline void slip_geometry(const Props& p, const double* Rpn,
-                                     double* gl, CState& s) {
+                                     double* gl, CState& s,
+                                     const double* rot, int64_t SG) {
   double tn[9], RE[36];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j) tn[i + 3 * j] = Rpn[j + 3 * i];
   rt2rve(tn, RE);
-  rt2rvw(tn, s.rw);
+  if (rot) {
+    double R[9], w[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) R[k] = rot[k * SG];
+    mul33(w, R, tn);
+    rt2rvw(w, s.rw);
+  } else {
+    rt2rvw(tn, s.rw);
+  }
   s.q0 = p.cr->qs;
 #pragma unroll
   for (int i = 0; i < NS; ++i) {
@@ -300,7 +325,8 @@ __device__ inline void slips(const Props& p, const CState& s, const double* x,
   }
 }
 
-// form_wp / form_wbarp (qc == qs)
+// form_wp (with qc) and form_wbarp (with qs: a state whose rw is that
+// of Rp_n^T)
 __device__ inline void form_w(const Props& p, const CState& s,
                               const double* rs, const double* gam, double* w) {
   w[0] = w[1] = w[2] = 0.0;
@@ -692,12 +718,28 @@ __device__ void solve(const Props& p, CState& s, const double* sn,
   *tt = x[6];
 }
 
-// mm10_update_euler_angles (Kocks): angles of g Rp (R = I)
-__device__ inline void euler_angles(const Props& p, const double* Rp,
-                                    double* eul) {
+// mm10_update_euler_angles (Kocks): angles of g (Rp R^T); with small
+// strains (rot null, R = I) Rp R^T is Rp exactly
+__device__ inline void euler_angles(const Props& p, const double* Rp_in,
+                                    double* eul, const double* rot,
+                                    int64_t SG) {
   const double pi = 3.141592653589793, one_eighty = 180.0;
   const double* g = p.cr->g;
-  // g (Rp R^T) with R = I: Rp R^T is Rp exactly
+  double Rp[9];
+  if (rot) {  // mm10_a_mult_type_3t: Rp R^T
+    double R[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) R[k] = rot[k * SG];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        Rp[i + 3 * j] = Rp_in[i] * R[j] + Rp_in[i + 3] * R[j + 3] +
+                        Rp_in[i + 6] * R[j + 6];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Rp[k] = Rp_in[k];
+  }
   double full[9];
 #pragma unroll
   for (int j = 0; j < 3; ++j)
@@ -744,6 +786,8 @@ __device__ __forceinline__ void mm10_item(const DevModel& M,
   double* sn1 = M.stress_n1 + 9 * bd.q_off + q;
   double* cp = M.cep + 21 * bd.q_off + q;
   const double* ud = M.uddt + 6 * bd.q_off + q;
+  // finite strains: [R n+1] of the point (the recovery's), else R = I
+  const double* rot = bd.geonl ? M.rot_tmp + 9 * bd.q_off + q : nullptr;
   // (inputs and sums are read from and kept in memory where they are
   // needed, not held in registers across the local solves)
   const double dt = M.dt;
@@ -798,7 +842,7 @@ __device__ __forceinline__ void mm10_item(const DevModel& M,
     // np1 (mm10_setup_np1 + mm10_setup)
     double* gl = s_geo + threadIdx.x;
     CState np1;
-    slip_geometry(p, Rpn, gl, np1);
+    slip_geometry(p, Rpn, gl, np1, rot, SG);
 #pragma unroll
     for (int k = 0; k < 6; ++k) np1.D[k] = ud[k * SG];
     np1.tinc = dt;
@@ -938,7 +982,18 @@ __device__ __forceinline__ void mm10_item(const DevModel& M,
           Rpe[k] = step == 1 ? ((k % 4 == 0) ? 1.0 : 0.0)
                              : hn[cb + (9 + k) * SG];
         double wb[3];
-        form_w(p, np1, rs, gam, wb);
+        if (rot) {  // form_wbarp takes qs, of Rp_n alone
+          CState nq = np1;
+          double tn[9];
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) tn[i + 3 * j] = Rpe[j + 3 * i];
+          rt2rvw(tn, nq.rw);
+          form_w(p, nq, rs, gam, wb);
+        } else {
+          form_w(p, np1, rs, gam, wb);
+        }
         double W[9] = {0.0, -wb[2], -wb[1], wb[2], 0.0, -wb[0],
                        wb[1], wb[0], 0.0};  // column-major of wv2wt
         double alpha = sqrt(W[1 + 3 * 2] * W[1 + 3 * 2] +
@@ -977,7 +1032,7 @@ __device__ __forceinline__ void mm10_item(const DevModel& M,
       // ---- mm10_output
 #pragma unroll
       for (int i = 0; i < NS; ++i) slip[i] = gam[i];
-      euler_angles(p, Rp1, eul);
+      euler_angles(p, Rp1, eul, rot, SG);
       double dif[NS], t1, t2;
 #pragma unroll
       for (int i = 0; i < NS; ++i) dif[i] = rs[i] * np1.tinc * p.iD_v;
@@ -997,7 +1052,22 @@ __device__ __forceinline__ void mm10_item(const DevModel& M,
       work_inc = dot6(stress, np1.D);
 #pragma unroll
       for (int k = 0; k < 6; ++k) ee[k] = stress[k];
-      chol_solve6(p.cr->C, ee);  // R = I: the rotation of ee is exact
+      chol_solve6(p.cr->C, ee);
+      if (rot) {  // ee = RT2RVE(R) C^-1 sigma (exact for R = I)
+        double R[9], er[36], t[6];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) R[k] = rot[k * SG];
+        rt2rve(R, er);
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+          double a = er[i] * ee[0];
+#pragma unroll
+          for (int j = 1; j < 6; ++j) a = a + er[i + 6 * j] * ee[j];
+          t[i] = a;
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) ee[k] = t[k];
+      }
       double dbarp[6] = {0, 0, 0, 0, 0, 0}, wp[3], ewwe[6], ep[6];
 #pragma unroll
       for (int i = 0; i < NS; ++i)
@@ -1134,7 +1204,8 @@ __device__ __forceinline__ void mm10_item(const DevModel& M,
 #pragma unroll
   for (int k = 36; k < 63; ++k) hn1[k * SG] = 0.0;
 #pragma unroll
-  for (int k = 0; k < 9; ++k) hn1[(63 + k) * SG] = (k % 4 == 0) ? 1.0 : 0.0;
+  for (int k = 0; k < 9; ++k)
+    hn1[(63 + k) * SG] = rot ? rot[k * SG] : ((k % 4 == 0) ? 1.0 : 0.0);
   double w72 = step == 1 ? 0.0 : hn[72 * SG];
   double w73 = step == 1 ? 0.0 : hn[73 * SG];
   double w74 = step == 1 ? 0.0 : hn[74 * SG];
@@ -1155,8 +1226,9 @@ __device__ __forceinline__ void mm10_item(const DevModel& M,
 }
 
 // ---------------------------------------------------------------------
-// grad Fe of small-strain mm10 blocks: rknstr_finish_cp (rknstr.f) with
-// mm10_calc_grads (mm10_a.f:1579).  For single-crystal elements the
+// grad Fe of mm10 blocks: rknstr_finish_cp (rknstr.f) with
+// mm10_calc_grads (mm10_a.f:1579).  Small strains: for single-crystal
+// elements the
 // reference fits each component of Rt3d(3,3,ngp) linearly over the Gauss
 // points (DGELS) and stores the three slopes of every component, the same
 // 27 words at every point; Rt3d is a flat copy (mm10_a_copy_vector) of
@@ -1164,7 +1236,8 @@ __device__ __forceinline__ void mm10_item(const DevModel& M,
 // elem_hist(span, hist, ngp) from (element 1, first Rp word of crystal 1,
 // point 1), so words of several elements of the block enter it.  At step
 // 1 those words are the values mm10 initialised in place.  Elements with
-// several crystals get zeros.  One 64-lane workgroup per mm10 block.
+// several crystals get zeros.  Finite strains: per element (below).  One
+// 64-lane workgroup per mm10 block.
 // ---------------------------------------------------------------------
 __device__ double mm10_init_word(const DevModel& M, const BlockDesc& bd,
                                  int w, int i, int64_t q, int64_t SG) {
@@ -1217,8 +1290,77 @@ __global__ void __launch_bounds__(64)
   }
   __syncthreads();
   double* hn1 = M.hist_n1 + bd.hist_off;
-  for (int64_t k = l; k < 27 * SG; k += 64)
-    hn1[(36 + k / SG) * SG + k % SG] = s_g[k / SG];
+  if (!bd.geonl) {
+    for (int64_t k = l; k < 27 * SG; k += 64)
+      hn1[(36 + k / SG) * SG + k % SG] = s_g[k / SG];
+    return;
+  }
+  // finite strains (mm10_calc_grads' geonl branch): per element, the fit
+  // of Rp R^T (Rp of crystal 1 from the element's own n-history, R = [R
+  // n+1]) taken to the current axes by the inverse (mm10_a_invert_33) of
+  // the element's n = 0 Jacobian at its last point; zeros for several
+  // crystals or a singular Jacobian
+  for (int i = l; i < S; i += 64) {
+    double gv[27];
+#pragma unroll
+    for (int k = 0; k < 27; ++k) gv[k] = 0.0;
+    const double* a = M.jac_tmp + 9 * ((int64_t)bd.felem - 1 + i);
+#define A3(r, c) a[3 * ((r)-1) + (c)-1]
+    const double det =
+        A3(1, 1) * A3(2, 2) * A3(3, 3) - A3(1, 1) * A3(2, 3) * A3(3, 2) -
+        A3(1, 2) * A3(2, 1) * A3(3, 3) + A3(1, 2) * A3(2, 3) * A3(3, 1) +
+        A3(1, 3) * A3(2, 1) * A3(3, 2) - A3(1, 3) * A3(2, 2) * A3(3, 1);
+    if (bd.ncry == 1 && fabs(det) > 1.0e-10) {
+      double ji[9];  // jacinv row-major: transpose(cofactor) / det
+      ji[0] = (A3(2, 2) * A3(3, 3) - A3(2, 3) * A3(3, 2)) / det;
+      ji[3] = -(A3(2, 1) * A3(3, 3) - A3(2, 3) * A3(3, 1)) / det;
+      ji[6] = (A3(2, 1) * A3(3, 2) - A3(2, 2) * A3(3, 1)) / det;
+      ji[1] = -(A3(1, 2) * A3(3, 3) - A3(1, 3) * A3(3, 2)) / det;
+      ji[4] = (A3(1, 1) * A3(3, 3) - A3(1, 3) * A3(3, 1)) / det;
+      ji[7] = -(A3(1, 1) * A3(3, 2) - A3(1, 2) * A3(3, 1)) / det;
+      ji[2] = (A3(1, 2) * A3(2, 3) - A3(1, 3) * A3(2, 2)) / det;
+      ji[5] = -(A3(1, 1) * A3(2, 3) - A3(1, 3) * A3(2, 1)) / det;
+      ji[8] = (A3(1, 1) * A3(2, 2) - A3(1, 2) * A3(2, 1)) / det;
+#undef A3
+      double sl[27];  // slopes (d) of component (a, b) at a + 3 b + 9 d
+#pragma unroll
+      for (int k = 0; k < 27; ++k) sl[k] = 0.0;
+      for (int g = 0; g < G; ++g) {
+        const int64_t q = (int64_t)i * G + g;
+        double Rp[9], R[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+          const int w = kMM10Common + 9 + k;  // Rp of crystal 1
+          Rp[k] = step == 1 ? mm10_init_word(M, bd, w, i, q, SG)
+                            : hn[w * SG + q];
+          R[k] = M.rot_tmp[9 * bd.q_off + k * SG + q];
+        }
+#pragma unroll
+        for (int ia = 0; ia < 3; ++ia)
+#pragma unroll
+          for (int ib = 0; ib < 3; ++ib) {
+            // mm10_a_mult_type_3t: (Rp R^T)(ia, ib)
+            const double v = Rp[ia] * R[ib] + Rp[ia + 3] * R[ib + 3] +
+                             Rp[ia + 6] * R[ib + 6];
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+              sl[ia + 3 * ib + 9 * d] = sl[ia + 3 * ib + 9 * d] +
+                                        gt->lsq[d][g] * v;
+          }
+      }
+#pragma unroll
+      for (int ab = 0; ab < 9; ++ab)
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          gv[ab + 9 * d] = ji[3 * d] * sl[ab] + ji[3 * d + 1] * sl[ab + 9] +
+                           ji[3 * d + 2] * sl[ab + 18];
+    }
+    for (int g = 0; g < G; ++g) {
+      const int64_t q = (int64_t)i * G + g;
+#pragma unroll
+      for (int k = 0; k < 27; ++k) hn1[(36 + k) * SG + q] = gv[k];
+    }
+  }
 }
 
 int launch_mm10_grads(const DevModel& M, const int32_t* d_blks, int nblks,

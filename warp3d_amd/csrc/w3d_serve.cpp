@@ -608,10 +608,8 @@ int w3d_batch_advance(w3d_batch* b, w3d_problem* const* ps, int32_t n,
       rc = launch_tangent_mfma(c->dev(), c->d_work_tan, c->tan_groups,
                                c->tan_start, c->tan_count, p->o.precision, s);
       if (!rc)
-        rc = launch_assemble(c->d_ek64, c->d_ek32,
-                             p->o.precision == W3D_FP32, c->d_asm_hdr,
-                             c->d_asm_src, c->d_asm_wbase, c->asm_nseg,
-                             c->neqns, p->d_k, p->d_k + p->neq, s);
+        rc = c->enqueue_assemble(p->o.precision == W3D_FP32, p->d_k,
+                                 p->d_k + p->neq, s);
       if (!rc) rc = enqueue_csr_values(c, p->d_k, p->d_k + p->neq);
       if (rc) return fail(rc, "w3d_batch_advance: tangent/assembly launch");
       c->tangent_precision = p->o.precision;
