@@ -136,9 +136,12 @@ def profiled_traffic(kernel, workload):
     return None
 
 
-def run_device(arr, cst, du, steps, warmup, precision, asym=False):
+def run_device(arr, cst, du, steps, warmup, precision, asym=False,
+               graph=False):
     """asym: asymmetric assembly (full FP64 Ke, CSR rows) in place of the
-    symmetric tangent and VSS assembly."""
+    symmetric tangent and VSS assembly.  graph: also time the same
+    iterations as one HIP graph launch each (w3d_iteration_dev), without
+    the per-phase events; info["graph_ms"]."""
     from warp3d_amd import hip
     from warp3d_amd.model import BlockModel
     dev = BlockModel(arr)
@@ -187,6 +190,18 @@ def run_device(arr, cst, du, steps, warmup, precision, asym=False):
     if not asym:
         info["k_coeffs"] = d_kc.download()
         info["k_diag"] = d_kd.download()
+    if graph and not asym:
+        it0 = 1 + warmup + steps
+        for w in range(2):   # capture, then one replay
+            dev.iteration_dev(2, it0 + w, d_u.ptr, d_du.ptr, d_ifv.ptr,
+                              precision, d_kd.ptr, d_kc.ptr)
+        hip.synchronize()
+        g0 = time.perf_counter()
+        for s in range(steps):
+            dev.iteration_dev(2, it0 + 2 + s, d_u.ptr, d_du.ptr, d_ifv.ptr,
+                              precision, d_kd.ptr, d_kc.ptr)
+        hip.synchronize()
+        info["graph_ms"] = (time.perf_counter() - g0) * 1e3 / steps
     return (t1 - t0) * 1e3 / steps, phases, info
 
 
@@ -484,7 +499,7 @@ def main():
     for prec, name in ((_abi.W3D_FP64, "fp64"), (_abi.W3D_FP32, "fp32")):
         t = time.perf_counter()
         ms, phases, info = run_device(arr, cst, du, args.steps, args.warmup,
-                                      prec)
+                                      prec, graph=True)
         res[name] = (ms, phases, info)
         log(f"{name}: {ms:.3f} ms/step  phases "
             + ", ".join(f"{k} {v[1] / max(v[0], 1):.3f} ms" for k, v in
@@ -568,6 +583,13 @@ def main():
                            "k_coeffs_rel_err": kc_err}
     out["speedup_bound_fp32_tangent"] = round(
         precision_bound(ph64, alg, parts, m.nonode, ncontrib, nout, ninv), 3)
+    # the same iteration as one HIP graph launch, without phase events
+    g32 = info.get("graph_ms")
+    if g32:
+        out["graph"] = {"ms_per_step": round(g32, 4),
+                        "value": round(ngp / (g32 * 1e-3), 1),
+                        "fp64_ms_per_step": round(
+                            res["fp64"][2].get("graph_ms", 0.0), 4)}
     if args.workload == "hex8" and not args.no_extra:
         out["fp32_tangent"].update(c1_precision_safe())
         out["hex20"] = extra_workload("hex20", args)
