@@ -61,7 +61,7 @@ def kernel_name(phase, workload, fp32=True):
     prec = "float" if fp32 else "double"
     return {"recover": f"k_recover<{tmpl}, 0, false>",
             "ifv_gather": "k_ifv_gather",
-            "tangent": f"k_tangent_mfma<{tmpl}, {prec}, false>",
+            "tangent": f"k_tangent_mfma<{tmpl}, {prec}, false, false>",
             "assemble": f"k_assemble<{prec}>",
             "energy_reduce": "k_energy_reduce",
             "mm10": "k_mm10"}[phase]
