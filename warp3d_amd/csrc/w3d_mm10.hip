@@ -69,6 +69,25 @@ struct CState {
 
 __device__ inline double sgn(double x) { return x >= 0.0 ? 1.0 : -1.0; }
 
+// x^y of the Voce functions: by repeated squaring when y is a whole number
+// in [0, 127] (the rate and Voce exponents of practice, e.g. n = 20, m = 1
+// or 5: some ulps from the correctly rounded power, far inside the local
+// solve's tolerances), ocml's pow otherwise.  y is a material constant, so
+// the branch is uniform.
+__device__ inline double vpow(double x, double y) {
+  const int k = (int)y;
+  if ((double)k == y && k >= 0 && k < 128) {
+    double r = 1.0, b = x;
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
+      if (k >> i & 1) r = r * b;
+      if ((k >> i) > 1) b = b * b;
+    }
+    return r;
+  }
+  return pow(x, y);
+}
+
 // [b] c with b 3x3 column-major
 __device__ inline void mv3(double* a, const double* b, const double* c) {
 #pragma unroll
@@ -321,7 +340,7 @@ __device__ inline void slips(const Props& p, const CState& s, const double* x,
 #pragma unroll
   for (int i = 0; i < NS; ++i) {
     rs[i] = rs_of(s, x, i);
-    gam[i] = s.dg / tt * pow(fabs(rs[i] / tt), p.rate_n - 1.0) * rs[i];
+    gam[i] = s.dg / tt * vpow(fabs(rs[i] / tt), p.rate_n - 1.0) * rs[i];
   }
 }
 
@@ -386,7 +405,7 @@ __device__ inline double h_voche(const Props& p, const double* gam, double tt,
                                  double tau_n) {
   double h = 0.0;
   double ht = 1.0 - (tt - p.tau_y) / p.tau_v + 0.0 / (tt - p.tau_y);
-  double f = pow(fabs(ht), p.voce_m) * sgn(ht);
+  double f = vpow(fabs(ht), p.voce_m) * sgn(ht);
 #pragma unroll
   for (int i = 0; i < NS; ++i) h = h + f * fabs(gam[i]);
   return tau_n + p.theta_0 * h;
@@ -411,12 +430,12 @@ __device__ inline void formJ11_slips(const Props& p, const CState& s,
                                      const double* rs, const double* gam,
                                      double* J11) {
   const double* C = p.cr->C;
-  const double ptt = pow(tt, p.rate_n);
+  const double ptt = vpow(tt, p.rate_n);
 #pragma unroll
   for (int k = 0; k < 36; ++k) J11[k] = 0.0;
 #pragma unroll
   for (int i = 0; i < NS; ++i) {
-    double d = pow(fabs(rs[i]), p.rate_n - 1.0);
+    double d = vpow(fabs(rs[i]), p.rate_n - 1.0);
     d = s.dg * p.rate_n / ptt * d;
     double dgdt = d + s.tinc * p.iD_v;
     double st[6], wv[6];
@@ -500,15 +519,15 @@ __device__ inline void formJ(const Props& p, const CState& s, const double* x,
     }
   }
   const double ht = 1.0 - (tt - p.tau_y) / p.tau_v + 0.0 / (tt - p.tau_y);
-  const double hm = pow(fabs(ht), p.voce_m);
+  const double hm = vpow(fabs(ht), p.voce_m);
   double et[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
   for (int i = 0; i < NS; ++i) {
-    double c = hm * sgn(ht) * pow(fabs(rs[i]), p.rate_n - 2.0) * rs[i];
+    double c = hm * sgn(ht) * vpow(fabs(rs[i]), p.rate_n - 2.0) * rs[i];
 #pragma unroll
     for (int k = 0; k < 6; ++k) et[k] = et[k] + c * s.ms(i, k);
   }
-  const double sc = p.theta_0 * s.dg * p.rate_n / pow(tt, p.rate_n);
+  const double sc = p.theta_0 * s.dg * p.rate_n / vpow(tt, p.rate_n);
   double etau = 0.0;
   const double dty = tt - p.tau_y;
 #pragma unroll
@@ -547,7 +566,7 @@ __device__ inline double dot6(const double* a, const double* b) {
 // mm10_solve: predictor (6x6 Newton at fixed extrapolated tau) then the
 // 7x7 corrector; Jg (49 words at stride jstride) receives the Jacobian of
 // the last corrector iteration
-__device__ void solve(const Props& p, CState& s, const double* sn,
+__device__ __forceinline__ void solve(const Props& p, CState& s, const double* sn,
                       double tau_n, double tt_rate_n, const double* Dn,
                       double* stress, double* tt, bool& fail, double dtinc,
                       double* Jg, int64_t jstride) {
@@ -1093,10 +1112,10 @@ __device__ __forceinline__ void mm10_item(const DevModel& M,
 #pragma unroll
         for (int k = 0; k < 6; ++k) ep[k] = ep[k] / np1.tinc;
         n_eff = 0.0;
-        const double ptt = pow(tt, p.rate_n);
+        const double ptt = vpow(tt, p.rate_n);
 #pragma unroll
         for (int i = 0; i < NS; ++i) {
-          double d = pow(fabs(rs[i]), p.rate_n - 1.0);
+          double d = vpow(fabs(rs[i]), p.rate_n - 1.0);
           d = np1.dg * p.rate_n / ptt * d;
           double dgdt = d + np1.tinc * p.iD_v;
           double a = np1.ms(i, 0) * ep[0] + np1.ms(i, 1) * ep[1] +
