@@ -159,6 +159,7 @@ __device__ inline void rt2rvw(const double* rt, double* rv) {
 template <int N, int NRHS>
 __device__ inline void lu_solve(double* a, double* b) {
   int piv[N];
+  double rp[N];  // the pivots' reciprocals, for the back substitution
 #pragma unroll
   for (int j = 0; j < N; ++j) {
     int p = j;
@@ -183,7 +184,8 @@ __device__ inline void lu_solve(double* a, double* b) {
         }
       }
     }
-    double r = 1.0 / a[j + N * j];
+    const double r = 1.0 / a[j + N * j];
+    rp[j] = r;
 #pragma unroll
     for (int i = j + 1; i < N; ++i) a[i + N * j] = a[i + N * j] * r;
 #pragma unroll
@@ -219,7 +221,7 @@ __device__ inline void lu_solve(double* a, double* b) {
 #pragma unroll
     for (int k = N - 1; k >= 0; --k)
       if (x[k] != 0.0) {
-        x[k] = x[k] / a[k + N * k];
+        x[k] = x[k] * rp[k];
 #pragma unroll
         for (int i = 0; i < k; ++i) x[i] = x[i] - x[k] * a[i + N * k];
       }
@@ -337,10 +339,13 @@ __device__ inline void symsw(const double* s, const double* w, double* sw) {
 // resolved shears and slip increments at x (stress) and tau
 __device__ inline void slips(const Props& p, const CState& s, const double* x,
                              double tt, double* rs, double* gam) {
+  // one reciprocal of tau for the twelve quotients (a few ulps from the
+  // quotients themselves, far inside the local solve's tolerances)
+  const double itt = 1.0 / tt, dgt = s.dg * itt;
 #pragma unroll
   for (int i = 0; i < NS; ++i) {
     rs[i] = rs_of(s, x, i);
-    gam[i] = s.dg / tt * vpow(fabs(rs[i] / tt), p.rate_n - 1.0) * rs[i];
+    gam[i] = dgt * vpow(fabs(rs[i] * itt), p.rate_n - 1.0) * rs[i];
   }
 }
 
@@ -424,11 +429,14 @@ __device__ inline void formR(const Props& p, CState& s, const double* sn,
   s.tt_rate = (h - tau_n) / s.tinc;
 }
 
-// formJ11 (6x6, column-major) from the slips (rs, gam) at (x, tt)
+// formJ11 (6x6, column-major) from the slips (rs, gam) at (x, tt); with
+// J12 also formJ's J12 column, from the same per-system vectors
+// 2 (sym(x) w_i) + C ms_i that J11 takes
+template <bool WITH_J12 = false>
 __device__ inline void formJ11_slips(const Props& p, const CState& s,
                                      const double* x, double tt,
                                      const double* rs, const double* gam,
-                                     double* J11) {
+                                     double* J11, double* J12 = nullptr) {
   const double* C = p.cr->C;
   const double ptt = vpow(tt, p.rate_n);
 #pragma unroll
@@ -454,6 +462,13 @@ __device__ inline void formJ11_slips(const Props& p, const CState& s,
       if (s.ms(i, c) != 0.0) {
 #pragma unroll
         for (int r = 0; r < 6; ++r) J11[r + 6 * c] = J11[r + 6 * c] + wv[r] * t;
+      }
+    }
+    if (WITH_J12) {
+      const double dgdh = -p.rate_n / tt * gam[i];
+      if (dgdh != 0.0) {
+#pragma unroll
+        for (int r = 0; r < 6; ++r) J12[r] = J12[r] + wv[r] * (1.0 * dgdh);
       }
     }
   }
@@ -497,27 +512,8 @@ __device__ inline void formJ(const Props& p, const CState& s, const double* x,
   double rs[NS], gam[NS];
   slips(p, s, x, tt, rs, gam);  // once for J11 and the rest (bitwise)
   double J11[36];
-  formJ11_slips(p, s, x, tt, rs, gam, J11);
-  const double* C = p.cr->C;
   double J12[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll
-  for (int i = 0; i < NS; ++i) {
-    double st[6], tm[6];
-    double qv[3];
-    s.qs3(i, qv);
-    symsw(x, qv, st);
-    double dgdh = -p.rate_n / tt * gam[i];
-#pragma unroll
-    for (int r = 0; r < 6; ++r) tm[r] = 2.0 * st[r];
-#pragma unroll
-    for (int j = 0; j < 6; ++j)
-#pragma unroll
-      for (int r = 0; r < 6; ++r) tm[r] = tm[r] + C[r + 6 * j] * s.ms(i, j);
-    if (dgdh != 0.0) {
-#pragma unroll
-      for (int r = 0; r < 6; ++r) J12[r] = J12[r] + tm[r] * (1.0 * dgdh);
-    }
-  }
+  formJ11_slips<true>(p, s, x, tt, rs, gam, J11, J12);
   const double ht = 1.0 - (tt - p.tau_y) / p.tau_v + 0.0 / (tt - p.tau_y);
   const double hm = vpow(fabs(ht), p.voce_m);
   double et[6] = {0, 0, 0, 0, 0, 0};
@@ -530,11 +526,12 @@ __device__ inline void formJ(const Props& p, const CState& s, const double* x,
   const double sc = p.theta_0 * s.dg * p.rate_n / vpow(tt, p.rate_n);
   double etau = 0.0;
   const double dty = tt - p.tau_y;
+  const double iht = 1.0 / fabs(ht), ntt = p.rate_n / tt;
 #pragma unroll
   for (int i = 0; i < NS; ++i)
     etau = etau + (p.voce_m * (-1.0 / p.tau_v - 0.0 / (dty * dty)) *
-                       fabs(gam[i]) / fabs(ht) -
-                   fabs(gam[i]) * p.rate_n / tt * sgn(ht) * sgn(gam[i])) *
+                       (fabs(gam[i]) * iht) -
+                   fabs(gam[i]) * ntt * sgn(ht) * sgn(gam[i])) *
                       hm;
   etau = p.theta_0 * etau;
   etau = 1.0 - etau;
