@@ -155,7 +155,11 @@ __device__ inline void rt2rvw(const double* rt, double* rv) {
 }
 
 // LU with partial pivoting (first largest |pivot|) and the solve of
-// nrhs right-hand sides, column-major with leading dimension N.
+// nrhs right-hand sides, column-major with leading dimension N.  The
+// eliminations and substitutions do not skip zero multipliers as LAPACK's
+// loops do: across a wave the skip would be a divergent branch that saves
+// nothing, and adding a zero product changes no value (signed zeros
+// aside).
 template <int N, int NRHS>
 __device__ inline void lu_solve(double* a, double* b) {
   int piv[N];
@@ -191,11 +195,9 @@ __device__ inline void lu_solve(double* a, double* b) {
 #pragma unroll
     for (int k = j + 1; k < N; ++k) {
       double t = -a[j + N * k];
-      if (t != 0.0) {
 #pragma unroll
-        for (int i = j + 1; i < N; ++i)
-          a[i + N * k] = a[i + N * k] + a[i + N * j] * t;
-      }
+      for (int i = j + 1; i < N; ++i)
+        a[i + N * k] = a[i + N * k] + a[i + N * j] * t;
     }
   }
 #pragma unroll
@@ -213,18 +215,16 @@ __device__ inline void lu_solve(double* a, double* b) {
       }
     }
 #pragma unroll
-    for (int k = 0; k < N; ++k)
-      if (x[k] != 0.0) {
+    for (int k = 0; k < N; ++k) {
 #pragma unroll
-        for (int i = k + 1; i < N; ++i) x[i] = x[i] - x[k] * a[i + N * k];
-      }
+      for (int i = k + 1; i < N; ++i) x[i] = x[i] - x[k] * a[i + N * k];
+    }
 #pragma unroll
-    for (int k = N - 1; k >= 0; --k)
-      if (x[k] != 0.0) {
-        x[k] = x[k] * rp[k];
+    for (int k = N - 1; k >= 0; --k) {
+      x[k] = x[k] * rp[k];
 #pragma unroll
-        for (int i = 0; i < k; ++i) x[i] = x[i] - x[k] * a[i + N * k];
-      }
+      for (int i = 0; i < k; ++i) x[i] = x[i] - x[k] * a[i + N * k];
+    }
   }
 }
 
@@ -459,17 +459,13 @@ __device__ inline void formJ11_slips(const Props& p, const CState& s,
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
       double t = dgdt * s.ms(i, c);
-      if (s.ms(i, c) != 0.0) {
 #pragma unroll
-        for (int r = 0; r < 6; ++r) J11[r + 6 * c] = J11[r + 6 * c] + wv[r] * t;
-      }
+      for (int r = 0; r < 6; ++r) J11[r + 6 * c] = J11[r + 6 * c] + wv[r] * t;
     }
     if (WITH_J12) {
       const double dgdh = -p.rate_n / tt * gam[i];
-      if (dgdh != 0.0) {
 #pragma unroll
-        for (int r = 0; r < 6; ++r) J12[r] = J12[r] + wv[r] * (1.0 * dgdh);
-      }
+      for (int r = 0; r < 6; ++r) J12[r] = J12[r] + wv[r] * (1.0 * dgdh);
     }
   }
   double wp[3];
@@ -565,6 +561,7 @@ __device__ inline double dot6(const double* a, const double* b) {
 // the last corrector iteration
 __device__ __forceinline__ void solve(const Props& p, CState& s, const double* sn,
                       double tau_n, double tt_rate_n, const double* Dn,
+                      int64_t dn_stride,
                       double* stress, double* tt, bool& fail, double dtinc,
                       double* Jg, int64_t jstride) {
   const double c = 1.0e-4, red = 0.5;
@@ -593,7 +590,7 @@ __device__ __forceinline__ void solve(const Props& p, CState& s, const double* s
       for (int k = 0; k < 6; ++k) d1[k] = d1[k] / sqrt(t1 + t2);
     }
 #pragma unroll
-    for (int k = 0; k < 6; ++k) d2[k] = Dn[k];
+    for (int k = 0; k < 6; ++k) d2[k] = Dn ? Dn[k * dn_stride] : 0.0;
     dtrace = (d2[0] + d2[1] + d2[2]) / 3.0;
     d2[0] = d2[0] - dtrace;
     d2[1] = d2[1] - dtrace;
@@ -835,22 +832,20 @@ __device__ __forceinline__ void mm10_item(const DevModel& M,
     const int64_t cb = (int64_t)(87 + 71 * c) * SG;
     // n-state of the crystal (mm10_copy_cc_hist; at step 1 the
     // mm10_init_cc_hist0 values)
-    double sn[6], Rpn[9], Dn[6], tau_n, tt_rate_n;
+    // (Dn, the n strain increment, is read where the predictor uses it)
+    double sn[6], Rpn[9], tau_n, tt_rate_n;
+    const double* Dn = step == 1 ? nullptr : hn + cb + 18 * SG;
     if (step == 1) {
 #pragma unroll
       for (int k = 0; k < 6; ++k) sn[k] = sn9[k * SG];
 #pragma unroll
       for (int k = 0; k < 9; ++k) Rpn[k] = (k % 4 == 0) ? 1.0 : 0.0;
-#pragma unroll
-      for (int k = 0; k < 6; ++k) Dn[k] = 0.0;
       tau_n = p.tau_y + 1.0e-5;
     } else {
 #pragma unroll
       for (int k = 0; k < 6; ++k) sn[k] = hn[cb + k * SG];
 #pragma unroll
       for (int k = 0; k < 9; ++k) Rpn[k] = hn[cb + (9 + k) * SG];
-#pragma unroll
-      for (int k = 0; k < 6; ++k) Dn[k] = hn[cb + (18 + k) * SG];
       tau_n = hn[cb + 42 * SG];
     }
     tt_rate_n = hn[cb + 58 * SG];
@@ -901,18 +896,22 @@ __device__ __forceinline__ void mm10_item(const DevModel& M,
         for (int k = 0; k < 6; ++k) stress[k] = stress[k] - R1[k];
       }
     } else {
+      // the sub-steps run on np1 itself, their increments scaled from
+      // the point's full-step values in memory (ud, dt); np1's full-step
+      // D, tinc and dg are formed again after the loop, so that they are
+      // not held across the local solves
       double frac = 0.0, sstep = 1.0;
       int cuts = 0;
-      CState curr = np1;
+      CState& curr = np1;
       while (frac < 1.0) {
 #pragma unroll
-        for (int k = 0; k < 6; ++k) curr.D[k] = np1.D[k] * (sstep + frac);
-        curr.tinc = np1.tinc * (sstep + frac);
+        for (int k = 0; k < 6; ++k) curr.D[k] = ud[k * SG] * (sstep + frac);
+        curr.tinc = dt * (sstep + frac);
         curr.tt_rate = 0.0;
         setup(p, curr);
         tt = tau_n;
-        solve(p, curr, sn, tau_n, tt_rate_n, Dn, stress, &tt, fail,
-              np1.tinc * sstep, Jg, SG);
+        solve(p, curr, sn, tau_n, tt_rate_n, Dn, SG, stress, &tt, fail,
+              dt * sstep, Jg, SG);
         if (fail) {
 #pragma unroll
           for (int k = 0; k < 6; ++k) stress[k] = ostress[k];
@@ -933,6 +932,10 @@ __device__ __forceinline__ void mm10_item(const DevModel& M,
         break;
       }
       curr_tt_rate = curr.tt_rate;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) np1.D[k] = ud[k * SG];
+      np1.tinc = dt;
+      setup(p, np1);
     }
     np1.tt_rate = curr_tt_rate;
     double eul[3] = {0, 0, 0}, Rp1[9], ee[6], slip[NS], u[15], ep6[6],
