@@ -88,6 +88,35 @@ __device__ inline double vpow(double x, double y) {
   return pow(x, y);
 }
 
+// vpow over an array, in place: the exponent's bits are tested once for all
+// N values (uniform branches around straight runs of N products); every
+// value takes the products vpow takes.
+template <int N>
+__device__ inline void vpow_arr(double* v, double y) {
+  const int k = (int)y;
+  if ((double)k == y && k >= 0 && k < 128) {
+    double r[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) r[j] = 1.0;
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
+      if (k >> i & 1) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) r[j] = r[j] * v[j];
+      }
+      if ((k >> i) > 1) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) v[j] = v[j] * v[j];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) v[j] = r[j];
+  } else {
+#pragma unroll
+    for (int j = 0; j < N; ++j) v[j] = pow(v[j], y);
+  }
+}
+
 // [b] c with b 3x3 column-major
 __device__ inline void mv3(double* a, const double* b, const double* c) {
 #pragma unroll
@@ -345,8 +374,11 @@ __device__ inline void slips(const Props& p, const CState& s, const double* x,
 #pragma unroll
   for (int i = 0; i < NS; ++i) {
     rs[i] = rs_of(s, x, i);
-    gam[i] = dgt * vpow(fabs(rs[i] * itt), p.rate_n - 1.0) * rs[i];
+    gam[i] = fabs(rs[i] * itt);
   }
+  vpow_arr<NS>(gam, p.rate_n - 1.0);
+#pragma unroll
+  for (int i = 0; i < NS; ++i) gam[i] = dgt * gam[i] * rs[i];
 }
 
 // form_wp (with qc) and form_wbarp (with qs: a state whose rw is that
@@ -441,9 +473,13 @@ __device__ inline void formJ11_slips(const Props& p, const CState& s,
   const double ptt = vpow(tt, p.rate_n);
 #pragma unroll
   for (int k = 0; k < 36; ++k) J11[k] = 0.0;
+  double dp[NS];
+#pragma unroll
+  for (int i = 0; i < NS; ++i) dp[i] = fabs(rs[i]);
+  vpow_arr<NS>(dp, p.rate_n - 1.0);
 #pragma unroll
   for (int i = 0; i < NS; ++i) {
-    double d = vpow(fabs(rs[i]), p.rate_n - 1.0);
+    double d = dp[i];
     d = s.dg * p.rate_n / ptt * d;
     double dgdt = d + s.tinc * p.iD_v;
     double st[6], wv[6];
@@ -513,9 +549,13 @@ __device__ inline void formJ(const Props& p, const CState& s, const double* x,
   const double ht = 1.0 - (tt - p.tau_y) / p.tau_v + 0.0 / (tt - p.tau_y);
   const double hm = vpow(fabs(ht), p.voce_m);
   double et[6] = {0, 0, 0, 0, 0, 0};
+  double dp[NS];
+#pragma unroll
+  for (int i = 0; i < NS; ++i) dp[i] = fabs(rs[i]);
+  vpow_arr<NS>(dp, p.rate_n - 2.0);
 #pragma unroll
   for (int i = 0; i < NS; ++i) {
-    double c = hm * sgn(ht) * vpow(fabs(rs[i]), p.rate_n - 2.0) * rs[i];
+    double c = hm * sgn(ht) * dp[i] * rs[i];
 #pragma unroll
     for (int k = 0; k < 6; ++k) et[k] = et[k] + c * s.ms(i, k);
   }
