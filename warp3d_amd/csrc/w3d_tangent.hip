@@ -90,12 +90,13 @@ __device__ inline void column_xyz(T bx, T by, T bz, const T* vol, int j,
   }
 }
 
-// Waves per SIMD the register allocation aims at: four for the 8-node
-// brick's small-strain tangent, whose LDS (under 40 KB a workgroup of
-// four waves) lets four workgroups share a CU; two for the others, whose
-// LDS holds the CU to fewer.
+// Waves per SIMD the register allocation aims at: three for the 8-node
+// brick's small-strain tangent (168 registers, no spills; at four its
+// 128-register cap spilled 14 words of the batch set-up and the FP64
+// tangent ran 11 % slower, the FP32 one the same); two for the others,
+// whose LDS holds the CU to fewer.
 __host__ __device__ constexpr int tan_waves_per_eu(int nnode, bool geonl) {
-  return nnode == 8 && !geonl ? 4 : 2;
+  return nnode == 8 && !geonl ? 3 : 2;
 }
 
 template <int NN, int G, bool BBAR, typename T, bool GEONL = false,
